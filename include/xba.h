@@ -1,0 +1,170 @@
+/*
+ * xba.h -- C ABI of the MI355X (gfx950) sliding-window bundle-adjustment solver (libxba.so).
+ *
+ * What each entry point replaces in the reference (paths relative to the reference checkout's src/xclean/):
+ *   xba_create / xba_upload / xba_iterate / xba_download
+ *       vio/window.py:889-982 `solve(pb, iters=5, lam0=1e-4)` for a BATCH of windows: the same LM iteration
+ *       (build_system :291-463, prior and IMU terms :831-886, damped Cholesky :933-941, back-substitution
+ *       :496-519, on-manifold update :522-540, trial cost :466-493, accept/reject :967-981), with every window's
+ *       accept/reject decided on the device.  The reference binds it as a Python function over a `Problem`
+ *       (window.py:768-792); xclean_amd/window.py is the Python mirror that packs Problems and calls these.
+ *   xba_linearize
+ *       vio/window.py:985-1008 `final_linearisation(pb)` (and one build_system at any damping): the reduced
+ *       system H, b of every window, plus per-observation whitened chi2 and the landmark marginals.
+ *   xba_np_solve
+ *       backend/ba.py:713-869 `_solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0, prior, rotf)` (and its
+ *       FP32 twin `_solve_gpu`, ba.py:930-1089): the local-BA drop-in that tests/test_ba_solver.py calls.
+ *   xba_backward
+ *       no reference counterpart (the reference has no differentiable solve): the implicit-function adjoint of
+ *       the last solved update with respect to observed pixels and per-observation weights.
+ *
+ * Conventions (vio/CONVENTIONS.md): body poses T_wb, R_wb <- R_wb Exp(dphi), p_wb <- p_wb + dp; VI blocks
+ * [dphi, dp, dv, dbg, dba]; landmarks (a, b, rho) hosted in a keyframe; whitened residual = (pred - meas) * isig;
+ * Huber on the whitened norm; normal equations H dx = -b.
+ *
+ * Memory: arrays marked (host) are read during the call only; arrays marked (device) are device pointers that
+ * must stay valid for the lifetime of the context (state arrays are read by xba_upload and written by
+ * xba_download).  Every call is asynchronous on `stream` (a hipStream_t, 0 = the null stream) unless noted.
+ * Return codes: 0 ok; < 0 an error (no exception ever crosses this ABI); xba_strerror(code) describes it.
+ */
+#ifndef XBA_H
+#define XBA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XBA_OK 0
+#define XBA_E_ARG (-1)          /* invalid argument (sizes, NULL pointer, index out of range) */
+#define XBA_E_HIP (-2)          /* a HIP runtime call failed */
+#define XBA_E_TOO_MANY_OBS (-3) /* a landmark has more observations than XBA_MAX_OBS_PER_LM */
+#define XBA_E_RANGE (-4)        /* a landmark couples slots further apart than the kernel's slot tile allows */
+#define XBA_E_SIZE (-5)         /* a reduced system is larger than XBA_MAX_STATE */
+#define XBA_E_STATE (-6)        /* call order: e.g. xba_iterate before xba_upload */
+
+#define XBA_MAX_OBS_PER_LM 64   /* one wavefront lane per observation of a landmark */
+#define XBA_MAX_STATE 1024      /* reduced-system dimension per window */
+#define XBA_IMU_REC 301         /* packed IMU factor: dR 9, dv 3, dp 3, dt 1, J 54, bg_lin 3, ba_lin 3, W 225 */
+#define XBA_WPAR 24             /* per-window parameters: fx fy cx cy B huber_k imu_k pad Rbc(9) pbc(3) g_w(3) pad */
+
+enum xba_precision {
+    XBA_FP64 = 0,  /* assembly, Schur and Cholesky in FP64 (the plain path) */
+    XBA_MIXED = 1  /* assembly and Schur in FP32 with FP32 accumulation; FP32 Cholesky of the Jacobi-scaled
+                      system with FP64 iterative refinement; FP64 SE(3) update and state */
+};
+
+/* A batch of windows, all arrays concatenated over windows in window order.  Slots, landmarks, observations,
+ * IMU factors and prior blocks of window w are the ranges [*_0[w], *_0[w+1]) of the concatenated arrays; slot
+ * indices stored in lm_host, ob_pose, imu_ij and prior_slot are window-local.  The observations of landmark l are
+ * [ob_start[l], ob_start[l+1]) (global indices). */
+typedef struct xba_batch {
+    int32_t nwin;
+    /* (host) structure, read by xba_create */
+    const int32_t *win_slot0;   /* (nwin+1) */
+    const int32_t *win_lm0;     /* (nwin+1) */
+    const int32_t *win_imu0;    /* (nwin+1) */
+    const int32_t *win_pblk0;   /* (nwin+1) prior blocks */
+    const int32_t *win_prow0;   /* (nwin+1) prior rows (rows of S, r) */
+    const int32_t *slot_dim;    /* (nslot) 6 or 15 */
+    const int32_t *ob_start;    /* (nlm+1) */
+    const int32_t *lm_host;     /* (nlm) */
+    const int32_t *ob_pose;     /* (nobs) */
+    const int32_t *imu_ij;      /* (nimu,2) */
+    const int32_t *prior_slot;  /* (npblk) */
+    const int32_t *prior_dim;   /* (npblk) 6 or 15 */
+    /* (device) data */
+    double *Rwb, *pwb;          /* (nslot,3,3) (nslot,3) state */
+    double *vel, *bgs, *bas;    /* (nslot,3) state of 15-dim slots (may be NULL when every slot is 6-dim) */
+    double *lmp;                /* (nlm,3) state (a, b, rho) */
+    const uint8_t *lm_mask;     /* (nlm) */
+    const double *rho_meas;     /* (nlm) */
+    const double *rho_isig;     /* (nlm) 0 = no inverse-depth prior */
+    const double *ob_z;         /* (nobs,2) measured pixel (cam 0: u, v; cam 1: u_R, unused) */
+    const double *ob_isig;      /* (nobs) whitening 1/sigma */
+    const uint8_t *ob_cam;      /* (nobs) 0 left 2-D, 1 right 1-D */
+    const uint8_t *ob_valid;    /* (nobs) */
+    const double *win_par;      /* (nwin, XBA_WPAR) */
+    const double *imu_rec;      /* (nimu, XBA_IMU_REC) */
+    const double *prior_x0;     /* (npblk, 21): R0 9, p0 3, v0 3, bg0 3, ba0 3 */
+    const double *prior_S;      /* window w: rows [win_prow0[w], win_prow0[w+1]) x (sum of its prior dims), packed */
+    const double *prior_r;      /* (nprow) */
+} xba_batch;
+
+typedef struct xba_options {
+    int32_t precision;          /* enum xba_precision */
+    int32_t lm_per_chunk;       /* landmarks per work item of the assembly kernels; 0 = automatic */
+    int32_t slot_tile;          /* slot range of one work item's LDS tile; 0 = automatic */
+    int32_t grid;               /* persistent grid of the assembly kernels (workgroups); 0 = automatic */
+    int32_t use_graph;          /* 1: capture one LM iteration in a hipGraph and replay it */
+    int32_t want_chi2;          /* 1: xba_linearize writes per-observation chi2 */
+} xba_options;
+
+typedef struct xba_ctx xba_ctx;
+
+/* Plan a batch: validate the structure, build the work list, allocate device workspace.  Synchronous. */
+int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt);
+int xba_destroy(xba_ctx *ctx);
+
+/* Copy the state (and measurements, in the working precision) from the batch's device arrays into the context. */
+int xba_upload(xba_ctx *ctx, double lam0, void *stream);
+
+/* Run `iters` LM iterations of every window on device-resident data (vio/window.py:889-982).  No host sync. */
+int xba_iterate(xba_ctx *ctx, int32_t iters, void *stream);
+
+/* Write the state back into the batch's device state arrays. */
+int xba_download(xba_ctx *ctx, void *stream);
+
+/* Per-window statistics after xba_iterate, (device) out (nwin, 8): cost0, cost, accepted, iters, lam,
+ * last Cholesky status (1 ok, 0 failed), last accept (1/0), refinement steps used. */
+int xba_stats(xba_ctx *ctx, double *out_dev, void *stream);
+
+/* The last iteration's full reduced update dx (sum of window state dims, FP64, device out). */
+int xba_last_dx(xba_ctx *ctx, double *out_dev, void *stream);
+
+/* One linearisation at damping `lam` at the context's current state (final_linearisation when lam == 0):
+ * (device) H_out: window w's n_w x n_w block at offset sum_{v<w} n_v^2; b_out: (sum n_w); cost_out (nwin):
+ * robust visual + prior + IMU cost; chi2_out (nobs, may be NULL): whitened squared residual, +inf when not
+ * evaluable; Hinv_out (nlm,3,3) and ok_out (nlm) (may be NULL): the damped landmark inverses; imu_chi2_out (nimu,
+ * may be NULL): the whitened chi2 of every IMU factor. */
+int xba_linearize(xba_ctx *ctx, double lam, double *H_out, double *b_out, double *cost_out, double *chi2_out,
+                  double *Hinv_out, uint8_t *ok_out, double *imu_chi2_out, void *stream);
+
+/* Sizes the caller needs for the outputs above: total state dimension and total H entries. */
+int xba_sizes(const xba_ctx *ctx, int64_t *n_state, int64_t *n_hentries);
+
+/* Implicit-function adjoint of the last iteration's solved update dx* = -H^-1 b (Gauss-Newton H, at the state
+ * the update was linearised at, reusing the Cholesky factor): given g = dL/d(dx) (device, sum n_w), writes
+ * dL/dz (device, nobs x 2) and dL/dw (device, nobs) where w scales each observation's whitened residual weight. */
+int xba_backward(xba_ctx *ctx, const double *g_dx, double *grad_z, double *grad_w, void *stream);
+
+/* ---- the Euclidean local-BA drop-in (backend/ba.py:713-869) ---------------------------------------------------
+ * poses (device, nk x 4 x 4, T_wc, in/out), pts (device, nl x 3, in/out), okf/olm (device int32, N), ouv (device,
+ * N x 2), free_mask (device uint8, nk), optional prior (pr_c (nl,3), pr_rho0 (nl), pr_valid (nl); NULL = none) and
+ * rotation factors (rot_ab (n_rot,2) int32, rot_M (n_rot,3,3), rot_sig (n_rot); n_rot = 0: none).  stats_out
+ * (device, 4): chi2_0, chi2_n, rms_0, rms_n.  Synchronous (host-side planning from okf/olm copies). */
+typedef struct xba_np_problem {
+    int32_t nk, nl, N;
+    double *poses, *pts;
+    const int32_t *okf, *olm;
+    const double *ouv;
+    const uint8_t *free_mask;
+    double fx, cx, cy, huber_k;
+    const double *pr_c, *pr_rho0, *pr_valid;
+    double pr_invsig;
+    int32_t n_rot;
+    const int32_t *rot_ab;
+    const double *rot_M, *rot_sig;
+} xba_np_problem;
+
+int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                 double *dx_hist_out, void *stream);
+
+const char *xba_strerror(int code);
+const char *xba_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XBA_H */

@@ -1,0 +1,97 @@
+"""ctypes binding of libxba.so (include/xba.h).
+
+The shared library is built in-tree by ``xclean_amd.build`` (``__graft_entry__.build()``).  There is no fallback:
+if the library is missing, or no GPU is visible, the solver raises instead of computing anything elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxba.so")
+_LIB = None
+
+XBA_FP64, XBA_MIXED = 0, 1
+XBA_WPAR = 24
+XBA_IMU_REC = 301
+
+vp = C.c_void_p
+i32p = C.POINTER(C.c_int32)
+
+
+class XbaBatch(C.Structure):
+    _fields_ = [("nwin", C.c_int32),
+                ("win_slot0", i32p), ("win_lm0", i32p), ("win_imu0", i32p), ("win_pblk0", i32p),
+                ("win_prow0", i32p), ("slot_dim", i32p), ("ob_start", i32p), ("lm_host", i32p), ("ob_pose", i32p),
+                ("imu_ij", i32p), ("prior_slot", i32p), ("prior_dim", i32p),
+                ("Rwb", vp), ("pwb", vp), ("vel", vp), ("bgs", vp), ("bas", vp), ("lmp", vp),
+                ("lm_mask", vp), ("rho_meas", vp), ("rho_isig", vp), ("ob_z", vp), ("ob_isig", vp),
+                ("ob_cam", vp), ("ob_valid", vp), ("win_par", vp), ("imu_rec", vp), ("prior_x0", vp),
+                ("prior_S", vp), ("prior_r", vp)]
+
+
+class XbaOptions(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("lm_per_chunk", C.c_int32), ("slot_tile", C.c_int32),
+                ("grid", C.c_int32), ("use_graph", C.c_int32), ("want_chi2", C.c_int32)]
+
+
+class XbaNpProblem(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32),
+                ("poses", vp), ("pts", vp), ("okf", vp), ("olm", vp), ("ouv", vp), ("free_mask", vp),
+                ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
+                ("pr_c", vp), ("pr_rho0", vp), ("pr_valid", vp), ("pr_invsig", C.c_double),
+                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
+
+
+class XbaError(RuntimeError):
+    pass
+
+
+# every symbol include/xba.h declares
+EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
+           "xba_linearize", "xba_sizes", "xba_backward", "xba_np_solve", "xba_strerror", "xba_version")
+
+
+def load():
+    """Load libxba.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise XbaError("libxba.so is not built (%s): run __graft_entry__.build() or python -m xclean_amd.build"
+                       % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.xba_create.argtypes = [C.POINTER(vp), C.POINTER(XbaBatch), C.POINTER(XbaOptions)]
+    L.xba_destroy.argtypes = [vp]
+    L.xba_upload.argtypes = [vp, C.c_double, vp]
+    L.xba_iterate.argtypes = [vp, C.c_int32, vp]
+    L.xba_download.argtypes = [vp, vp]
+    L.xba_stats.argtypes = [vp, vp, vp]
+    L.xba_last_dx.argtypes = [vp, vp, vp]
+    L.xba_linearize.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.xba_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.xba_backward.argtypes = [vp, vp, vp, vp, vp]
+    L.xba_np_solve.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
+    L.xba_strerror.restype = C.c_char_p
+    L.xba_strerror.argtypes = [C.c_int]
+    L.xba_version.restype = C.c_char_p
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if fn.restype is C.c_int:
+            pass
+    _LIB = L
+    return L
+
+
+def check(code, what):
+    if code != 0:
+        msg = load().xba_strerror(int(code)).decode()
+        raise XbaError("%s failed: %s (%d)" % (what, msg, code))
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise XbaError("xclean_amd needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path")
+    load()

@@ -1,0 +1,618 @@
+// libxba: the C ABI of include/xba.h -- planning (work list, offsets, workspace), upload/download of the working
+// state, the LM iteration as a fixed kernel sequence (optionally replayed from one captured hipGraph), and the
+// linearisation entry point.  One translation unit: the kernels are included below.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "xba_internal.h"
+#include "xba_lin.hip"
+#include "xba_dense.hip"
+
+using namespace xba;
+
+#define HIPCK(x)                                                   \
+    do {                                                           \
+        hipError_t e_ = (x);                                       \
+        if (e_ != hipSuccess) {                                    \
+            fprintf(stderr, "xba: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+            return XBA_E_HIP;                                      \
+        }                                                          \
+    } while (0)
+
+namespace {
+
+template <typename T>
+T *dalloc(size_t n, std::vector<void *> &owned) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    owned.push_back(p);
+    return (T *)p;
+}
+
+// Working-precision conversion / packing kernels (upload and download).
+template <typename T>
+__global__ void k_upload(int nobs, int nlm, const int32_t *ob_pose, const uint8_t *ob_cam, const uint8_t *ob_valid,
+                         const double *isig, int32_t *ob_info, T *isigT, const int32_t *lm_host,
+                         const uint8_t *lm_mask, const double *rho_isig, int32_t *lm_info, T *riT) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nobs || i < nlm; i += gridDim.x * blockDim.x) {
+        if (i < nobs) {
+            ob_info[i] = pack_obs(ob_pose[i], ob_cam[i] ? 1 : 0, ob_valid[i] ? 1 : 0);
+            isigT[i] = (T)isig[i];
+        }
+        if (i < nlm) {
+            lm_info[i] = lm_host[i] | ((lm_mask[i] ? 1 : 0) << 12);
+            riT[i] = (T)rho_isig[i];
+        }
+    }
+}
+
+__global__ void k_init_ws(int nwin, double *ws, double lam0) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwin) return;
+    for (int k = 0; k < NWS; k++) ws[(size_t)NWS * w + k] = 0.0;
+    ws[(size_t)NWS * w + WS_LAM] = lam0;
+    ws[(size_t)NWS * w + WS_CHOLOK] = 1.0;
+}
+
+__global__ void k_stats(int nwin, const double *ws, double *out) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwin) return;
+    const double *s = ws + (size_t)NWS * w;
+    double *o = out + 8 * (size_t)w;
+    o[0] = s[WS_COST0]; o[1] = s[WS_CUR]; o[2] = s[WS_ACC]; o[3] = s[WS_ITERS]; o[4] = s[WS_LAM];
+    o[5] = s[WS_CHOLOK]; o[6] = s[WS_ACCEPT]; o[7] = s[WS_NREF];
+}
+
+__global__ void k_cost_out(int nwin, const double *ws, double *out) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwin) out[w] = ws[(size_t)NWS * w + WS_CTOT];
+}
+
+}  // namespace
+
+struct xba_ctx {
+    xba_batch B;
+    xba_options opt;
+    int nwin = 0, nslot = 0, nlm = 0, nobs = 0, nimu = 0, npblk = 0;
+    int nchunks = 0, Rmax = 1, maxn = 0, grid = 1;
+    int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
+    int lin_lds = 0, solve_lds_cap = 0;
+    bool uploaded = false;
+    std::vector<void *> owned;
+    // device
+    Chunk *chunks = nullptr;
+    WinInfo *wins = nullptr;
+    int32_t *colmap = nullptr, *slot_off = nullptr, *slot_dim = nullptr, *ob_start = nullptr, *lm_host = nullptr,
+            *ob_pose = nullptr, *imu_ij = nullptr, *prior_slot = nullptr, *prior_dim = nullptr, *lm_win = nullptr;
+    int32_t *ob_info = nullptr, *lm_info = nullptr;
+    void *isigT = nullptr, *lmpT = nullptr, *lmp_trial = nullptr, *riT = nullptr;
+    void *partH = nullptr, *partb = nullptr, *partH_lo = nullptr, *partb_lo = nullptr, *lmHi = nullptr, *lmbl = nullptr;
+    double *partc = nullptr, *partc2 = nullptr;
+    uint8_t *lmok = nullptr;
+    double *Rwb = nullptr, *pwb = nullptr, *vel = nullptr, *bgs = nullptr, *bas = nullptr;
+    double *R2 = nullptr, *p2 = nullptr, *v2 = nullptr, *g2 = nullptr, *a2 = nullptr;
+    double *H = nullptr, *b = nullptr, *dx = nullptr, *fact64 = nullptr, *imu_scr = nullptr, *wstate = nullptr;
+    float *fact32 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t gstream = nullptr;
+};
+
+namespace {
+
+template <typename T>
+KArgs<T> kargs(xba_ctx *c) {
+    KArgs<T> A;
+    memset(&A, 0, sizeof(A));
+    A.nchunks = c->nchunks;
+    A.nwin = c->nwin;
+    A.nlm = c->nlm;
+    A.nobs = c->nobs;
+    A.chunks = c->chunks;
+    A.wins = c->wins;
+    A.ob_start = c->ob_start;
+    A.lm_info = c->lm_info;
+    A.ob_info = c->ob_info;
+    A.slot_off = c->slot_off;
+    A.z = c->B.ob_z;
+    A.isig = (const T *)c->isigT;
+    A.lmp = (double *)c->lmpT;
+    A.lmp_trial = (double *)c->lmp_trial;
+    A.rho_meas = c->B.rho_meas;
+    A.rho_isig = (const T *)c->riT;
+    A.Rwb = c->Rwb;
+    A.pwb = c->pwb;
+    A.R2 = c->R2;
+    A.p2 = c->p2;
+    A.dx = c->dx;
+    A.win_par = c->B.win_par;
+    A.partH = (T *)c->partH;
+    A.partb = (T *)c->partb;
+    A.partH_lo = (T *)c->partH_lo;
+    A.partb_lo = (T *)c->partb_lo;
+    A.partc = c->partc;
+    A.partc2 = c->partc2;
+    A.lmHi = (T *)c->lmHi;
+    A.lmbl = (T *)c->lmbl;
+    A.lmok = c->lmok;
+    A.wstate = c->wstate;
+    A.lam_override = -1.0;
+    return A;
+}
+
+DArgs dargs(xba_ctx *c) {
+    DArgs D;
+    memset(&D, 0, sizeof(D));
+    D.nwin = c->nwin;
+    D.chunks = c->chunks;
+    D.wins = c->wins;
+    D.colmap = c->colmap;
+    D.slot_off = c->slot_off;
+    D.slot_dim = c->slot_dim;
+    D.imu_ij = c->imu_ij;
+    D.prior_slot = c->prior_slot;
+    D.prior_dim = c->prior_dim;
+    D.imu_rec = c->B.imu_rec;
+    D.prior_x0 = c->B.prior_x0;
+    D.prior_S = c->B.prior_S;
+    D.prior_r = c->B.prior_r;
+    D.win_par = c->B.win_par;
+    D.Rwb = c->Rwb; D.pwb = c->pwb; D.vel = c->vel; D.bgs = c->bgs; D.bas = c->bas;
+    D.R2 = c->R2; D.p2 = c->p2; D.v2 = c->v2; D.g2 = c->g2; D.a2 = c->a2;
+    D.H = c->H;
+    D.b = c->b;
+    D.dx = c->dx;
+    D.fact64 = c->fact64;
+    D.fact32 = c->fact32;
+    D.imu_scr = c->imu_scr;
+    D.wstate = c->wstate;
+    D.partc = c->partc;
+    D.partc2 = c->partc2;
+    D.nref_max = 6;
+    return D;
+}
+
+bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
+
+// The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
+template <typename T>
+void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 = nullptr) {
+    hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    DArgs D = dargs(c);
+    D.imu_chi2 = imu_chi2;
+    int gx = (int)(((int64_t)c->maxn * c->maxn + 255) / 256);
+    hipLaunchKernelGGL(k_reduce<T>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D, (const T *)c->partH,
+                       (const T *)c->partb, (const T *)c->partH_lo, (const T *)c->partb_lo);
+    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
+}
+
+template <typename T>
+void launch_iter(xba_ctx *c, hipStream_t s) {
+    KArgs<T> A = kargs<T>(c);
+    launch_lin<T>(c, A, s);
+    DArgs D = dargs(c);
+    hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
+    hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
+    hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
+    int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *xba_version(void) { return "xba 0.1 (gfx950)"; }
+
+const char *xba_strerror(int code) {
+    switch (code) {
+        case XBA_OK: return "ok";
+        case XBA_E_ARG: return "invalid argument";
+        case XBA_E_HIP: return "HIP runtime error";
+        case XBA_E_TOO_MANY_OBS: return "a landmark has more than XBA_MAX_OBS_PER_LM observations";
+        case XBA_E_RANGE: return "a landmark couples slots further apart than the slot tile allows";
+        case XBA_E_SIZE: return "a reduced system exceeds XBA_MAX_STATE";
+        case XBA_E_STATE: return "call order (upload before iterate)";
+        default: return "unknown error";
+    }
+}
+
+int xba_destroy(xba_ctx *c) {
+    if (!c) return XBA_OK;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (void *p : c->owned) (void)hipFree(p);
+    delete c;
+    return XBA_OK;
+}
+
+int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
+    if (!out || !batch || !opt || batch->nwin <= 0) return XBA_E_ARG;
+    const xba_batch &B = *batch;
+    if (!B.win_slot0 || !B.win_lm0 || !B.slot_dim || !B.ob_start || !B.lm_host || !B.ob_pose) return XBA_E_ARG;
+    if (opt->precision != XBA_FP64 && opt->precision != XBA_MIXED) return XBA_E_ARG;
+    xba_ctx *c = new xba_ctx();
+    c->B = B;
+    c->opt = *opt;
+    const int nwin = B.nwin;
+    c->nwin = nwin;
+    c->nslot = B.win_slot0[nwin];
+    c->nlm = B.win_lm0[nwin];
+    c->nobs = B.ob_start[c->nlm];
+    c->nimu = B.win_imu0 ? B.win_imu0[nwin] : 0;
+    c->npblk = B.win_pblk0 ? B.win_pblk0[nwin] : 0;
+    const bool mx = opt->precision == XBA_MIXED;
+    const int tsz = mx ? 4 : 8;
+    auto fail = [&](int code) {
+        xba_destroy(c);
+        return code;
+    };
+    // ---- per-window records
+    std::vector<WinInfo> wins(nwin);
+    std::vector<int32_t> slot_off(std::max(c->nslot, 1)), colmap, lm_win(std::max(c->nlm, 1));
+    int64_t st = 0, h0 = 0, prS = 0;
+    int maxn = 0;
+    bool any_vi = false;
+    for (int w = 0; w < nwin; w++) {
+        WinInfo &W = wins[w];
+        memset(&W, 0, sizeof(W));
+        W.slot0 = B.win_slot0[w];
+        W.P = B.win_slot0[w + 1] - W.slot0;
+        W.lm0 = B.win_lm0[w];
+        W.L = B.win_lm0[w + 1] - W.lm0;
+        if (W.P <= 0 || W.P > 4095 || W.L < 0) return fail(XBA_E_ARG);
+        int n = 0;
+        for (int s = 0; s < W.P; s++) {
+            int d = B.slot_dim[W.slot0 + s];
+            if (d != 6 && d != 15) return fail(XBA_E_ARG);
+            if (d == 15) W.vi = 1;
+            slot_off[W.slot0 + s] = n;
+            for (int i = 0; i < d; i++) colmap.push_back(i < 6 ? s * 8 + i : -1);
+            n += d;
+        }
+        if (n > XBA_MAX_STATE) return fail(XBA_E_SIZE);
+        W.n = n;
+        W.st0 = st;
+        W.h0 = h0;
+        st += n;
+        h0 += (int64_t)n * n;
+        maxn = std::max(maxn, n);
+        any_vi |= W.vi != 0;
+        W.imu0 = B.win_imu0 ? B.win_imu0[w] : 0;
+        W.imu1 = B.win_imu0 ? B.win_imu0[w + 1] : 0;
+        W.pblk0 = B.win_pblk0 ? B.win_pblk0[w] : 0;
+        W.pblk1 = B.win_pblk0 ? B.win_pblk0[w + 1] : 0;
+        W.prow0 = B.win_prow0 ? B.win_prow0[w] : 0;
+        W.prow1 = B.win_prow0 ? B.win_prow0[w + 1] : 0;
+        int np_ = 0;
+        for (int k = W.pblk0; k < W.pblk1; k++) {
+            int s = B.prior_slot[k], d = B.prior_dim[k];
+            if (s < 0 || s >= W.P || (d != 6 && d != 15) || d > B.slot_dim[W.slot0 + s]) return fail(XBA_E_ARG);
+            np_ += d;
+        }
+        if (np_ > XBA_MAX_STATE || W.prow1 - W.prow0 > XBA_MAX_STATE) return fail(XBA_E_SIZE);
+        W.np = np_;
+        W.prS0 = prS;
+        prS += (int64_t)(W.prow1 - W.prow0) * np_;
+        for (int f = W.imu0; f < W.imu1; f++) {
+            int i = B.imu_ij[2 * f], j = B.imu_ij[2 * f + 1];
+            if (i < 0 || i >= W.P || j < 0 || j >= W.P || B.slot_dim[W.slot0 + i] != 15 ||
+                B.slot_dim[W.slot0 + j] != 15)
+                return fail(XBA_E_ARG);
+        }
+        for (int l = W.lm0; l < W.lm0 + W.L; l++) lm_win[l] = w;
+    }
+    if (any_vi && (!B.vel || !B.bgs || !B.bas)) return fail(XBA_E_ARG);
+    c->nstate = st;
+    c->nh = h0;
+    c->maxn = maxn;
+    // ---- work list: landmark chunks whose observations stay inside a slot tile
+    int tile = opt->slot_tile > 0 ? opt->slot_tile : (mx ? 16 : 12);
+    const int tile_cap = mx ? 32 : 24;
+    int lpc = opt->lm_per_chunk;
+    if (lpc <= 0) lpc = std::max(4, std::min(1024, (int)std::lround(c->nlm / 2048.0)));
+    std::vector<int> llo(std::max(c->nlm, 1)), lhi(std::max(c->nlm, 1));
+    for (int w = 0; w < nwin; w++) {
+        const WinInfo &W = wins[w];
+        for (int l = W.lm0; l < W.lm0 + W.L; l++) {
+            int h = B.lm_host[l];
+            if (h < 0 || h >= W.P) return fail(XBA_E_ARG);
+            int o0 = B.ob_start[l], o1 = B.ob_start[l + 1];
+            if (o1 < o0) return fail(XBA_E_ARG);
+            if (o1 - o0 > XBA_MAX_OBS_PER_LM) return fail(XBA_E_TOO_MANY_OBS);
+            int lo = h, hi = h;
+            for (int k = o0; k < o1; k++) {
+                int t = B.ob_pose[k];
+                if (t < 0 || t >= W.P) return fail(XBA_E_ARG);
+                lo = std::min(lo, t);
+                hi = std::max(hi, t);
+            }
+            llo[l] = lo;
+            lhi[l] = hi;
+            if (hi - lo + 1 > tile) tile = hi - lo + 1;
+        }
+    }
+    if (tile > tile_cap) return fail(XBA_E_RANGE);
+    std::vector<Chunk> chunks;
+    int64_t pH = 0, pb = 0;
+    int Rmax = 1;
+    for (int w = 0; w < nwin; w++) {
+        const WinInfo &W = wins[w];
+        wins[w].c0 = (int)chunks.size();
+        int l = W.lm0, lend = W.lm0 + W.L;
+        while (l < lend) {
+            int cs = l, lo = llo[l], hi = lhi[l];
+            l++;
+            while (l < lend && l - cs < lpc) {
+                int nlo = std::min(lo, llo[l]), nhi = std::max(hi, lhi[l]);
+                if (nhi - nlo + 1 > tile) break;
+                lo = nlo;
+                hi = nhi;
+                l++;
+            }
+            Chunk ch;
+            memset(&ch, 0, sizeof(ch));
+            ch.win = w;
+            ch.l0 = cs;
+            ch.l1 = l;
+            ch.lo = lo;
+            ch.R = hi - lo + 1;
+            ch.partH = pH;
+            ch.partb = pb;
+            pH += (int64_t)(ch.R * (ch.R + 1) / 2) * 36;
+            pb += 6 * ch.R;
+            Rmax = std::max(Rmax, ch.R);
+            chunks.push_back(ch);
+        }
+        wins[w].c1 = (int)chunks.size();
+    }
+    c->nchunks = (int)chunks.size();
+    c->Rmax = Rmax;
+    c->npartH = pH;
+    c->npartb = pb;
+    c->lin_lds = lds_layout(Rmax, tsz).total;
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    int ncu = 0;
+    HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = std::max(1, std::min(8, (160 * 1024) / std::max(c->lin_lds, 1)));
+    c->grid = opt->grid > 0 ? opt->grid : std::max(1, std::min(c->nchunks, ncu * per_cu));
+    if (c->nchunks == 0) c->grid = 1;
+    // ---- device allocations
+    auto &O = c->owned;
+    c->chunks = dalloc<Chunk>(chunks.size(), O);
+    c->wins = dalloc<WinInfo>(nwin, O);
+    c->colmap = dalloc<int32_t>(colmap.size(), O);
+    c->slot_off = dalloc<int32_t>(c->nslot, O);
+    c->slot_dim = dalloc<int32_t>(c->nslot, O);
+    c->ob_start = dalloc<int32_t>(c->nlm + 1, O);
+    c->lm_host = dalloc<int32_t>(c->nlm, O);
+    c->ob_pose = dalloc<int32_t>(c->nobs, O);
+    c->lm_win = dalloc<int32_t>(c->nlm, O);
+    c->imu_ij = dalloc<int32_t>(2 * (size_t)c->nimu, O);
+    c->prior_slot = dalloc<int32_t>(c->npblk, O);
+    c->prior_dim = dalloc<int32_t>(c->npblk, O);
+    c->ob_info = dalloc<int32_t>(c->nobs, O);
+    c->lm_info = dalloc<int32_t>(c->nlm, O);
+    c->isigT = mx ? (void *)dalloc<float>(c->nobs, O) : (void *)dalloc<double>(c->nobs, O);
+    c->lmpT = (void *)dalloc<double>(3 * (size_t)c->nlm, O);
+    c->lmp_trial = (void *)dalloc<double>(3 * (size_t)c->nlm, O);
+    c->riT = mx ? (void *)dalloc<float>(c->nlm, O) : (void *)dalloc<double>(c->nlm, O);
+    c->partH = mx ? (void *)dalloc<float>(pH, O) : (void *)dalloc<double>(pH, O);
+    c->partb = mx ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(pb, O);
+    c->partH_lo = mx ? (void *)dalloc<float>(pH, O) : (void *)dalloc<double>(1, O);
+    c->partb_lo = mx ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(1, O);
+    c->lmHi = mx ? (void *)dalloc<float>(6 * (size_t)c->nlm, O) : (void *)dalloc<double>(6 * (size_t)c->nlm, O);
+    c->lmbl = mx ? (void *)dalloc<float>(3 * (size_t)c->nlm, O) : (void *)dalloc<double>(3 * (size_t)c->nlm, O);
+    c->partc = dalloc<double>(c->nchunks, O);
+    c->partc2 = dalloc<double>(c->nchunks, O);
+    c->lmok = dalloc<uint8_t>(c->nlm, O);
+    c->Rwb = dalloc<double>(9 * (size_t)c->nslot, O);
+    c->pwb = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->R2 = dalloc<double>(9 * (size_t)c->nslot, O);
+    c->p2 = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->vel = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->bgs = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->bas = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->v2 = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->g2 = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->a2 = dalloc<double>(3 * (size_t)c->nslot, O);
+    c->H = dalloc<double>(h0, O);
+    c->b = dalloc<double>(st, O);
+    c->dx = dalloc<double>(st, O);
+    c->fact64 = mx ? dalloc<double>(1, O) : dalloc<double>(h0, O);
+    c->fact32 = mx ? dalloc<float>(h0, O) : dalloc<float>(1, O);
+    c->imu_scr = dalloc<double>(465 * (size_t)std::max(c->nimu, 1), O);
+    c->wstate = dalloc<double>((size_t)NWS * nwin, O);
+    for (void *p : O)
+        if (!p) return fail(XBA_E_HIP);
+    auto up = [&](void *dst, const void *src, size_t bytes) -> bool {
+        if (bytes == 0 || !src) return true;
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(c->chunks, chunks.data(), chunks.size() * sizeof(Chunk)) &&
+              up(c->wins, wins.data(), nwin * sizeof(WinInfo)) &&
+              up(c->colmap, colmap.data(), colmap.size() * 4) && up(c->slot_off, slot_off.data(), c->nslot * 4) &&
+              up(c->slot_dim, B.slot_dim, c->nslot * 4) && up(c->ob_start, B.ob_start, (c->nlm + 1) * 4) &&
+              up(c->lm_host, B.lm_host, c->nlm * 4) && up(c->ob_pose, B.ob_pose, (size_t)c->nobs * 4) &&
+              up(c->lm_win, lm_win.data(), c->nlm * 4) && up(c->imu_ij, B.imu_ij, 2 * (size_t)c->nimu * 4) &&
+              up(c->prior_slot, B.prior_slot, c->npblk * 4) && up(c->prior_dim, B.prior_dim, c->npblk * 4);
+    if (!ok) return fail(XBA_E_HIP);
+    // LDS budgets
+    if (c->lin_lds > 64 * 1024) {
+        if (mx) {
+            HIPCK(hipFuncSetAttribute((const void *)k_lin<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
+        } else {
+            HIPCK(hipFuncSetAttribute((const void *)k_lin<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
+        }
+    }
+    hipFuncAttributes fa;
+    HIPCK(hipFuncGetAttributes(&fa, mx ? (const void *)k_solve<float> : (const void *)k_solve<double>));
+    int cap = 160 * 1024 - (int)fa.sharedSizeBytes - 256;
+    int need = (int)std::min<int64_t>((int64_t)maxn * maxn * tsz, (int64_t)cap);
+    c->solve_lds_cap = std::max(0, need);
+    if (c->solve_lds_cap > 64 * 1024)
+        HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, c->solve_lds_cap));
+    *out = c;
+    return XBA_OK;
+}
+
+int xba_sizes(const xba_ctx *c, int64_t *n_state, int64_t *n_h) {
+    if (!c) return XBA_E_ARG;
+    if (n_state) *n_state = c->nstate;
+    if (n_h) *n_h = c->nh;
+    return XBA_OK;
+}
+
+int xba_upload(xba_ctx *c, double lam0, void *stream) {
+    if (!c) return XBA_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const xba_batch &B = c->B;
+    int n = std::max(c->nobs, c->nlm);
+    int g = std::max(1, std::min((n + 255) / 256, 8192));
+    if (mixed(c))
+        hipLaunchKernelGGL(k_upload<float>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
+                           B.ob_valid, B.ob_isig, c->ob_info, (float *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
+                           c->lm_info, (float *)c->riT);
+    else
+        hipLaunchKernelGGL(k_upload<double>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
+                           B.ob_valid, B.ob_isig, c->ob_info, (double *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
+                           c->lm_info, (double *)c->riT);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(c->lmpT, B.lmp, 3 * (size_t)c->nlm * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipGetLastError());
+    size_t ns = (size_t)c->nslot;
+    HIPCK(hipMemcpyAsync(c->Rwb, B.Rwb, 9 * ns * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(c->pwb, B.pwb, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.vel) HIPCK(hipMemcpyAsync(c->vel, B.vel, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.bgs) HIPCK(hipMemcpyAsync(c->bgs, B.bgs, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.bas) HIPCK(hipMemcpyAsync(c->bas, B.bas, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_init_ws, dim3((c->nwin + 255) / 256), dim3(256), 0, s, c->nwin, c->wstate, lam0);
+    HIPCK(hipGetLastError());
+    c->uploaded = true;
+    return XBA_OK;
+}
+
+int xba_iterate(xba_ctx *c, int32_t iters, void *stream) {
+    if (!c || iters < 0) return XBA_E_ARG;
+    if (!c->uploaded) return XBA_E_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->opt.use_graph) {
+        if (!c->gexec || c->gstream != s) {
+            if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+            if (c->graph) (void)hipGraphDestroy(c->graph);
+            c->gexec = nullptr;
+            c->graph = nullptr;
+            hipStream_t cs = s;
+            bool own = false;
+            if (cs == nullptr) {  // the null stream cannot be captured: capture on a private stream
+                HIPCK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+                own = true;
+            }
+            HIPCK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            if (mixed(c))
+                launch_iter<float>(c, cs);
+            else
+                launch_iter<double>(c, cs);
+            HIPCK(hipStreamEndCapture(cs, &c->graph));
+            HIPCK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+            if (own) HIPCK(hipStreamDestroy(cs));
+            c->gstream = s;
+        }
+        for (int it = 0; it < iters; it++) HIPCK(hipGraphLaunch(c->gexec, s));
+        return XBA_OK;
+    }
+    for (int it = 0; it < iters; it++) {
+        if (mixed(c))
+            launch_iter<float>(c, s);
+        else
+            launch_iter<double>(c, s);
+        HIPCK(hipGetLastError());
+    }
+    return XBA_OK;
+}
+
+int xba_download(xba_ctx *c, void *stream) {
+    if (!c) return XBA_E_ARG;
+    if (!c->uploaded) return XBA_E_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    const xba_batch &B = c->B;
+    size_t ns = (size_t)c->nslot;
+    HIPCK(hipMemcpyAsync(B.Rwb, c->Rwb, 9 * ns * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(B.pwb, c->pwb, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.vel) HIPCK(hipMemcpyAsync(B.vel, c->vel, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.bgs) HIPCK(hipMemcpyAsync(B.bgs, c->bgs, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    if (B.bas) HIPCK(hipMemcpyAsync(B.bas, c->bas, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(B.lmp, c->lmpT, 3 * (size_t)c->nlm * 8, hipMemcpyDeviceToDevice, s));
+    return XBA_OK;
+}
+
+int xba_stats(xba_ctx *c, double *out, void *stream) {
+    if (!c || !out) return XBA_E_ARG;
+    hipLaunchKernelGGL(k_stats, dim3((c->nwin + 255) / 256), dim3(256), 0, (hipStream_t)stream, c->nwin, c->wstate, out);
+    HIPCK(hipGetLastError());
+    return XBA_OK;
+}
+
+int xba_last_dx(xba_ctx *c, double *out, void *stream) {
+    if (!c || !out) return XBA_E_ARG;
+    HIPCK(hipMemcpyAsync(out, c->dx, (size_t)c->nstate * 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return XBA_OK;
+}
+
+int xba_linearize(xba_ctx *c, double lam, double *H_out, double *b_out, double *cost_out, double *chi2_out,
+                  double *Hinv_out, uint8_t *ok_out, double *imu_chi2_out, void *stream) {
+    if (!c || lam < 0.0) return XBA_E_ARG;
+    if (!c->uploaded) return XBA_E_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    if (mixed(c)) {
+        KArgs<float> A = kargs<float>(c);
+        A.lam_override = lam;
+        A.chi2_out = chi2_out;
+        A.Hinv_out = Hinv_out;
+        A.ok_out = ok_out;
+        launch_lin<float>(c, A, s, imu_chi2_out);
+    } else {
+        KArgs<double> A = kargs<double>(c);
+        A.lam_override = lam;
+        A.chi2_out = chi2_out;
+        A.Hinv_out = Hinv_out;
+        A.ok_out = ok_out;
+        launch_lin<double>(c, A, s, imu_chi2_out);
+    }
+    HIPCK(hipGetLastError());
+    if (H_out) HIPCK(hipMemcpyAsync(H_out, c->H, (size_t)c->nh * 8, hipMemcpyDeviceToDevice, s));
+    if (b_out) HIPCK(hipMemcpyAsync(b_out, c->b, (size_t)c->nstate * 8, hipMemcpyDeviceToDevice, s));
+    if (cost_out) {
+        hipLaunchKernelGGL(k_cost_out, dim3((c->nwin + 255) / 256), dim3(256), 0, s, c->nwin, c->wstate, cost_out);
+        HIPCK(hipGetLastError());
+    }
+    return XBA_OK;
+}
+
+int xba_backward(xba_ctx *c, const double *g_dx, double *grad_z, double *grad_w, void *stream) {
+    (void)g_dx;
+    (void)grad_z;
+    (void)grad_w;
+    (void)stream;
+    if (!c) return XBA_E_ARG;
+    return XBA_E_STATE;
+}
+
+int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                 double *dx_hist_out, void *stream) {
+    (void)q; (void)iters; (void)lam0; (void)precision; (void)stats_out; (void)dx_hist_out; (void)stream;
+    return XBA_E_STATE;
+}
+
+}  // extern "C"

@@ -1,0 +1,672 @@
+// Per-window kernels of the window solve: the deterministic reduction of the chunk partials into each window's
+// reduced system, the prior and IMU terms, the damped Cholesky solve (FP64, or FP32 with FP64 iterative
+// refinement), the on-manifold pose update, and the device-side accept/reject.
+//
+// Reference arithmetic (paths relative to the reference's src/xclean/):
+//   _full_system (scatter the 6P visual system, add prior and IMU)  vio/window.py:865-886
+//   _imu_terms (Huber on the whitened norm with huber_k_dof(15))    vio/window.py:831-862
+//   factor_core                                                     vio/imu_factor.py:771-875
+//   Prior.terms / cost, box-minus                                   vio/marg.py:47-57, 102-127
+//   damping Hd = H + diag(lam diag(H) + 1e-9), Cholesky, dx = -H^-1 b   vio/window.py:933-941
+//   apply_update (R <- R Exp(dphi), p += dp; v, bg, ba additive)    vio/window.py:522-532, 945-955
+//   accept iff c_new < c_tot: lam = max(lam/10, 1e-12), else lam *= 10   vio/window.py:967-981
+#include "xba_internal.h"
+
+namespace xba {
+
+// ------------------------------------------------------------------------------------------ SO(3) helpers --
+// vio/window.py:43-61
+__device__ inline void so3_exp_nb(const double *w, double *R) {
+    double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2], th = sqrt(th2), a, b;
+    double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    if (th < 1e-8) {
+        a = 1.0 - th2 / 6.0;
+        b = 0.5 - th2 / 24.0;
+    } else {
+        a = sin(th) / th;
+        b = (1.0 - cos(th)) / th2;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double kk = K[3 * i] * K[j] + K[3 * i + 1] * K[3 + j] + K[3 * i + 2] * K[6 + j];
+            R[3 * i + j] = (i == j ? 1.0 : 0.0) + a * K[3 * i + j] + b * kk;
+        }
+}
+
+// vio/window.py:64-105
+__device__ inline void so3_log_nb(const double *R, double *w) {
+    double c = (R[0] + R[4] + R[8] - 1.0) * 0.5;
+    c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+    double th = acos(c);
+    double v0 = R[7] - R[5], v1 = R[2] - R[6], v2 = R[3] - R[1];
+    if (th < 1e-6) {
+        w[0] = 0.5 * v0; w[1] = 0.5 * v1; w[2] = 0.5 * v2;
+        return;
+    }
+    if (th > M_PI - 1e-4) {
+        double Bm[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Bm[3 * i + j] = (R[3 * i + j] + R[3 * j + i]) * 0.5;
+        int i = 0;
+        if (Bm[4] > Bm[4 * i]) i = 1;
+        if (Bm[8] > Bm[4 * i]) i = 2;
+        double den = sqrt(fmax((Bm[4 * i] - c) / (1.0 - c), 0.0)), ax[3];
+        for (int j = 0; j < 3; j++) ax[j] = (Bm[3 * j + i] - (j == i ? c : 0.0)) / fmax(den * (1.0 - c), 1e-12);
+        ax[i] = den;
+        double n = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]), s = 1.0;
+        if (ax[0] * v0 + ax[1] * v1 + ax[2] * v2 < 0) s = -1.0;
+        for (int j = 0; j < 3; j++) w[j] = s * th * ax[j] / n;
+        return;
+    }
+    double f = th / (2.0 * sin(th));
+    w[0] = f * v0; w[1] = f * v1; w[2] = f * v2;
+}
+
+// series coefficients with the derived switch of core/lie.py:47-79 (TAYLOR_ANGLE = (7257600 eps)^0.1)
+#define XBA_TAYLOR_ANGLE 0.12610312423896003
+#define XBA_ANGLE_ZERO 3.650024149988857e-08
+__device__ inline void abc(double t2, double *a, double *b, double *c) {
+    double t = sqrt(t2);
+    if (t < XBA_TAYLOR_ANGLE) {
+        double x = t2;
+        *a = 1.0 - x / 6.0 + x * x / 120.0 - x * x * x / 5040.0;
+        *b = 0.5 - x / 24.0 + x * x / 720.0 - x * x * x / 40320.0;
+        *c = 1.0 / 6.0 - x / 120.0 + x * x / 5040.0 - x * x * x / 362880.0;
+    } else {
+        double s = sin(t);
+        *a = s / t;
+        *b = (1.0 - cos(t)) / t2;
+        *c = (t - s) / (t2 * t);
+    }
+}
+
+// vio/imu_factor.py:214-241
+__device__ inline void exp_jr(const double *v, double *R, double *Jr) {
+    double x = v[0], y = v[1], z = v[2], a, b, c;
+    abc(x * x + y * y + z * z, &a, &b, &c);
+    double xy = x * y, xz = x * z, yz = y * z;
+    double k00 = -(y * y + z * z), k11 = -(x * x + z * z), k22 = -(x * x + y * y);
+    R[0] = 1.0 + b * k00; R[1] = -a * z + b * xy; R[2] = a * y + b * xz;
+    R[3] = a * z + b * xy; R[4] = 1.0 + b * k11; R[5] = -a * x + b * yz;
+    R[6] = -a * y + b * xz; R[7] = a * x + b * yz; R[8] = 1.0 + b * k22;
+    Jr[0] = 1.0 + c * k00; Jr[1] = b * z + c * xy; Jr[2] = -b * y + c * xz;
+    Jr[3] = -b * z + c * xy; Jr[4] = 1.0 + c * k11; Jr[5] = b * x + c * yz;
+    Jr[6] = b * y + c * xz; Jr[7] = -b * x + c * yz; Jr[8] = 1.0 + c * k22;
+}
+
+// vio/imu_factor.py:244-263
+__device__ inline void jr_inv(const double *v, double *o) {
+    double x = v[0], y = v[1], z = v[2], t2 = x * x + y * y + z * z, t = sqrt(t2), co;
+    if (t < XBA_TAYLOR_ANGLE)
+        co = 1.0 / 12.0 + t2 / 720.0 + t2 * t2 / 30240.0 + t2 * t2 * t2 / 1209600.0;
+    else
+        co = (1.0 - (t * sin(t)) / (2.0 * (1.0 - cos(t)))) / t2;
+    o[0] = 1.0 - co * (y * y + z * z); o[1] = -0.5 * z + co * x * y; o[2] = 0.5 * y + co * x * z;
+    o[3] = 0.5 * z + co * x * y; o[4] = 1.0 - co * (x * x + z * z); o[5] = -0.5 * x + co * y * z;
+    o[6] = -0.5 * y + co * x * z; o[7] = 0.5 * x + co * y * z; o[8] = 1.0 - co * (x * x + y * y);
+}
+
+// vio/imu_factor.py:266-330 (Shepperd's quaternion branch, atan2 angle)
+__device__ inline void log_quat(const double *R, double *o) {
+    double m00 = R[0], m11 = R[4], m22 = R[8], tr = m00 + m11 + m22, qw;
+    int br = 0;
+    if (!(tr > 0.0)) {
+        double best = tr;
+        if (m00 > best) { br = 1; best = m00; }
+        if (m11 > best) { br = 2; best = m11; }
+        if (m22 > best) { br = 3; best = m22; }
+        if (br == 0) br = 1;
+    }
+    if (br == 0) {
+        double s = sqrt(fmax(tr + 1.0, 0.0)) * 2.0;
+        qw = 0.25 * s;
+        o[0] = (R[7] - R[5]) / s; o[1] = (R[2] - R[6]) / s; o[2] = (R[3] - R[1]) / s;
+    } else {
+        int a = br - 1, c = (a + 1) % 3, d = (a + 2) % 3;
+        double s = sqrt(fmax(1.0 + R[4 * a] - R[4 * c] - R[4 * d], 0.0)) * 2.0;
+        qw = (R[3 * d + c] - R[3 * c + d]) / s;
+        o[a] = 0.25 * s;
+        o[c] = (R[3 * c + a] + R[3 * a + c]) / s;
+        o[d] = (R[3 * d + a] + R[3 * a + d]) / s;
+    }
+    if (qw < 0.0) {
+        qw = -qw;
+        o[0] = -o[0]; o[1] = -o[1]; o[2] = -o[2];
+    }
+    double n = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    double sc = (n < XBA_ANGLE_ZERO) ? 2.0 : 2.0 * atan2(n, qw) / n;
+    o[0] *= sc; o[1] *= sc; o[2] *= sc;
+}
+
+__device__ inline void mm3(const double *A, const double *B, double *C) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+__device__ inline void mtm3(const double *A, const double *B, double *C) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) C[3 * i + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+}
+__device__ inline void mmt3(const double *A, const double *B, double *C) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            C[3 * i + j] = A[3 * i] * B[3 * j] + A[3 * i + 1] * B[3 * j + 1] + A[3 * i + 2] * B[3 * j + 2];
+}
+
+// ---------------------------------------------------------------------------------------- the IMU factor --
+// Unwhitened residual r (15) and Jacobians Ji, Jj (15 x 15, row-major) of vio/imu_factor.py:771-846.
+__device__ void imu_raw(const double *Ri, const double *pi, const double *vi, const double *bgi, const double *bai,
+                        const double *Rj, const double *pj, const double *vj, const double *bgj, const double *baj,
+                        const double *g, const double *rec, double *r, double *Ji, double *Jj) {
+    const double *dR = rec, *dv = rec + 9, *dp = rec + 12, dt = rec[15], *J = rec + 16, *bg_lin = rec + 70,
+                 *ba_lin = rec + 73;
+    double dbg[3], dba[3], phib[3], rR[3], uv[3], up[3], Dv[3], Dp[3];
+    double Eb[9], Jrb[9], dRc[9], Mij[9], E[9], Jri[9], T1[9], T2[9], T3[9], T4[9];
+    for (int c = 0; c < 3; c++) {
+        dbg[c] = bgi[c] - bg_lin[c];
+        dba[c] = bai[c] - ba_lin[c];
+    }
+    for (int i = 0; i < 3; i++) phib[i] = J[6 * i] * dbg[0] + J[6 * i + 1] * dbg[1] + J[6 * i + 2] * dbg[2];
+    exp_jr(phib, Eb, Jrb);
+    mm3(dR, Eb, dRc);
+    mtm3(Ri, Rj, Mij);
+    mtm3(dRc, Mij, E);
+    log_quat(E, rR);
+    double dt2 = dt * dt;
+    for (int c = 0; c < 3; c++) {
+        Dv[c] = vj[c] - vi[c] - g[c] * dt;
+        Dp[c] = pj[c] - pi[c] - vi[c] * dt - 0.5 * g[c] * dt2;
+    }
+    for (int i = 0; i < 3; i++) {
+        uv[i] = Ri[i] * Dv[0] + Ri[3 + i] * Dv[1] + Ri[6 + i] * Dv[2];
+        up[i] = Ri[i] * Dp[0] + Ri[3 + i] * Dp[1] + Ri[6 + i] * Dp[2];
+    }
+    for (int i = 0; i < 3; i++) {
+        double cv = dv[i], cp = dp[i];
+        for (int k = 0; k < 3; k++) {
+            cv += J[6 * (3 + i) + k] * dbg[k] + J[6 * (3 + i) + 3 + k] * dba[k];
+            cp += J[6 * (6 + i) + k] * dbg[k] + J[6 * (6 + i) + 3 + k] * dba[k];
+        }
+        r[i] = rR[i];
+        r[3 + i] = uv[i] - cv;
+        r[6 + i] = up[i] - cp;
+        r[9 + i] = bgj[i] - bgi[i];
+        r[12 + i] = baj[i] - bai[i];
+    }
+    if (!Ji) return;
+    jr_inv(rR, Jri);
+    mmt3(Jri, Mij, T1);
+    mmt3(Jri, E, T2);
+    mm3(T2, Jrb, T3);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T4[3 * i + j] = T3[3 * i] * J[j] + T3[3 * i + 1] * J[6 + j] + T3[3 * i + 2] * J[12 + j];
+    for (int i = 0; i < 225; i++) Ji[i] = Jj[i] = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double RiT = Ri[3 * j + i];
+            Ji[15 * i + j] = -T1[3 * i + j];
+            Ji[15 * i + 9 + j] = -T4[3 * i + j];
+            Jj[15 * i + j] = Jri[3 * i + j];
+            Ji[15 * (3 + i) + 6 + j] = -RiT;
+            Ji[15 * (3 + i) + 9 + j] = -J[6 * (3 + i) + j];
+            Ji[15 * (3 + i) + 12 + j] = -J[6 * (3 + i) + 3 + j];
+            Ji[15 * (6 + i) + 3 + j] = -RiT;
+            Ji[15 * (6 + i) + 6 + j] = -RiT * dt;
+            Ji[15 * (6 + i) + 9 + j] = -J[6 * (6 + i) + j];
+            Ji[15 * (6 + i) + 12 + j] = -J[6 * (6 + i) + 3 + j];
+            Jj[15 * (3 + i) + 6 + j] = RiT;
+            Jj[15 * (6 + i) + 3 + j] = RiT;
+        }
+    Ji[15 * 3 + 1] = -uv[2]; Ji[15 * 3 + 2] = uv[1];
+    Ji[15 * 4 + 0] = uv[2]; Ji[15 * 4 + 2] = -uv[0];
+    Ji[15 * 5 + 0] = -uv[1]; Ji[15 * 5 + 1] = uv[0];
+    Ji[15 * 6 + 1] = -up[2]; Ji[15 * 6 + 2] = up[1];
+    Ji[15 * 7 + 0] = up[2]; Ji[15 * 7 + 2] = -up[0];
+    Ji[15 * 8 + 0] = -up[1]; Ji[15 * 8 + 1] = up[0];
+    for (int i = 0; i < 3; i++) {
+        Ji[15 * (9 + i) + 9 + i] = -1.0;
+        Ji[15 * (12 + i) + 12 + i] = -1.0;
+        Jj[15 * (9 + i) + 9 + i] = 1.0;
+        Jj[15 * (12 + i) + 12 + i] = 1.0;
+    }
+}
+
+struct DArgs {
+    int32_t nwin;
+    const Chunk *chunks;
+    const WinInfo *wins;
+    const int32_t *colmap;      // (sum n): slot*8 + i for pose columns, -1 otherwise
+    const int32_t *slot_off;    // (nslot)
+    const int32_t *slot_dim;    // (nslot)
+    const int32_t *imu_ij;      // (nimu,2)
+    const int32_t *prior_slot, *prior_dim;
+    const double *imu_rec, *prior_x0, *prior_S, *prior_r, *win_par;
+    double *Rwb, *pwb, *vel, *bgs, *bas;  // state (nslot,...)
+    double *R2, *p2, *v2, *g2, *a2;       // trial state
+    double *H, *b, *dx, *fact64;          // (sum n^2), (sum n), (sum n), (sum n^2)
+    float *fact32;                        // (sum n^2)
+    double *imu_scr;                      // (nimu, 465)
+    double *wstate;
+    const double *partc, *partc2;
+    double *imu_chi2;                     // (nimu) or null: whitened chi2 of every IMU factor
+    int32_t nref_max;
+};
+
+// ------------------------------------------------------------------------------------------- k_reduce --
+// H[w] (n x n, FP64 out) = sum over the window's chunks, in chunk order, of their lower-triangular partial tiles
+// (mirrored), with zeros in the non-pose columns; b likewise; the visual cost = sum of the chunk costs.
+template <typename T>
+struct Acc {  // FP64: plain sum; FP32: Kahan-compensated sum
+    T s = 0, c = 0;
+    __device__ inline void add(T v) {
+        if (sizeof(T) == 4) {
+            T y = v - c;
+            T t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+    __device__ inline double value() const { return (double)s - (double)c; }
+};
+
+template <typename T>
+__global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH_lo, const T *partb_lo) {
+    const int w = blockIdx.y;
+    const WinInfo W = D.wins[w];
+    const int n = W.n;
+    const bool kah = sizeof(T) == 4;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < (int64_t)n * n) {
+        int a = (int)(idx / n), bcol = (int)(idx % n);
+        int ca = D.colmap[W.st0 + a], cb = D.colmap[W.st0 + bcol];
+        Acc<T> v;
+        if (ca >= 0 && cb >= 0) {
+            int s2 = ca >> 3, i = ca & 7, s1 = cb >> 3, j = cb & 7;
+            if (s2 < s1 || (s2 == s1 && i < j)) {
+                int t = s2; s2 = s1; s1 = t;
+                t = i; i = j; j = t;
+            }
+            for (int c = W.c0; c < W.c1; c++) {
+                const Chunk ch = D.chunks[c];
+                int r2 = s2 - ch.lo, r1 = s1 - ch.lo;
+                if (r1 < 0 || r2 >= ch.R) continue;
+                int64_t e = ch.partH + (int64_t)(r2 * (r2 + 1) / 2 + r1) * 36 + 6 * i + j;
+                v.add(partH[e]);
+                if (kah) v.add(partH_lo[e]);
+            }
+        }
+        D.H[W.h0 + idx] = v.value();
+    }
+    if (idx < n) {
+        int ca = D.colmap[W.st0 + idx];
+        Acc<T> v;
+        if (ca >= 0) {
+            int s = ca >> 3, i = ca & 7;
+            for (int c = W.c0; c < W.c1; c++) {
+                const Chunk ch = D.chunks[c];
+                int r = s - ch.lo;
+                if (r < 0 || r >= ch.R) continue;
+                v.add(partb[ch.partb + 6 * r + i]);
+                if (kah) v.add(partb_lo[ch.partb + 6 * r + i]);
+            }
+        }
+        D.b[W.st0 + idx] = v.value();
+    }
+    if (idx == 0) {
+        double cs = 0.0;
+        for (int c = W.c0; c < W.c1; c++) cs += D.partc[c];
+        D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
+    }
+}
+
+// ------------------------------------------------------------------------------------------- k_terms --
+// The prior and IMU terms of one window (one workgroup per window).  trial == 0: add H, b and the cost at the
+// current state, then c_tot = visual + rest (cost0 on the first call); trial == 1: cost only, at the trial state.
+__global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
+    __shared__ double sd[XBA_MAX_STATE], sres[XBA_MAX_STATE];
+    __shared__ int scol[XBA_MAX_STATE];
+    __shared__ double sW[225], sr[15], sJi[225], sJj[225], sred[256];
+    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const WinInfo W = D.wins[w];
+    const int n = W.n;
+    const double *R = trial ? D.R2 : D.Rwb, *p = trial ? D.p2 : D.pwb;
+    const double *v = trial ? D.v2 : D.vel, *bg = trial ? D.g2 : D.bgs, *ba = trial ? D.a2 : D.bas;
+    double *H = D.H + W.h0, *b = D.b + W.st0;
+    double cost = 0.0;  // thread 0
+    // ---- prior: d = x [-] x0 per block, res = S d + r
+    if (W.pblk1 > W.pblk0) {
+        int np_ = W.np, m = W.prow1 - W.prow0;
+        if (tid == 0) {
+            int o = 0;
+            for (int k = W.pblk0; k < W.pblk1; k++) {
+                int s = D.prior_slot[k], dim = D.prior_dim[k], gs = W.slot0 + s;
+                const double *x0 = D.prior_x0 + 21 * (size_t)k;
+                double M[9];
+                mtm3(x0, R + 9 * (size_t)gs, M);
+                so3_log_nb(M, sd + o);
+                for (int i = 0; i < 3; i++) sd[o + 3 + i] = p[3 * (size_t)gs + i] - x0[9 + i];
+                if (dim == 15)
+                    for (int i = 0; i < 3; i++) {
+                        sd[o + 6 + i] = v[3 * (size_t)gs + i] - x0[12 + i];
+                        sd[o + 9 + i] = bg[3 * (size_t)gs + i] - x0[15 + i];
+                        sd[o + 12 + i] = ba[3 * (size_t)gs + i] - x0[18 + i];
+                    }
+                for (int i = 0; i < dim; i++) scol[o + i] = D.slot_off[gs] + i;
+                o += dim;
+            }
+        }
+        __syncthreads();
+        const double *S = D.prior_S + W.prS0;
+        for (int r = tid; r < m; r += nt) {
+            double acc = 0.0;
+            for (int c = 0; c < np_; c++) acc += S[(size_t)r * np_ + c] * sd[c];
+            sres[r] = acc + D.prior_r[W.prow0 + r];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double cs = 0.0;
+            for (int r = 0; r < m; r++) cs += sres[r] * sres[r];
+            cost += 0.5 * cs;
+        }
+        if (!trial) {
+            for (int e = tid; e < np_ * np_; e += nt) {
+                int a = e / np_, c = e % np_;
+                double acc = 0.0;
+                for (int r = 0; r < m; r++) acc += S[(size_t)r * np_ + a] * S[(size_t)r * np_ + c];
+                H[(size_t)scol[a] * n + scol[c]] += acc;
+            }
+            for (int a = tid; a < np_; a += nt) {
+                double acc = 0.0;
+                for (int r = 0; r < m; r++) acc += S[(size_t)r * np_ + a] * sres[r];
+                b[scol[a]] += acc;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- IMU factors (vio/window.py:831-862)
+    const int nf = W.imu1 - W.imu0;
+    if (nf > 0) {
+        const double kd = D.win_par[(size_t)XBA_WPAR * w + 6];
+        const double *g = D.win_par + (size_t)XBA_WPAR * w + 20;
+        // raw residuals and Jacobians, one thread per factor
+        for (int f = tid; f < nf; f += nt) {
+            int fi = W.imu0 + f, i = W.slot0 + D.imu_ij[2 * fi], j = W.slot0 + D.imu_ij[2 * fi + 1];
+            double *scr = D.imu_scr + 465 * (size_t)fi;
+            imu_raw(R + 9 * (size_t)i, p + 3 * (size_t)i, v + 3 * (size_t)i, bg + 3 * (size_t)i, ba + 3 * (size_t)i,
+                    R + 9 * (size_t)j, p + 3 * (size_t)j, v + 3 * (size_t)j, bg + 3 * (size_t)j, ba + 3 * (size_t)j,
+                    g, D.imu_rec + (size_t)XBA_IMU_REC * fi, scr, trial ? nullptr : scr + 15,
+                    trial ? nullptr : scr + 240);
+        }
+        __syncthreads();
+        for (int f = 0; f < nf; f++) {
+            int fi = W.imu0 + f;
+            const double *scr = D.imu_scr + 465 * (size_t)fi, *Wm = D.imu_rec + (size_t)XBA_IMU_REC * fi + 76;
+            // whiten: X <- W X (W lower triangular)
+            for (int e = tid; e < 15; e += nt) {
+                double s = 0.0;
+                for (int k = 0; k <= e; k++) s += Wm[15 * e + k] * scr[k];
+                sr[e] = s;
+            }
+            if (!trial)
+                for (int e = tid; e < 450; e += nt) {
+                    int which = e / 225, ic = e % 225, i = ic / 15, c = ic % 15;
+                    const double *X = scr + 15 + 225 * which;
+                    double s = 0.0;
+                    for (int k = 0; k <= i; k++) s += Wm[15 * i + k] * X[15 * k + c];
+                    (which ? sJj : sJi)[ic] = s;
+                }
+            __syncthreads();
+            double e2 = 0.0;
+            for (int k = 0; k < 15; k++) e2 += sr[k] * sr[k];
+            double e = sqrt(e2), wgt = (e <= kd) ? 1.0 : kd / e;
+            if (tid == 0) {
+                cost += (e <= kd) ? 0.5 * e2 : kd * (e - 0.5 * kd);
+                if (D.imu_chi2 && !trial) D.imu_chi2[fi] = e2;
+            }
+            if (!trial) {
+                int i = D.imu_ij[2 * fi], j = D.imu_ij[2 * fi + 1];
+                int ci = D.slot_off[W.slot0 + i], cj = D.slot_off[W.slot0 + j];
+                for (int q = tid; q < 3 * 225; q += nt) {
+                    int which = q / 225, ac = q % 225, a = ac / 15, c = ac % 15;
+                    double s = 0.0;
+                    if (which == 0) {
+                        for (int k = 0; k < 15; k++) s += sJi[15 * k + a] * sJi[15 * k + c];
+                        H[(size_t)(ci + a) * n + ci + c] += wgt * s;
+                    } else if (which == 1) {
+                        for (int k = 0; k < 15; k++) s += sJj[15 * k + a] * sJj[15 * k + c];
+                        H[(size_t)(cj + a) * n + cj + c] += wgt * s;
+                    } else {
+                        for (int k = 0; k < 15; k++) s += sJi[15 * k + a] * sJj[15 * k + c];
+                        H[(size_t)(ci + a) * n + cj + c] += wgt * s;
+                        H[(size_t)(cj + c) * n + ci + a] += wgt * s;
+                    }
+                }
+                for (int q = tid; q < 30; q += nt) {
+                    int which = q / 15, a = q % 15;
+                    double s = 0.0;
+                    const double *X = which ? sJj : sJi;
+                    for (int k = 0; k < 15; k++) s += X[15 * k + a] * sr[k];
+                    b[(which ? cj : ci) + a] += wgt * s;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        double *ws = D.wstate + (size_t)NWS * w;
+        if (trial) {
+            ws[WS_COSTTRIAL_REST] = cost;
+        } else {
+            ws[WS_COSTREST] = cost;
+            double ct = ws[WS_COSTVIS] + cost;
+            ws[WS_CTOT] = ct;
+            ws[WS_CUR] = ct;
+            if (ws[WS_HAVE0] == 0.0) {
+                ws[WS_COST0] = ct;
+                ws[WS_HAVE0] = 1.0;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------- dense Cholesky --
+// In-place lower Cholesky of the n x n row-major A by one workgroup (right-looking, one column per step).
+// Returns false (uniformly) on a non-positive or non-finite pivot.
+template <typename T>
+__device__ bool chol_wg(T *A, int n, int *sflag) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = 0; j < n; j++) {
+        if (tid == 0) {
+            T d = A[(size_t)j * n + j];
+            bool ok = d > (T)0 && isfinite(d);
+            A[(size_t)j * n + j] = ok ? sqrt(d) : (T)1;
+            *sflag = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!*sflag) return false;
+        const T djj = A[(size_t)j * n + j];
+        for (int i = j + 1 + tid; i < n; i += nt) A[(size_t)i * n + j] /= djj;
+        __syncthreads();
+        const int m = n - j - 1;
+        for (int e = tid; e < m * m; e += nt) {
+            int i = j + 1 + e / m, k = j + 1 + e % m;
+            if (k <= i) A[(size_t)i * n + k] -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// x <- (L L^T)^-1 x by forward and back substitution (column-oriented).
+template <typename T>
+__device__ void chol_solve_wg(const T *Lm, int n, T *x) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = 0; j < n; j++) {
+        if (tid == 0) x[j] /= Lm[(size_t)j * n + j];
+        __syncthreads();
+        const T xj = x[j];
+        for (int i = j + 1 + tid; i < n; i += nt) x[i] -= Lm[(size_t)i * n + j] * xj;
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        if (tid == 0) x[j] /= Lm[(size_t)j * n + j];
+        __syncthreads();
+        const T xj = x[j];
+        for (int i = tid; i < j; i += nt) x[i] -= Lm[(size_t)j * n + i] * xj;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------- k_solve --
+// Damped Cholesky solve of every window, dx = -Hd^-1 b, then the trial state.  FP64 (T = double) factors Hd
+// itself; MIXED (T = float) factors the Jacobi-scaled Hd in FP32 and refines x in FP64 against Hd.
+// `lam_mode`: 0 the window's lambda; the factor is kept in fact64/fact32 for the adjoint.
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve(DArgs D, int lds_cap) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
+    __shared__ T sy[XBA_MAX_STATE];
+    __shared__ int sflag;
+    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const WinInfo W = D.wins[w];
+    const int n = W.n;
+    double *ws = D.wstate + (size_t)NWS * w;
+    const double lam = ws[WS_LAM];
+    const double *H = D.H + W.h0, *bv = D.b + W.st0;
+    const bool in_lds = (size_t)n * n * sizeof(T) <= (size_t)lds_cap;
+    T *F = in_lds ? (T *)smem : (sizeof(T) == 8 ? (T *)(D.fact64 + W.h0) : (T *)(D.fact32 + W.h0));
+    // scaling (mixed) and the damped matrix in the working precision
+    for (int i = tid; i < n; i += nt) {
+        double hd = H[(size_t)i * n + i] + (lam * H[(size_t)i * n + i] + 1e-9);
+        ss[i] = (sizeof(T) == 8) ? 1.0 : (hd > 0.0 ? 1.0 / sqrt(hd) : 1.0);
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += nt) {
+        int i = e / n, j = e % n;
+        double h = H[e];
+        if (i == j) h += lam * h + 1e-9;
+        F[e] = (T)(h * ss[i] * ss[j]);
+    }
+    __syncthreads();
+    bool ok = chol_wg<T>(F, n, &sflag);
+    int nref = 0;
+    if (ok) {
+        for (int i = tid; i < n; i += nt) {
+            sx[i] = 0.0;
+            sr[i] = bv[i];
+        }
+        __syncthreads();
+        const int maxit = (sizeof(T) == 8) ? 1 : D.nref_max;
+        for (int it = 0; it < maxit; it++) {
+            for (int i = tid; i < n; i += nt) sy[i] = (T)(sr[i] * ss[i]);
+            __syncthreads();
+            chol_solve_wg<T>(F, n, sy);
+            for (int i = tid; i < n; i += nt) sx[i] += (double)sy[i] * ss[i];
+            __syncthreads();
+            nref = it + 1;
+            if (it + 1 == maxit) break;
+            // residual r = b - Hd x in FP64
+            for (int i = tid; i < n; i += nt) {
+                double acc = bv[i];
+                const double *Hi = H + (size_t)i * n;
+                for (int j = 0; j < n; j++) acc -= Hi[j] * sx[j];
+                acc -= (lam * Hi[i] + 1e-9) * sx[i];
+                sr[i] = acc;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double rn = 0.0, bn = 0.0;
+                for (int i = 0; i < n; i++) {
+                    rn = fmax(rn, fabs(sr[i]));
+                    bn = fmax(bn, fabs(bv[i]));
+                }
+                sflag = (rn <= 1e-15 * bn) ? 1 : 0;
+            }
+            __syncthreads();
+            if (sflag) break;
+        }
+        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
+        // keep the factor for the adjoint when it lives in LDS
+        if (in_lds)
+            for (int e = tid; e < n * n; e += nt) {
+                if (sizeof(T) == 8)
+                    D.fact64[W.h0 + e] = (double)F[e];
+                else
+                    D.fact32[W.h0 + e] = (float)F[e];
+            }
+        __syncthreads();
+        // trial state (vio/window.py:522-532, 945-955)
+        for (int s = tid; s < W.P; s += nt) {
+            int gs = W.slot0 + s, o = D.slot_off[gs];
+            const double *dxs = D.dx + W.st0 + o;
+            double E[9];
+            so3_exp_nb(dxs, E);
+            mm3(D.Rwb + 9 * (size_t)gs, E, D.R2 + 9 * (size_t)gs);
+            for (int i = 0; i < 3; i++) D.p2[3 * (size_t)gs + i] = D.pwb[3 * (size_t)gs + i] + dxs[3 + i];
+            if (W.vi) {
+                bool full = D.slot_dim[gs] == 15;
+                for (int i = 0; i < 3; i++) {
+                    D.v2[3 * (size_t)gs + i] = D.vel[3 * (size_t)gs + i] + (full ? dxs[6 + i] : 0.0);
+                    D.g2[3 * (size_t)gs + i] = D.bgs[3 * (size_t)gs + i] + (full ? dxs[9 + i] : 0.0);
+                    D.a2[3 * (size_t)gs + i] = D.bas[3 * (size_t)gs + i] + (full ? dxs[12 + i] : 0.0);
+                }
+            }
+        }
+    } else {
+        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
+    }
+    if (tid == 0) {
+        ws[WS_CHOLOK] = ok ? 1.0 : 0.0;
+        ws[WS_NREF] = nref;
+    }
+}
+
+// ------------------------------------------------------------------------------------------- k_accept --
+// c_new = trial visual cost (chunk sums in order) + prior/IMU trial cost; accept iff c_new < c_tot.
+__global__ __launch_bounds__(256) void k_accept(DArgs D) {
+    __shared__ int sacc;
+    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const WinInfo W = D.wins[w];
+    double *ws = D.wstate + (size_t)NWS * w;
+    if (tid == 0) {
+        int acc = 0;
+        if (ws[WS_CHOLOK] != 0.0) {
+            double cn = 0.0;
+            for (int c = W.c0; c < W.c1; c++) cn += D.partc2[c];
+            cn += ws[WS_COSTTRIAL_REST];
+            ws[WS_CNEW] = cn;
+            acc = cn < ws[WS_CTOT];
+        }
+        if (acc) {
+            ws[WS_LAM] = fmax(ws[WS_LAM] / 10.0, 1e-12);
+            ws[WS_ACC] += 1.0;
+            ws[WS_CUR] = ws[WS_CNEW];
+        } else {
+            ws[WS_LAM] *= 10.0;
+        }
+        ws[WS_ACCEPT] = acc ? 1.0 : 0.0;
+        ws[WS_ITERS] += 1.0;
+        sacc = acc;
+    }
+    __syncthreads();
+    if (!sacc) return;
+    for (int s = tid; s < W.P; s += nt) {
+        size_t gs = W.slot0 + s;
+        for (int i = 0; i < 9; i++) D.Rwb[9 * gs + i] = D.R2[9 * gs + i];
+        for (int i = 0; i < 3; i++) D.pwb[3 * gs + i] = D.p2[3 * gs + i];
+        if (W.vi)
+            for (int i = 0; i < 3; i++) {
+                D.vel[3 * gs + i] = D.v2[3 * gs + i];
+                D.bgs[3 * gs + i] = D.g2[3 * gs + i];
+                D.bas[3 * gs + i] = D.a2[3 * gs + i];
+            }
+    }
+}
+
+template __global__ void k_reduce<float>(DArgs, const float *, const float *, const float *, const float *);
+template __global__ void k_reduce<double>(DArgs, const double *, const double *, const double *, const double *);
+template __global__ void k_solve<float>(DArgs, int);
+template __global__ void k_solve<double>(DArgs, int);
+
+}  // namespace xba

@@ -1,0 +1,81 @@
+// Internal declarations of libxba: work-list records, per-window records and the kernel argument block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/xba.h"
+
+#define XBA_WAVE 64
+
+// One work item of the landmark kernels: a contiguous landmark range of one window whose observations touch only
+// the slots [lo, lo + R).  Its reduced-system partial lives at partH (R(R+1)/2 lower-triangular 6x6 blocks) and
+// partb (6R entries) of the partial buffers.
+struct Chunk {
+    int32_t win, l0, l1, lo, R, pad;
+    int64_t partH, partb;
+};
+
+// Per-window record.
+struct WinInfo {
+    int32_t slot0, P, lm0, L, n, c0, c1, imu0, imu1, pblk0, pblk1, prow0, prow1, np, vi, pad;
+    int64_t st0;   // offset of the window's state vector (dx, b)
+    int64_t h0;    // offset of the window's n x n matrices
+    int64_t prS0;  // offset of the window's prior S
+};
+
+// Per-window scalar state (double[nwin * NWS]).
+enum {
+    WS_LAM = 0,
+    WS_COSTVIS,
+    WS_COSTREST,
+    WS_CTOT,
+    WS_CNEW,
+    WS_COST0,
+    WS_ACC,
+    WS_CHOLOK,
+    WS_ACCEPT,
+    WS_NREF,
+    WS_HAVE0,
+    WS_CUR,
+    WS_ITERS,
+    WS_COSTTRIAL_REST,
+    NWS = 16
+};
+
+// Packed observation word: slot (bits 0-11) | cam << 12 | valid << 13.  Packed landmark word: host slot (bits
+// 0-11) | mask << 12.
+__host__ __device__ inline int32_t pack_obs(int slot, int cam, int valid) {
+    return slot | (cam << 12) | (valid << 13);
+}
+
+template <typename T>
+struct KArgs {
+    int32_t nchunks, nwin, nlm, nobs;
+    const Chunk *chunks;
+    const WinInfo *wins;
+    const int32_t *ob_start;   // (nlm+1)
+    const int32_t *lm_info;    // (nlm) packed landmark word
+    const int32_t *ob_info;    // (nobs) packed observation word
+    const int32_t *slot_off;   // (nslot) offset of the slot's block inside its window's state vector
+    const double *z;           // (nobs,2) measurements stay FP64: residuals are formed in FP64
+    const T *isig;             // (nobs)
+    double *lmp;               // (nlm,3) landmark state (FP64)
+    double *lmp_trial;         // (nlm,3)
+    const double *rho_meas;    // (nlm)
+    const T *rho_isig;         // (nlm)
+    const double *Rwb, *pwb, *R2, *p2;  // (nslot,9), (nslot,3)
+    const double *dx;          // (sum n)
+    const double *win_par;     // (nwin, XBA_WPAR)
+    T *partH, *partb;
+    T *partH_lo, *partb_lo;    // FP32 partials: the low parts of the compensated tile sums
+    double *partc, *partc2;
+    T *lmHi, *lmbl;            // (nlm,6) symmetric damped inverse, (nlm,3)
+    uint8_t *lmok;
+    double *wstate;            // (nwin, NWS)
+    double lam_override;       // >= 0: damping for every window (linearize); < 0: per-window lambda
+    double *chi2_out;          // (nobs) or null
+    double *Hinv_out;          // (nlm,9) or null
+    uint8_t *ok_out;           // (nlm) or null
+    double *adj_gz, *adj_gw;   // adjoint outputs (nobs,2), (nobs) or null
+    const double *adj_y;       // adjoint solve H^-1 g (sum n) or null
+};
