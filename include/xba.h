@@ -99,6 +99,10 @@ typedef struct xba_options {
     int32_t grid;               /* persistent grid of the assembly kernels (workgroups); 0 = automatic */
     int32_t use_graph;          /* 1: capture one LM iteration in a hipGraph and replay it */
     int32_t want_chi2;          /* 1: xba_linearize writes per-observation chi2 */
+    int32_t deterministic;      /* 1: single-wavefront assembly kernels whose every sum runs in a fixed order
+                                   (bitwise reproducible; FP32 tiles use Kahan-compensated FP32 accumulation);
+                                   0: multi-wavefront kernels accumulating the reduced system in FP64 LDS atomics
+                                   (reproducible to rounding only) */
 } xba_options;
 
 typedef struct xba_ctx xba_ctx;
@@ -133,6 +137,16 @@ int xba_linearize(xba_ctx *ctx, double lam, double *H_out, double *b_out, double
 
 /* Sizes the caller needs for the outputs above: total state dimension and total H entries. */
 int xba_sizes(const xba_ctx *ctx, int64_t *n_state, int64_t *n_hentries);
+
+/* Diagnostics (no reference counterpart): run ONE LM iteration eagerly (it advances the solve like
+ * xba_iterate(ctx, 1)) with HIP events around each kernel, and write the kernel durations in ms to ms_out (host,
+ * 8): linearise+Schur, reduce, prior/IMU terms, Cholesky solve + update, back-substitution + trial cost, trial
+ * prior/IMU cost, accept, landmark commit.  Synchronous. */
+int xba_profile_iteration(xba_ctx *ctx, double *ms_out, void *stream);
+
+/* Plan summary (host out, 8): work items, persistent grid, max slot tile, LDS bytes of the landmark kernels, LDS
+ * bytes of the dense solve, partial-tile entries, partial-b entries, max state dimension. */
+int xba_plan_info(const xba_ctx *ctx, int64_t *out);
 
 /* Implicit-function adjoint of the last iteration's solved update dx* = -H^-1 b (Gauss-Newton H, at the state
  * the update was linearised at, reusing the Cholesky factor): given g = dL/d(dx) (device, sum n_w), writes

@@ -1,0 +1,335 @@
+"""Pins the CPU oracle (oracle/xba_oracle.c) before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  - golden vectors written by the reference's own pure-numpy `_solve_np` / `make_problem` (tests/golden/);
+  - central finite differences of the oracle's own residuals (visual factor, IMU factor);
+  - brute force: the Schur-reduced system equals the dense full system reduced with numpy, and one damped LM step
+    equals the dense solve of the full damped system (Schur identity);
+  - closed forms and identities (SO(3)/SE(3) as in the reference's tests/test_lie.py, Cholesky vs numpy,
+    huber_k_dof vs scipy, W^T W = C^-1).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xclean_amd import synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+# ------------------------------------------------------------------------------------------- golden vectors --
+def test_make_problem_reproduces_the_reference_rng_stream():
+    g = np.load(os.path.join(GOLD, "make_problem_nk6_nl60_s1.npz"))
+    a = synth.make_problem(nk=6, nl=60, seed=1)
+    for k in ("okf", "olm"):
+        np.testing.assert_array_equal(a[k], g[k])
+    for k in ("poses0", "pts0", "ouv", "poses_true", "pts_true"):
+        np.testing.assert_allclose(a[k], g[k], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["default", "mask", "rejects", "prior_rotf"])
+def test_oracle_solve_np_matches_reference_golden(case):
+    g = np.load(os.path.join(GOLD, "solve_np_%s.npz" % case))
+
+    class Rig:
+        fx_rect, cx, cy = float(g["fx"]), float(g["cx"]), float(g["cy"])
+    prior = None
+    rotf = None
+    if "pr_c" in g:
+        prior = (g["pr_c"], g["pr_rho0"], float(g["pr_invsig"]), g["pr_valid"])
+    if "rot_ab" in g:
+        rotf = [(int(a), int(b), M, float(s)) for (a, b), M, s in zip(g["rot_ab"], g["rot_M"], g["rot_sig"])]
+    free = g["free"]
+    free = int(free) if free.ndim == 0 else free
+    p, q, st = O.solve_np(g["poses0"], g["pts0"], g["okf"], g["olm"], g["ouv"], Rig, free, int(g["iters"]),
+                          float(g["lam0"]), prior=prior, rotf=rotf)
+    assert _rel(p, g["out_poses"]) < 1e-10
+    assert _rel(q, g["out_pts"]) < 1e-10
+    np.testing.assert_allclose([st["chi2_0"], st["chi2_n"], st["rms_0"], st["rms_n"]], g["stats"], rtol=1e-10)
+
+
+def test_oracle_solve_np_moves_poses_toward_the_truth():
+    """The invariant of the reference's tests/test_ba_solver.py:26-33."""
+    pb = synth.make_problem()
+    p, _q, st = O.solve_np(pb["poses0"], pb["pts0"], pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 5)
+    err = lambda a, b: float(np.sqrt(np.mean(np.sum((a[:, :3, 3] - b[:, :3, 3]) ** 2, 1))))
+    assert err(p, pb["poses_true"]) < 0.6 * err(pb["poses0"], pb["poses_true"])
+    assert st["rms_n"] < st["rms_0"]
+
+
+# ----------------------------------------------------------------------------------------------- SO(3)/SE(3) --
+MAGS = [0.0, 1e-14, 1e-10, 1e-8, 1e-6, 0.063, 0.126, 0.252, 1e-2, 0.1, 1.0, 2.0, 3.0, np.pi - 1e-6]
+
+
+def _axes(seed, n=6):
+    a = np.random.RandomState(seed).standard_normal((n, 3))
+    return a / np.linalg.norm(a, axis=1, keepdims=True)
+
+
+def _expm_so3(w):
+    from scipy.linalg import expm
+    return expm(synth.hat(w))
+
+
+def test_so3_exp_and_log_against_expm_and_round_trips():
+    for t in MAGS:
+        for ax in _axes(11):
+            w = ax * t
+            R = _expm_so3(w)
+            assert np.abs(O.so3_exp_nb(w) - R).max() < 1e-12
+            Rq, Jr = O.exp_jr(w)
+            assert np.abs(Rq - R).max() < 1e-12
+            assert np.abs(O.log_quat(R) - w).max() < 1e-8
+            if t < np.pi - 1e-3:
+                assert np.abs(O.so3_log_nb(R) - w).max() < 1e-8
+
+
+def test_jr_inverse_identity_and_jr_by_finite_differences():
+    for t in MAGS:
+        for ax in _axes(7, 4):
+            w = ax * t
+            _R, Jr = O.exp_jr(w)
+            assert np.abs(Jr @ O.jr_inv(w) - np.eye(3)).max() < 1e-12
+            if t < 3.0:
+                # Exp(w + dv) = Exp(w) Exp(Jr dv)
+                h = 1e-7
+                R0 = O.so3_exp_nb(w)
+                for k in range(3):
+                    dv = np.zeros(3)
+                    dv[k] = h
+                    Rp = O.so3_exp_nb(w + dv)
+                    d = O.log_quat(R0.T @ Rp) / h
+                    assert np.abs(d - Jr[:, k]).max() < 1e-5
+
+
+def test_se3_exp_against_matrix_exponential():
+    from scipy.linalg import expm
+    rng = np.random.RandomState(3)
+    for t in MAGS[:-1]:
+        for ax in _axes(13, 3):
+            xi = np.concatenate([rng.standard_normal(3), ax * t])
+            M = np.zeros((4, 4))
+            M[:3, :3] = synth.hat(xi[3:])
+            M[:3, 3] = xi[:3]
+            assert np.abs(O.se3_exp(xi) - expm(M)).max() < 1e-11
+
+
+def test_so3_project_equals_svd_projection():
+    rng = np.random.RandomState(5)
+    for _ in range(20):
+        R = _expm_so3(rng.standard_normal(3)) + rng.standard_normal((3, 3)) * 1e-7
+        U, _s, Vt = np.linalg.svd(R)
+        D = np.eye(3)
+        D[2, 2] = np.sign(np.linalg.det(U @ Vt))
+        assert np.abs(O.so3_project(R) - U @ D @ Vt).max() < 1e-14
+
+
+def test_cholesky_solve_against_numpy():
+    rng = np.random.RandomState(1)
+    A = rng.standard_normal((40, 40))
+    A = A @ A.T + 40 * np.eye(40)
+    b = rng.standard_normal(40)
+    assert _rel(O.chol_solve(A, b), np.linalg.solve(A, b)) < 1e-13
+    assert O.chol_solve(-np.eye(3), np.ones(3)) is None
+
+
+# --------------------------------------------------------------------------------------------- visual factor --
+def _perturbed_state_residual(pb, l, k, which, d):
+    """Residual of obs k after perturbing: 'l' landmark (a, b, rho) additively; 't'/'h' target/host body pose
+    [dphi, dp] (R <- R Exp(dphi), p <- p + dp)."""
+    q = pb.copy()
+    if which == "l":
+        q.lmp[l] = q.lmp[l] + d
+    else:
+        s = q.ob_pose[k] if which == "t" else q.lm_host[l]
+        q.Rwb[s] = q.Rwb[s] @ O.so3_exp_nb(d[:3])
+        q.pwb[s] = q.pwb[s] + d[3:]
+    return O.obs_residual_jac(q, l, k)[0]
+
+
+@pytest.mark.parametrize("stereo,vi", [("prior", False), ("ur", False), ("prior", True)])
+def test_visual_jacobians_by_central_finite_differences(stereo, vi):
+    pb, _ = synth.make_window(P=5, L=40, views=(2, 5), seed=[9, 1], stereo=stereo, vi=vi)
+    rng = np.random.RandomState(0)
+    h = 1e-6
+    checked = 0
+    for l in rng.choice(pb.L, 12, replace=False):
+        for k in range(pb.ob_start[l], pb.ob_start[l + 1]):
+            e, Jl, Jt, Jh = O.obs_residual_jac(pb, l, k)
+            same = pb.ob_pose[k] == pb.lm_host[l]
+            for which, J, dim in (("l", Jl, 3), ("t", Jt, 6), ("h", Jh, 6)):
+                if same and which != "l":
+                    assert np.all(J == 0)
+                    continue
+                Jfd = np.zeros_like(J)
+                for c in range(dim):
+                    d = np.zeros(dim)
+                    d[c] = h
+                    Jfd[:, c] = (_perturbed_state_residual(pb, l, k, which, d) -
+                                 _perturbed_state_residual(pb, l, k, which, -d)) / (2 * h)
+                scale = max(np.abs(Jfd).max(), 1.0)
+                assert np.abs(J - Jfd).max() < 2e-5 * scale, (which, J, Jfd)
+                checked += 1
+    assert checked > 20
+
+
+def _dense_problem(pb, lam):
+    """Stack every valid observation's whitened, Huber-weighted rows into the dense [Jp | Jl] system, add the
+    inverse-depth prior rows and the landmark damping, and return (Hpp, Hpl, Hll, bp, bl)."""
+    P, L = pb.P, pb.L
+    n6 = 6 * P
+    Hfull = np.zeros((n6 + 3 * L, n6 + 3 * L))
+    bfull = np.zeros(n6 + 3 * L)
+    for l in range(L):
+        for k in range(pb.ob_start[l], pb.ob_start[l + 1]):
+            e, Jl, Jt, Jh = O.obs_residual_jac(pb, l, k)
+            isg = pb.ob_isig[k]
+            e, Jl, Jt, Jh = e * isg, Jl * isg, Jt * isg, Jh * isg
+            en = np.linalg.norm(e)
+            w = 1.0 if en <= pb.huber_k else pb.huber_k / en
+            J = np.zeros((len(e), n6 + 3 * L))
+            t, hh = pb.ob_pose[k], pb.lm_host[l]
+            if t != hh:
+                J[:, 6 * t:6 * t + 6] += Jt
+                J[:, 6 * hh:6 * hh + 6] += Jh
+            J[:, n6 + 3 * l:n6 + 3 * l + 3] = Jl
+            Hfull += w * J.T @ J
+            bfull += w * J.T @ e
+        if pb.rho_isig[l] > 0:
+            r0 = (pb.lmp[l, 2] - pb.rho_meas[l]) * pb.rho_isig[l]
+            w = 1.0 if abs(r0) <= pb.huber_k else pb.huber_k / abs(r0)
+            i = n6 + 3 * l + 2
+            Hfull[i, i] += w * pb.rho_isig[l] ** 2
+            bfull[i] += w * pb.rho_isig[l] * r0
+    for l in range(L):
+        for j in range(3):
+            i = n6 + 3 * l + j
+            Hfull[i, i] += lam * Hfull[i, i] + 1e-12
+    return Hfull, bfull, n6
+
+
+def test_schur_reduced_system_equals_dense_brute_force():
+    pb, _ = synth.make_window(P=5, L=60, views=(2, 5), seed=[9, 2], stereo="ur")
+    lam = 1e-3
+    _c, H6, b6, *_ = O.window_build(pb.copy(), lam)
+    Hf, bf, n6 = _dense_problem(pb, lam)
+    App, Apl, All = Hf[:n6, :n6], Hf[:n6, n6:], Hf[n6:, n6:]
+    Hred = App - Apl @ np.linalg.solve(All, Apl.T)
+    bred = bf[:n6] - Apl @ np.linalg.solve(All, bf[n6:])
+    assert _rel(H6, Hred) < 1e-9
+    assert _rel(b6, bred) < 1e-9
+
+
+def test_one_lm_step_equals_dense_damped_solve():
+    """solve()'s update equals the dense solve of the full system damped on the landmark blocks (Marquardt) and on
+    the reduced diagonal (vio/window.py:933-935), prior included: a Schur-identity check of the whole step."""
+    pb, _ = synth.make_window(P=5, L=50, views=(2, 5), seed=[9, 3])
+    lam = 1e-4
+    ref = pb.copy()
+    st, dxh, _acc = O.window_solve(ref, iters=1, lam0=lam, history=True)
+    _c, H6, b6, *_ = O.window_build(pb.copy(), lam)
+    H, b, _cr = O.window_full_system(pb.copy(), H6, b6)
+    Hf, bf, n6 = _dense_problem(pb, lam)
+    Hf[:n6, :n6] += H - H6          # the prior's information on the pose block
+    bf[:n6] += b - b6
+    Hf[np.arange(n6), np.arange(n6)] += lam * np.diag(H) + 1e-9
+    x = -np.linalg.solve(Hf, bf)
+    assert _rel(dxh[0], x[:n6]) < 1e-8
+
+
+def test_gauss_newton_step_at_noise_free_truth_is_zero():
+    pb, tr = synth.make_window(P=5, L=80, views=(2, 5), seed=[9, 4], outlier_frac=0.0, px_sigma=1.0)
+    # replace the measurements by exact projections of the truth and start at the truth
+    pb.Rwb[:] = tr["Rwb"]
+    pb.pwb[:] = tr["pwb"]
+    pb.lmp[:] = tr["lmp"]
+    pb.rho_meas[:] = tr["lmp"][:, 2]
+    pb.prior = None
+    for l in range(pb.L):
+        for k in range(pb.ob_start[l], pb.ob_start[l + 1]):
+            e, *_ = O.obs_residual_jac(pb, l, k)
+            pb.ob_z[k, :len(e)] += e
+    st, dxh, _ = O.window_solve(pb.copy(), iters=1, history=True)
+    assert st["cost0"] < 1e-12
+    assert np.abs(dxh[0]).max() < 1e-9
+
+
+# ----------------------------------------------------------------------------------------------- IMU factor --
+def _imu_case(seed=0):
+    pb, tr = synth.make_window(P=3, L=5, views=(2, 3), seed=[9, 5, seed], vi=True)
+    i, j, pre, W = pb.imu[0]
+    st = lambda s: (pb.Rwb[s], pb.pwb[s], pb.vel[s], pb.bgs[s] + 1e-3, pb.bas[s] - 2e-3)
+    return pb, st(i), st(j), pre, W
+
+
+def _box_plus(s, d):
+    R, p, v, bg, ba = s
+    return (R @ O.so3_exp_nb(d[0:3]), p + d[3:6], v + d[6:9], bg + d[9:12], ba + d[12:15])
+
+
+def test_imu_factor_jacobians_by_central_finite_differences():
+    pb, si, sj, pre, W = _imu_case()
+    g = pb.g_w
+    r, Ji, Jj, Jg = O.imu_factor(si, sj, pre, g, W, whiten=True)
+    h = 1e-6
+    for which, J in (("i", Ji), ("j", Jj)):
+        Jfd = np.zeros((15, 15))
+        for c in range(15):
+            d = np.zeros(15)
+            d[c] = h
+            a = (_box_plus(si, d), sj) if which == "i" else (si, _box_plus(sj, d))
+            b_ = (_box_plus(si, -d), sj) if which == "i" else (si, _box_plus(sj, -d))
+            Jfd[:, c] = (O.imu_factor(*a, pre, g, W)[0] - O.imu_factor(*b_, pre, g, W)[0]) / (2 * h)
+        assert np.abs(J - Jfd).max() < 1e-5 * max(1.0, np.abs(Jfd).max())
+    Jgfd = np.zeros((15, 3))
+    for c in range(3):
+        d = np.zeros(3)
+        d[c] = h
+        Jgfd[:, c] = (O.imu_factor(si, sj, pre, g + d, W)[0] - O.imu_factor(si, sj, pre, g - d, W)[0]) / (2 * h)
+    assert np.abs(Jg - Jgfd).max() < 1e-5 * max(1.0, np.abs(Jgfd).max())
+
+
+def test_imu_residual_is_small_at_the_truth_and_whitening_is_the_inverse_sqrt_covariance():
+    pb, tr = synth.make_window(P=3, L=5, views=(2, 3), seed=[9, 6], vi=True)
+    i, j, pre, W = pb.imu[0]
+    si = (tr["Rwb"][i], tr["pwb"][i], tr["vel"][i], tr["bg"], tr["ba"])
+    sj = (tr["Rwb"][j], tr["pwb"][j], tr["vel"][j], tr["bg"], tr["ba"])
+    r, *_ = O.imu_factor(si, sj, pre, pb.g_w, W, whiten=True)
+    assert np.linalg.norm(r[:9]) < 20.0             # a consistent factor: whitened chi ~ sqrt(9) (noise)
+    assert np.abs(W.T @ W @ pre.cov - np.eye(15)).max() < 1e-6
+
+
+def test_huber_k_dof_matches_the_chi2_equal_tail_rule():
+    assert abs(synth.huber_k_dof(1) - 1.345) < 1e-12
+    assert abs(synth.huber_k_dof(15) - 4.46) < 0.01
+
+
+# ---------------------------------------------------------------------------------------------- LM driver --
+def test_window_solve_costs_are_monotone_and_lambda_follows_the_rule():
+    pb, _ = synth.make_window(P=6, L=300, views=(2, 6), seed=[9, 7])
+    costs = []
+    lam = 1e-4
+    for it in range(5):
+        st = O.window_solve(pb, iters=1, lam0=lam)
+        costs.append(st["cost"])
+        expect = max(lam / 10, 1e-12) if st["accepted"] else lam * 10
+        assert abs(st["lam"] - expect) <= 1e-15 * expect
+        lam = st["lam"]
+    assert all(b <= a for a, b in zip(costs, costs[1:]))
+
+
+def test_posefree_block_equals_the_full_solve():
+    """vio/CONVENTIONS.md: moving pose-free landmarks into Problem.pf gives the same solve."""
+    full, _ = synth.make_window(P=6, L=200, views=(1, 6), seed=[9, 8], posefree=False)
+    split, _ = synth.make_window(P=6, L=200, views=(1, 6), seed=[9, 8], posefree=True)
+    assert split.pf is not None and len(split.pf.host) > 0
+    sf = O.window_solve(full, iters=3)
+    ss = O.window_solve(split, iters=3)
+    assert abs(sf["cost"] - ss["cost"]) <= 1e-12 * abs(sf["cost"])
+    assert _rel(split.pwb, full.pwb) < 1e-12

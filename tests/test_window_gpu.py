@@ -37,15 +37,16 @@ def _case(name):
     raise KeyError(name)
 
 
+@pytest.mark.parametrize("det", [False, True])
 @pytest.mark.parametrize("name", ["vo_small", "vi_small", "ur", "posefree"])
-def test_linearize_matches_oracle(name):
+def test_linearize_matches_oracle(name, det):
     from xclean_amd.window import BatchSolver
     pb = _case(name)
     lam = 1e-4
     c, H6, b6, Hi, bl, Hlp, ok, chi = O.window_build(pb.copy(), lam)
     ref = pb.copy()
     H, b, crest = O.window_full_system(ref, H6, b6)
-    bs = BatchSolver([pb], precision="fp64", use_graph=False)
+    bs = BatchSolver([pb], precision="fp64", use_graph=False, deterministic=det)
     bs.upload(lam)
     r = bs.linearize(lam)[0]
     n = H.shape[0]
@@ -65,12 +66,13 @@ def test_linearize_matches_oracle(name):
         np.testing.assert_allclose(r["Hll_inv"][ok], Hi[ok], rtol=1e-10, atol=1e-14)
 
 
+@pytest.mark.parametrize("det", [False, True])
 @pytest.mark.parametrize("name", ["vo_small", "cfg0", "vi_small", "ur", "posefree"])
-def test_fp64_first_update_matches_oracle(name):
+def test_fp64_first_update_matches_oracle(name, det):
     pb = _case(name)
     ref = pb.copy()
     st_ref, dx_ref, acc_ref = O.window_solve(ref, iters=1, history=True)
-    bs, st = _gpu_solve([pb], 1, "fp64", use_graph=False)
+    bs, st = _gpu_solve([pb], 1, "fp64", use_graph=False, deterministic=det)
     dx = bs.last_dx()[0]
     e = _rel(dx, dx_ref[0])
     print(name, "dx rel err", e, "cost", st[0]["cost"], st_ref["cost"])
@@ -108,20 +110,22 @@ def test_batch_of_uneven_windows_matches_oracle_per_window():
         assert _rel(pb.lmp, ref.lmp) < 1e-9
 
 
-def test_graph_replay_is_bitwise_equal_to_eager():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_deterministic_kernels_graph_replay_is_bitwise_equal_to_eager(precision):
     a = _case("vo_small")
     b = a.copy()
-    _gpu_solve([a], 4, "fp64", use_graph=False)
-    _gpu_solve([b], 4, "fp64", use_graph=True)
+    _gpu_solve([a], 4, precision, use_graph=False, deterministic=True)
+    _gpu_solve([b], 4, precision, use_graph=True, deterministic=True)
     np.testing.assert_array_equal(a.Rwb, b.Rwb)
     np.testing.assert_array_equal(a.lmp, b.lmp)
 
 
-def test_mixed_precision_update_error_vs_fp64_oracle():
+@pytest.mark.parametrize("det", [False, True])
+def test_mixed_precision_update_error_vs_fp64_oracle(det):
     pb = _case("cfg0")
     ref = pb.copy()
     st_ref, dx_ref, _ = O.window_solve(ref, iters=1, history=True)
-    bs, st = _gpu_solve([pb], 1, "mixed", use_graph=False)
+    bs, st = _gpu_solve([pb], 1, "mixed", use_graph=False, deterministic=det)
     dx = bs.last_dx()[0]
     e = _rel(dx, dx_ref[0])
     print("mixed dx rel err vs FP64 oracle:", e, "refinement steps", st[0]["refine_steps"])
@@ -137,3 +141,12 @@ def test_masked_invalid_and_empty_landmarks():
     bs, st = _gpu_solve([pb], 2, "fp64")
     assert _rel(bs.last_dx()[0], dx_ref[1]) < 1e-9
     assert _rel(pb.lmp, ref.lmp) < 1e-9
+
+
+def test_fast_graph_replay_matches_eager_to_rounding():
+    a = _case("vo_small")
+    b = a.copy()
+    _gpu_solve([a], 4, "fp64", use_graph=False)
+    _gpu_solve([b], 4, "fp64", use_graph=True)
+    assert _rel(a.lmp, b.lmp) < 1e-12
+    assert _rel(a.pwb, b.pwb) < 1e-12

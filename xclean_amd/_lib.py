@@ -33,7 +33,8 @@ class XbaBatch(C.Structure):
 
 class XbaOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("lm_per_chunk", C.c_int32), ("slot_tile", C.c_int32),
-                ("grid", C.c_int32), ("use_graph", C.c_int32), ("want_chi2", C.c_int32)]
+                ("grid", C.c_int32), ("use_graph", C.c_int32), ("want_chi2", C.c_int32),
+                ("deterministic", C.c_int32)]
 
 
 class XbaNpProblem(C.Structure):
@@ -50,7 +51,8 @@ class XbaError(RuntimeError):
 
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
-           "xba_linearize", "xba_sizes", "xba_backward", "xba_np_solve", "xba_strerror", "xba_version")
+           "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_backward", "xba_np_solve",
+           "xba_strerror", "xba_version")
 
 
 def load():
@@ -71,6 +73,8 @@ def load():
     L.xba_last_dx.argtypes = [vp, vp, vp]
     L.xba_linearize.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
     L.xba_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.xba_profile_iteration.argtypes = [vp, vp, vp]
+    L.xba_plan_info.argtypes = [vp, vp]
     L.xba_backward.argtypes = [vp, vp, vp, vp, vp]
     L.xba_np_solve.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_strerror.restype = C.c_char_p

@@ -12,6 +12,7 @@
 #include "xba_internal.h"
 #include "xba_lin.hip"
 #include "xba_dense.hip"
+#include "xba_fast.hip"
 
 using namespace xba;
 
@@ -82,7 +83,8 @@ struct xba_ctx {
     int nwin = 0, nslot = 0, nlm = 0, nobs = 0, nimu = 0, npblk = 0;
     int nchunks = 0, Rmax = 1, maxn = 0, grid = 1;
     int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
-    int lin_lds = 0, solve_lds_cap = 0;
+    int lin_lds = 0, solve_lds_cap = 0, fast_nw = 8, fast_nw_trial = 8, fast_lds = 0, fast_lds_trial = 0;
+    bool fast = true;
     bool uploaded = false;
     std::vector<void *> owned;
     // device
@@ -93,6 +95,7 @@ struct xba_ctx {
     int32_t *ob_info = nullptr, *lm_info = nullptr;
     void *isigT = nullptr, *lmpT = nullptr, *lmp_trial = nullptr, *riT = nullptr;
     void *partH = nullptr, *partb = nullptr, *partH_lo = nullptr, *partb_lo = nullptr, *lmHi = nullptr, *lmbl = nullptr;
+    double *partH64 = nullptr, *partb64 = nullptr;
     double *partc = nullptr, *partc2 = nullptr;
     uint8_t *lmok = nullptr;
     double *Rwb = nullptr, *pwb = nullptr, *vel = nullptr, *bgs = nullptr, *bas = nullptr;
@@ -136,6 +139,8 @@ KArgs<T> kargs(xba_ctx *c) {
     A.partb = (T *)c->partb;
     A.partH_lo = (T *)c->partH_lo;
     A.partb_lo = (T *)c->partb_lo;
+    A.partH64 = c->partH64;
+    A.partb64 = c->partb64;
     A.partc = c->partc;
     A.partc2 = c->partc2;
     A.lmHi = (T *)c->lmHi;
@@ -182,13 +187,40 @@ bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
 
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
 template <typename T>
+void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
+    if (c->fast)
+        hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A, c->Rmax);
+    else
+        hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+}
+
+template <typename T>
+void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
+    if (c->fast)
+        hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s, A,
+                           c->Rmax);
+    else
+        hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+}
+
+template <typename T>
+void launch_reduce(xba_ctx *c, const DArgs &D, hipStream_t s) {
+    int gx = (int)(((int64_t)c->maxn * c->maxn + 255) / 256);
+    if (c->fast)
+        hipLaunchKernelGGL(k_reduce<double>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D,
+                           (const double *)c->partH64, (const double *)c->partb64, (const double *)nullptr,
+                           (const double *)nullptr);
+    else
+        hipLaunchKernelGGL(k_reduce<T>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D, (const T *)c->partH,
+                           (const T *)c->partb, (const T *)c->partH_lo, (const T *)c->partb_lo);
+}
+
+template <typename T>
 void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 = nullptr) {
-    hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    launch_lin_kernel<T>(c, A, s);
     DArgs D = dargs(c);
     D.imu_chi2 = imu_chi2;
-    int gx = (int)(((int64_t)c->maxn * c->maxn + 255) / 256);
-    hipLaunchKernelGGL(k_reduce<T>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D, (const T *)c->partH,
-                       (const T *)c->partb, (const T *)c->partH_lo, (const T *)c->partb_lo);
+    launch_reduce<T>(c, D, s);
     hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
 }
 
@@ -198,13 +230,32 @@ void launch_iter(xba_ctx *c, hipStream_t s) {
     launch_lin<T>(c, A, s);
     DArgs D = dargs(c);
     hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
-    hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    launch_trial_kernel<T>(c, A, s);
     hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
     hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
     int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
     hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
 }
 
+}  // namespace
+
+namespace {
+// Working-precision copies of the measurements and the packed observation / landmark words (once per context).
+int pack_measurements(xba_ctx *c, hipStream_t s) {
+    const xba_batch &B = c->B;
+    int n = std::max(c->nobs, c->nlm);
+    int g = std::max(1, std::min((n + 255) / 256, 8192));
+    if (mixed(c))
+        hipLaunchKernelGGL(k_upload<float>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
+                           B.ob_valid, B.ob_isig, c->ob_info, (float *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
+                           c->lm_info, (float *)c->riT);
+    else
+        hipLaunchKernelGGL(k_upload<double>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
+                           B.ob_valid, B.ob_isig, c->ob_info, (double *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
+                           c->lm_info, (double *)c->riT);
+    HIPCK(hipGetLastError());
+    return XBA_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -314,10 +365,20 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->nh = h0;
     c->maxn = maxn;
     // ---- work list: landmark chunks whose observations stay inside a slot tile
-    int tile = opt->slot_tile > 0 ? opt->slot_tile : (mx ? 16 : 12);
-    const int tile_cap = mx ? 32 : 24;
+    c->fast = opt->deterministic == 0;
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    int ncu = 0;
+    HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int tile = opt->slot_tile > 0 ? opt->slot_tile : (c->fast ? 16 : (mx ? 16 : 12));
+    const int tile_cap = c->fast ? 28 : (mx ? 32 : 24);
     int lpc = opt->lm_per_chunk;
-    if (lpc <= 0) lpc = std::max(4, std::min(1024, (int)std::lround(c->nlm / 2048.0)));
+    if (lpc <= 0) {
+        if (c->fast)
+            lpc = std::max(8, std::min(4096, (int)std::lround(c->nlm / (4.0 * ncu))));
+        else
+            lpc = std::max(4, std::min(1024, (int)std::lround(c->nlm / 2048.0)));
+    }
     std::vector<int> llo(std::max(c->nlm, 1)), lhi(std::max(c->nlm, 1));
     for (int w = 0; w < nwin; w++) {
         const WinInfo &W = wins[w];
@@ -378,11 +439,20 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->npartH = pH;
     c->npartb = pb;
     c->lin_lds = lds_layout(Rmax, tsz).total;
-    int dev = 0;
-    HIPCK(hipGetDevice(&dev));
-    int ncu = 0;
-    HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu = std::max(1, std::min(8, (160 * 1024) / std::max(c->lin_lds, 1)));
+    const int lds_max = 160 * 1024;
+    int per_cu = 1;
+    if (c->fast) {
+        int nw = 8;
+        while (nw > 1 && fast_layout(Rmax, tsz, nw, false).total > lds_max) nw--;
+        if (fast_layout(Rmax, tsz, nw, false).total > lds_max) return fail(XBA_E_RANGE);
+        c->fast_nw = nw;
+        c->fast_lds = fast_layout(Rmax, tsz, nw, false).total;
+        c->fast_nw_trial = 8;
+        c->fast_lds_trial = fast_layout(Rmax, tsz, 8, true).total;
+        per_cu = 1;
+    } else {
+        per_cu = std::max(1, std::min(8, lds_max / std::max(c->lin_lds, 1)));
+    }
     c->grid = opt->grid > 0 ? opt->grid : std::max(1, std::min(c->nchunks, ncu * per_cu));
     if (c->nchunks == 0) c->grid = 1;
     // ---- device allocations
@@ -405,10 +475,13 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->lmpT = (void *)dalloc<double>(3 * (size_t)c->nlm, O);
     c->lmp_trial = (void *)dalloc<double>(3 * (size_t)c->nlm, O);
     c->riT = mx ? (void *)dalloc<float>(c->nlm, O) : (void *)dalloc<double>(c->nlm, O);
-    c->partH = mx ? (void *)dalloc<float>(pH, O) : (void *)dalloc<double>(pH, O);
-    c->partb = mx ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(pb, O);
-    c->partH_lo = mx ? (void *)dalloc<float>(pH, O) : (void *)dalloc<double>(1, O);
-    c->partb_lo = mx ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(1, O);
+    const int64_t pHd = c->fast ? 1 : pH, pbd = c->fast ? 1 : pb;
+    c->partH = mx ? (void *)dalloc<float>(pHd, O) : (void *)dalloc<double>(pHd, O);
+    c->partb = mx ? (void *)dalloc<float>(pbd, O) : (void *)dalloc<double>(pbd, O);
+    c->partH_lo = (mx && !c->fast) ? (void *)dalloc<float>(pH, O) : (void *)dalloc<double>(1, O);
+    c->partb_lo = (mx && !c->fast) ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(1, O);
+    c->partH64 = c->fast ? dalloc<double>(pH, O) : dalloc<double>(1, O);
+    c->partb64 = c->fast ? dalloc<double>(pb, O) : dalloc<double>(1, O);
     c->lmHi = mx ? (void *)dalloc<float>(6 * (size_t)c->nlm, O) : (void *)dalloc<double>(6 * (size_t)c->nlm, O);
     c->lmbl = mx ? (void *)dalloc<float>(3 * (size_t)c->nlm, O) : (void *)dalloc<double>(3 * (size_t)c->nlm, O);
     c->partc = dalloc<double>(c->nchunks, O);
@@ -446,6 +519,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
               up(c->prior_slot, B.prior_slot, c->npblk * 4) && up(c->prior_dim, B.prior_dim, c->npblk * 4);
     if (!ok) return fail(XBA_E_HIP);
     // LDS budgets
+    if (c->fast) {
+        if (mx) {
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds_trial));
+        } else {
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds_trial));
+        }
+    }
     if (c->lin_lds > 64 * 1024) {
         if (mx) {
             HIPCK(hipFuncSetAttribute((const void *)k_lin<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
@@ -463,6 +545,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     if (c->solve_lds_cap > 64 * 1024)
         HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, c->solve_lds_cap));
+    {
+        int rc = pack_measurements(c, nullptr);
+        if (rc != XBA_OK) return fail(rc);
+        if (hipDeviceSynchronize() != hipSuccess) return fail(XBA_E_HIP);
+    }
     *out = c;
     return XBA_OK;
 }
@@ -478,19 +565,7 @@ int xba_upload(xba_ctx *c, double lam0, void *stream) {
     if (!c) return XBA_E_ARG;
     hipStream_t s = (hipStream_t)stream;
     const xba_batch &B = c->B;
-    int n = std::max(c->nobs, c->nlm);
-    int g = std::max(1, std::min((n + 255) / 256, 8192));
-    if (mixed(c))
-        hipLaunchKernelGGL(k_upload<float>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
-                           B.ob_valid, B.ob_isig, c->ob_info, (float *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
-                           c->lm_info, (float *)c->riT);
-    else
-        hipLaunchKernelGGL(k_upload<double>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
-                           B.ob_valid, B.ob_isig, c->ob_info, (double *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
-                           c->lm_info, (double *)c->riT);
-    HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(c->lmpT, B.lmp, 3 * (size_t)c->nlm * 8, hipMemcpyDeviceToDevice, s));
-    HIPCK(hipGetLastError());
     size_t ns = (size_t)c->nslot;
     HIPCK(hipMemcpyAsync(c->Rwb, B.Rwb, 9 * ns * 8, hipMemcpyDeviceToDevice, s));
     HIPCK(hipMemcpyAsync(c->pwb, B.pwb, 3 * ns * 8, hipMemcpyDeviceToDevice, s));
@@ -597,6 +672,58 @@ int xba_linearize(xba_ctx *c, double lam, double *H_out, double *b_out, double *
         hipLaunchKernelGGL(k_cost_out, dim3((c->nwin + 255) / 256), dim3(256), 0, s, c->nwin, c->wstate, cost_out);
         HIPCK(hipGetLastError());
     }
+    return XBA_OK;
+}
+
+int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
+    if (!c || !ms_out) return XBA_E_ARG;
+    if (!c->uploaded) return XBA_E_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[9];
+    for (int i = 0; i < 9; i++) HIPCK(hipEventCreate(&ev[i]));
+    DArgs D = dargs(c);
+    int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        KArgs<T> A = kargs<T>(c);
+        (void)hipEventRecord(ev[0], s);
+        launch_lin_kernel<T>(c, A, s);
+        (void)hipEventRecord(ev[1], s);
+        launch_reduce<T>(c, D, s);
+        (void)hipEventRecord(ev[2], s);
+        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
+        (void)hipEventRecord(ev[3], s);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
+        (void)hipEventRecord(ev[4], s);
+        launch_trial_kernel<T>(c, A, s);
+        (void)hipEventRecord(ev[5], s);
+        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
+        (void)hipEventRecord(ev[6], s);
+        hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
+        (void)hipEventRecord(ev[7], s);
+        hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
+        (void)hipEventRecord(ev[8], s);
+    };
+    if (mixed(c))
+        run(0.0f);
+    else
+        run(0.0);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventSynchronize(ev[8]));
+    for (int i = 0; i < 8; i++) {
+        float ms = 0.f;
+        HIPCK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        ms_out[i] = ms;
+    }
+    for (int i = 0; i < 9; i++) (void)hipEventDestroy(ev[i]);
+    return XBA_OK;
+}
+
+int xba_plan_info(const xba_ctx *c, int64_t *out) {
+    if (!c || !out) return XBA_E_ARG;
+    out[0] = c->nchunks; out[1] = c->grid; out[2] = c->Rmax; out[3] = c->fast ? c->fast_lds : c->lin_lds;
+    out[4] = c->solve_lds_cap;
+    out[5] = c->npartH; out[6] = c->npartb; out[7] = c->maxn;
     return XBA_OK;
 }
 

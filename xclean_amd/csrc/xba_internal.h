@@ -68,6 +68,7 @@ struct KArgs {
     const double *win_par;     // (nwin, XBA_WPAR)
     T *partH, *partb;
     T *partH_lo, *partb_lo;    // FP32 partials: the low parts of the compensated tile sums
+    double *partH64, *partb64; // partials of the multi-wave kernels (FP64 tiles)
     double *partc, *partc2;
     T *lmHi, *lmbl;            // (nlm,6) symmetric damped inverse, (nlm,3)
     uint8_t *lmok;

@@ -1,0 +1,67 @@
+"""Algorithmic work of one LM iteration of the window solve, and the MI355X peaks the measurements are priced
+against.  The per-unit figures are fixed here so every report computes them the same way.
+
+Symbols: M observation rows (entries of ob_pose), L landmarks, n reduced-system dimension of a window, k_l slots
+landmark l couples (its host plus every distinct target slot of an observation with t != h).
+
+Bytes (compulsory traffic of one LM iteration):
+  FP32-assembly path:  16 M + 40 L + 8 n^2 + 8 n   (observation row: u, v + slot + whitening = 16 B; landmark:
+                       (a, b, rho) 12 + host 4 + rho_meas 4 + rho_isig 4 + CSR offset 4 read, update 12 written;
+                       reduced system written once and read once)
+  FP64 path:           28 M + 72 L + 16 n^2 + 16 n
+FLOPs (an FMA counts 2): 810 M + sum_l (60 + 144 k_l + 216 k_l^2) + n^3/3 + 2 n^2
+  (per observation: projection + Jacobians ~150, Hll 36, Hlp 144, b 48, pose-pose 432; per landmark: 3x3 inverse 60,
+  Hi Hlp 108 + Hlp^T Hi bl 36 per slot, A2^T (Hi A1) 216 per slot pair; dense Cholesky and two triangular solves)
+
+The landmark kernel (linearise + Schur, the dominant one) is priced with the observation and landmark terms only:
+  bytes  16 M + 28 L (+ 4 (n^2 + n) per window, the reduced-system partial it writes once)   [FP32 path]
+         28 M + 48 L (+ 8 (n^2 + n))                                                       [FP64 path]
+  flops  810 M + sum_l (60 + 144 k_l + 216 k_l^2)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
+FP32_PEAK_TFLOPS = 157.3       # MI355X dense FP32 (matrix cores; equal to the packed-FP32 vector rate)
+FP64_PEAK_TFLOPS = 78.6        # MI355X dense FP64 (vector = matrix)
+
+
+def touched_slots(lm_host, ob_start, ob_pose):
+    """k_l per landmark: its host plus every distinct target slot t != host of its observations (0 if none)."""
+    L = len(lm_host)
+    cnt = np.diff(ob_start)
+    owner = np.repeat(np.arange(L), cnt)
+    t = np.asarray(ob_pose, np.int64)
+    h = np.asarray(lm_host, np.int64)[owner]
+    sel = t != h
+    pairs = np.unique(np.stack([owner[sel], t[sel]], 1), axis=0) if sel.any() else np.zeros((0, 2), np.int64)
+    k = np.bincount(pairs[:, 0], minlength=L) if len(pairs) else np.zeros(L, np.int64)
+    return np.where(k > 0, k + 1, 0)
+
+
+def iteration_bytes(M, L, ns, fp32=True):
+    ns = np.asarray(ns, np.float64)
+    if fp32:
+        return 16.0 * M + 40.0 * L + float(np.sum(8 * ns * ns + 8 * ns))
+    return 28.0 * M + 72.0 * L + float(np.sum(16 * ns * ns + 16 * ns))
+
+
+def iteration_flops(M, k, ns):
+    k = np.asarray(k, np.float64)
+    ns = np.asarray(ns, np.float64)
+    lm = k > 0
+    return 810.0 * M + float(np.sum(60 + 144 * k[lm] + 216 * k[lm] ** 2)) + float(np.sum(ns ** 3 / 3 + 2 * ns ** 2))
+
+
+def lin_kernel_bytes(M, L, ns, fp32=True):
+    ns = np.asarray(ns, np.float64)
+    if fp32:
+        return 16.0 * M + 28.0 * L + float(np.sum(4 * (ns * ns + ns)))
+    return 28.0 * M + 48.0 * L + float(np.sum(8 * (ns * ns + ns)))
+
+
+def lin_kernel_flops(M, k):
+    k = np.asarray(k, np.float64)
+    lm = k > 0
+    return 810.0 * M + float(np.sum(60 + 144 * k[lm] + 216 * k[lm] ** 2))

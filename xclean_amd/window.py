@@ -129,7 +129,7 @@ class BatchSolver:
     """Solve many windows at once on the GPU.  ``problems`` is a list of ``Problem``s (mutated by ``download``)."""
 
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
-                 device="cuda"):
+                 device="cuda", deterministic=False):
         import torch
         _lib.require_gpu()
         self.torch = torch
@@ -206,6 +206,7 @@ class BatchSolver:
         o = _lib.XbaOptions()
         o.precision, o.lm_per_chunk, o.slot_tile, o.grid, o.use_graph = self.prec, lm_per_chunk, slot_tile, grid, int(
             bool(use_graph))
+        o.deterministic = int(bool(deterministic))
         self.opts = o
         ctx = C.c_void_p()
         _lib.check(self.L.xba_create(C.byref(ctx), C.byref(b), C.byref(o)), "xba_create")
@@ -306,6 +307,20 @@ class BatchSolver:
             out.append(dict(H=Hw, b=bw, cost=cost[i], ob_chi2=cw, lm_ok=kw, Hll_inv=hw,
                             imu_chi2=list(ichi[self.win_imu0[i]:self.win_imu0[i + 1]]), packed=w))
         return out
+
+    KERNELS = ("lin_schur", "reduce", "terms", "chol_solve", "backsub_trial", "trial_terms", "accept", "commit")
+
+    def profile_iteration(self):
+        """Run one LM iteration eagerly with HIP events around every kernel; returns {kernel: ms}."""
+        out = np.zeros(8)
+        _lib.check(self.L.xba_profile_iteration(self.ctx, out.ctypes.data, self.stream), "xba_profile_iteration")
+        return dict(zip(self.KERNELS, out.tolist()))
+
+    def plan_info(self):
+        out = np.zeros(8, np.int64)
+        _lib.check(self.L.xba_plan_info(self.ctx, out.ctypes.data), "xba_plan_info")
+        return dict(zip(("chunks", "grid", "slot_tile", "lin_lds", "solve_lds", "partH", "partb", "max_n"),
+                        out.tolist()))
 
     def solve(self, iters=5, lam0=1e-4, write_back=True):
         self.upload(lam0)
