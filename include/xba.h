@@ -155,7 +155,7 @@ int xba_backward(xba_ctx *ctx, const double *g_dx, double *grad_z, double *grad_
 
 /* ---- the Euclidean local-BA drop-in (backend/ba.py:713-869) ---------------------------------------------------
  * poses (device, nk x 4 x 4, T_wc, in/out), pts (device, nl x 3, in/out), okf/olm (device int32, N), ouv (device,
- * N x 2), free_mask (device uint8, nk), optional prior (pr_c (nl,3), pr_rho0 (nl), pr_valid (nl); NULL = none) and
+ * N x 2), free_mask (device uint8, nk), optional prior (pr_c (nl,3), pr_rho0 (nl), pr_invsig (nl), pr_valid (nl); pr_c NULL = none) and
  * rotation factors (rot_ab (n_rot,2) int32, rot_M (n_rot,3,3), rot_sig (n_rot); n_rot = 0: none).  stats_out
  * (device, 4): chi2_0, chi2_n, rms_0, rms_n.  Synchronous (host-side planning from okf/olm copies). */
 typedef struct xba_np_problem {
@@ -165,8 +165,7 @@ typedef struct xba_np_problem {
     const double *ouv;
     const uint8_t *free_mask;
     double fx, cx, cy, huber_k;
-    const double *pr_c, *pr_rho0, *pr_valid;
-    double pr_invsig;
+    const double *pr_c, *pr_rho0, *pr_invsig, *pr_valid;
     int32_t n_rot;
     const int32_t *rot_ab;
     const double *rot_M, *rot_sig;

@@ -51,7 +51,7 @@ class OnpProblem(C.Structure):
     _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32),
                 ("poses", f64p), ("pts", f64p), ("okf", i32p), ("olm", i32p), ("ouv", f64p),
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
-                ("free_mask", u8p), ("pr_c", f64p), ("pr_rho0", f64p), ("pr_invsig", C.c_double),
+                ("free_mask", u8p), ("pr_c", f64p), ("pr_rho0", f64p), ("pr_invsig", f64p),
                 ("pr_valid", f64p), ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p)]
 
 
@@ -281,8 +281,9 @@ def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
         pc = np.ascontiguousarray(c_, np.float64)
         pr0 = np.ascontiguousarray(np.broadcast_to(rho0, (nl,)), np.float64)
         pv = np.ascontiguousarray(np.broadcast_to(valid, (nl,)), np.float64)
-        keep += [pc, pr0, pv]
-        s.pr_c, s.pr_rho0, s.pr_invsig, s.pr_valid = _p(pc, f64p), _p(pr0, f64p), float(invsig), _p(pv, f64p)
+        pis = np.ascontiguousarray(np.broadcast_to(invsig, (nl,)), np.float64)
+        keep += [pc, pr0, pv, pis]
+        s.pr_c, s.pr_rho0, s.pr_invsig, s.pr_valid = _p(pc, f64p), _p(pr0, f64p), _p(pis, f64p), _p(pv, f64p)
     if rotf:
         ab = np.ascontiguousarray([[f[0], f[1]] for f in rotf], np.int32)
         Ms = np.ascontiguousarray(np.stack([f[2] for f in rotf]), np.float64)

@@ -1180,7 +1180,7 @@ typedef struct {
     /* optional inverse-range prior (ba.py:774-794) */
     const double *pr_c;         /* (nl,3) or NULL */
     const double *pr_rho0;      /* (nl) */
-    double pr_invsig;
+    const double *pr_invsig;    /* (nl) */
     const double *pr_valid;     /* (nl) */
     /* optional gyro rotation factors (ba.py:644-697) */
     int32_t n_rot;
@@ -1346,9 +1346,9 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
                                     pts[3 * j + 2] - q->pr_c[3 * j + 2]};
                     double dist = sqrt(dP[0] * dP[0] + dP[1] * dP[1] + dP[2] * dP[2]);
                     int ok = q->pr_valid[j] > 0.5 && dist > 1e-6;
-                    double rho = ok ? 1.0 / dist : 0.0, rp = (rho - q->pr_rho0[j]) * q->pr_invsig;
+                    double rho = ok ? 1.0 / dist : 0.0, rp = (rho - q->pr_rho0[j]) * q->pr_invsig[j];
                     double wp = fabs(rp) <= K ? 1.0 : K / fmax(fabs(rp), 1e-9), swp = sqrt(wp);
-                    double gg = ok ? -rho * rho / dist * q->pr_invsig : 0.0, Jp_[3], rpw = ok ? rp * swp : 0.0;
+                    double gg = ok ? -rho * rho / dist * q->pr_invsig[j] : 0.0, Jp_[3], rpw = ok ? rp * swp : 0.0;
                     for (int i = 0; i < 3; i++) Jp_[i] = ok ? dP[i] * (gg * swp) : 0.0;
                     for (int i = 0; i < 3; i++) {
                         for (int c = 0; c < 3; c++) Hll[9 * j + 3 * i + c] += Jp_[i] * Jp_[c];

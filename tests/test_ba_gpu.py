@@ -1,0 +1,140 @@
+"""Parity of the GPU local-BA drop-in (xclean_amd.ba._solve_np -> libxba xba_np_solve) with the reference's
+backend/ba.py:713 `_solve_np`.
+
+Bars (FP64 throughout): poses and points within 1e-9 relative of the golden outputs that the reference's own
+`_solve_np` wrote (tests/golden/solve_np_*.npz) and of the CPU oracle on larger seeded problems; per-iteration pose
+updates within 1e-9 relative of the oracle's; chi2/rms statistics within 1e-9 relative.  Plus the invariant of the
+reference's tests/test_ba_solver.py:26-33 and the edge cases of the interface (no iterations, landmarks without
+observations, every keyframe held, out-of-range indices).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xclean_amd import synth
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _stats(st):
+    return np.array([st["chi2_0"], st["chi2_n"], st["rms_0"], st["rms_n"]])
+
+
+def _golden(case):
+    g = np.load(os.path.join(GOLD, "solve_np_%s.npz" % case))
+
+    class Rig:
+        fx_rect, cx, cy = float(g["fx"]), float(g["cx"]), float(g["cy"])
+    prior = rotf = None
+    if "pr_c" in g:
+        prior = (g["pr_c"], g["pr_rho0"], float(g["pr_invsig"]), g["pr_valid"])
+    if "rot_ab" in g:
+        rotf = [(int(a), int(b), M, float(s)) for (a, b), M, s in zip(g["rot_ab"], g["rot_M"], g["rot_sig"])]
+    free = g["free"]
+    free = int(free) if free.ndim == 0 else free
+    return g, Rig, free, prior, rotf
+
+
+@pytest.mark.parametrize("case", ["default", "mask", "rejects", "prior_rotf"])
+def test_solve_np_matches_reference_golden(case):
+    from xclean_amd.ba import _solve_np
+    g, Rig, free, prior, rotf = _golden(case)
+    print("solve_np golden", case, flush=True)
+    p, q, st = _solve_np(g["poses0"].copy(), g["pts0"].copy(), g["okf"], g["olm"], g["ouv"], Rig, free,
+                         int(g["iters"]), float(g["lam0"]), prior=prior, rotf=rotf)
+    assert _rel(p, g["out_poses"]) < 1e-9
+    assert _rel(q, g["out_pts"]) < 1e-9
+    np.testing.assert_allclose(_stats(st), g["stats"], rtol=1e-9)
+
+
+def _problem(seed, nk=15, nl=600, obs_per=4):
+    return synth.make_problem(nk=nk, nl=nl, obs_per=obs_per, seed=seed)
+
+
+@pytest.mark.parametrize("seed,nk,nl,obs_per", [(5, 15, 600, 4), (11, 20, 3000, 6), (12, 8, 200, 8)])
+def test_solve_np_matches_oracle_iteration_by_iteration(seed, nk, nl, obs_per):
+    from xclean_amd.ba import _solve_np
+    pb = _problem(seed, nk, nl, obs_per)
+    print("solve_np oracle seed", seed, nk, nl, obs_per, flush=True)
+    args = (pb["poses0"].copy(), pb["pts0"].copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 6)
+    p, q, st, dxh = _solve_np(*args, history=True)
+    po, qo, sto, dxo, acc = O.solve_np(*args, history=True)
+    for i in range(len(dxo)):
+        assert _rel(dxh[i], dxo[i]) < 1e-9, i
+    assert _rel(p, po) < 1e-9
+    assert _rel(q, qo) < 1e-9
+    np.testing.assert_allclose(_stats(st), _stats(sto), rtol=1e-9)
+
+
+def test_solve_np_prior_per_landmark_sigma_mask_and_rotation_factors():
+    """Per-landmark inverse sigma (the reference's _gather_prior output), a held-neighbour mask, rotation factors."""
+    from xclean_amd.ba import _solve_np
+    pb = _problem(21, 12, 800, 5)
+    rng = np.random.RandomState(3)
+    nl, nk = len(pb["pts0"]), len(pb["poses0"])
+    c = pb["poses_true"][rng.randint(0, nk, nl), :3, 3]
+    rho0 = 1.0 / np.linalg.norm(pb["pts_true"] - c, axis=1) * (1 + 0.05 * rng.standard_normal(nl))
+    invsig = rng.uniform(5.0, 50.0, nl)
+    valid = (rng.rand(nl) > 0.2).astype(np.float64)
+    prior = (c, rho0, invsig, valid)
+    free = np.ones(nk, bool)
+    free[[0, 1, 5]] = False
+    Tt = np.stack([np.linalg.inv(T) for T in pb["poses_true"]])
+    rotf = []
+    for a in range(nk - 1):
+        M = Tt[a, :3, :3] @ Tt[a + 1, :3, :3].T @ synth.so3_exp(0.002 * rng.standard_normal(3))
+        rotf.append((a, a + 1, M, 0.01))
+    args = (pb["poses0"].copy(), pb["pts0"].copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"], free, 5)
+    p, q, st, dxh = _solve_np(*args, prior=prior, rotf=rotf, history=True)
+    po, qo, sto, dxo, _acc = O.solve_np(*args, prior=prior, rotf=rotf, history=True)
+    for i in range(len(dxo)):
+        assert _rel(dxh[i], dxo[i]) < 1e-9, i
+    assert _rel(p, po) < 1e-9 and _rel(q, qo) < 1e-9
+    np.testing.assert_allclose(_stats(st), _stats(sto), rtol=1e-9)
+    for k in np.nonzero(~free)[0]:          # held keyframes do not move
+        np.testing.assert_allclose(p[k], pb["poses0"][k], rtol=0, atol=1e-13)
+
+
+def test_solve_np_moves_poses_toward_the_truth():
+    """The reference's tests/test_ba_solver.py:26-33 on the GPU path."""
+    from xclean_amd.ba import _solve_np
+    pb = synth.make_problem()
+    p, _q, st = _solve_np(pb["poses0"].copy(), pb["pts0"].copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 5)
+    err = lambda a, b: float(np.sqrt(np.mean(np.sum((a[:, :3, 3] - b[:, :3, 3]) ** 2, 1))))
+    assert err(p, pb["poses_true"]) < 0.6 * err(pb["poses0"], pb["poses_true"])
+    assert st["rms_n"] < st["rms_0"]
+
+
+def test_solve_np_edge_cases():
+    from xclean_amd import _lib
+    from xclean_amd.ba import _solve_np
+    pb = _problem(7, 6, 120, 3)
+    base = (pb["poses0"].copy(), pb["pts0"].copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"])
+    # no iterations: the inputs come back, chi2_n = chi2_0
+    p, q, st = _solve_np(*base, 0, 0)
+    np.testing.assert_allclose(p, pb["poses0"], rtol=0, atol=1e-14)
+    np.testing.assert_array_equal(q, pb["pts0"])
+    assert st["chi2_0"] == st["chi2_n"]
+    # every keyframe held: poses stay, points still move
+    p, q, st = _solve_np(*base, np.zeros(6, bool), 3)
+    po, qo, sto = O.solve_np(*base, np.zeros(6, bool), 3)
+    np.testing.assert_allclose(p, pb["poses0"], rtol=0, atol=1e-14)
+    assert _rel(q, qo) < 1e-9
+    # extra landmarks that no observation sees: they keep their position
+    pts = np.concatenate([pb["pts0"], [[1.0, 2.0, 3.0], [0.0, 0.0, 5.0]]])
+    p, q, st = _solve_np(pb["poses0"].copy(), pts.copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 3)
+    po, qo, sto = O.solve_np(pb["poses0"].copy(), pts.copy(), pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 3)
+    assert _rel(p, po) < 1e-9 and _rel(q, qo) < 1e-9
+    np.testing.assert_array_equal(q[-2:], pts[-2:])
+    # out-of-range indices are rejected before any kernel runs
+    bad = pb["olm"].copy()
+    bad[3] = len(pb["pts0"])
+    with pytest.raises(_lib.XbaError):
+        _solve_np(pb["poses0"].copy(), pb["pts0"].copy(), pb["okf"], bad, pb["ouv"], pb["rig"], 0, 2)

@@ -41,7 +41,7 @@ class XbaNpProblem(C.Structure):
     _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32),
                 ("poses", vp), ("pts", vp), ("okf", vp), ("olm", vp), ("ouv", vp), ("free_mask", vp),
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
-                ("pr_c", vp), ("pr_rho0", vp), ("pr_valid", vp), ("pr_invsig", C.c_double),
+                ("pr_c", vp), ("pr_rho0", vp), ("pr_invsig", vp), ("pr_valid", vp),
                 ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
 
 

@@ -1,0 +1,103 @@
+"""The local bundle adjustment drop-in: ``_solve_np`` with the reference's signature and results, solved on the GPU.
+
+backend/ba.py:713-869 (``_solve_np``): Euclidean world points, camera poses T_cw = T_wc^-1 perturbed on the left,
+Huber IRLS on the pixel-error norm, Marquardt damping of every 3x3 landmark block, Schur complement onto the poses,
+held (non-free) keyframes taken out of the solve, damped reduced system, SE(3) update projected onto SO(3), and the
+accept rule on the unrobust chi2 with the lambda schedule x0.5 (floor 1e-7) / x4 (cap 1e2).  The optional
+inverse-range prior (ba.py:774-794) and gyro rotation factors (ba.py:644-697, which enter the gradient) are taken in
+the reference's forms.
+
+The whole loop runs in FP64 on the device as one stream of kernels (libxba ``xba_np_solve``); the host only plans
+the landmark grouping.  There is no CPU path: without the built library or a GPU this raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+HUBER_K = 1.345      # core/derive.py:47
+D_POSE = 6           # backend/ba.py:58
+
+
+def _free_mask(free, nk):
+    """backend/ba.py:700-710: a boolean mask, or the single held (gauge) index."""
+    if np.ndim(free) == 0:
+        m = np.ones(nk, bool)
+        m[int(free)] = False
+        return m
+    return np.asarray(free, bool)
+
+
+def _rig_intrinsics(rig):
+    return float(rig.fx_rect), float(rig.cx), float(rig.cy)
+
+
+def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None, rotf=None, device=None,
+              history=False):
+    """backend/ba.py:713 ``_solve_np``.  Returns (poses T_wc (nk,4,4), pts (nl,3), stats dict) as numpy arrays,
+    stats = {chi2_0, chi2_n, rms_0, rms_n}.  ``history=True`` also returns the (iters, 6 nk) pose updates.
+
+    ``prior`` = (c (nl,3), rho0 (nl), invsig (nl) or scalar, valid (nl)); ``rotf`` = [(a, b, R (3,3), sigma), ...].
+    Inputs may be numpy arrays or torch tensors; they are copied to the GPU once per call."""
+    import torch
+    _lib.require_gpu()
+    L = _lib.load()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    poses = np.asarray(poses, np.float64).reshape(-1, 4, 4)
+    pts = np.asarray(pts, np.float64).reshape(-1, 3)
+    nk, nl = len(poses), len(pts)
+    okf = np.asarray(okf).astype(np.int32).ravel()
+    olm = np.asarray(olm).astype(np.int32).ravel()
+    ouv = np.asarray(ouv, np.float64).reshape(-1, 2)
+    N = len(okf)
+    if len(olm) != N or len(ouv) != N:
+        raise ValueError("okf, olm and ouv must have one row per observation")
+    fm = _free_mask(free, nk)
+    if fm.shape != (nk,):
+        raise ValueError("free mask must have one entry per keyframe")
+
+    def dt(x, dtype=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).to(dev).contiguous()
+
+    d_poses, d_pts = dt(poses), dt(pts)
+    d_okf, d_olm, d_ouv = dt(okf, torch.int32), dt(olm, torch.int32), dt(ouv)
+    d_free = dt(fm.astype(np.uint8), torch.uint8)
+    keep = [d_poses, d_pts, d_okf, d_olm, d_ouv, d_free]
+    q = _lib.XbaNpProblem()
+    q.nk, q.nl, q.N = nk, nl, N
+    q.poses, q.pts, q.okf, q.olm, q.ouv = (d_poses.data_ptr(), d_pts.data_ptr(), d_okf.data_ptr(),
+                                           d_olm.data_ptr(), d_ouv.data_ptr())
+    q.free_mask = d_free.data_ptr()
+    q.fx, q.cx, q.cy = _rig_intrinsics(rig)
+    q.huber_k = HUBER_K
+    if prior is not None:
+        c_, rho0, invsig, valid = prior
+        pc = dt(np.asarray(c_, np.float64).reshape(nl, 3))
+        pr0 = dt(np.broadcast_to(np.asarray(rho0, np.float64), (nl,)))
+        pis = dt(np.broadcast_to(np.asarray(invsig, np.float64), (nl,)))
+        pv = dt(np.broadcast_to(np.asarray(valid, np.float64), (nl,)))
+        keep += [pc, pr0, pis, pv]
+        q.pr_c, q.pr_rho0, q.pr_invsig, q.pr_valid = pc.data_ptr(), pr0.data_ptr(), pis.data_ptr(), pv.data_ptr()
+    if rotf:
+        ab = dt(np.array([[f[0], f[1]] for f in rotf], np.int32), torch.int32)
+        Ms = dt(np.stack([np.asarray(f[2], np.float64) for f in rotf]))
+        sg = dt(np.array([float(f[3]) for f in rotf], np.float64))
+        keep += [ab, Ms, sg]
+        q.n_rot, q.rot_ab, q.rot_M, q.rot_sig = len(rotf), ab.data_ptr(), Ms.data_ptr(), sg.data_ptr()
+    stats = torch.zeros(4, dtype=torch.float64, device=dev)
+    dxh = torch.zeros((max(iters, 0), D_POSE * nk), dtype=torch.float64, device=dev) if history else None
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        rc = L.xba_np_solve(C.byref(q), int(iters), float(lam0), _lib.XBA_FP64, C.c_void_p(stats.data_ptr()),
+                            C.c_void_p(dxh.data_ptr()) if dxh is not None else None, C.c_void_p(stream))
+    _lib.check(rc, "xba_np_solve")
+    s = stats.cpu().numpy()
+    st = {"chi2_0": float(s[0]), "chi2_n": float(s[1]), "rms_0": float(s[2]), "rms_n": float(s[3])}
+    out = (d_poses.cpu().numpy(), d_pts.cpu().numpy(), st)
+    del keep
+    if history:
+        return out + (dxh.cpu().numpy(),)
+    return out

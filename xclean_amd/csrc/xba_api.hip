@@ -13,6 +13,7 @@
 #include "xba_lin.hip"
 #include "xba_dense.hip"
 #include "xba_fast.hip"
+#include "xba_np.hip"
 
 using namespace xba;
 
@@ -84,6 +85,7 @@ struct xba_ctx {
     int nchunks = 0, Rmax = 1, maxn = 0, grid = 1;
     int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
     int lin_lds = 0, solve_lds_cap = 0, fast_nw = 8, fast_nw_trial = 8, fast_lds = 0, fast_lds_trial = 0;
+    int nnarrow = 0, Rnarrow = 1, Rwide = 1, wide_nw = 8, wide_nw_trial = 8, wide_lds = 0, wide_lds_trial = 0, grid_wide = 1;
     bool fast = true;
     bool uploaded = false;
     std::vector<void *> owned;
@@ -188,19 +190,30 @@ bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
 template <typename T>
 void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
-    if (c->fast)
-        hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A, c->Rmax);
-    else
+    if (c->fast) {
+        if (c->nnarrow > 0)
+            hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A,
+                               c->Rnarrow, 0, c->nnarrow);
+        if (c->nchunks > c->nnarrow)
+            hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, A,
+                               c->Rwide, c->nnarrow, c->nchunks);
+    } else {
         hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    }
 }
 
 template <typename T>
 void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
-    if (c->fast)
-        hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s, A,
-                           c->Rmax);
-    else
+    if (c->fast) {
+        if (c->nnarrow > 0)
+            hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s,
+                               A, c->Rnarrow, 0, c->nnarrow);
+        if (c->nchunks > c->nnarrow)
+            hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
+                               c->wide_lds_trial, s, A, c->Rwide, c->nnarrow, c->nchunks);
+    } else {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
+    }
 }
 
 template <typename T>
@@ -371,7 +384,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     int ncu = 0;
     HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int tile = opt->slot_tile > 0 ? opt->slot_tile : (c->fast ? 16 : (mx ? 16 : 12));
-    const int tile_cap = c->fast ? 28 : (mx ? 32 : 24);
+    int tile_cap = mx ? 32 : 24;
+    if (c->fast) {  // the widest slot tile whose one-wavefront layout fits the LDS
+        tile_cap = 32;
+        while (tile_cap > 1 && fast_layout(tile_cap, tsz, 1, false).total > 160 * 1024) tile_cap--;
+    }
     int lpc = opt->lm_per_chunk;
     if (lpc <= 0) {
         if (c->fast)
@@ -401,39 +418,64 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         }
     }
     if (tile > tile_cap) return fail(XBA_E_RANGE);
-    std::vector<Chunk> chunks;
+    std::vector<Chunk> chunks, wide;
     int64_t pH = 0, pb = 0;
-    int Rmax = 1;
+    int Rmax = 1, Rn = 1, Rw = 1;
+    // the fast family keeps landmarks that couple more slots than the narrow tile in separate (wide) work items
+    const int narrow = c->fast ? std::min(tile, opt->slot_tile > 0 ? opt->slot_tile : 16) : tile;
+    auto emit = [&](std::vector<Chunk> &dst, int w, int cs, int ce, int lo, int hi) {
+        Chunk ch;
+        memset(&ch, 0, sizeof(ch));
+        ch.win = w;
+        ch.l0 = cs;
+        ch.l1 = ce;
+        ch.lo = lo;
+        ch.R = hi - lo + 1;
+        dst.push_back(ch);
+    };
+    std::vector<int> wc0(nwin), wc1(nwin), ww0(nwin), ww1(nwin);
     for (int w = 0; w < nwin; w++) {
         const WinInfo &W = wins[w];
-        wins[w].c0 = (int)chunks.size();
+        wc0[w] = (int)chunks.size();
+        ww0[w] = (int)wide.size();
         int l = W.lm0, lend = W.lm0 + W.L;
         while (l < lend) {
+            const bool isw = lhi[l] - llo[l] + 1 > narrow;
+            const int lim = isw ? tile_cap : narrow;
             int cs = l, lo = llo[l], hi = lhi[l];
             l++;
-            while (l < lend && l - cs < lpc) {
+            while (l < lend && l - cs < lpc && ((lhi[l] - llo[l] + 1 > narrow) == isw)) {
                 int nlo = std::min(lo, llo[l]), nhi = std::max(hi, lhi[l]);
-                if (nhi - nlo + 1 > tile) break;
+                if (nhi - nlo + 1 > lim) break;
                 lo = nlo;
                 hi = nhi;
                 l++;
             }
-            Chunk ch;
-            memset(&ch, 0, sizeof(ch));
-            ch.win = w;
-            ch.l0 = cs;
-            ch.l1 = l;
-            ch.lo = lo;
-            ch.R = hi - lo + 1;
-            ch.partH = pH;
-            ch.partb = pb;
-            pH += (int64_t)(ch.R * (ch.R + 1) / 2) * 36;
-            pb += 6 * ch.R;
-            Rmax = std::max(Rmax, ch.R);
-            chunks.push_back(ch);
+            emit(isw ? wide : chunks, w, cs, l, lo, hi);
         }
-        wins[w].c1 = (int)chunks.size();
+        wc1[w] = (int)chunks.size();
+        ww1[w] = (int)wide.size();
     }
+    const int nn = (int)chunks.size();
+    for (int w = 0; w < nwin; w++) {
+        wins[w].c0 = wc0[w];
+        wins[w].c1 = wc1[w];
+        wins[w].c2 = nn + ww0[w];
+        wins[w].c3 = nn + ww1[w];
+    }
+    for (auto &ch : chunks) Rn = std::max(Rn, ch.R);
+    for (auto &ch : wide) Rw = std::max(Rw, ch.R);
+    chunks.insert(chunks.end(), wide.begin(), wide.end());
+    for (auto &ch : chunks) {
+        ch.partH = pH;
+        ch.partb = pb;
+        pH += (int64_t)(ch.R * (ch.R + 1) / 2) * 36;
+        pb += 6 * ch.R;
+        Rmax = std::max(Rmax, ch.R);
+    }
+    c->nnarrow = c->fast ? nn : (int)chunks.size();
+    c->Rnarrow = Rn;
+    c->Rwide = Rw;
     c->nchunks = (int)chunks.size();
     c->Rmax = Rmax;
     c->npartH = pH;
@@ -442,18 +484,24 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     const int lds_max = 160 * 1024;
     int per_cu = 1;
     if (c->fast) {
-        int nw = 8;
-        while (nw > 1 && fast_layout(Rmax, tsz, nw, false).total > lds_max) nw--;
-        if (fast_layout(Rmax, tsz, nw, false).total > lds_max) return fail(XBA_E_RANGE);
-        c->fast_nw = nw;
-        c->fast_lds = fast_layout(Rmax, tsz, nw, false).total;
-        c->fast_nw_trial = 8;
-        c->fast_lds_trial = fast_layout(Rmax, tsz, 8, true).total;
+        auto pick = [&](int R, int &nw, int &lds, int &nwt, int &ldst) -> bool {
+            nw = 8;
+            while (nw > 1 && fast_layout(R, tsz, nw, false).total > lds_max) nw--;
+            if (fast_layout(R, tsz, nw, false).total > lds_max) return false;
+            lds = fast_layout(R, tsz, nw, false).total;
+            nwt = 8;
+            while (nwt > 1 && fast_layout(R, tsz, nwt, true).total > lds_max) nwt--;
+            ldst = fast_layout(R, tsz, nwt, true).total;
+            return true;
+        };
+        if (!pick(c->Rnarrow, c->fast_nw, c->fast_lds, c->fast_nw_trial, c->fast_lds_trial)) return fail(XBA_E_RANGE);
+        if (!pick(c->Rwide, c->wide_nw, c->wide_lds, c->wide_nw_trial, c->wide_lds_trial)) return fail(XBA_E_RANGE);
         per_cu = 1;
+        c->grid_wide = std::max(1, std::min(c->nchunks - c->nnarrow, ncu));
     } else {
         per_cu = std::max(1, std::min(8, lds_max / std::max(c->lin_lds, 1)));
     }
-    c->grid = opt->grid > 0 ? opt->grid : std::max(1, std::min(c->nchunks, ncu * per_cu));
+    c->grid = opt->grid > 0 ? opt->grid : std::max(1, std::min(c->fast ? c->nnarrow : c->nchunks, ncu * per_cu));
     if (c->nchunks == 0) c->grid = 1;
     // ---- device allocations
     auto &O = c->owned;
@@ -521,11 +569,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     // LDS budgets
     if (c->fast) {
         if (mx) {
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds_trial));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         } else {
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, c->fast_lds_trial));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         }
     }
     if (c->lin_lds > 64 * 1024) {
@@ -723,6 +771,7 @@ int xba_plan_info(const xba_ctx *c, int64_t *out) {
     if (!c || !out) return XBA_E_ARG;
     out[0] = c->nchunks; out[1] = c->grid; out[2] = c->Rmax; out[3] = c->fast ? c->fast_lds : c->lin_lds;
     out[4] = c->solve_lds_cap;
+    if (c->fast) out[1] = c->grid * 100000 + c->fast_nw * 1000 + (c->nchunks - c->nnarrow);
     out[5] = c->npartH; out[6] = c->npartb; out[7] = c->maxn;
     return XBA_OK;
 }
@@ -738,8 +787,99 @@ int xba_backward(xba_ctx *c, const double *g_dx, double *grad_z, double *grad_w,
 
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                  double *dx_hist_out, void *stream) {
-    (void)q; (void)iters; (void)lam0; (void)precision; (void)stats_out; (void)dx_hist_out; (void)stream;
-    return XBA_E_STATE;
+    if (!q || iters < 0 || q->nk < 1 || q->nl < 0 || q->N < 0 || !q->poses || !stats_out) return XBA_E_ARG;
+    if (precision != XBA_FP64) return XBA_E_ARG;  // the reference solve is FP64 throughout
+    if ((q->nl > 0 && !q->pts) || (q->N > 0 && (!q->okf || !q->olm || !q->ouv)) || !q->free_mask) return XBA_E_ARG;
+    if (q->pr_c && (!q->pr_rho0 || !q->pr_invsig || !q->pr_valid)) return XBA_E_ARG;
+    if (q->n_rot < 0 || (q->n_rot > 0 && (!q->rot_ab || !q->rot_M || !q->rot_sig))) return XBA_E_ARG;
+    const int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
+    if (n > XBA_MAX_STATE) return XBA_E_SIZE;
+    hipStream_t s = (hipStream_t)stream;
+
+    // host planning: observations grouped by landmark (stable), index validation
+    std::vector<int32_t> okf(N), olm(N), order(N), lstart(nl + 1, 0), rab(2 * (size_t)q->n_rot);
+    if (N) {
+        HIPCK(hipMemcpyAsync(okf.data(), q->okf, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(olm.data(), q->olm, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+    }
+    if (q->n_rot)
+        HIPCK(hipMemcpyAsync(rab.data(), q->rot_ab, sizeof(int32_t) * rab.size(), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (int k = 0; k < N; k++) {
+        if (okf[k] < 0 || okf[k] >= nk || olm[k] < 0 || olm[k] >= nl) return XBA_E_RANGE;
+        lstart[olm[k] + 1]++;
+    }
+    for (int f = 0; f < 2 * q->n_rot; f++)
+        if (rab[f] < 0 || rab[f] >= nk) return XBA_E_RANGE;
+    for (int j = 0; j < nl; j++) lstart[j + 1] += lstart[j];
+    {
+        std::vector<int32_t> fill(lstart.begin(), lstart.end() - 1);
+        for (int k = 0; k < N; k++) order[fill[olm[k]]++] = k;
+    }
+
+    std::vector<void *> owned;
+    auto release = [&]() {
+        for (void *p : owned) (void)hipFree(p);
+    };
+    NpArgs A{};
+    A.nk = nk; A.nl = nl; A.N = N; A.n_rot = q->n_rot; A.has_prior = q->pr_c ? 1 : 0;
+    A.fx = q->fx; A.cx = q->cx; A.cy = q->cy; A.K = q->huber_k;
+    A.pts = q->pts; A.okf = q->okf; A.olm = q->olm; A.ouv = q->ouv; A.free_mask = q->free_mask;
+    A.pr_c = q->pr_c; A.pr_rho0 = q->pr_rho0; A.pr_invsig = q->pr_invsig; A.pr_valid = q->pr_valid;
+    A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
+    A.dx_hist = dx_hist_out;
+    A.Tcw = dalloc<double>(16 * (size_t)nk, owned);
+    A.Tnew = dalloc<double>(16 * (size_t)nk, owned);
+    A.pnew = dalloc<double>(3 * (size_t)nl, owned);
+    A.J = dalloc<double>(20 * (size_t)N, owned);
+    A.r2 = dalloc<double>(N, owned);
+    A.Hlli = dalloc<double>(9 * (size_t)nl, owned);
+    A.bl = dalloc<double>(3 * (size_t)nl, owned);
+    A.H = dalloc<double>((size_t)n * n, owned);
+    A.bp = dalloc<double>(n, owned);
+    A.dx = dalloc<double>(n, owned);
+    A.st = dalloc<double>(8, owned);
+    int32_t *d_order = dalloc<int32_t>(N, owned), *d_lstart = dalloc<int32_t>(nl + 1, owned);
+    if (owned.size() != 13) {
+        release();
+        return XBA_E_HIP;
+    }
+    A.order = d_order;
+    A.lstart = d_lstart;
+    const double st0[8] = {lam0, 0, 0, 0, 1, 0, 0, 0};
+    auto run = [&]() -> int {
+        if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(A.st, st0, sizeof(st0), hipMemcpyHostToDevice, s));
+        if (dx_hist_out && iters)
+            HIPCK(hipMemsetAsync(dx_hist_out, 0, sizeof(double) * (size_t)iters * n, s));
+        const int gk = (nk + 255) / 256, gN = (N + 255) / 256, gl = (nl + 255) / 256;
+        const int gz = std::min((n * n + 255) / 256, 1024), gp = (3 * nl + 255) / 256;
+        k_np_inv_poses<<<gk, 256, 0, s>>>(nk, q->poses, A.Tcw);
+        if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
+        k_np_chi<<<1, 256, 0, s>>>(A, 5);
+        for (int it = 0; it < iters; it++) {
+            if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
+            if (gl) k_np_lm<<<gl, 256, 0, s>>>(A);
+            k_np_zero<<<gz, 256, 0, s>>>(A, n);
+            if (gl) k_np_schur<<<gl, 256, 0, s>>>(A, n);
+            k_np_solve<<<1, 256, 0, s>>>(A, n, it);
+            if (gl) k_np_backsub<<<gl, 256, 0, s>>>(A);
+            if (gN) k_np_trial_r2<<<gN, 256, 0, s>>>(A);
+            k_np_accept<<<1, 256, 0, s>>>(A);
+            if (gp) k_np_commit_pts<<<gp, 256, 0, s>>>(A);
+        }
+        if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
+        k_np_chi<<<1, 256, 0, s>>>(A, 6);
+        k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tcw, q->poses);
+        k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(s));
+        return XBA_OK;
+    };
+    const int rc = run();
+    release();
+    return rc;
 }
 
 }  // extern "C"

@@ -287,7 +287,9 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
                 int t = s2; s2 = s1; s1 = t;
                 t = i; i = j; j = t;
             }
-            for (int c = W.c0; c < W.c1; c++) {
+            for (int c = W.c0; c < W.c3; c++) {
+                if (c == W.c1) c = W.c2;
+                if (c >= W.c3) break;
                 const Chunk ch = D.chunks[c];
                 int r2 = s2 - ch.lo, r1 = s1 - ch.lo;
                 if (r1 < 0 || r2 >= ch.R) continue;
@@ -303,7 +305,9 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
         Acc<T> v;
         if (ca >= 0) {
             int s = ca >> 3, i = ca & 7;
-            for (int c = W.c0; c < W.c1; c++) {
+            for (int c = W.c0; c < W.c3; c++) {
+                if (c == W.c1) c = W.c2;
+                if (c >= W.c3) break;
                 const Chunk ch = D.chunks[c];
                 int r = s - ch.lo;
                 if (r < 0 || r >= ch.R) continue;
@@ -316,6 +320,7 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
     if (idx == 0) {
         double cs = 0.0;
         for (int c = W.c0; c < W.c1; c++) cs += D.partc[c];
+        for (int c = W.c2; c < W.c3; c++) cs += D.partc[c];
         D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
     }
 }
@@ -634,6 +639,7 @@ __global__ __launch_bounds__(256) void k_accept(DArgs D) {
         if (ws[WS_CHOLOK] != 0.0) {
             double cn = 0.0;
             for (int c = W.c0; c < W.c1; c++) cn += D.partc2[c];
+            for (int c = W.c2; c < W.c3; c++) cn += D.partc2[c];
             cn += ws[WS_COSTTRIAL_REST];
             ws[WS_CNEW] = cn;
             acc = cn < ws[WS_CTOT];

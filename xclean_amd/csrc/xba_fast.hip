@@ -17,7 +17,8 @@ namespace xba {
 #define FB_OBS 64   // observations per pass (one lane each)
 #define FB_LM 32    // landmarks per pass
 #define FB_HLP 96   // sum over a pass's landmarks of their touched slots
-#define FB_LREC 20  // per landmark: Hll 6 (00 01 02 11 12 22), bl 3, Hi 6 (same order), HiB 3, ok, pad
+#define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
+#define FB_LREC 32  // per landmark: Hll 6 (00 01 02 11 12 22), bl 3, Hi 6, HiB 3, ok, L^-1 6 (00 10 11 20 21 22), g 3
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
 // plus a wait on the LDS counter makes every lane's earlier LDS writes visible to every lane's later reads.
@@ -32,7 +33,8 @@ namespace xba {
 struct FastLayout {
     int sc, sc64, wc, wd, tile, btile, wcost, wave, wave_stride, total;
     // offsets inside one wavefront's region
-    int o_oblm, o_obt, o_obact, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_slots, o_hlp, o_lrec, o_ldbl, o_lrow;
+    int o_oblm, o_obt, o_obact, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
+        o_lrow;
 };
 
 __host__ __device__ inline FastLayout fast_layout(int R, int ts, int nw, bool trial) {
@@ -43,28 +45,42 @@ __host__ __device__ inline FastLayout fast_layout(int R, int ts, int nw, bool tr
     L.sc64 = o;  o = align16(o + nc * R * SC64_STRIDE * 8);
     L.wc = o;    o = align16(o + 32 * ts);
     L.wd = o;    o = align16(o + 8 * 8);
-    L.tile = o;  o = align16(o + (trial ? 6 * R * ts : (R * (R + 1) / 2) * 36 * 8));  // trial: dp per slot
+    L.tile = o;  o = align16(o + (trial ? 6 * R * ts : (R * (R + 1) / 2) * TBLK * 8));  // trial: dp per slot
     L.btile = o; o = align16(o + (trial ? 0 : 6 * R * 8));
     L.wcost = o; o = align16(o + 16 * 8);
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
     L.o_obt = w;   w = align16(w + FB_OBS * 4);
-    L.o_obact = w; w = align16(w + FB_OBS * 4);
+    L.o_obact = w; w = align16(w + (trial ? FB_OBS * 4 : 0));
     L.o_rcq = w;   w = align16(w + FB_OBS * 8);
     L.o_u = w;     w = align16(w + (trial ? FB_OBS * 3 * ts : 0));
     L.o_lmask = w; w = align16(w + FB_LM * 4);
     L.o_lobs0 = w; w = align16(w + FB_LM * 4);
     L.o_lnobs = w; w = align16(w + FB_LM * 4);
     L.o_loff = w;  w = align16(w + FB_LM * 4);
+    L.o_poff = w;  w = align16(w + FB_LM * 4);
     L.o_slots = w; w = align16(w + (trial ? 0 : FB_HLP * 4));
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + FB_LM * FB_LREC * ts);
-    L.o_ldbl = w;  w = align16(w + FB_LM * 4 * 8);
-    L.o_lrow = w;  w = align16(w + (trial ? 0 : FB_OBS * 9 * ts));  // per observation: Hll 6 + bl 3 terms
+    L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
+    L.o_lrow = w;  w = align16(w + (trial ? 0 : NF * FB_OBS * ts));  // per observation: the whitened record
     L.wave_stride = w;
     L.total = o + nw * w;
     return L;
+}
+
+// Largest b in [0, B) with off[b] <= o (off non-decreasing, off[0] == 0).
+__device__ inline int owner_of(const int *off, int B, int o) {
+    int lo = 0, hi = B - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= o)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 __device__ inline int wave_incl_scan(int v, int lane) {
@@ -109,7 +125,14 @@ __device__ inline int select_pass(const int32_t *ob_start, int lb, int lend, int
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
 template <typename T>
-__global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
+__device__ inline void rows6(const T *rec, int q, int fx, int fy, int i, T w, T acc[6]) {
+    // acc[j] += w * sum_r X[r][i] Y[r][j] for the record fields fx (row stride 6) and fy (row stride 6)
+    const T x0 = REC(fx + i, q), x1 = REC(fx + 6 + i, q);
+    for (int j = 0; j < 6; j++) acc[j] += w * (x0 * REC(fy + j, q) + x1 * REC(fy + 6 + j, q));
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
     const FastLayout FL = fast_layout(Rmax, (int)sizeof(T), NW, false);
@@ -119,14 +142,15 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
     double *wcost = (double *)(smem + FL.wcost);
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
-    int *ob_lm = (int *)(wb + FL.o_oblm), *ob_t = (int *)(wb + FL.o_obt), *ob_act = (int *)(wb + FL.o_obact);
+    int *ob_lm = (int *)(wb + FL.o_oblm), *rt = (int *)(wb + FL.o_obt);
     double *rcq = (double *)(wb + FL.o_rcq);
     unsigned *lmask = (unsigned *)(wb + FL.o_lmask);
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs), *loff = (int *)(wb + FL.o_loff);
+    int *poff = (int *)(wb + FL.o_poff);
     int *slots = (int *)(wb + FL.o_slots);
-    T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec), *lrow = (T *)(wb + FL.o_lrow);
+    T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec), *rec = (T *)(wb + FL.o_lrow);
 
-    for (int c = blockIdx.x; c < A.nchunks; c += gridDim.x) {
+    for (int c = cbeg + blockIdx.x; c < cend; c += gridDim.x) {
         const Chunk ch = A.chunks[c];
         const WinInfo W = A.wins[ch.win];
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;
@@ -137,7 +161,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
             load_wconst<T>(wc, wp, lane);
             load_wd(wd, wp, lane);
         }
-        for (int i = tid; i < nblk * 36; i += blockDim.x) tile[i] = 0.0;
+        for (int i = tid; i < nblk * TBLK; i += blockDim.x) tile[i] = 0.0;
         for (int i = tid; i < 6 * R; i += blockDim.x) btile[i] = 0.0;
         __syncthreads();
         // this wavefront's share of the chunk
@@ -147,158 +171,62 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
         for (int lb = wbeg; lb < wend;) {
             int nob;
             const int B = select_pass(A.ob_start, lb, wend, lane, lobs0, lnobs, ob_lm, &nob);
-            if (lane < B) lmask[lane] = 0u;
             WAVE_SYNC();
-            // ---- observations: projection, activity, touched slots
-            int q_b = 0, q_l = 0, q_k = 0, q_t = 0, q_h = 0, q_cam = 0;
-            bool q_in = lane < nob;
-            double a = 0, b = 0, rho = 0;
-            bool q_mask = false;
-            if (q_in) {
-                q_b = ob_lm[lane];
-                q_l = lb + q_b;
-                q_k = A.ob_start[q_l] + (lane - lobs0[q_b]);
-                const int li = A.lm_info[q_l];
-                q_mask = (li >> 12) & 1;
-                q_h = (li & 0xfff) - ch.lo;
-                const int info = A.ob_info[q_k];
-                q_t = (info & 0xfff) - ch.lo;
-                q_cam = (info >> 12) & 1;
-                const int valid = (info >> 13) & 1;
-                a = A.lmp[3 * (size_t)q_l];
-                b = A.lmp[3 * (size_t)q_l + 1];
-                rho = A.lmp[3 * (size_t)q_l + 2];
-                bool act = false;
-                if (q_mask && valid) {
-                    const double *Rh = sc64 + q_h * SC64_STRIDE, *Rt = sc64 + q_t * SC64_STRIDE;
-                    double d0 = Rh[9] - Rt[9], d1 = Rh[10] - Rt[10], d2 = Rh[11] - Rt[11];
-                    double Y0 = Rh[0] * a + Rh[1] * b + Rh[2] + rho * d0;
-                    double Y1 = Rh[3] * a + Rh[4] * b + Rh[5] + rho * d1;
-                    double Y2 = Rh[6] * a + Rh[7] * b + Rh[8] + rho * d2;
-                    double q2 = Rt[2] * Y0 + Rt[5] * Y1 + Rt[8] * Y2;
-                    act = q2 > 1e-9;
-                }
-                ob_act[lane] = act ? 1 : 0;
-                ob_t[lane] = q_t;
-                if (act && q_t != q_h) atomicOr(&lmask[q_b], (1u << q_t) | (1u << q_h));
-            }
-            WAVE_SYNC();
-            // ---- landmarks: touched-slot lists (compact, ascending) and their offsets
-            int kb = 0;
-            if (lane < B) kb = __popc(lmask[lane]);
-            const int koff = wave_incl_scan(kb, lane) - kb;
-            if (lane < B) {
-                loff[lane] = koff;
-                unsigned m = lmask[lane];
-                int i = 0;
-                while (m) {
-                    int s = __ffs(m) - 1;
-                    m &= m - 1u;
-                    slots[koff + i] = s;
-                    for (int e = 0; e < 18; e++) hlp[(koff + i) * 18 + e] = (T)0;
-                    i++;
-                }
-            }
-            WAVE_SYNC();
-            // ---- observations: residual, Huber, Jacobians; direct pose blocks into the tile; Hlp terms
-            if (q_in) {
-                const bool act = ob_act[lane] != 0;
-                double e[2] = {0.0, 0.0}, w = 0.0, cst = 0.0, en2 = 0.0;
-                T Jl[2][3], Jt[2][6], Jh[2][6];
-                const int nd = q_cam == 0 ? 2 : 1;
-                T *row = lrow + lane * 9;
-                for (int i = 0; i < 9; i++) row[i] = (T)0;
-                if (act) {
-                    obs_eval<T>(sc64 + q_h * SC64_STRIDE, sc64 + q_t * SC64_STRIDE, sc + q_h * SC_STRIDE,
-                                sc + q_t * SC_STRIDE, wc, wd, a, b, rho, q_cam, A.z[2 * (size_t)q_k],
-                                A.z[2 * (size_t)q_k + 1], q_t == q_h, true, e, Jl, Jt, Jh);
-                    const double isg = (double)A.isig[q_k];
-                    for (int i = 0; i < nd; i++) {
-                        e[i] *= isg;
-                        en2 += e[i] * e[i];
-                    }
-                    huber(sqrt(en2), wd[5], &w, &cst);
-                    const T isgT = A.isig[q_k], wT = (T)w;
-                    T eT[2] = {(T)e[0], nd == 2 ? (T)e[1] : (T)0};
-                    for (int r = 0; r < 2; r++) {
-                        const T sgn = r < nd ? isgT : (T)0;
-                        for (int j = 0; j < 3; j++) Jl[r][j] *= sgn;
-                        for (int j = 0; j < 6; j++) {
-                            Jt[r][j] *= sgn;
-                            Jh[r][j] *= sgn;
-                        }
-                    }
-                    // Hll (6) and bl (3) terms, summed per landmark in observation order below
-                    row[0] = wT * (Jl[0][0] * Jl[0][0] + Jl[1][0] * Jl[1][0]);
-                    row[1] = wT * (Jl[0][0] * Jl[0][1] + Jl[1][0] * Jl[1][1]);
-                    row[2] = wT * (Jl[0][0] * Jl[0][2] + Jl[1][0] * Jl[1][2]);
-                    row[3] = wT * (Jl[0][1] * Jl[0][1] + Jl[1][1] * Jl[1][1]);
-                    row[4] = wT * (Jl[0][1] * Jl[0][2] + Jl[1][1] * Jl[1][2]);
-                    row[5] = wT * (Jl[0][2] * Jl[0][2] + Jl[1][2] * Jl[1][2]);
-                    for (int i = 0; i < 3; i++) row[6 + i] = wT * (Jl[0][i] * eT[0] + Jl[1][i] * eT[1]);
-                    if (q_t != q_h) {
-                        const unsigned m = lmask[q_b];
-                        const int base = loff[q_b];
-                        const int it = __popc(m & ((1u << q_t) - 1u)), ih = __popc(m & ((1u << q_h) - 1u));
-                        T *At = hlp + (base + it) * 18, *Ah = hlp + (base + ih) * 18;
-                        for (int i = 0; i < 3; i++)
-                            for (int j = 0; j < 6; j++) {
-                                lds_add<T>(At + 6 * i + j, wT * (Jl[0][i] * Jt[0][j] + Jl[1][i] * Jt[1][j]));
-                                lds_add<T>(Ah + 6 * i + j, wT * (Jl[0][i] * Jh[0][j] + Jl[1][i] * Jh[1][j]));
-                            }
-                        // direct blocks (t,t), (h,h) lower triangles and the cross block
-                        double *Btt = tile + (q_t * (q_t + 1) / 2 + q_t) * 36;
-                        double *Bhh = tile + (q_h * (q_h + 1) / 2 + q_h) * 36;
-                        for (int i = 0; i < 6; i++)
-                            for (int j = 0; j <= i; j++) {
-                                lds_add<double>(Btt + 6 * i + j,
-                                                (double)(wT * (Jt[0][i] * Jt[0][j] + Jt[1][i] * Jt[1][j])));
-                                lds_add<double>(Bhh + 6 * i + j,
-                                                (double)(wT * (Jh[0][i] * Jh[0][j] + Jh[1][i] * Jh[1][j])));
-                            }
-                        if (q_t > q_h) {
-                            double *Bx = tile + (q_t * (q_t + 1) / 2 + q_h) * 36;
-                            for (int i = 0; i < 6; i++)
-                                for (int j = 0; j < 6; j++)
-                                    lds_add<double>(Bx + 6 * i + j,
-                                                    (double)(wT * (Jt[0][i] * Jh[0][j] + Jt[1][i] * Jh[1][j])));
-                        } else {
-                            double *Bx = tile + (q_h * (q_h + 1) / 2 + q_t) * 36;
-                            for (int i = 0; i < 6; i++)
-                                for (int j = 0; j < 6; j++)
-                                    lds_add<double>(Bx + 6 * i + j,
-                                                    (double)(wT * (Jh[0][i] * Jt[0][j] + Jh[1][i] * Jt[1][j])));
-                        }
-                        for (int i = 0; i < 6; i++) {
-                            lds_add<double>(btile + 6 * q_t + i, (double)(wT * (Jt[0][i] * eT[0] + Jt[1][i] * eT[1])));
-                            lds_add<double>(btile + 6 * q_h + i, (double)(wT * (Jh[0][i] * eT[0] + Jh[1][i] * eT[1])));
-                        }
-                    }
-                }
-                rcq[lane] = act ? cst : 0.0;
-                if (A.chi2_out) A.chi2_out[q_k] = act ? en2 : (double)INFINITY;
-            }
-            WAVE_SYNC();
-            // ---- landmarks: Hll, bl in observation order, prior, damping, det guard, FP64 inverse
-            double lcost = 0.0;
-            if (lane < B) {
-                const int l = lb + lane;
+            // ---- observations: whitened residual, Huber weight and cost, whitened Jacobians into the record
+            if (lane < nob) {
+                const int qb = ob_lm[lane], l = lb + qb;
+                const int k = A.ob_start[l] + (lane - lobs0[qb]);
                 const int li = A.lm_info[l];
-                T *rec = lrec + lane * FB_LREC;
-                rec[18] = (T)0;
+                const int h = (li & 0xfff) - ch.lo;
                 if ((li >> 12) & 1) {
-                    const int q0 = lobs0[lane], n = lnobs[lane];
-                    T s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    lin_observations<T>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
+                                        A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1], A.lmp[3 * (size_t)l + 2],
+                                        lane, true);
+                } else {
+                    REC(F_W, lane) = (T)0;
+                    rcq[lane] = 0.0;
+                    rt[lane] = 0;
+                }
+            }
+            WAVE_SYNC();
+            // ---- landmarks (one lane each): touched slots, Hll, bl, prior, damping, det guard, FP64 inverse,
+            //      Hll = L L^T
+            double lcost = 0.0;
+            int kb = 0;
+            unsigned msk = 0u;
+            if (lane < B) {
+                const int l = lb + lane, li = A.lm_info[l];
+                T *lr = lrec + lane * FB_LREC;
+                lr[18] = (T)0;
+                lr[28] = (T)0;
+                if ((li >> 12) & 1) {
+                    const int h = (li & 0xfff) - ch.lo, q0 = lobs0[lane], n = lnobs[lane];
+                    T s6[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                     int nact = 0;
                     for (int q = q0; q < q0 + n; q++) {
-                        if (!ob_act[q]) continue;
+                        const T w = REC(F_W, q);
+                        if (w == (T)0) continue;
                         nact++;
                         lcost += rcq[q];
-                        for (int i = 0; i < 9; i++) s[i] += lrow[q * 9 + i];
+                        const T l00 = REC(F_JL, q), l01 = REC(F_JL + 1, q), l02 = REC(F_JL + 2, q);
+                        const T l10 = REC(F_JL + 3, q), l11 = REC(F_JL + 4, q), l12 = REC(F_JL + 5, q);
+                        const T e0 = REC(F_E, q), e1 = REC(F_E + 1, q);
+                        s6[0] += w * (l00 * l00 + l10 * l10);
+                        s6[1] += w * (l00 * l01 + l10 * l11);
+                        s6[2] += w * (l00 * l02 + l10 * l12);
+                        s6[3] += w * (l01 * l01 + l11 * l11);
+                        s6[4] += w * (l01 * l02 + l11 * l12);
+                        s6[5] += w * (l02 * l02 + l12 * l12);
+                        s6[6] += w * (l00 * e0 + l10 * e1);
+                        s6[7] += w * (l01 * e0 + l11 * e1);
+                        s6[8] += w * (l02 * e0 + l12 * e1);
+                        const int t = rt[q];
+                        if (t != h) msk |= (1u << t) | (1u << h);
                     }
+                    kb = __popc(msk);
                     double H6[6], bl[3];
-                    for (int i = 0; i < 6; i++) H6[i] = (double)s[i];
-                    for (int i = 0; i < 3; i++) bl[i] = (double)s[6 + i];
+                    for (int i = 0; i < 6; i++) H6[i] = (double)s6[i];
+                    for (int i = 0; i < 3; i++) bl[i] = (double)s6[6 + i];
                     const double rho_l = A.lmp[3 * (size_t)l + 2];
                     const T ris = A.rho_isig[l];
                     if (ris > (T)0) {
@@ -327,62 +255,175 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
                         Hi[6] = (Hll[3] * Hll[7] - Hll[4] * Hll[6]) * id;
                         Hi[7] = (Hll[1] * Hll[6] - Hll[0] * Hll[7]) * id;
                         Hi[8] = (Hll[0] * Hll[4] - Hll[1] * Hll[3]) * id;
+                        // Hll = L L^T: the Schur blocks become (L^-1 A2)^T (L^-1 A1)
+                        const double c00 = sqrt(fmax(Hll[0], 1e-300));
+                        const double c10 = Hll[3] / c00, c20 = Hll[6] / c00;
+                        const double c11 = sqrt(fmax(Hll[4] - c10 * c10, 1e-300));
+                        const double c21 = (Hll[7] - c20 * c10) / c11;
+                        const double c22 = sqrt(fmax(Hll[8] - c20 * c20 - c21 * c21, 1e-300));
+                        const double i00 = 1.0 / c00, i11 = 1.0 / c11, i22 = 1.0 / c22;
+                        const double i10 = -c10 * i00 * i11;
+                        const double i21 = -c21 * i11 * i22;
+                        const double i20 = -(c20 * i00 + c21 * i10) * i22;
+                        lr[19] = (T)i00; lr[20] = (T)i10; lr[21] = (T)i11;
+                        lr[22] = (T)i20; lr[23] = (T)i21; lr[24] = (T)i22;
+                        lr[25] = (T)(i00 * bl[0]);
+                        lr[26] = (T)(i10 * bl[0] + i11 * bl[1]);
+                        lr[27] = (T)(i20 * bl[0] + i21 * bl[1] + i22 * bl[2]);
                     }
-                    const double HiB0 = Hi[0] * bl[0] + Hi[1] * bl[1] + Hi[2] * bl[2];
-                    const double HiB1 = Hi[3] * bl[0] + Hi[4] * bl[1] + Hi[5] * bl[2];
-                    const double HiB2 = Hi[6] * bl[0] + Hi[7] * bl[1] + Hi[8] * bl[2];
-                    const T hs[6] = {(T)Hi[0], (T)Hi[1], (T)Hi[2], (T)Hi[4], (T)Hi[5], (T)Hi[8]};
-                    for (int i = 0; i < 6; i++) rec[9 + i] = hs[i];
-                    rec[15] = (T)HiB0;
-                    rec[16] = (T)HiB1;
-                    rec[17] = (T)HiB2;
-                    rec[18] = ok ? (T)1 : (T)0;
+                    lr[18] = ok ? (T)1 : (T)0;
+                    lr[28] = (T)1;  // masked in: its pose-pose blocks are added even when the det guard fails
                     A.lmok[l] = ok ? 1 : 0;
                     if (A.ok_out) A.ok_out[l] = ok ? 1 : 0;
                     T *ho = A.lmHi + 6 * (size_t)l;
-                    for (int i = 0; i < 6; i++) ho[i] = hs[i];
+                    ho[0] = (T)Hi[0]; ho[1] = (T)Hi[1]; ho[2] = (T)Hi[2]; ho[3] = (T)Hi[4]; ho[4] = (T)Hi[5];
+                    ho[5] = (T)Hi[8];
                     T *bo = A.lmbl + 3 * (size_t)l;
-                    bo[0] = (T)bl[0];
-                    bo[1] = (T)bl[1];
-                    bo[2] = (T)bl[2];
+                    bo[0] = (T)bl[0]; bo[1] = (T)bl[1]; bo[2] = (T)bl[2];
                     if (A.Hinv_out)
                         for (int i = 0; i < 9; i++) A.Hinv_out[9 * (size_t)l + i] = ok ? Hi[i] : 0.0;
                 } else {
                     A.lmok[l] = 0;
                 }
+                lmask[lane] = msk;
+            }
+            const int koff = wave_incl_scan(kb, lane) - kb;
+            const int ktot = __shfl(koff + kb, B - 1, XBA_WAVE);
+            const int npb = kb * (kb + 1) / 2;
+            const int pof = wave_incl_scan(npb, lane) - npb;
+            const int ptot = __shfl(pof + npb, B - 1, XBA_WAVE);
+            if (lane < B) {
+                poff[lane] = pof;
+                loff[lane] = koff;
+                unsigned m = msk;
+                int i = 0;
+                while (m) {
+                    slots[koff + i] = __ffs(m) - 1;
+                    m &= m - 1u;
+                    i++;
+                }
             }
             const double pcost = wave_sum(lcost);
             if (lane == 0) wcst += pcost;
             WAVE_SYNC();
-            // ---- Schur: every touched block pair of every landmark of the pass, lanes over the entries
-            for (int bb = 0; bb < B; bb++) {
-                const T *rec = lrec + bb * FB_LREC;
-                if (rec[18] == (T)0) continue;
-                const int k = __popc(lmask[bb]), base = loff[bb];
-                const T h00 = rec[9], h01 = rec[10], h02 = rec[11], h11 = rec[12], h12 = rec[13], h22 = rec[14];
-                const int npair = k * (k + 1) / 2;
-                for (int o = lane; o < npair * 36; o += XBA_WAVE) {
-                    const int p = o / 36, ij = o % 36, i = ij / 6, j = ij % 6;
-                    int si = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
-                    while ((si + 1) * (si + 2) / 2 <= p) si++;
-                    while (si * (si + 1) / 2 > p) si--;
-                    const int sj = p - si * (si + 1) / 2;
-                    if (si == sj && j > i) continue;
-                    const int s2 = slots[base + si], s1 = slots[base + sj];
-                    const T *A2 = hlp + (base + si) * 18, *A1 = hlp + (base + sj) * 18;
-                    const T c0 = A1[j], c1 = A1[6 + j], c2 = A1[12 + j];
-                    const T t0 = h00 * c0 + h01 * c1 + h02 * c2;
-                    const T t1 = h01 * c0 + h11 * c1 + h12 * c2;
-                    const T t2 = h02 * c0 + h12 * c1 + h22 * c2;
-                    const T v = A2[i] * t0 + A2[6 + i] * t1 + A2[12 + i] * t2;
-                    lds_add<double>(tile + (s2 * (s2 + 1) / 2 + s1) * 36 + ij, -(double)v);
+            // ---- U = L^-1 Hlp, one lane per column of a (landmark, slot) block:
+            //      Hlp[s][:, j] = sum_q w_q Jl_q^T J_{q,s}[:, j]  (J_{q,s} = Jt if t_q == s != h, Jh if s == h)
+            for (int o = lane; o < ktot * 6; o += XBA_WAVE) {
+                const int kk = o / 6, j = o % 6;
+                const int bb = owner_of(loff, B, kk);
+                const T *lr = lrec + bb * FB_LREC;
+                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo, s = slots[kk];
+                const int q0 = lobs0[bb], n = lnobs[bb];
+                T a0 = 0, a1 = 0, a2 = 0;
+                for (int q = q0; q < q0 + n; q++) {
+                    const T w = REC(F_W, q);
+                    const int t = rt[q];
+                    if (w == (T)0 || t == h) continue;
+                    int f;
+                    if (s == h) f = F_JH;
+                    else if (t == s) f = F_JT;
+                    else continue;
+                    const T y0 = REC(f + j, q), y1 = REC(f + 6 + j, q);
+                    a0 += w * (REC(F_JL, q) * y0 + REC(F_JL + 3, q) * y1);
+                    a1 += w * (REC(F_JL + 1, q) * y0 + REC(F_JL + 4, q) * y1);
+                    a2 += w * (REC(F_JL + 2, q) * y0 + REC(F_JL + 5, q) * y1);
                 }
-                for (int o = lane; o < k * 6; o += XBA_WAVE) {
-                    const int si = o / 6, i = o % 6, s = slots[base + si];
-                    const T *A1 = hlp + (base + si) * 18;
-                    const T v = A1[i] * rec[15] + A1[6 + i] * rec[16] + A1[12 + i] * rec[17];
-                    lds_add<double>(btile + 6 * s + i, -(double)v);
+                T *U = hlp + kk * 18;
+                U[j] = lr[19] * a0;
+                U[6 + j] = lr[20] * a0 + lr[21] * a1;
+                U[12 + j] = lr[22] * a0 + lr[23] * a1 + lr[24] * a2;
+            }
+            WAVE_SYNC();
+            // ---- reduced system: one lane per (landmark, touched block pair), the 6x6 block in registers:
+            //      direct pose-pose terms of the observations that touch it minus U2^T U1, then 21 or 36 atomics
+            for (int o = lane; o < ptot; o += XBA_WAVE) {
+                const int bb = owner_of(poff, B, o);
+                const T *lr = lrec + bb * FB_LREC;
+                const int p = o - poff[bb];
+                int si = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+                while ((si + 1) * (si + 2) / 2 <= p) si++;
+                while (si * (si + 1) / 2 > p) si--;
+                const int sj = p - si * (si + 1) / 2;
+                const int base = loff[bb];
+                const int s2 = slots[base + si], s1 = slots[base + sj];
+                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo;
+                T acc[36];
+                if (lr[18] != (T)0) {
+                    const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
+                    T a2[18], a1[18];
+                    for (int e = 0; e < 18; e++) {
+                        a2[e] = U2[e];
+                        a1[e] = U1[e];
+                    }
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++)
+                            acc[6 * i + j] = -(a2[i] * a1[j] + a2[6 + i] * a1[6 + j] + a2[12 + i] * a1[12 + j]);
+                } else {
+                    for (int e = 0; e < 36; e++) acc[e] = (T)0;
                 }
+                if (s2 == s1 || s1 == h || s2 == h) {
+                    const int q0 = lobs0[bb], n = lnobs[bb];
+                    for (int q = q0; q < q0 + n; q++) {
+                        const T w = REC(F_W, q);
+                        const int t = rt[q];
+                        if (w == (T)0 || t == h) continue;
+                        int fx, fy;
+                        if (s2 == s1) {
+                            if (s2 == h) fx = fy = F_JH;
+                            else if (t == s2) fx = fy = F_JT;
+                            else continue;
+                        } else if (s1 == h) {
+                            if (t != s2) continue;
+                            fx = F_JT;
+                            fy = F_JH;
+                        } else {
+                            if (t != s1) continue;
+                            fx = F_JH;
+                            fy = F_JT;
+                        }
+                        T x0[6], x1[6], y0[6], y1[6];
+                        for (int e = 0; e < 6; e++) {
+                            x0[e] = w * REC(fx + e, q);
+                            x1[e] = w * REC(fx + 6 + e, q);
+                            y0[e] = REC(fy + e, q);
+                            y1[e] = REC(fy + 6 + e, q);
+                        }
+                        for (int i = 0; i < 6; i++)
+                            for (int j = 0; j < 6; j++) acc[6 * i + j] += x0[i] * y0[j] + x1[i] * y1[j];
+                    }
+                }
+                double *dst = tile + (s2 * (s2 + 1) / 2 + s1) * TBLK;
+                if (si == sj) {
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j <= i; j++) lds_add<double>(dst + 6 * i + j, (double)acc[6 * i + j]);
+                } else {
+                    for (int e = 0; e < 36; e++) lds_add<double>(dst + e, (double)acc[e]);
+                }
+            }
+            // ---- gradient: one lane per (landmark, touched slot), 6 entries
+            for (int o = lane; o < ktot; o += XBA_WAVE) {
+                const int bb = owner_of(loff, B, o);
+                const T *lr = lrec + bb * FB_LREC;
+                const int s = slots[o];
+                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo;
+                const int q0 = lobs0[bb], n = lnobs[bb];
+                T acc[6] = {0, 0, 0, 0, 0, 0};
+                for (int q = q0; q < q0 + n; q++) {
+                    const T w = REC(F_W, q);
+                    const int t = rt[q];
+                    if (w == (T)0 || t == h) continue;
+                    int f;
+                    if (s == h) f = F_JH;
+                    else if (t == s) f = F_JT;
+                    else continue;
+                    const T e0 = w * REC(F_E, q), e1 = w * REC(F_E + 1, q);
+                    for (int i = 0; i < 6; i++) acc[i] += REC(f + i, q) * e0 + REC(f + 6 + i, q) * e1;
+                }
+                if (lr[18] != (T)0) {
+                    const T *U = hlp + o * 18;
+                    for (int i = 0; i < 6; i++) acc[i] -= U[i] * lr[25] + U[6 + i] * lr[26] + U[12 + i] * lr[27];
+                }
+                for (int i = 0; i < 6; i++) lds_add<double>(btile + 6 * s + i, (double)acc[i]);
             }
             WAVE_SYNC();
             lb += B;
@@ -390,7 +431,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
         if (lane == 0) wcost[wv] = wcst;
         __syncthreads();
         double *pH = A.partH64 + ch.partH, *pb = A.partb64 + ch.partb;
-        for (int i = tid; i < nblk * 36; i += blockDim.x) pH[i] = tile[i];
+        for (int i = tid; i < nblk * 36; i += blockDim.x) pH[i] = tile[(i / 36) * TBLK + i % 36];
         for (int i = tid; i < 6 * R; i += blockDim.x) pb[i] = btile[i];
         if (tid == 0) {
             double cs = 0.0;
@@ -403,7 +444,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax) {
 
 // ---------------------------------------------------------------------------------------- k_trial_fast --
 template <typename T>
-__global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax) {
+__global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
     const FastLayout FL = fast_layout(Rmax, (int)sizeof(T), NW, true);
@@ -416,7 +457,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax) {
     T *u = (T *)(wb + FL.o_u);
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs);
 
-    for (int c = blockIdx.x; c < A.nchunks; c += gridDim.x) {
+    for (int c = cbeg + blockIdx.x; c < cend; c += gridDim.x) {
         const Chunk ch = A.chunks[c];
         const WinInfo W = A.wins[ch.win];
         if (A.wstate[(size_t)NWS * ch.win + WS_CHOLOK] == 0.0) {
@@ -578,9 +619,9 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax) {
     }
 }
 
-template __global__ void k_lin_fast<float>(KArgs<float>, int);
-template __global__ void k_lin_fast<double>(KArgs<double>, int);
-template __global__ void k_trial_fast<float>(KArgs<float>, int);
-template __global__ void k_trial_fast<double>(KArgs<double>, int);
+template __global__ void k_lin_fast<float>(KArgs<float>, int, int, int);
+template __global__ void k_lin_fast<double>(KArgs<double>, int, int, int);
+template __global__ void k_trial_fast<float>(KArgs<float>, int, int, int);
+template __global__ void k_trial_fast<double>(KArgs<double>, int, int, int);
 
 }  // namespace xba

@@ -18,6 +18,7 @@ struct Chunk {
 // Per-window record.
 struct WinInfo {
     int32_t slot0, P, lm0, L, n, c0, c1, imu0, imu1, pblk0, pblk1, prow0, prow1, np, vi, pad;
+    int32_t c2, c3;  // second work-item range of the window (landmarks coupling more slots than the narrow tile)
     int64_t st0;   // offset of the window's state vector (dx, b)
     int64_t h0;    // offset of the window's n x n matrices
     int64_t prS0;  // offset of the window's prior S

@@ -1,0 +1,373 @@
+// The local-BA drop-in: backend/ba.py:713-869 `_solve_np` (Euclidean world points, poses T_cw perturbed on the left
+// T <- exp(xi) T with xi = [rho, phi], Huber IRLS on the pixel norm, Marquardt damping of the 3x3 landmark blocks by
+// lam * max(tr/3, 1e-9), Schur complement, held slots, damping of the reduced diagonal by lam * max(d, 1e-9), SE(3)
+// update + projection onto SO(3), accept iff the unrobust chi2 did not increase: lam = max(lam/2, 1e-7), else
+// lam = min(4 lam, 1e2)), in FP64 on the device.  The accept/reject decision and lambda live on the device; the whole
+// fixed-iteration loop is one stream of kernels with no host round trip.
+#include "xba_internal.h"
+
+namespace xba {
+
+struct NpArgs {
+    int32_t nk, nl, N, n_rot, has_prior;
+    double fx, cx, cy, K;
+    double *Tcw, *Tnew;            // (nk,16) row-major
+    double *pts, *pnew;            // (nl,3)
+    const int32_t *okf, *olm;      // (N)
+    const double *ouv;             // (N,2)
+    const uint8_t *free_mask;      // (nk)
+    const int32_t *order, *lstart; // observations sorted by landmark (stable), CSR offsets (nl+1)
+    const double *pr_c, *pr_rho0, *pr_invsig, *pr_valid;
+    const int32_t *rot_ab;
+    const double *rot_M, *rot_sig;
+    double *J;                     // (N,20): Jpose 12, Jlm 6, rw 2 (sqrt(w)-scaled)
+    double *r2;                    // (N) squared residual (unrobust)
+    double *Hlli, *bl;             // (nl,9), (nl,3)
+    double *H, *bp, *dx;           // (n,n), (n), (n)
+    double *st;                    // device scalars: [0] lam [1] chi_cur [2] chi_new [3] accept [4] ok [5] chi0 [6] chi_n
+    double *dx_hist;               // (iters, n) or null
+};
+
+__device__ inline void np_residual(const NpArgs &A, const double *T, const double *X, const double *uv, double *P,
+                                   double *r) {
+    for (int i = 0; i < 3; i++) P[i] = (T[4 * i] * X[0] + T[4 * i + 1] * X[1] + T[4 * i + 2] * X[2]) + T[4 * i + 3];
+    const double z = fmax(P[2], 1e-6);
+    r[0] = A.fx * P[0] / z + A.cx - uv[0];
+    r[1] = A.fx * P[1] / z + A.cy - uv[1];
+}
+
+// Residual, Huber weight and sqrt(w)-scaled Jacobians of every observation (ba.py:723-757).
+__global__ void k_np_obs(NpArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.N) return;
+    const double *T = A.Tcw + 16 * (size_t)A.okf[k], *X = A.pts + 3 * (size_t)A.olm[k];
+    double P[3], r[2];
+    np_residual(A, T, X, A.ouv + 2 * (size_t)k, P, r);
+    A.r2[k] = r[0] * r[0] + r[1] * r[1];
+    const double e = sqrt(r[0] * r[0] + r[1] * r[1]);
+    const double w = (e <= A.K) ? 1.0 : A.K / fmax(e, 1e-9), sw = sqrt(w);
+    const double x = P[0], y = P[1], zz = fmax(P[2], 1e-6), iz = 1.0 / zz, iz2 = iz * iz, fx = A.fx;
+    const double Jp3[2][3] = {{fx * iz, 0.0, -fx * x * iz2}, {0.0, fx * iz, -fx * y * iz2}};
+    const double dP[3][6] = {{1, 0, 0, 0, zz, -y}, {0, 1, 0, -zz, 0, x}, {0, 0, 1, y, -x, 0}};
+    double *J = A.J + 20 * (size_t)k;
+    for (int a = 0; a < 2; a++) {
+        for (int c = 0; c < 6; c++) J[6 * a + c] = (Jp3[a][0] * dP[0][c] + Jp3[a][1] * dP[1][c] + Jp3[a][2] * dP[2][c]) * sw;
+        for (int c = 0; c < 3; c++) J[12 + 3 * a + c] = (Jp3[a][0] * T[c] + Jp3[a][1] * T[4 + c] + Jp3[a][2] * T[8 + c]) * sw;
+        J[18 + a] = r[a] * sw;
+    }
+}
+
+// Per landmark (observations in original order): Hll, bl, the inverse-range prior, damping, inverse (ba.py:759-797).
+__global__ void k_np_lm(NpArgs A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl) return;
+    const double lam = A.st[0];
+    double Hll[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bl[3] = {0, 0, 0};
+    for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
+        const double *J = A.J + 20 * (size_t)A.order[q];
+        for (int i = 0; i < 3; i++) {
+            for (int c = 0; c < 3; c++) Hll[3 * i + c] += J[12 + i] * J[12 + c] + J[15 + i] * J[15 + c];
+            bl[i] += -(J[12 + i] * J[18] + J[15 + i] * J[19]);
+        }
+    }
+    if (A.has_prior) {  // ba.py:774-794 (the reference applies it when any landmark is valid; rows of invalid ones are 0)
+        const double *X = A.pts + 3 * (size_t)j, *c = A.pr_c + 3 * (size_t)j;
+        const double dP[3] = {X[0] - c[0], X[1] - c[1], X[2] - c[2]};
+        const double dist = sqrt(dP[0] * dP[0] + dP[1] * dP[1] + dP[2] * dP[2]);
+        const bool ok = A.pr_valid[j] > 0.5 && dist > 1e-6;
+        if (ok) {
+            const double rho = 1.0 / dist, rp = (rho - A.pr_rho0[j]) * A.pr_invsig[j];
+            const double wp = fabs(rp) <= A.K ? 1.0 : A.K / fmax(fabs(rp), 1e-9), swp = sqrt(wp);
+            const double gg = -rho * rho / dist * A.pr_invsig[j], rpw = rp * swp;
+            double Jp_[3];
+            for (int i = 0; i < 3; i++) Jp_[i] = dP[i] * (gg * swp);
+            for (int i = 0; i < 3; i++) {
+                for (int c2 = 0; c2 < 3; c2++) Hll[3 * i + c2] += Jp_[i] * Jp_[c2];
+                bl[i] -= Jp_[i] * rpw;
+            }
+        }
+    }
+    const double tr = Hll[0] + Hll[4] + Hll[8], d = lam * fmax(tr / 3.0, 1e-9);
+    for (int i = 0; i < 3; i++) Hll[4 * i] += d;
+    const double c00 = Hll[4] * Hll[8] - Hll[5] * Hll[7], c01 = Hll[5] * Hll[6] - Hll[3] * Hll[8],
+                 c02 = Hll[3] * Hll[7] - Hll[4] * Hll[6];
+    const double det = Hll[0] * c00 + Hll[1] * c01 + Hll[2] * c02, id = 1.0 / det;
+    double *Hi = A.Hlli + 9 * (size_t)j;
+    Hi[0] = c00 * id; Hi[1] = (Hll[2] * Hll[7] - Hll[1] * Hll[8]) * id; Hi[2] = (Hll[1] * Hll[5] - Hll[2] * Hll[4]) * id;
+    Hi[3] = c01 * id; Hi[4] = (Hll[0] * Hll[8] - Hll[2] * Hll[6]) * id; Hi[5] = (Hll[2] * Hll[3] - Hll[0] * Hll[5]) * id;
+    Hi[6] = c02 * id; Hi[7] = (Hll[1] * Hll[6] - Hll[0] * Hll[7]) * id; Hi[8] = (Hll[0] * Hll[4] - Hll[1] * Hll[3]) * id;
+    for (int i = 0; i < 3; i++) A.bl[3 * (size_t)j + i] = bl[i];
+}
+
+__device__ inline void hpl_of(const double *J, double *Hpl) {  // (6x3) Jpose^T Jlm
+    for (int i = 0; i < 6; i++)
+        for (int c = 0; c < 3; c++) Hpl[3 * i + c] = J[i] * J[12 + c] + J[6 + i] * J[15 + c];
+}
+
+__global__ void k_np_zero(NpArgs A, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) A.H[i] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) A.bp[i] = 0.0;
+}
+
+// Reduced system: H = Hpp - sum_l sum_{a,b} (Hpl_a Hlli) Hpl_b^T, bp = -Jpose^T rw - (Hpl_a Hlli) bl (ba.py:766-823).
+// One thread per landmark; FP64 atomics into the small n x n system.
+__global__ void k_np_schur(NpArgs A, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl) return;
+    const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
+    for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
+        const int ka = A.order[qa], ki = 6 * A.okf[ka];
+        const double *Ja = A.J + 20 * (size_t)ka;
+        double Hpa[18], T3[18];
+        hpl_of(Ja, Hpa);
+        for (int i = 0; i < 6; i++)
+            for (int c = 0; c < 3; c++)
+                T3[3 * i + c] = Hpa[3 * i] * Hi[c] + Hpa[3 * i + 1] * Hi[3 + c] + Hpa[3 * i + 2] * Hi[6 + c];
+        for (int i = 0; i < 6; i++) {
+            double g = -(Ja[i] * Ja[18] + Ja[6 + i] * Ja[19]) - (T3[3 * i] * bl[0] + T3[3 * i + 1] * bl[1] + T3[3 * i + 2] * bl[2]);
+            atomicAdd(A.bp + ki + i, g);
+            for (int c = 0; c < 6; c++) atomicAdd(A.H + (size_t)(ki + i) * n + ki + c, Ja[i] * Ja[c] + Ja[6 + i] * Ja[6 + c]);
+        }
+        for (int qb = A.lstart[j]; qb < A.lstart[j + 1]; qb++) {
+            const int kb = A.order[qb], kk = 6 * A.okf[kb];
+            double Hpb[18];
+            hpl_of(A.J + 20 * (size_t)kb, Hpb);
+            for (int i = 0; i < 6; i++)
+                for (int c = 0; c < 6; c++)
+                    atomicAdd(A.H + (size_t)(ki + i) * n + kk + c,
+                              -(T3[3 * i] * Hpb[3 * c] + T3[3 * i + 1] * Hpb[3 * c + 1] + T3[3 * i + 2] * Hpb[3 * c + 2]));
+        }
+    }
+}
+
+// so3 log via the quaternion and Jr^-1 (for the rotation factors) come from xba_dense.hip.
+// One workgroup: chi2, rotation-factor gradient, held slots, damping, Cholesky, solve, trial poses (ba.py:825-856).
+__global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
+    __shared__ double sred[256];
+    __shared__ int sflag;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // chi2 of the current residuals, fixed-order reduction
+    double s = 0.0;
+    for (int k = tid; k < A.N; k += nt) s += A.r2[k];
+    sred[tid] = s;
+    __syncthreads();
+    for (int d = nt / 2; d > 0; d >>= 1) {
+        if (tid < d) sred[tid] += sred[tid + d];
+        __syncthreads();
+    }
+    if (tid == 0) A.st[1] = sred[0];
+    // gyro rotation factors: only their gradient enters (ba.py:644-697, 825-828)
+    if (tid == 0)
+        for (int f = 0; f < A.n_rot; f++) {
+            const int a = A.rot_ab[2 * f], b = A.rot_ab[2 * f + 1];
+            const double w = 1.0 / fmax(A.rot_sig[f], 1e-9);
+            double Ri[9], Rj[9], Am[9], E[9], e[3], Jr[9], Ja[9], Jb[9], r[3];
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) {
+                    Ri[3 * i + c] = A.Tcw[16 * a + 4 * i + c];
+                    Rj[3 * i + c] = A.Tcw[16 * b + 4 * i + c];
+                }
+            mmt3(Ri, Rj, Am);
+            mtm3(A.rot_M + 9 * f, Am, E);
+            log_quat(E, e);
+            jr_inv(e, Jr);
+            mmt3(Jr, Am, Ja);
+            for (int i = 0; i < 9; i++) {
+                Ja[i] *= w;
+                Jb[i] = -w * Jr[i];
+            }
+            for (int i = 0; i < 3; i++) r[i] = w * e[i];
+            for (int i = 0; i < 3; i++) {
+                double ga = 0.0, gb = 0.0;
+                for (int c = 0; c < 3; c++) {
+                    ga += Ja[3 * c + i] * r[c];
+                    gb += Jb[3 * c + i] * r[c];
+                }
+                A.bp[6 * a + 3 + i] += -ga;
+                A.bp[6 * b + 3 + i] += -gb;
+            }
+        }
+    __syncthreads();
+    // held slots: rows and columns leave the solve (ba.py:829-836)
+    for (int e = tid; e < n * n; e += nt) {
+        const int i = e / n, c = e % n;
+        const bool hi_ = !A.free_mask[i / 6], hc = !A.free_mask[c / 6];
+        if (hi_ || hc) A.H[e] = (hi_ && hc && i / 6 == c / 6 && i == c) ? 1.0 : 0.0;
+    }
+    for (int i = tid; i < n; i += nt)
+        if (!A.free_mask[i / 6]) A.bp[i] = 0.0;
+    __syncthreads();
+    const double lam = A.st[0];
+    for (int i = tid; i < n; i += nt) A.H[(size_t)i * n + i] += lam * fmax(A.H[(size_t)i * n + i], 1e-9);
+    __syncthreads();
+    // Cholesky (the damped reduced system is symmetric positive definite) in place, then solve
+    bool ok = chol_wg<double>(A.H, n, &sflag);
+    if (ok) {
+        for (int i = tid; i < n; i += nt) A.dx[i] = A.bp[i];
+        __syncthreads();
+        chol_solve_wg<double>(A.H, n, A.dx);
+        if (A.dx_hist)
+            for (int i = tid; i < n; i += nt) A.dx_hist[(size_t)it * n + i] = A.dx[i];
+        // trial poses: T_new = exp(dx) T (free slots), rotation projected onto SO(3) by polar iteration
+        for (int k = tid; k < A.nk; k += nt) {
+            double *Tn = A.Tnew + 16 * (size_t)k;
+            const double *T = A.Tcw + 16 * (size_t)k;
+            if (!A.free_mask[k]) {
+                for (int i = 0; i < 16; i++) Tn[i] = T[i];
+                continue;
+            }
+            const double *xi = A.dx + 6 * k;
+            double a, b, c, K[9] = {0, -xi[5], xi[4], xi[5], 0, -xi[3], -xi[4], xi[3], 0}, KK[9], R[9], V[9];
+            abc(xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5], &a, &b, &c);
+            mm3(K, K, KK);
+            for (int i = 0; i < 9; i++) {
+                const double e = (i % 4 == 0) ? 1.0 : 0.0;
+                R[i] = e + a * K[i] + b * KK[i];
+                V[i] = e + b * K[i] + c * KK[i];
+            }
+            double E[16] = {0};
+            for (int i = 0; i < 3; i++) {
+                for (int j = 0; j < 3; j++) E[4 * i + j] = R[3 * i + j];
+                E[4 * i + 3] = V[3 * i] * xi[0] + V[3 * i + 1] * xi[1] + V[3 * i + 2] * xi[2];
+            }
+            E[15] = 1.0;
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    double acc = 0.0;
+                    for (int m = 0; m < 4; m++) acc += E[4 * i + m] * T[4 * m + j];
+                    Tn[4 * i + j] = acc;
+                }
+            double X[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) X[3 * i + j] = Tn[4 * i + j];
+            for (int itp = 0; itp < 30; itp++) {  // X <- (X + X^-T) / 2 converges to the orthogonal polar factor
+                double c00 = X[4] * X[8] - X[5] * X[7], c01 = X[5] * X[6] - X[3] * X[8], c02 = X[3] * X[7] - X[4] * X[6];
+                double det = X[0] * c00 + X[1] * c01 + X[2] * c02, id = 1.0 / det;
+                double Xi[9] = {c00 * id, (X[2] * X[7] - X[1] * X[8]) * id, (X[1] * X[5] - X[2] * X[4]) * id,
+                                c01 * id, (X[0] * X[8] - X[2] * X[6]) * id, (X[2] * X[3] - X[0] * X[5]) * id,
+                                c02 * id, (X[1] * X[6] - X[0] * X[7]) * id, (X[0] * X[4] - X[1] * X[3]) * id};
+                double dmax = 0.0;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        double nv = 0.5 * (X[3 * i + j] + Xi[3 * j + i]);
+                        dmax = fmax(dmax, fabs(nv - X[3 * i + j]));
+                        X[3 * i + j] = nv;
+                    }
+                if (dmax < 1e-16) break;
+            }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) Tn[4 * i + j] = X[3 * i + j];
+        }
+    }
+    if (tid == 0) A.st[4] = ok ? 1.0 : 0.0;
+}
+
+// Landmark back-substitution dxl = Hlli (bl - sum Hpl^T dxp_k) and the trial points (ba.py:844-856).
+__global__ void k_np_backsub(NpArgs A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl || A.st[4] == 0.0) return;
+    double rhs[3] = {A.bl[3 * (size_t)j], A.bl[3 * (size_t)j + 1], A.bl[3 * (size_t)j + 2]};
+    for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
+        const int k = A.order[q];
+        double Hp[18];
+        hpl_of(A.J + 20 * (size_t)k, Hp);
+        const double *d = A.dx + 6 * A.okf[k];
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+            for (int i = 0; i < 6; i++) s += Hp[3 * i + c] * d[i];
+            rhs[c] += -s;
+        }
+    }
+    const double *Hi = A.Hlli + 9 * (size_t)j;
+    for (int i = 0; i < 3; i++)
+        A.pnew[3 * (size_t)j + i] = A.pts[3 * (size_t)j + i] + (Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2]);
+}
+
+__global__ void k_np_trial_r2(NpArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.N || A.st[4] == 0.0) return;
+    double P[3], r[2];
+    np_residual(A, A.Tnew + 16 * (size_t)A.okf[k], A.pnew + 3 * (size_t)A.olm[k], A.ouv + 2 * (size_t)k, P, r);
+    A.r2[k] = r[0] * r[0] + r[1] * r[1];
+}
+
+// chi_new, accept (chi_new <= chi_cur), lambda rule, commit of the poses (ba.py:858-863).
+__global__ __launch_bounds__(256) void k_np_accept(NpArgs A) {
+    __shared__ double sred[256];
+    __shared__ int sacc;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (A.st[4] == 0.0) {  // a singular reduced system stops the loop (LinAlgError): keep the state, freeze lambda
+        if (tid == 0) A.st[3] = 0.0;
+        return;
+    }
+    double s = 0.0;
+    for (int k = tid; k < A.N; k += nt) s += A.r2[k];
+    sred[tid] = s;
+    __syncthreads();
+    for (int d = nt / 2; d > 0; d >>= 1) {
+        if (tid < d) sred[tid] += sred[tid + d];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double cn = sred[0];
+        A.st[2] = cn;
+        const bool acc = cn <= A.st[1];
+        A.st[3] = acc ? 1.0 : 0.0;
+        A.st[0] = acc ? fmax(A.st[0] * 0.5, 1e-7) : fmin(A.st[0] * 4.0, 1e2);
+        sacc = acc;
+    }
+    __syncthreads();
+    if (sacc)
+        for (int i = tid; i < 16 * A.nk; i += nt) A.Tcw[i] = A.Tnew[i];
+}
+
+__global__ void k_np_commit_pts(NpArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * A.nl && A.st[3] != 0.0) A.pts[i] = A.pnew[i];
+}
+
+__global__ void k_np_inv_poses(int nk, const double *in, double *out) {  // T_wc <-> T_cw
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double *T = in + 16 * (size_t)k;
+    double *o = out + 16 * (size_t)k;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[4 * i + j] = T[4 * j + i];
+    for (int i = 0; i < 3; i++) o[4 * i + 3] = -(T[i] * T[3] + T[4 + i] * T[7] + T[8 + i] * T[11]);
+    o[12] = o[13] = o[14] = 0.0;
+    o[15] = 1.0;
+}
+
+// Fixed-order sum of r2 into st[slot] (slot 5: chi2 of the initial state; 6: chi2 of the returned state).
+__global__ __launch_bounds__(256) void k_np_chi(NpArgs A, int slot) {
+    __shared__ double sred[256];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double s = 0.0;
+    for (int k = tid; k < A.N; k += nt) s += A.r2[k];
+    sred[tid] = s;
+    __syncthreads();
+    for (int d = nt / 2; d > 0; d >>= 1) {
+        if (tid < d) sred[tid] += sred[tid + d];
+        __syncthreads();
+    }
+    if (tid == 0) A.st[slot] = sred[0];
+}
+
+__global__ void k_np_r2_only(NpArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.N) return;
+    double P[3], r[2];
+    np_residual(A, A.Tcw + 16 * (size_t)A.okf[k], A.pts + 3 * (size_t)A.olm[k], A.ouv + 2 * (size_t)k, P, r);
+    A.r2[k] = r[0] * r[0] + r[1] * r[1];
+}
+
+// stats (device, 4): chi2_0, chi2_n, rms_0, rms_n (ba.py:720-722, 865-869).
+__global__ void k_np_stats(const double *st, int N, double *stats) {
+    if (threadIdx.x) return;
+    stats[0] = st[5];
+    stats[1] = st[6];
+    stats[2] = N ? sqrt(st[5] / N) : NAN;
+    stats[3] = N ? sqrt(st[6] / N) : NAN;
+}
+
+}  // namespace xba

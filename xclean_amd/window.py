@@ -22,6 +22,7 @@ from . import _lib
 from .problem import pack_imu
 
 _PREC = {"fp64": _lib.XBA_FP64, "mixed": _lib.XBA_MIXED}
+NARROW_SLOTS = 16   # the solver's default narrow slot tile
 
 
 def _huber_k_dof15():
@@ -65,7 +66,16 @@ class _Packed:
             start = np.concatenate([[0], np.cumsum(cnt)])
         self.P, self.L0, self.M0, self.npf = P, L0, M0, npf
         self.L, self.M = L0 + npf, M0 + npf
-        perm = np.argsort(host, kind="stable")
+        # landmark order: those coupling at most NARROW_SLOTS consecutive slots first, then by host slot (the
+        # solver's work items are contiguous landmark ranges, so this keeps them short in slots)
+        owner = np.repeat(np.arange(self.L), cnt)
+        lo = host.copy()
+        hi = host.copy()
+        if len(owner):
+            np.minimum.at(lo, owner, ob_pose)
+            np.maximum.at(hi, owner, ob_pose)
+        wide = (hi - lo + 1) > NARROW_SLOTS
+        perm = np.lexsort((host, wide))
         self.perm = perm
         cnt_s = cnt[perm]
         st_s = np.zeros(self.L + 1, np.int64)
