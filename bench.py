@@ -27,6 +27,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 LM_ITERS = 5
+NARROW_SLOTS = 16
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lin_fast_narrow_batched256_traffic.md")
+
+
+def _measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (each counter scaled by its reading on a plain copy of known size); None when that summary is absent."""
+    import re
+    try:
+        txt = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -37,6 +51,8 @@ def main():
     ap.add_argument("--windows", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (three launches) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (windows would shard across GPUs with "
@@ -114,14 +130,26 @@ def main():
     plain = pb0.M / t_plain
     log("bench: plain FP64 config-0 step %.3f ms -> %.3e obs/s per LM iteration" % (t_plain * 1e3, plain))
 
-    # roofline of the landmark kernel (linearise + Schur), priced with the algorithmic figures of roofline.py
-    ks = np.concatenate([roofline.touched_slots(w.lm_host, w.ob_start, w.ob_pose) for w in bs.packed])
+    # roofline of the dominant kernel: k_lin_fast<float, 0>, the linearise + Schur launch over the narrow work items
+    # (landmarks coupling at most 16 consecutive slots), priced with the algorithmic figures of roofline.py on the
+    # observations and landmarks that launch covers
+    M_n = L_n = 0
+    ks_n = []
+    for w in bs.packed:
+        narrow = roofline.landmark_span(w.lm_host, w.ob_start, w.ob_pose) <= NARROW_SLOTS
+        cnt = np.diff(w.ob_start)
+        M_n += int(cnt[narrow].sum())
+        L_n += int(narrow.sum())
+        ks_n.append(roofline.touched_slots(w.lm_host, w.ob_start, w.ob_pose)[narrow])
+    ks_n = np.concatenate(ks_n)
     t_lin = kern["lin_schur"] * 1e-3
-    lin_bytes = roofline.lin_kernel_bytes(M, L, ns, fp32=True)
-    lin_flops = roofline.lin_kernel_flops(M, ks)
+    lin_bytes = roofline.lin_kernel_bytes(M_n, L_n, ns, fp32=True)
+    lin_flops = roofline.lin_kernel_flops(M_n, ks_n)
     gbs = lin_bytes / t_lin / 1e9
     tfl = lin_flops / t_lin / 1e12
+    ks = np.concatenate([roofline.touched_slots(w.lm_host, w.ob_start, w.ob_pose) for w in bs.packed])
     it_flops = roofline.iteration_flops(M, ks, ns)
+    traffic = _measured_traffic()
 
     cpu = None
     if not args.no_cpu:
@@ -170,8 +198,9 @@ def main():
         "accepted_steps_mean": float(np.mean([s["accepted"] for s in st])),
         "kernel_ms": kern,
         "roofline": {"bound": "mfma", "achieved": tfl, "peak": roofline.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tfl / roofline.FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_lin<float> (linearise + Schur)",
+                     "frac": tfl / roofline.FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_lin_fast<float, 0> (linearise + Schur, narrow work items)",
+                     "launch_ms": kern["lin_schur"], "launch_obs_rows": M_n, "launch_landmarks": L_n,
                      "algorithmic_flops": lin_flops, "algorithmic_bytes": lin_bytes,
                      "hbm_achieved_gbs": gbs, "hbm_frac": gbs / roofline.HBM_PEAK_GBS,
                      "note": "compute-bound: intensity %.0f FLOP/B is above the FP32 ridge (~20 FLOP/B); FP32 work "
@@ -180,6 +209,20 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    sys.stdout.flush()
+    if args.calib_copy:
+        import ctypes
+        from xclean_amd import _lib
+        nb = 1 << 30
+        src = torch.ones(nb, dtype=torch.uint8, device=dev)
+        dst = torch.empty(nb, dtype=torch.uint8, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        for _ in range(3):
+            _lib.check(_lib.load().xba_copy_kernel(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                                   nb, ctypes.c_void_p(stream)), "xba_copy_kernel")
+        torch.cuda.synchronize(dev)
+        assert bool(torch.equal(src, dst))
+        log("bench: calibration copy of %d bytes x3 done" % nb)
 
 
 if __name__ == "__main__":

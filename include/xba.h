@@ -139,9 +139,11 @@ int xba_linearize(xba_ctx *ctx, double lam, double *H_out, double *b_out, double
 int xba_sizes(const xba_ctx *ctx, int64_t *n_state, int64_t *n_hentries);
 
 /* Diagnostics (no reference counterpart): run ONE LM iteration eagerly (it advances the solve like
- * xba_iterate(ctx, 1)) with HIP events around each kernel, and write the kernel durations in ms to ms_out (host,
- * 8): linearise+Schur, reduce, prior/IMU terms, Cholesky solve + update, back-substitution + trial cost, trial
- * prior/IMU cost, accept, landmark commit.  Synchronous. */
+ * xba_iterate(ctx, 1)) with HIP events around each launch, and write the durations in ms to ms_out (host,
+ * XBA_NPROF): linearise+Schur of the narrow work items, of the wide work items, reduce, prior/IMU terms, Cholesky
+ * solve + update, back-substitution + trial cost (narrow, wide), trial prior/IMU cost, accept, landmark commit.
+ * A class with no work items reads ~0.  Synchronous. */
+#define XBA_NPROF 10
 int xba_profile_iteration(xba_ctx *ctx, double *ms_out, void *stream);
 
 /* Plan summary (host out, 8): work items, persistent grid, max slot tile, LDS bytes of the landmark kernels, LDS
@@ -173,6 +175,11 @@ typedef struct xba_np_problem {
 
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                  double *dx_hist_out, void *stream);
+
+/* Diagnostics (no reference counterpart): a plain device-to-device copy kernel of `bytes` (a multiple of 16;
+ * 16-byte aligned pointers) on `stream`, so HBM counters (FETCH_SIZE, WRITE_SIZE) can be calibrated against a
+ * kernel whose traffic is known exactly.  Asynchronous. */
+int xba_copy_kernel(const void *src, void *dst, int64_t bytes, void *stream);
 
 const char *xba_strerror(int code);
 const char *xba_version(void);

@@ -54,6 +54,11 @@ __global__ void k_upload(int nobs, int nlm, const int32_t *ob_pose, const uint8_
     }
 }
 
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_init_ws(int nwin, double *ws, double lam0) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwin) return;
@@ -189,29 +194,29 @@ bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
 
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
 template <typename T>
-void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
+void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
-        if (c->nnarrow > 0)
-            hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A,
+        if (c->nnarrow > 0 && part != 1)
+            hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A,
                                c->Rnarrow, 0, c->nnarrow);
-        if (c->nchunks > c->nnarrow)
-            hipLaunchKernelGGL(k_lin_fast<T>, dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, A,
+        if (c->nchunks > c->nnarrow && part != 0)
+            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, A,
                                c->Rwide, c->nnarrow, c->nchunks);
-    } else {
+    } else if (part != 1) {
         hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
 }
 
 template <typename T>
-void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s) {
+void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
-        if (c->nnarrow > 0)
-            hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s,
+        if (c->nnarrow > 0 && part != 1)
+            hipLaunchKernelGGL((k_trial_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s,
                                A, c->Rnarrow, 0, c->nnarrow);
-        if (c->nchunks > c->nnarrow)
-            hipLaunchKernelGGL(k_trial_fast<T>, dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
+        if (c->nchunks > c->nnarrow && part != 0)
+            hipLaunchKernelGGL((k_trial_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
                                c->wide_lds_trial, s, A, c->Rwide, c->nnarrow, c->nchunks);
-    } else {
+    } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
 }
@@ -272,6 +277,16 @@ int pack_measurements(xba_ctx *c, hipStream_t s) {
 }  // namespace
 
 extern "C" {
+
+int xba_copy_kernel(const void *src, void *dst, int64_t bytes, void *stream) {
+    if (!src || !dst || bytes < 0 || bytes % 16 || ((uintptr_t)src | (uintptr_t)dst) % 16) return XBA_E_ARG;
+    const int64_t n = bytes / 16;
+    if (n == 0) return XBA_OK;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst, n);
+    HIPCK(hipGetLastError());
+    return XBA_OK;
+}
 
 const char *xba_version(void) { return "xba 0.1 (gfx950)"; }
 
@@ -569,11 +584,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     // LDS budgets
     if (c->fast) {
         if (mx) {
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         } else {
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         }
     }
     if (c->lin_lds > 64 * 1024) {
@@ -727,43 +746,47 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
     if (!c || !ms_out) return XBA_E_ARG;
     if (!c->uploaded) return XBA_E_STATE;
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t ev[9];
-    for (int i = 0; i < 9; i++) HIPCK(hipEventCreate(&ev[i]));
+    hipEvent_t ev[XBA_NPROF + 1];
+    for (int i = 0; i <= XBA_NPROF; i++) HIPCK(hipEventCreate(&ev[i]));
     DArgs D = dargs(c);
     int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
     auto run = [&](auto tag) {
         using T = decltype(tag);
         KArgs<T> A = kargs<T>(c);
         (void)hipEventRecord(ev[0], s);
-        launch_lin_kernel<T>(c, A, s);
+        launch_lin_kernel<T>(c, A, s, 0);
         (void)hipEventRecord(ev[1], s);
-        launch_reduce<T>(c, D, s);
+        launch_lin_kernel<T>(c, A, s, 1);
         (void)hipEventRecord(ev[2], s);
-        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
+        launch_reduce<T>(c, D, s);
         (void)hipEventRecord(ev[3], s);
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
+        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
         (void)hipEventRecord(ev[4], s);
-        launch_trial_kernel<T>(c, A, s);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
         (void)hipEventRecord(ev[5], s);
-        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
+        launch_trial_kernel<T>(c, A, s, 0);
         (void)hipEventRecord(ev[6], s);
-        hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
+        launch_trial_kernel<T>(c, A, s, 1);
         (void)hipEventRecord(ev[7], s);
-        hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
+        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
         (void)hipEventRecord(ev[8], s);
+        hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
+        (void)hipEventRecord(ev[9], s);
+        hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
+        (void)hipEventRecord(ev[10], s);
     };
     if (mixed(c))
         run(0.0f);
     else
         run(0.0);
     HIPCK(hipGetLastError());
-    HIPCK(hipEventSynchronize(ev[8]));
-    for (int i = 0; i < 8; i++) {
+    HIPCK(hipEventSynchronize(ev[XBA_NPROF]));
+    for (int i = 0; i < XBA_NPROF; i++) {
         float ms = 0.f;
         HIPCK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         ms_out[i] = ms;
     }
-    for (int i = 0; i < 9; i++) (void)hipEventDestroy(ev[i]);
+    for (int i = 0; i <= XBA_NPROF; i++) (void)hipEventDestroy(ev[i]);
     return XBA_OK;
 }
 

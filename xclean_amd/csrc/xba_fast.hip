@@ -131,7 +131,9 @@ __device__ inline void rows6(const T *rec, int q, int fx, int fy, int i, T w, T 
     for (int j = 0; j < 6; j++) acc[j] += w * (x0 * REC(fy + j, q) + x1 * REC(fy + 6 + j, q));
 }
 
-template <typename T>
+// CLS names the work-item class the launch covers (0: narrow chunks, 1: wide chunks); the code is the same, the
+// two classes get separate symbols so that profiles report them apart.
+template <typename T, int CLS>
 __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
 }
 
 // ---------------------------------------------------------------------------------------- k_trial_fast --
-template <typename T>
+template <typename T, int CLS>
 __global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
@@ -619,9 +621,13 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax, int cb
     }
 }
 
-template __global__ void k_lin_fast<float>(KArgs<float>, int, int, int);
-template __global__ void k_lin_fast<double>(KArgs<double>, int, int, int);
-template __global__ void k_trial_fast<float>(KArgs<float>, int, int, int);
-template __global__ void k_trial_fast<double>(KArgs<double>, int, int, int);
+template __global__ void k_lin_fast<float, 0>(KArgs<float>, int, int, int);
+template __global__ void k_lin_fast<float, 1>(KArgs<float>, int, int, int);
+template __global__ void k_lin_fast<double, 0>(KArgs<double>, int, int, int);
+template __global__ void k_lin_fast<double, 1>(KArgs<double>, int, int, int);
+template __global__ void k_trial_fast<float, 0>(KArgs<float>, int, int, int);
+template __global__ void k_trial_fast<float, 1>(KArgs<float>, int, int, int);
+template __global__ void k_trial_fast<double, 0>(KArgs<double>, int, int, int);
+template __global__ void k_trial_fast<double, 1>(KArgs<double>, int, int, int);
 
 }  // namespace xba

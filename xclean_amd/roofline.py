@@ -40,6 +40,18 @@ def touched_slots(lm_host, ob_start, ob_pose):
     return np.where(k > 0, k + 1, 0)
 
 
+def landmark_span(lm_host, ob_start, ob_pose):
+    """Number of consecutive slots each landmark couples: max - min + 1 over its host and its observations' slots."""
+    L = len(lm_host)
+    cnt = np.diff(ob_start)
+    owner = np.repeat(np.arange(L), cnt)
+    h = np.asarray(lm_host, np.int64)
+    lo, hi = h.copy(), h.copy()
+    np.minimum.at(lo, owner, np.asarray(ob_pose, np.int64))
+    np.maximum.at(hi, owner, np.asarray(ob_pose, np.int64))
+    return hi - lo + 1
+
+
 def iteration_bytes(M, L, ns, fp32=True):
     ns = np.asarray(ns, np.float64)
     if fp32:

@@ -318,11 +318,12 @@ class BatchSolver:
                             imu_chi2=list(ichi[self.win_imu0[i]:self.win_imu0[i + 1]]), packed=w))
         return out
 
-    KERNELS = ("lin_schur", "reduce", "terms", "chol_solve", "backsub_trial", "trial_terms", "accept", "commit")
+    KERNELS = ("lin_schur", "lin_schur_wide", "reduce", "terms", "chol_solve", "backsub_trial", "backsub_trial_wide",
+               "trial_terms", "accept", "commit")
 
     def profile_iteration(self):
         """Run one LM iteration eagerly with HIP events around every kernel; returns {kernel: ms}."""
-        out = np.zeros(8)
+        out = np.zeros(len(self.KERNELS))
         _lib.check(self.L.xba_profile_iteration(self.ctx, out.ctypes.data, self.stream), "xba_profile_iteration")
         return dict(zip(self.KERNELS, out.tolist()))
 
