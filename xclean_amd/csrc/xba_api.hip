@@ -192,15 +192,36 @@ DArgs dargs(xba_ctx *c) {
 
 bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
 
+// The multi-wave kernels compute in FP64 whatever the precision (only their observation record is stored in the
+// working type): their arguments take the FP64 whitening weights the caller uploaded and FP64 landmark blocks.
+template <typename T>
+KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
+    KArgs<double> F;
+    memset(&F, 0, sizeof(F));
+    F.nchunks = A.nchunks; F.nwin = A.nwin; F.nlm = A.nlm; F.nobs = A.nobs;
+    F.chunks = A.chunks; F.wins = A.wins; F.ob_start = A.ob_start; F.lm_info = A.lm_info; F.ob_info = A.ob_info;
+    F.slot_off = A.slot_off; F.z = A.z;
+    F.isig = c->B.ob_isig;
+    F.lmp = A.lmp; F.lmp_trial = A.lmp_trial; F.rho_meas = A.rho_meas;
+    F.rho_isig = c->B.rho_isig;
+    F.Rwb = A.Rwb; F.pwb = A.pwb; F.R2 = A.R2; F.p2 = A.p2; F.dx = A.dx; F.win_par = A.win_par;
+    F.partH64 = A.partH64; F.partb64 = A.partb64; F.partc = A.partc; F.partc2 = A.partc2;
+    F.lmHi = (double *)c->lmHi; F.lmbl = (double *)c->lmbl; F.lmok = A.lmok;
+    F.wstate = A.wstate; F.lam_override = A.lam_override; F.chi2_out = A.chi2_out; F.Hinv_out = A.Hinv_out;
+    F.ok_out = A.ok_out; F.adj_gz = A.adj_gz; F.adj_gw = A.adj_gw; F.adj_y = A.adj_y;
+    return F;
+}
+
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
 template <typename T>
 void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
+        const KArgs<double> F = fast_args<T>(c, A);
         if (c->nnarrow > 0 && part != 1)
-            hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, A,
+            hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, F,
                                c->Rnarrow, 0, c->nnarrow);
         if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, A,
+            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, F,
                                c->Rwide, c->nnarrow, c->nchunks);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
@@ -210,12 +231,13 @@ void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = 
 template <typename T>
 void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
+        const KArgs<double> F = fast_args<T>(c, A);
         if (c->nnarrow > 0 && part != 1)
-            hipLaunchKernelGGL((k_trial_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial, s,
-                               A, c->Rnarrow, 0, c->nnarrow);
+            hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial,
+                               s, F, c->Rnarrow, 0, c->nnarrow);
         if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_trial_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
-                               c->wide_lds_trial, s, A, c->Rwide, c->nnarrow, c->nchunks);
+            hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
+                               c->wide_lds_trial, s, F, c->Rwide, c->nnarrow, c->nchunks);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -545,8 +567,9 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->partb_lo = (mx && !c->fast) ? (void *)dalloc<float>(pb, O) : (void *)dalloc<double>(1, O);
     c->partH64 = c->fast ? dalloc<double>(pH, O) : dalloc<double>(1, O);
     c->partb64 = c->fast ? dalloc<double>(pb, O) : dalloc<double>(1, O);
-    c->lmHi = mx ? (void *)dalloc<float>(6 * (size_t)c->nlm, O) : (void *)dalloc<double>(6 * (size_t)c->nlm, O);
-    c->lmbl = mx ? (void *)dalloc<float>(3 * (size_t)c->nlm, O) : (void *)dalloc<double>(3 * (size_t)c->nlm, O);
+    const bool lm32 = mx && !c->fast;  // the multi-wave kernels keep the landmark blocks in FP64
+    c->lmHi = lm32 ? (void *)dalloc<float>(6 * (size_t)c->nlm, O) : (void *)dalloc<double>(6 * (size_t)c->nlm, O);
+    c->lmbl = lm32 ? (void *)dalloc<float>(3 * (size_t)c->nlm, O) : (void *)dalloc<double>(3 * (size_t)c->nlm, O);
     c->partc = dalloc<double>(c->nchunks, O);
     c->partc2 = dalloc<double>(c->nchunks, O);
     c->lmok = dalloc<uint8_t>(c->nlm, O);
@@ -586,14 +609,12 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         if (mx) {
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         } else {
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         }
+        HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
     if (c->lin_lds > 64 * 1024) {
         if (mx) {

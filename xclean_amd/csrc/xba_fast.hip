@@ -16,9 +16,10 @@ namespace xba {
 
 #define FB_OBS 64   // observations per pass (one lane each)
 #define FB_LM 32    // landmarks per pass
-#define FB_HLP 96   // sum over a pass's landmarks of their touched slots
+#define FB_HLP 64   // sum over a pass's landmarks of their touched slots
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
-#define FB_LREC 32  // per landmark: Hll 6 (00 01 02 11 12 22), bl 3, Hi 6, HiB 3, ok, L^-1 6 (00 10 11 20 21 22), g 3
+#define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, masked-in flag
+enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
 // plus a wait on the LDS counter makes every lane's earlier LDS writes visible to every lane's later reads.
@@ -37,10 +38,11 @@ struct FastLayout {
         o_lrow;
 };
 
-__host__ __device__ inline FastLayout fast_layout(int R, int ts, int nw, bool trial) {
+// rts: bytes per record entry (the observation record's storage type); everything else is FP64.
+__host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool trial) {
     FastLayout L;
     int o = 0;
-    const int nc = trial ? 2 : 1;
+    const int nc = trial ? 2 : 1, ts = 8;
     L.sc = o;    o = align16(o + nc * R * SC_STRIDE * ts);
     L.sc64 = o;  o = align16(o + nc * R * SC64_STRIDE * 8);
     L.wc = o;    o = align16(o + 32 * ts);
@@ -64,7 +66,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int ts, int nw, bool tr
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + FB_LM * FB_LREC * ts);
     L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
-    L.o_lrow = w;  w = align16(w + (trial ? 0 : NF * FB_OBS * ts));  // per observation: the whitened record
+    L.o_lrow = w;  w = align16(w + (trial ? 0 : NF * FB_OBS * rts));  // per observation: the whitened record
     L.wave_stride = w;
     L.total = o + nw * w;
     return L;
@@ -124,20 +126,15 @@ __device__ inline int select_pass(const int32_t *ob_start, int lb, int lend, int
 }
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
-template <typename T>
-__device__ inline void rows6(const T *rec, int q, int fx, int fy, int i, T w, T acc[6]) {
-    // acc[j] += w * sum_r X[r][i] Y[r][j] for the record fields fx (row stride 6) and fy (row stride 6)
-    const T x0 = REC(fx + i, q), x1 = REC(fx + 6 + i, q);
-    for (int j = 0; j < 6; j++) acc[j] += w * (x0 * REC(fy + j, q) + x1 * REC(fy + 6 + j, q));
-}
-
-// CLS names the work-item class the launch covers (0: narrow chunks, 1: wide chunks); the code is the same, the
-// two classes get separate symbols so that profiles report them apart.
-template <typename T, int CLS>
-__global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
+// RT is the type the per-observation record (whitened residual, weight and Jacobians) is stored in; every sum and
+// product is formed in FP64.  CLS names the work-item class the launch covers (0: narrow chunks, 1: wide chunks);
+// the code is the same, the two classes get separate symbols so that profiles report them apart.
+template <typename RT, int CLS>
+__global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int cbeg, int cend) {
+    using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
-    const FastLayout FL = fast_layout(Rmax, (int)sizeof(T), NW, false);
+    const FastLayout FL = fast_layout(Rmax, (int)sizeof(RT), NW, false);
     T *sc = (T *)(smem + FL.sc), *wc = (T *)(smem + FL.wc);
     double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd);
     double *tile = (double *)(smem + FL.tile), *btile = (double *)(smem + FL.btile);
@@ -150,7 +147,8 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs), *loff = (int *)(wb + FL.o_loff);
     int *poff = (int *)(wb + FL.o_poff);
     int *slots = (int *)(wb + FL.o_slots);
-    T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec), *rec = (T *)(wb + FL.o_lrow);
+    T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
+    RT *rec = (RT *)(wb + FL.o_lrow);
 
     for (int c = cbeg + blockIdx.x; c < cend; c += gridDim.x) {
         const Chunk ch = A.chunks[c];
@@ -181,11 +179,11 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                 const int li = A.lm_info[l];
                 const int h = (li & 0xfff) - ch.lo;
                 if ((li >> 12) & 1) {
-                    lin_observations<T>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
+                    lin_observations<T, RT>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
                                         A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1], A.lmp[3 * (size_t)l + 2],
                                         lane, true);
                 } else {
-                    REC(F_W, lane) = (T)0;
+                    REC(F_W, lane) = (RT)0;
                     rcq[lane] = 0.0;
                     rt[lane] = 0;
                 }
@@ -199,15 +197,15 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
             if (lane < B) {
                 const int l = lb + lane, li = A.lm_info[l];
                 T *lr = lrec + lane * FB_LREC;
-                lr[18] = (T)0;
-                lr[28] = (T)0;
+                lr[LR_OK] = 0.0;
+                lr[LR_MASKED] = 0.0;
                 if ((li >> 12) & 1) {
                     const int h = (li & 0xfff) - ch.lo, q0 = lobs0[lane], n = lnobs[lane];
                     T s6[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                     int nact = 0;
                     for (int q = q0; q < q0 + n; q++) {
-                        const T w = REC(F_W, q);
-                        if (w == (T)0) continue;
+                        const T w = (T)REC(F_W, q);
+                        if (w == 0.0) continue;
                         nact++;
                         lcost += rcq[q];
                         const T l00 = REC(F_JL, q), l01 = REC(F_JL + 1, q), l02 = REC(F_JL + 2, q);
@@ -267,21 +265,21 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                         const double i10 = -c10 * i00 * i11;
                         const double i21 = -c21 * i11 * i22;
                         const double i20 = -(c20 * i00 + c21 * i10) * i22;
-                        lr[19] = (T)i00; lr[20] = (T)i10; lr[21] = (T)i11;
-                        lr[22] = (T)i20; lr[23] = (T)i21; lr[24] = (T)i22;
-                        lr[25] = (T)(i00 * bl[0]);
-                        lr[26] = (T)(i10 * bl[0] + i11 * bl[1]);
-                        lr[27] = (T)(i20 * bl[0] + i21 * bl[1] + i22 * bl[2]);
+                        lr[LR_L + 0] = i00; lr[LR_L + 1] = i10; lr[LR_L + 2] = i11;
+                        lr[LR_L + 3] = i20; lr[LR_L + 4] = i21; lr[LR_L + 5] = i22;
+                        lr[LR_G + 0] = i00 * bl[0];
+                        lr[LR_G + 1] = i10 * bl[0] + i11 * bl[1];
+                        lr[LR_G + 2] = i20 * bl[0] + i21 * bl[1] + i22 * bl[2];
                     }
-                    lr[18] = ok ? (T)1 : (T)0;
-                    lr[28] = (T)1;  // masked in: its pose-pose blocks are added even when the det guard fails
+                    lr[LR_OK] = ok ? 1.0 : 0.0;
+                    lr[LR_MASKED] = 1.0;  // masked in: its pose-pose blocks are added even when the det guard fails
                     A.lmok[l] = ok ? 1 : 0;
                     if (A.ok_out) A.ok_out[l] = ok ? 1 : 0;
                     T *ho = A.lmHi + 6 * (size_t)l;
-                    ho[0] = (T)Hi[0]; ho[1] = (T)Hi[1]; ho[2] = (T)Hi[2]; ho[3] = (T)Hi[4]; ho[4] = (T)Hi[5];
-                    ho[5] = (T)Hi[8];
+                    ho[0] = Hi[0]; ho[1] = Hi[1]; ho[2] = Hi[2]; ho[3] = Hi[4]; ho[4] = Hi[5];
+                    ho[5] = Hi[8];
                     T *bo = A.lmbl + 3 * (size_t)l;
-                    bo[0] = (T)bl[0]; bo[1] = (T)bl[1]; bo[2] = (T)bl[2];
+                    bo[0] = bl[0]; bo[1] = bl[1]; bo[2] = bl[2];
                     if (A.Hinv_out)
                         for (int i = 0; i < 9; i++) A.Hinv_out[9 * (size_t)l + i] = ok ? Hi[i] : 0.0;
                 } else {
@@ -320,20 +318,20 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                 for (int q = q0; q < q0 + n; q++) {
                     const T w = REC(F_W, q);
                     const int t = rt[q];
-                    if (w == (T)0 || t == h) continue;
+                    if (w == 0.0 || t == h) continue;
                     int f;
                     if (s == h) f = F_JH;
                     else if (t == s) f = F_JT;
                     else continue;
                     const T y0 = REC(f + j, q), y1 = REC(f + 6 + j, q);
-                    a0 += w * (REC(F_JL, q) * y0 + REC(F_JL + 3, q) * y1);
-                    a1 += w * (REC(F_JL + 1, q) * y0 + REC(F_JL + 4, q) * y1);
-                    a2 += w * (REC(F_JL + 2, q) * y0 + REC(F_JL + 5, q) * y1);
+                    a0 += w * ((T)REC(F_JL, q) * y0 + (T)REC(F_JL + 3, q) * y1);
+                    a1 += w * ((T)REC(F_JL + 1, q) * y0 + (T)REC(F_JL + 4, q) * y1);
+                    a2 += w * ((T)REC(F_JL + 2, q) * y0 + (T)REC(F_JL + 5, q) * y1);
                 }
                 T *U = hlp + kk * 18;
-                U[j] = lr[19] * a0;
-                U[6 + j] = lr[20] * a0 + lr[21] * a1;
-                U[12 + j] = lr[22] * a0 + lr[23] * a1 + lr[24] * a2;
+                U[j] = lr[LR_L] * a0;
+                U[6 + j] = lr[LR_L + 1] * a0 + lr[LR_L + 2] * a1;
+                U[12 + j] = lr[LR_L + 3] * a0 + lr[LR_L + 4] * a1 + lr[LR_L + 5] * a2;
             }
             WAVE_SYNC();
             // ---- reduced system: one lane per (landmark, touched block pair), the 6x6 block in registers:
@@ -350,7 +348,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                 const int s2 = slots[base + si], s1 = slots[base + sj];
                 const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo;
                 T acc[36];
-                if (lr[18] != (T)0) {
+                if (lr[LR_OK] != 0.0) {
                     const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
                     T a2[18], a1[18];
                     for (int e = 0; e < 18; e++) {
@@ -368,7 +366,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                     for (int q = q0; q < q0 + n; q++) {
                         const T w = REC(F_W, q);
                         const int t = rt[q];
-                        if (w == (T)0 || t == h) continue;
+                        if (w == 0.0 || t == h) continue;
                         int fx, fy;
                         if (s2 == s1) {
                             if (s2 == h) fx = fy = F_JH;
@@ -413,17 +411,18 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
                 for (int q = q0; q < q0 + n; q++) {
                     const T w = REC(F_W, q);
                     const int t = rt[q];
-                    if (w == (T)0 || t == h) continue;
+                    if (w == 0.0 || t == h) continue;
                     int f;
                     if (s == h) f = F_JH;
                     else if (t == s) f = F_JT;
                     else continue;
-                    const T e0 = w * REC(F_E, q), e1 = w * REC(F_E + 1, q);
+                    const T e0 = w * (T)REC(F_E, q), e1 = w * (T)REC(F_E + 1, q);
                     for (int i = 0; i < 6; i++) acc[i] += REC(f + i, q) * e0 + REC(f + 6 + i, q) * e1;
                 }
-                if (lr[18] != (T)0) {
+                if (lr[LR_OK] != 0.0) {
                     const T *U = hlp + o * 18;
-                    for (int i = 0; i < 6; i++) acc[i] -= U[i] * lr[25] + U[6 + i] * lr[26] + U[12 + i] * lr[27];
+                    for (int i = 0; i < 6; i++)
+                        acc[i] -= U[i] * lr[LR_G] + U[6 + i] * lr[LR_G + 1] + U[12 + i] * lr[LR_G + 2];
                 }
                 for (int i = 0; i < 6; i++) lds_add<double>(btile + 6 * s + i, (double)acc[i]);
             }
@@ -445,11 +444,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<T> A, int Rmax, int cbeg
 }
 
 // ---------------------------------------------------------------------------------------- k_trial_fast --
-template <typename T, int CLS>
-__global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax, int cbeg, int cend) {
+template <int CLS>
+__global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, int cbeg, int cend) {
+    using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
-    const FastLayout FL = fast_layout(Rmax, (int)sizeof(T), NW, true);
+    const FastLayout FL = fast_layout(Rmax, 8, NW, true);
     T *sc = (T *)(smem + FL.sc), *wc = (T *)(smem + FL.wc), *dps = (T *)(smem + FL.tile);
     double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd), *wcost = (double *)(smem + FL.wcost);
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
@@ -621,13 +621,11 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<T> A, int Rmax, int cb
     }
 }
 
-template __global__ void k_lin_fast<float, 0>(KArgs<float>, int, int, int);
-template __global__ void k_lin_fast<float, 1>(KArgs<float>, int, int, int);
+template __global__ void k_lin_fast<float, 0>(KArgs<double>, int, int, int);
+template __global__ void k_lin_fast<float, 1>(KArgs<double>, int, int, int);
 template __global__ void k_lin_fast<double, 0>(KArgs<double>, int, int, int);
 template __global__ void k_lin_fast<double, 1>(KArgs<double>, int, int, int);
-template __global__ void k_trial_fast<float, 0>(KArgs<float>, int, int, int);
-template __global__ void k_trial_fast<float, 1>(KArgs<float>, int, int, int);
-template __global__ void k_trial_fast<double, 0>(KArgs<double>, int, int, int);
-template __global__ void k_trial_fast<double, 1>(KArgs<double>, int, int, int);
+template __global__ void k_trial_fast<0>(KArgs<double>, int, int, int);
+template __global__ void k_trial_fast<1>(KArgs<double>, int, int, int);
 
 }  // namespace xba

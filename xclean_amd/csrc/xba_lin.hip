@@ -230,9 +230,9 @@ __device__ inline void tile_add(T *acc, T *cmp, int i, T v) {
 
 // ----------------------------------------------------------------- one landmark: observations into LDS --
 // Lanes over the landmark's observations: whitened residual (FP64), Huber weight and cost (FP64), whitened
-// Jacobians (working precision), stored per lane in the wave's record.
-template <typename T>
-__device__ inline void lin_observations(const KArgs<T> &A, T *rec, double *rc, int *rt, const double *sc64,
+// Jacobians (computed in T), stored per lane in the wave's record in the record type RT.
+template <typename T, typename RT = T>
+__device__ inline void lin_observations(const KArgs<T> &A, RT *rec, double *rc, int *rt, const double *sc64,
                                         const T *sc, const T *wc, const double *wd, int o0, int no, int lo, int h,
                                         double a, double b, double rho, int lane, bool jac) {
     if (lane < no) {
@@ -268,20 +268,20 @@ __device__ inline void lin_observations(const KArgs<T> &A, T *rec, double *rc, i
             }
         }
         if (A.chi2_out) A.chi2_out[k] = act ? en2 : (double)INFINITY;
-        rec[F_W * XBA_WAVE + lane] = act ? (T)w : (T)0;
+        rec[F_W * XBA_WAVE + lane] = act ? (RT)w : (RT)0;
         rc[lane] = act ? c : 0.0;
-        rec[F_ND * XBA_WAVE + lane] = (T)nd;
+        rec[F_ND * XBA_WAVE + lane] = (RT)nd;
         rt[lane] = t;
         if (act) {
-            rec[(F_E + 0) * XBA_WAVE + lane] = (T)e[0];
-            rec[(F_E + 1) * XBA_WAVE + lane] = nd == 2 ? (T)e[1] : (T)0;
+            rec[(F_E + 0) * XBA_WAVE + lane] = (RT)e[0];
+            rec[(F_E + 1) * XBA_WAVE + lane] = nd == 2 ? (RT)e[1] : (RT)0;
             if (jac)
                 for (int r = 0; r < 2; r++) {
                     bool on = r < nd;
-                    for (int j = 0; j < 3; j++) rec[(F_JL + 3 * r + j) * XBA_WAVE + lane] = on ? Jl[r][j] : (T)0;
+                    for (int j = 0; j < 3; j++) rec[(F_JL + 3 * r + j) * XBA_WAVE + lane] = on ? (RT)Jl[r][j] : (RT)0;
                     for (int j = 0; j < 6; j++) {
-                        rec[(F_JT + 6 * r + j) * XBA_WAVE + lane] = on ? Jt[r][j] : (T)0;
-                        rec[(F_JH + 6 * r + j) * XBA_WAVE + lane] = on ? Jh[r][j] : (T)0;
+                        rec[(F_JT + 6 * r + j) * XBA_WAVE + lane] = on ? (RT)Jt[r][j] : (RT)0;
+                        rec[(F_JH + 6 * r + j) * XBA_WAVE + lane] = on ? (RT)Jh[r][j] : (RT)0;
                     }
                 }
         }
