@@ -91,6 +91,9 @@ struct xba_ctx {
     int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
     int lin_lds = 0, solve_lds_cap = 0, fast_nw = 8, fast_nw_trial = 8, fast_lds = 0, fast_lds_trial = 0;
     int nnarrow = 0, Rnarrow = 1, Rwide = 1, wide_nw = 8, wide_nw_trial = 8, wide_lds = 0, wide_lds_trial = 0, grid_wide = 1;
+    int npass = 0;
+    int32_t *pass_l0 = nullptr, *sched = nullptr, *sched_ctr = nullptr;
+    double *pass_cost = nullptr;
     bool fast = true;
     bool uploaded = false;
     std::vector<void *> owned;
@@ -155,6 +158,10 @@ KArgs<T> kargs(xba_ctx *c) {
     A.lmok = c->lmok;
     A.wstate = c->wstate;
     A.lam_override = -1.0;
+    A.pass_l0 = c->pass_l0;
+    A.sched = c->sched;
+    A.sched_ctr = c->sched_ctr;
+    A.pass_cost = c->pass_cost;
     return A;
 }
 
@@ -209,6 +216,7 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
     F.lmHi = (double *)c->lmHi; F.lmbl = (double *)c->lmbl; F.lmok = A.lmok;
     F.wstate = A.wstate; F.lam_override = A.lam_override; F.chi2_out = A.chi2_out; F.Hinv_out = A.Hinv_out;
     F.ok_out = A.ok_out; F.adj_gz = A.adj_gz; F.adj_gw = A.adj_gw; F.adj_y = A.adj_y;
+    F.pass_l0 = A.pass_l0; F.sched = A.sched; F.sched_ctr = A.sched_ctr; F.pass_cost = A.pass_cost;
     return F;
 }
 
@@ -219,10 +227,10 @@ void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = 
         const KArgs<double> F = fast_args<T>(c, A);
         if (c->nnarrow > 0 && part != 1)
             hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, F,
-                               c->Rnarrow, 0, c->nnarrow);
+                               c->Rnarrow, 0, c->nnarrow, 0);
         if (c->nchunks > c->nnarrow && part != 0)
             hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, F,
-                               c->Rwide, c->nnarrow, c->nchunks);
+                               c->Rwide, c->nnarrow, c->nchunks, 1);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -234,10 +242,10 @@ void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part 
         const KArgs<double> F = fast_args<T>(c, A);
         if (c->nnarrow > 0 && part != 1)
             hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial,
-                               s, F, c->Rnarrow, 0, c->nnarrow);
+                               s, F, c->Rnarrow, 0, c->nnarrow, 2);
         if (c->nchunks > c->nnarrow && part != 0)
             hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
-                               c->wide_lds_trial, s, F, c->Rwide, c->nnarrow, c->nchunks);
+                               c->wide_lds_trial, s, F, c->Rwide, c->nnarrow, c->nchunks, 3);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -510,6 +518,39 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         pb += 6 * ch.R;
         Rmax = std::max(Rmax, ch.R);
     }
+    // ---- pass plan and hand-out order of the multi-wave kernels: every work item is cut into passes whose
+    // observations fit one lane each and whose touched slots fit the pass's U storage; work items are handed out
+    // largest first within each class
+    std::vector<int32_t> pass_l0, sched;
+    if (c->fast) {
+        std::vector<double> cost(chunks.size(), 0.0);
+        for (size_t ci = 0; ci < chunks.size(); ci++) {
+            Chunk &ch = chunks[ci];
+            ch.p0 = (int)pass_l0.size();
+            int l = ch.l0;
+            while (l < ch.l1) {
+                int nl = 0, S = 0, K = 0;
+                pass_l0.push_back(l);
+                while (l + nl < ch.l1 && nl < FB_LM) {
+                    const int n = B.ob_start[l + nl + 1] - B.ob_start[l + nl];
+                    const int k = std::min(n + 1, ch.R);
+                    if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP)) break;
+                    S += n;
+                    K += k;
+                    nl++;
+                    cost[ci] += (double)(n + 1) * (n + 1) + 4.0 * n + 8.0;
+                }
+                l += nl;
+            }
+            ch.p1 = (int)pass_l0.size();
+        }
+        sched.resize(chunks.size());
+        for (size_t ci = 0; ci < chunks.size(); ci++) sched[ci] = (int32_t)ci;
+        auto by_cost = [&](int32_t a, int32_t b) { return cost[a] > cost[b] || (cost[a] == cost[b] && a < b); };
+        std::sort(sched.begin(), sched.begin() + nn, by_cost);
+        std::sort(sched.begin() + nn, sched.end(), by_cost);
+    }
+    c->npass = (int)pass_l0.size();
     c->nnarrow = c->fast ? nn : (int)chunks.size();
     c->Rnarrow = Rn;
     c->Rwide = Rw;
@@ -543,6 +584,10 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     // ---- device allocations
     auto &O = c->owned;
     c->chunks = dalloc<Chunk>(chunks.size(), O);
+    c->pass_l0 = dalloc<int32_t>(pass_l0.size(), O);
+    c->sched = dalloc<int32_t>(sched.size(), O);
+    c->sched_ctr = dalloc<int32_t>(8, O);
+    c->pass_cost = dalloc<double>(pass_l0.size(), O);
     c->wins = dalloc<WinInfo>(nwin, O);
     c->colmap = dalloc<int32_t>(colmap.size(), O);
     c->slot_off = dalloc<int32_t>(c->nslot, O);
@@ -597,6 +642,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
     bool ok = up(c->chunks, chunks.data(), chunks.size() * sizeof(Chunk)) &&
+              (pass_l0.empty() || up(c->pass_l0, pass_l0.data(), pass_l0.size() * 4)) &&
+              (sched.empty() || up(c->sched, sched.data(), sched.size() * 4)) &&
               up(c->wins, wins.data(), nwin * sizeof(WinInfo)) &&
               up(c->colmap, colmap.data(), colmap.size() * 4) && up(c->slot_off, slot_off.data(), c->nslot * 4) &&
               up(c->slot_dim, B.slot_dim, c->nslot * 4) && up(c->ob_start, B.ob_start, (c->nlm + 1) * 4) &&
@@ -604,6 +651,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
               up(c->lm_win, lm_win.data(), c->nlm * 4) && up(c->imu_ij, B.imu_ij, 2 * (size_t)c->nimu * 4) &&
               up(c->prior_slot, B.prior_slot, c->npblk * 4) && up(c->prior_dim, B.prior_dim, c->npblk * 4);
     if (!ok) return fail(XBA_E_HIP);
+    if (hipMemset(c->sched_ctr, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(XBA_E_HIP);
     // LDS budgets
     if (c->fast) {
         if (mx) {

@@ -32,7 +32,7 @@ enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
     } while (0)
 
 struct FastLayout {
-    int sc, sc64, wc, wd, tile, btile, wcost, wave, wave_stride, total;
+    int sc, sc64, wc, wd, tile, btile, ctl, wave, wave_stride, total;
     // offsets inside one wavefront's region
     int o_oblm, o_obt, o_obact, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
         o_lrow;
@@ -49,7 +49,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.wd = o;    o = align16(o + 8 * 8);
     L.tile = o;  o = align16(o + (trial ? 6 * R * ts : (R * (R + 1) / 2) * TBLK * 8));  // trial: dp per slot
     L.btile = o; o = align16(o + (trial ? 0 : 6 * R * 8));
-    L.wcost = o; o = align16(o + 16 * 8);
+    L.ctl = o;   o = align16(o + 4 * 4);
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
@@ -104,25 +104,51 @@ __device__ inline void lds_add(T *p, T v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// One pass's landmark set: as many landmarks from `lb` as fit the pass limits.  Writes lobs0/lnobs/ob_lm and
-// returns the count B (>= 1) and the pass's observation count.
-__device__ inline int select_pass(const int32_t *ob_start, int lb, int lend, int lane, int *lobs0, int *lnobs,
-                                  int *ob_lm, int *nob_out) {
+// One pass (the B landmarks from lb, planned on the host so that their observations fit one lane each and their
+// touched slots fit the pass's U storage): per landmark its first record row and observation count, per record row
+// its landmark.  Returns the pass's observation count.
+__device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane, int *lobs0, int *lnobs, int *ob_lm) {
     const int l = lb + lane;
-    const bool in = lane < FB_LM && l < lend;
-    const int n = in ? (ob_start[l + 1] - ob_start[l]) : 1000;
+    const bool in = lane < B;
+    const int n = in ? (ob_start[l + 1] - ob_start[l]) : 0;
     const int S = wave_incl_scan(n, lane);
-    const int K = wave_incl_scan(n + 1, lane);
-    const bool fit = in && S <= FB_OBS && K <= FB_HLP;
-    const uint64_t bal = __ballot(fit);
-    const int B = __popcll(bal);
-    *nob_out = __shfl(S, B - 1, XBA_WAVE);
-    if (lane < B) {
+    if (in) {
         lobs0[lane] = S - n;
         lnobs[lane] = n;
         for (int q = 0; q < n; q++) ob_lm[S - n + q] = lane;
     }
-    return B;
+    return __shfl(S, B - 1, XBA_WAVE);
+}
+
+// Work items are handed out dynamically, largest first (the host orders `sched`): thread 0 takes the next index of
+// sched[cbeg, cend) from the launch kind's counter, and the last workgroup to find the list exhausted resets both
+// counters for the next launch of that kind.  ctl[1] is the chunk's next pass, taken by the wavefronts in turn.
+// Returns the work item, or -1 when there is none left.
+__device__ inline int next_chunk(const KArgs<double> &A, int kind, int cbeg, int cend, int *ctl) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t *ctr = A.sched_ctr + 2 * kind;
+        const int i = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int c = -1;
+        if (cbeg + i < cend) {
+            c = A.sched[cbeg + i];
+        } else if (__hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                   (int)gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        ctl[0] = c;
+        ctl[1] = c >= 0 ? A.chunks[c].p0 : 0;
+    }
+    __syncthreads();
+    return ctl[0];
+}
+
+// The next pass of the current work item for this wavefront (uniform across the wavefront).
+__device__ inline int next_pass(int *ctl, int lane) {
+    int p = 0;
+    if (lane == 0) p = __hip_atomic_fetch_add(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __shfl(p, 0, XBA_WAVE);
 }
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
@@ -130,7 +156,7 @@ __device__ inline int select_pass(const int32_t *ob_start, int lb, int lend, int
 // product is formed in FP64.  CLS names the work-item class the launch covers (0: narrow chunks, 1: wide chunks);
 // the code is the same, the two classes get separate symbols so that profiles report them apart.
 template <typename RT, int CLS>
-__global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int cbeg, int cend) {
+__global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int cbeg, int cend, int kind) {
     using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
@@ -138,7 +164,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     T *sc = (T *)(smem + FL.sc), *wc = (T *)(smem + FL.wc);
     double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd);
     double *tile = (double *)(smem + FL.tile), *btile = (double *)(smem + FL.btile);
-    double *wcost = (double *)(smem + FL.wcost);
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
     int *ob_lm = (int *)(wb + FL.o_oblm), *rt = (int *)(wb + FL.o_obt);
@@ -149,8 +174,11 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     int *slots = (int *)(wb + FL.o_slots);
     T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
     RT *rec = (RT *)(wb + FL.o_lrow);
+    int *ctl = (int *)(smem + FL.ctl);
 
-    for (int c = cbeg + blockIdx.x; c < cend; c += gridDim.x) {
+    for (;;) {
+        const int c = next_chunk(A, kind, cbeg, cend, ctl);
+        if (c < 0) break;
         const Chunk ch = A.chunks[c];
         const WinInfo W = A.wins[ch.win];
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;
@@ -164,13 +192,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
         for (int i = tid; i < nblk * TBLK; i += blockDim.x) tile[i] = 0.0;
         for (int i = tid; i < 6 * R; i += blockDim.x) btile[i] = 0.0;
         __syncthreads();
-        // this wavefront's share of the chunk
-        const int nl = ch.l1 - ch.l0;
-        const int wbeg = ch.l0 + (int)(((int64_t)nl * wv) / NW), wend = ch.l0 + (int)(((int64_t)nl * (wv + 1)) / NW);
-        double wcst = 0.0;
-        for (int lb = wbeg; lb < wend;) {
-            int nob;
-            const int B = select_pass(A.ob_start, lb, wend, lane, lobs0, lnobs, ob_lm, &nob);
+        // the chunk's passes, taken by the wavefronts in turn
+        for (;;) {
+            const int p = next_pass(ctl, lane);
+            if (p >= ch.p1) break;
+            const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
+            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm);
             WAVE_SYNC();
             // ---- observations: whitened residual, Huber weight and cost, whitened Jacobians into the record
             if (lane < nob) {
@@ -304,7 +331,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 }
             }
             const double pcost = wave_sum(lcost);
-            if (lane == 0) wcst += pcost;
+            if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
             // ---- U = L^-1 Hlp, one lane per column of a (landmark, slot) block:
             //      Hlp[s][:, j] = sum_q w_q Jl_q^T J_{q,s}[:, j]  (J_{q,s} = Jt if t_q == s != h, Jh if s == h)
@@ -427,39 +454,39 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 for (int i = 0; i < 6; i++) lds_add<double>(btile + 6 * s + i, (double)acc[i]);
             }
             WAVE_SYNC();
-            lb += B;
         }
-        if (lane == 0) wcost[wv] = wcst;
         __syncthreads();
         double *pH = A.partH64 + ch.partH, *pb = A.partb64 + ch.partb;
         for (int i = tid; i < nblk * 36; i += blockDim.x) pH[i] = tile[(i / 36) * TBLK + i % 36];
         for (int i = tid; i < 6 * R; i += blockDim.x) pb[i] = btile[i];
         if (tid == 0) {
             double cs = 0.0;
-            for (int w = 0; w < NW; w++) cs += wcost[w];
+            for (int q = ch.p0; q < ch.p1; q++) cs += A.pass_cost[q];
             A.partc[c] = cs;
         }
-        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------------------- k_trial_fast --
 template <int CLS>
-__global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, int cbeg, int cend) {
+__global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, int cbeg, int cend, int kind) {
     using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
     const FastLayout FL = fast_layout(Rmax, 8, NW, true);
     T *sc = (T *)(smem + FL.sc), *wc = (T *)(smem + FL.wc), *dps = (T *)(smem + FL.tile);
-    double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd), *wcost = (double *)(smem + FL.wcost);
+    double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd);
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
     int *ob_lm = (int *)(wb + FL.o_oblm);
     double *rcq = (double *)(wb + FL.o_rcq), *ldbl = (double *)(wb + FL.o_ldbl);
     T *u = (T *)(wb + FL.o_u);
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs);
+    int *ctl = (int *)(smem + FL.ctl);
 
-    for (int c = cbeg + blockIdx.x; c < cend; c += gridDim.x) {
+    for (;;) {
+        const int c = next_chunk(A, kind, cbeg, cend, ctl);
+        if (c < 0) break;
         const Chunk ch = A.chunks[c];
         const WinInfo W = A.wins[ch.win];
         if (A.wstate[(size_t)NWS * ch.win + WS_CHOLOK] == 0.0) {
@@ -483,12 +510,11 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
             dps[i] = (T)A.dx[W.st0 + A.slot_off[W.slot0 + ch.lo + s] + j];
         }
         __syncthreads();
-        const int nl = ch.l1 - ch.l0;
-        const int wbeg = ch.l0 + (int)(((int64_t)nl * wv) / NW), wend = ch.l0 + (int)(((int64_t)nl * (wv + 1)) / NW);
-        double wcst = 0.0;
-        for (int lb = wbeg; lb < wend;) {
-            int nob;
-            const int B = select_pass(A.ob_start, lb, wend, lane, lobs0, lnobs, ob_lm, &nob);
+        for (;;) {
+            const int p = next_pass(ctl, lane);
+            if (p >= ch.p1) break;
+            const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
+            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm);
             WAVE_SYNC();
             // ---- observations at the current state: u_q = w Jl^T (Jt dp_t + Jh dp_h)
             int q_l = 0, q_k = 0, q_t = 0, q_h = 0, q_cam = 0, q_valid = 0;
@@ -606,26 +632,23 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
             if (lane < B)
                 for (int q = lobs0[lane]; q < lobs0[lane] + lnobs[lane]; q++) lcost += rcq[q];
             const double pcost = wave_sum(lcost);
-            if (lane == 0) wcst += pcost;
+            if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
-            lb += B;
         }
-        if (lane == 0) wcost[wv] = wcst;
         __syncthreads();
         if (tid == 0) {
             double cs = 0.0;
-            for (int w = 0; w < NW; w++) cs += wcost[w];
+            for (int q = ch.p0; q < ch.p1; q++) cs += A.pass_cost[q];
             A.partc2[c] = cs;
         }
-        __syncthreads();
     }
 }
 
-template __global__ void k_lin_fast<float, 0>(KArgs<double>, int, int, int);
-template __global__ void k_lin_fast<float, 1>(KArgs<double>, int, int, int);
-template __global__ void k_lin_fast<double, 0>(KArgs<double>, int, int, int);
-template __global__ void k_lin_fast<double, 1>(KArgs<double>, int, int, int);
-template __global__ void k_trial_fast<0>(KArgs<double>, int, int, int);
-template __global__ void k_trial_fast<1>(KArgs<double>, int, int, int);
+template __global__ void k_lin_fast<float, 0>(KArgs<double>, int, int, int, int);
+template __global__ void k_lin_fast<float, 1>(KArgs<double>, int, int, int, int);
+template __global__ void k_lin_fast<double, 0>(KArgs<double>, int, int, int, int);
+template __global__ void k_lin_fast<double, 1>(KArgs<double>, int, int, int, int);
+template __global__ void k_trial_fast<0>(KArgs<double>, int, int, int, int);
+template __global__ void k_trial_fast<1>(KArgs<double>, int, int, int, int);
 
 }  // namespace xba

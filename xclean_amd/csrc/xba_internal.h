@@ -9,9 +9,10 @@
 
 // One work item of the landmark kernels: a contiguous landmark range of one window whose observations touch only
 // the slots [lo, lo + R).  Its reduced-system partial lives at partH (R(R+1)/2 lower-triangular 6x6 blocks) and
-// partb (6R entries) of the partial buffers.
+// partb (6R entries) of the partial buffers.  The multi-wave kernels walk it in the passes [p0, p1) of the pass
+// list (each pass: the landmarks [pass_l0[p], pass_l0[p + 1])).
 struct Chunk {
-    int32_t win, l0, l1, lo, R, pad;
+    int32_t win, l0, l1, lo, R, p0, p1, pad;
     int64_t partH, partb;
 };
 
@@ -80,4 +81,8 @@ struct KArgs {
     uint8_t *ok_out;           // (nlm) or null
     double *adj_gz, *adj_gw;   // adjoint outputs (nobs,2), (nobs) or null
     const double *adj_y;       // adjoint solve H^-1 g (sum n) or null
+    const int32_t *pass_l0;    // multi-wave kernels: first landmark of every pass (npass + 1)
+    const int32_t *sched;      // multi-wave kernels: work items in the order they are handed out
+    int32_t *sched_ctr;        // multi-wave kernels: per launch kind, a hand-out counter and a finished counter
+    double *pass_cost;         // multi-wave kernels: robust cost of every pass (summed per work item in pass order)
 };
