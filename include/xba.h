@@ -90,6 +90,9 @@ typedef struct xba_batch {
     const double *prior_x0;     /* (npblk, 21): R0 9, p0 3, v0 3, bg0 3, ba0 3 */
     const double *prior_S;      /* window w: rows [win_prow0[w], win_prow0[w+1]) x (sum of its prior dims), packed */
     const double *prior_r;      /* (nprow) */
+    /* optional (NULL: 1): per-observation multiplier of the robust weight and cost of each visual row.  No
+     * reference counterpart; it is the input xba_backward differentiates with respect to. */
+    const double *ob_w;         /* (nobs) */
 } xba_batch;
 
 typedef struct xba_options {
@@ -150,9 +153,16 @@ int xba_profile_iteration(xba_ctx *ctx, double *ms_out, void *stream);
  * bytes of the dense solve, partial-tile entries, partial-b entries, max state dimension. */
 int xba_plan_info(const xba_ctx *ctx, int64_t *out);
 
-/* Implicit-function adjoint of the last iteration's solved update dx* = -H^-1 b (Gauss-Newton H, at the state
- * the update was linearised at, reusing the Cholesky factor): given g = dL/d(dx) (device, sum n_w), writes
- * dL/dz (device, nobs x 2) and dL/dw (device, nobs) where w scales each observation's whitened residual weight. */
+/* The Gauss-Newton step of every window at the current state, dx = -(H + 1e-9 I)^-1 b with H, b the undamped
+ * (lambda = 0) reduced system (vio/window.py:291-463, 865-886, 933-941 at lambda = 0), factored in FP64; the factor
+ * is kept for xba_backward.  dx_out (device, sum n) may be NULL.  No reference counterpart: it is the forward half
+ * of the differentiable solve. */
+int xba_gn_step(xba_ctx *ctx, double *dx_out, void *stream);
+
+/* Implicit-function adjoint of the step xba_gn_step computed (which must be the last call on ctx that touched the
+ * state), reusing its Cholesky factor: given g = dL/d(dx) (device, sum n), writes dL/dz (device, nobs x 2, the
+ * observed pixel coordinates ob_z) and dL/dw (device, nobs, the per-observation weights ob_w, 1 when the batch has
+ * none), in the batch's observation order.  No reference counterpart. */
 int xba_backward(xba_ctx *ctx, const double *g_dx, double *grad_z, double *grad_w, void *stream);
 
 /* ---- the Euclidean local-BA drop-in (backend/ba.py:713-869) ---------------------------------------------------

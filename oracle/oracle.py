@@ -44,7 +44,7 @@ class OwProblem(C.Structure):
                 ("prior_nb", C.c_int32), ("prior_m", C.c_int32), ("prior_slot", i32p), ("prior_dim", i32p),
                 ("prior_x0", f64p), ("prior_S", f64p), ("prior_r", f64p),
                 ("n_pf", C.c_int32), ("pf_host", i32p), ("pf_z", f64p), ("pf_isg", C.c_double),
-                ("pf_rmeas", f64p), ("pf_ris", f64p), ("pf_lmp", f64p)]
+                ("pf_rmeas", f64p), ("pf_ris", f64p), ("pf_lmp", f64p), ("ob_w", f64p)]
 
 
 class OnpProblem(C.Structure):
@@ -132,6 +132,9 @@ class _Packed:
         s.ob_start, s.ob_pose = _p(self.ob_start, i32p), _p(self.ob_pose, i32p)
         s.ob_cam, s.ob_valid = _p(self.ob_cam, u8p), _p(self.ob_valid, u8p)
         s.ob_z, s.ob_isig = _p(self.ob_z, f64p), _p(self.ob_isig, f64p)
+        ob_w = getattr(pb, "ob_w", None)
+        self.ob_w = c(ob_w, np.float64) if ob_w is not None else None
+        s.ob_w = _p(self.ob_w, f64p)
         s.fx, s.fy, s.cx, s.cy, s.B, s.huber_k = pb.fx, pb.fy, pb.cx, pb.cy, pb.B, pb.huber_k
         s.dims = _p(self.dims, i32p)
         s.vel, s.bgs, s.bas = _p(pb.vel, f64p), _p(pb.bgs, f64p), _p(pb.bas, f64p)

@@ -72,6 +72,9 @@ typedef struct {
     double pf_isg;
     const double *pf_rmeas, *pf_ris;
     double *pf_lmp;             /* (n_pf,3) */
+    /* optional per-observation weight multiplying the robust weight and cost of each visual row (NULL: 1).  Not
+     * in the reference: it is the input the adjoint differentiates with respect to. */
+    const double *ob_w;         /* (M) */
 } ow_problem;
 
 /* ------------------------------------------------------------------------------------------- small helpers -- */
@@ -462,6 +465,10 @@ double ow_build(const ow_problem *pb, const double *Rwb, const double *pwb, cons
             ob_chi2[k] = en2;
             double w, c;
             huber(sqrt(en2), pb->huber_k, &w, &c);
+            if (pb->ob_w) {
+                w *= pb->ob_w[k];
+                c *= pb->ob_w[k];
+            }
             cost += c;
             obs_jacobians(Rwb, Rwc, pwc, Rcb, Px, lmp + 3 * l, h, t, cam, pb->fx, pb->fy, pb->B, Jl, Jt, Jh);
             for (int i = 0; i < nd; i++) {
@@ -591,7 +598,7 @@ double ow_visual_cost(const ow_problem *pb, const double *Rwb, const double *pwb
             double en2 = 0.0, w, c;
             for (int i = 0; i < nd; i++) en2 += (e[i] * pb->ob_isig[k]) * (e[i] * pb->ob_isig[k]);
             huber(sqrt(en2), pb->huber_k, &w, &c);
-            cost += c;
+            cost += pb->ob_w ? c * pb->ob_w[k] : c;
         }
         if (pb->rho_isig[l] > 0.0) {
             double r0 = (lmp[3 * l + 2] - pb->rho_meas[l]) * pb->rho_isig[l], w, c;

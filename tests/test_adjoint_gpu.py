@@ -1,0 +1,124 @@
+"""The differentiable window solve on the GPU (xclean_amd.adjoint -> libxba xba_gn_step / xba_backward).
+
+Bars: the forward Gauss-Newton step equals the FP64 oracle's step at the same state (1e-9 relative in FP64, 1e-5
+in mixed precision, where the step near convergence is a small difference of large terms); the adjoint gradients
+with respect to observed pixels and per-observation weights agree with FP64 central finite differences (h = 1e-6)
+of the oracle's step within 1e-5 relative, on rows away from the Huber kink; a second backward reuses the kept
+factor and gives the same gradients.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xclean_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _windows():
+    return [synth.make_window(P=8, L=300, views=(2, 8), seed=[9, 0])[0],
+            synth.make_window(P=6, L=250, views=(1, 6), seed=[9, 1], stereo="ur", outlier_frac=0.15)[0],
+            synth.make_window(P=6, L=300, views=(2, 6), seed=[9, 2], vi=True)[0],
+            synth.make_window(P=7, L=300, views=(1, 7), seed=[9, 3], posefree=True)[0]]
+
+
+def _oracle_step(pb):
+    _c, H6, b6, *_ = O.window_build(pb, 0.0)
+    H, b, _ = O.window_full_system(pb, H6, b6)
+    return -np.linalg.solve(H + 1e-9 * np.eye(len(H)), b)
+
+
+def _coords(pb, rng, n):
+    out = []
+    for k in rng.permutation(pb.M):
+        l = int(np.searchsorted(pb.ob_start, k, side="right") - 1)
+        if not pb.ob_valid[k] or not pb.lm_mask[l]:
+            continue
+        e, *_ = O.obs_residual_jac(pb, l, k)
+        if len(e) == 0 or abs(np.linalg.norm(e * pb.ob_isig[k]) - pb.huber_k) < 1e-3:
+            continue
+        out.append((k, len(e)))
+        if len(out) == n:
+            break
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adjoint_matches_finite_differences(precision):
+    import torch
+    from xclean_amd.adjoint import WindowStep
+    from xclean_amd.window import BatchSolver
+    pbs = _windows()
+    rng = np.random.default_rng(4)
+    for pb in pbs:
+        pb.ob_w = rng.uniform(0.5, 1.5, pb.M)
+    print("adjoint", precision, [(pb.P, pb.L, pb.M) for pb in pbs], flush=True)
+    solver = BatchSolver(pbs, precision=precision, with_weights=True, use_graph=False)
+    solver.upload(1e-4)
+    solver.iterate(3)
+    solver.download(write_back=True)          # the oracle linearises at the same state
+    dev = solver.dev
+    z = torch.tensor(np.concatenate([pb.ob_z for pb in pbs]), dtype=torch.float64, device=dev, requires_grad=True)
+    w = torch.tensor(np.concatenate([pb.ob_w for pb in pbs]), dtype=torch.float64, device=dev, requires_grad=True)
+    dx = WindowStep.apply(z, w, solver)
+    n0 = np.concatenate([[0], np.cumsum([pb.n_state() for pb in pbs])])
+    # near convergence the step is a small difference of large terms: the mixed path's FP32 Jacobian record
+    # costs ~1e-6 of it there (the 1e-6 bar on the first, large update is checked in test_window_gpu.py)
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    for i, pb in enumerate(pbs):
+        ref = _oracle_step(pb)
+        got = dx[n0[i]:n0[i + 1]].detach().cpu().numpy()
+        print("  window", i, "step rel err %.2e" % (np.linalg.norm(got - ref) / np.linalg.norm(ref)), flush=True)
+        assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), (i, np.linalg.norm(got - ref))
+    g = torch.tensor(rng.standard_normal(int(n0[-1])), dtype=torch.float64, device=dev)
+    (g * dx).sum().backward()
+    gz, gw = z.grad.cpu().numpy(), w.grad.cpu().numpy()
+    m0 = np.concatenate([[0], np.cumsum([pb.M for pb in pbs])])
+    gnp = g.cpu().numpy()
+    h = 1e-6
+    for i, pb in enumerate(pbs):
+        gi = gnp[n0[i]:n0[i + 1]]
+        L = lambda: float(gi @ _oracle_step(pb))
+        fd, ad = [], []
+        for k, nd in _coords(pb, rng, 12):
+            for d in range(nd):
+                z0 = pb.ob_z[k, d]
+                pb.ob_z[k, d] = z0 + h
+                lp = L()
+                pb.ob_z[k, d] = z0 - h
+                lm = L()
+                pb.ob_z[k, d] = z0
+                fd.append((lp - lm) / (2 * h))
+                ad.append(gz[m0[i] + k, d])
+            w0 = pb.ob_w[k]
+            pb.ob_w[k] = w0 + h
+            lp = L()
+            pb.ob_w[k] = w0 - h
+            lm = L()
+            pb.ob_w[k] = w0
+            fd.append((lp - lm) / (2 * h))
+            ad.append(gw[m0[i] + k])
+        fd, ad = np.array(fd), np.array(ad)
+        err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+        print("  window", i, "coords", len(fd), "rel err vs FD %.2e" % err, flush=True)
+        assert err < 1e-5, (i, err)
+        if pb.pf is not None:                 # pose-free rows are not in the batch's visual rows
+            assert gz.shape[0] == m0[-1]
+    # a second backward through the same kept factor gives the same gradients
+    gz2, gw2 = solver.adjoint(g)
+    np.testing.assert_allclose(gz2.cpu().numpy(), gz, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(gw2.cpu().numpy(), gw, rtol=1e-12, atol=1e-300)
+
+
+def test_backward_needs_a_fresh_step():
+    from xclean_amd import _lib
+    from xclean_amd.window import BatchSolver
+    pb = synth.make_window(P=5, L=100, views=(2, 5), seed=[9, 5])[0]
+    solver = BatchSolver([pb], precision="fp64", with_weights=True, use_graph=False)
+    solver.upload(1e-4)
+    with pytest.raises(_lib.XbaError):
+        solver.adjoint(np.zeros(pb.n_state()))
+    solver.gn_step()
+    solver.iterate(1)                          # the state moved: the kept factor no longer matches
+    with pytest.raises(_lib.XbaError):
+        solver.adjoint(np.zeros(pb.n_state()))

@@ -28,7 +28,7 @@ class XbaBatch(C.Structure):
                 ("Rwb", vp), ("pwb", vp), ("vel", vp), ("bgs", vp), ("bas", vp), ("lmp", vp),
                 ("lm_mask", vp), ("rho_meas", vp), ("rho_isig", vp), ("ob_z", vp), ("ob_isig", vp),
                 ("ob_cam", vp), ("ob_valid", vp), ("win_par", vp), ("imu_rec", vp), ("prior_x0", vp),
-                ("prior_S", vp), ("prior_r", vp)]
+                ("prior_S", vp), ("prior_r", vp), ("ob_w", vp)]
 
 
 class XbaOptions(C.Structure):
@@ -51,7 +51,8 @@ class XbaError(RuntimeError):
 
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
-           "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_backward", "xba_np_solve",
+           "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
+           "xba_np_solve",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
 
@@ -75,6 +76,7 @@ def load():
     L.xba_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xba_profile_iteration.argtypes = [vp, vp, vp]
     L.xba_plan_info.argtypes = [vp, vp]
+    L.xba_gn_step.argtypes = [vp, vp, vp]
     L.xba_backward.argtypes = [vp, vp, vp, vp, vp]
     L.xba_np_solve.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_copy_kernel.argtypes = [vp, vp, C.c_int64, vp]

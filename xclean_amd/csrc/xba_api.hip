@@ -14,6 +14,7 @@
 #include "xba_dense.hip"
 #include "xba_fast.hip"
 #include "xba_np.hip"
+#include "xba_adj.hip"
 
 using namespace xba;
 
@@ -96,6 +97,8 @@ struct xba_ctx {
     double *pass_cost = nullptr;
     bool fast = true;
     bool uploaded = false;
+    bool gn_ready = false;  // xba_gn_step ran and nothing has changed the state or the factor since
+    double *adj_y = nullptr;
     std::vector<void *> owned;
     // device
     Chunk *chunks = nullptr;
@@ -162,6 +165,7 @@ KArgs<T> kargs(xba_ctx *c) {
     A.sched = c->sched;
     A.sched_ctr = c->sched_ctr;
     A.pass_cost = c->pass_cost;
+    A.ob_w = c->B.ob_w;
     return A;
 }
 
@@ -217,6 +221,7 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
     F.wstate = A.wstate; F.lam_override = A.lam_override; F.chi2_out = A.chi2_out; F.Hinv_out = A.Hinv_out;
     F.ok_out = A.ok_out; F.adj_gz = A.adj_gz; F.adj_gw = A.adj_gw; F.adj_y = A.adj_y;
     F.pass_l0 = A.pass_l0; F.sched = A.sched; F.sched_ctr = A.sched_ctr; F.pass_cost = A.pass_cost;
+    F.ob_w = A.ob_w;
     return F;
 }
 
@@ -631,7 +636,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->H = dalloc<double>(h0, O);
     c->b = dalloc<double>(st, O);
     c->dx = dalloc<double>(st, O);
-    c->fact64 = mx ? dalloc<double>(1, O) : dalloc<double>(h0, O);
+    c->fact64 = dalloc<double>(h0, O);  // FP64 factor: the FP64 solve and the Gauss-Newton step of the adjoint
+    c->adj_y = dalloc<double>(c->nstate, O);
     c->fact32 = mx ? dalloc<float>(h0, O) : dalloc<float>(1, O);
     c->imu_scr = dalloc<double>(465 * (size_t)std::max(c->nimu, 1), O);
     c->wstate = dalloc<double>((size_t)NWS * nwin, O);
@@ -711,12 +717,14 @@ int xba_upload(xba_ctx *c, double lam0, void *stream) {
     hipLaunchKernelGGL(k_init_ws, dim3((c->nwin + 255) / 256), dim3(256), 0, s, c->nwin, c->wstate, lam0);
     HIPCK(hipGetLastError());
     c->uploaded = true;
+    c->gn_ready = false;
     return XBA_OK;
 }
 
 int xba_iterate(xba_ctx *c, int32_t iters, void *stream) {
     if (!c || iters < 0) return XBA_E_ARG;
     if (!c->uploaded) return XBA_E_STATE;
+    c->gn_ready = false;
     hipStream_t s = (hipStream_t)stream;
     if (c->opt.use_graph) {
         if (!c->gexec || c->gstream != s) {
@@ -785,6 +793,7 @@ int xba_linearize(xba_ctx *c, double lam, double *H_out, double *b_out, double *
                   double *Hinv_out, uint8_t *ok_out, double *imu_chi2_out, void *stream) {
     if (!c || lam < 0.0) return XBA_E_ARG;
     if (!c->uploaded) return XBA_E_STATE;
+    c->gn_ready = false;
     hipStream_t s = (hipStream_t)stream;
     if (mixed(c)) {
         KArgs<float> A = kargs<float>(c);
@@ -814,6 +823,7 @@ int xba_linearize(xba_ctx *c, double lam, double *H_out, double *b_out, double *
 int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
     if (!c || !ms_out) return XBA_E_ARG;
     if (!c->uploaded) return XBA_E_STATE;
+    c->gn_ready = false;
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t ev[XBA_NPROF + 1];
     for (int i = 0; i <= XBA_NPROF; i++) HIPCK(hipEventCreate(&ev[i]));
@@ -868,13 +878,39 @@ int xba_plan_info(const xba_ctx *c, int64_t *out) {
     return XBA_OK;
 }
 
-int xba_backward(xba_ctx *c, const double *g_dx, double *grad_z, double *grad_w, void *stream) {
-    (void)g_dx;
-    (void)grad_z;
-    (void)grad_w;
-    (void)stream;
+int xba_gn_step(xba_ctx *c, double *dx_out, void *stream) {
     if (!c) return XBA_E_ARG;
-    return XBA_E_STATE;
+    if (!c->uploaded) return XBA_E_STATE;
+    if (mixed(c) && !c->fast) return XBA_E_STATE;  // the deterministic mixed family keeps FP32 landmark blocks
+    hipStream_t s = (hipStream_t)stream;
+    c->gn_ready = false;
+    if (mixed(c)) {
+        KArgs<float> A = kargs<float>(c);
+        A.lam_override = 0.0;
+        launch_lin<float>(c, A, s);
+    } else {
+        KArgs<double> A = kargs<double>(c);
+        A.lam_override = 0.0;
+        launch_lin<double>(c, A, s);
+    }
+    hipLaunchKernelGGL(k_gn_solve, dim3(c->nwin), dim3(256), 0, s, dargs(c));
+    HIPCK(hipGetLastError());
+    if (dx_out) HIPCK(hipMemcpyAsync(dx_out, c->dx, (size_t)c->nstate * 8, hipMemcpyDeviceToDevice, s));
+    c->gn_ready = true;
+    return XBA_OK;
+}
+
+int xba_backward(xba_ctx *c, const double *g_dx, double *grad_z, double *grad_w, void *stream) {
+    if (!c || !g_dx || !grad_z || !grad_w) return XBA_E_ARG;
+    if (!c->gn_ready) return XBA_E_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_adj_y, dim3(c->nwin), dim3(256), 0, s, dargs(c), g_dx, c->adj_y);
+    const KArgs<double> F = mixed(c) ? fast_args<float>(c, kargs<float>(c)) : kargs<double>(c);
+    if (c->nlm > 0)
+        hipLaunchKernelGGL(k_adj_obs, dim3((c->nlm + 127) / 128), dim3(128), 0, s, F, (const int32_t *)c->lm_win,
+                           (const double *)c->adj_y, grad_z, grad_w);
+    HIPCK(hipGetLastError());
+    return XBA_OK;
 }
 
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,

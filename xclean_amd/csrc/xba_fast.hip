@@ -545,6 +545,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                         double en2 = 0.0, w, cst;
                         for (int i = 0; i < nd; i++) en2 += (e[i] * isg) * (e[i] * isg);
                         huber(sqrt(en2), wd[5], &w, &cst);
+                        if (A.ob_w) w *= A.ob_w[q_k];
                         const T isgT = A.isig[q_k];
                         T s[3] = {0, 0, 0};
                         for (int r = 0; r < nd; r++) {
@@ -624,6 +625,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                         double isg = (double)A.isig[q_k], en2 = e[0] * isg * (e[0] * isg), w;
                         if (q_cam == 0) en2 += e[1] * isg * (e[1] * isg);
                         huber(sqrt(en2), wd[5], &w, &cq);
+                        if (A.ob_w) cq *= A.ob_w[q_k];
                     }
                 }
                 rcq[lane] = cq;

@@ -85,4 +85,5 @@ struct KArgs {
     const int32_t *sched;      // multi-wave kernels: work items in the order they are handed out
     int32_t *sched_ctr;        // multi-wave kernels: per launch kind, a hand-out counter and a finished counter
     double *pass_cost;         // multi-wave kernels: robust cost of every pass (summed per work item in pass order)
+    const double *ob_w;        // (nobs) per-observation weight multiplier, or null (1)
 };

@@ -256,6 +256,10 @@ __device__ inline void lin_observations(const KArgs<T> &A, RT *rec, double *rc, 
                 en2 += e[i] * e[i];
             }
             huber(sqrt(en2), wd[5], &w, &c);
+            if (A.ob_w) {
+                w *= A.ob_w[k];
+                c *= A.ob_w[k];
+            }
             if (jac) {
                 const T isgT = A.isig[k];
                 for (int i = 0; i < nd; i++) {
@@ -610,6 +614,7 @@ __global__ __launch_bounds__(XBA_WAVE) void k_trial(KArgs<T> A, int Rmax) {
                         double isg = (double)A.isig[k], en2 = e[0] * isg * (e[0] * isg), w;
                         if (cam == 0) en2 += e[1] * isg * (e[1] * isg);
                         huber(sqrt(en2), wd[5], &w, &cq);
+                        if (A.ob_w) cq *= A.ob_w[k];
                     }
                 }
             }

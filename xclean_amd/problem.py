@@ -20,11 +20,13 @@ class Problem:
 
     VO: every slot is a 6-dim pose block [dphi, dp] (R_wb <- R_wb Exp(dphi), p_wb <- p_wb + dp).  VI: ``dims`` (P,)
     gives 6 or 15 per slot and ``vel``/``bgs``/``bas`` (P,3) hold the extra state; ``imu`` is a list of
-    (i, j, Preint, W) factors between 15-dim slots, ``g_w`` the window's gravity (held fixed)."""
+    (i, j, Preint, W) factors between 15-dim slots, ``g_w`` the window's gravity (held fixed).  ``ob_w`` (M,),
+    optional, multiplies each observation row's robust weight and cost (1 when absent; not a field of the
+    reference: it is what the adjoint differentiates with respect to)."""
 
     __slots__ = ("Rwb", "pwb", "Rbc", "pbc", "lmp", "lm_host", "lm_mask", "rho_meas", "rho_isig", "ob_start",
                  "ob_pose", "ob_cam", "ob_z", "ob_isig", "ob_valid", "fx", "fy", "cx", "cy", "B", "huber_k",
-                 "prior", "dims", "vel", "bgs", "bas", "imu", "g_w", "pf")
+                 "prior", "dims", "vel", "bgs", "bas", "imu", "g_w", "pf", "ob_w")
 
     def __init__(self, **kw):
         for k in self.__slots__:

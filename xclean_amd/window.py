@@ -49,6 +49,8 @@ class _Packed:
         ob_z = np.asarray(pb.ob_z, np.float64).reshape(-1, 2)
         ob_isig = np.asarray(pb.ob_isig, np.float64)
         ob_valid = np.asarray(pb.ob_valid, bool)
+        has_w = getattr(pb, "ob_w", None) is not None
+        ob_w = np.asarray(pb.ob_w, np.float64) if has_w else np.ones(M0)
         npf = 0
         if pf is not None:
             npf = len(pf.host)
@@ -63,6 +65,7 @@ class _Packed:
             ob_z = np.concatenate([ob_z, pf.z])
             ob_isig = np.concatenate([ob_isig, np.full(npf, pf.isg)])
             ob_valid = np.concatenate([ob_valid, np.ones(npf, bool)])
+            ob_w = np.concatenate([ob_w, np.ones(npf)])
             start = np.concatenate([[0], np.cumsum(cnt)])
         self.P, self.L0, self.M0, self.npf = P, L0, M0, npf
         self.L, self.M = L0 + npf, M0 + npf
@@ -98,6 +101,8 @@ class _Packed:
         self.ob_z = np.ascontiguousarray(ob_z[o])
         self.ob_isig = ob_isig[o]
         self.ob_valid = ob_valid[o].astype(np.uint8)
+        self.has_w = has_w
+        self.ob_w = ob_w[o]
         self.dims = (np.full(P, 6, np.int32) if pb.dims is None else np.asarray(pb.dims, np.int32))
         self.n = int(self.dims.sum())
         self.vi = pb.vel is not None
@@ -139,7 +144,7 @@ class BatchSolver:
     """Solve many windows at once on the GPU.  ``problems`` is a list of ``Problem``s (mutated by ``download``)."""
 
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
-                 device="cuda", deterministic=False):
+                 device="cuda", deterministic=False, with_weights=False):
         import torch
         _lib.require_gpu()
         self.torch = torch
@@ -201,6 +206,18 @@ class BatchSolver:
             prior_S=dev(cat([w.prior_S.reshape(-1) for w in W]), np.float64),
             prior_r=dev(cat([w.prior_r for w in W]), np.float64),
         )
+        if with_weights or any(w.has_w for w in W):
+            self.d["ob_w"] = dev(cat([w.ob_w for w in W]), np.float64)
+        # packed row -> row of the concatenated original observations (pose-free rows have none)
+        orig0 = np.concatenate([[0], np.cumsum([w.M0 for w in W])]).astype(np.int64)
+        rows_p, rows_o = [], []
+        for i, w in enumerate(W):
+            keep = np.nonzero(w.obs_src < w.M0)[0]
+            rows_p.append(self.win_obs0[i] + keep)
+            rows_o.append(orig0[i] + w.obs_src[keep])
+        self.rows_packed = torch.from_numpy(cat(rows_p).astype(np.int64)).to(self.dev)
+        self.rows_orig = torch.from_numpy(cat(rows_o).astype(np.int64)).to(self.dev)
+        self.n_orig_obs = int(orig0[-1])
         b = _lib.XbaBatch()
         b.nwin = nw
         i32 = lambda a: a.ctypes.data_as(_lib.i32p)
@@ -332,6 +349,42 @@ class BatchSolver:
         _lib.check(self.L.xba_plan_info(self.ctx, out.ctypes.data), "xba_plan_info")
         return dict(zip(("chunks", "grid", "slot_tile", "lin_lds", "solve_lds", "partH", "partb", "max_n"),
                         out.tolist()))
+
+    def set_measurements(self, z=None, w=None):
+        """Overwrite the observed pixels z (M, 2) and/or the per-observation weights w (M,), given in the
+        concatenated original observation order of ``problems`` (device tensors or arrays)."""
+        t = self.torch
+        if z is not None:
+            z = t.as_tensor(z, dtype=t.float64, device=self.dev).reshape(-1, 2)
+            self.d["ob_z"].view(-1, 2)[self.rows_packed] = z[self.rows_orig]
+        if w is not None:
+            if "ob_w" not in self.d:
+                raise _lib.XbaError("this solver was built without per-observation weights (with_weights=True)")
+            w = t.as_tensor(w, dtype=t.float64, device=self.dev).reshape(-1)
+            self.d["ob_w"][self.rows_packed] = w[self.rows_orig]
+
+    def gn_step(self):
+        """The Gauss-Newton step of every window at the current state (lambda = 0), FP64 factor kept for
+        ``adjoint``.  Returns the concatenated step (sum n,) as a device tensor."""
+        t = self.torch
+        dx = t.empty(max(self.n_state, 1), dtype=t.float64, device=self.dev)
+        _lib.check(self.L.xba_gn_step(self.ctx, dx.data_ptr(), self.stream), "xba_gn_step")
+        return dx[:self.n_state]
+
+    def adjoint(self, g):
+        """dL/dz (M, 2) and dL/dw (M,) in the original observation order, for g = dL/d(dx) of the last
+        ``gn_step``."""
+        t = self.torch
+        g = t.as_tensor(g, dtype=t.float64, device=self.dev).reshape(-1).contiguous()
+        gz = t.empty((max(self.nobs, 1), 2), dtype=t.float64, device=self.dev)
+        gw = t.empty(max(self.nobs, 1), dtype=t.float64, device=self.dev)
+        _lib.check(self.L.xba_backward(self.ctx, g.data_ptr(), gz.data_ptr(), gw.data_ptr(), self.stream),
+                   "xba_backward")
+        oz = t.zeros((self.n_orig_obs, 2), dtype=t.float64, device=self.dev)
+        ow = t.zeros(self.n_orig_obs, dtype=t.float64, device=self.dev)
+        oz[self.rows_orig] = gz[self.rows_packed]
+        ow[self.rows_orig] = gw[self.rows_packed]
+        return oz, ow
 
     def solve(self, iters=5, lam0=1e-4, write_back=True):
         self.upload(lam0)
