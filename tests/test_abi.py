@@ -34,10 +34,15 @@ def test_strerror_and_version_need_no_gpu():
     assert L.xba_strerror(-3).decode().startswith("a landmark has more")
 
 
-def test_packing_sorts_landmarks_by_host_and_round_trips():
+def test_packing_sorts_landmarks_by_slot_range_and_round_trips():
     pb, _ = synth.make_window(P=7, L=300, views=(1, 7), seed=[4, 4], posefree=True, stereo="prior")
     w = _Packed(pb)
-    assert np.all(np.diff(w.lm_host) >= 0)
+    owner = np.repeat(np.arange(w.L), np.diff(w.ob_start))
+    lo, hi = w.lm_host.astype(np.int64).copy(), w.lm_host.astype(np.int64).copy()
+    np.minimum.at(lo, owner, w.ob_pose)
+    np.maximum.at(hi, owner, w.ob_pose)
+    key = lo * 1000 + hi              # narrow landmarks only here (P = 7): ordered by (first slot, last slot)
+    assert np.all(np.diff(key) >= 0)
     assert w.L == pb.L + len(pb.pf.host) and w.M == pb.M + len(pb.pf.host)
     lm_all = np.concatenate([pb.lmp, pb.pf.lmp])
     np.testing.assert_array_equal(w.unpermute_lm(w.lmp), lm_all)

@@ -473,6 +473,14 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     int Rmax = 1, Rn = 1, Rw = 1;
     // the fast family keeps landmarks that couple more slots than the narrow tile in separate (wide) work items
     const int narrow = c->fast ? std::min(tile, opt->slot_tile > 0 ? opt->slot_tile : 16) : tile;
+    // wide work items run one wavefront per workgroup (their tile fills the LDS): keep them short so that they
+    // spread over the CUs
+    int lpc_wide = lpc;
+    if (c->fast && opt->lm_per_chunk <= 0) {
+        int64_t nwl = 0;
+        for (int l = 0; l < c->nlm; l++) nwl += (lhi[l] - llo[l] + 1 > narrow) ? 1 : 0;
+        lpc_wide = (int)std::max<int64_t>(16, std::min<int64_t>(lpc, nwl / std::max(ncu, 1)));
+    }
     auto emit = [&](std::vector<Chunk> &dst, int w, int cs, int ce, int lo, int hi) {
         Chunk ch;
         memset(&ch, 0, sizeof(ch));
@@ -494,7 +502,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             const int lim = isw ? tile_cap : narrow;
             int cs = l, lo = llo[l], hi = lhi[l];
             l++;
-            while (l < lend && l - cs < lpc && ((lhi[l] - llo[l] + 1 > narrow) == isw)) {
+            const int cap = isw ? lpc_wide : lpc;
+            while (l < lend && l - cs < cap && ((lhi[l] - llo[l] + 1 > narrow) == isw)) {
                 int nlo = std::min(lo, llo[l]), nhi = std::max(hi, lhi[l]);
                 if (nhi - nlo + 1 > lim) break;
                 lo = nlo;
@@ -534,14 +543,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             ch.p0 = (int)pass_l0.size();
             int l = ch.l0;
             while (l < ch.l1) {
-                int nl = 0, S = 0, K = 0;
+                int nl = 0, S = 0, K = 0, PP = 0;
                 pass_l0.push_back(l);
                 while (l + nl < ch.l1 && nl < FB_LM) {
                     const int n = B.ob_start[l + nl + 1] - B.ob_start[l + nl];
                     const int k = std::min(n + 1, ch.R);
-                    if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP)) break;
+                    if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP || PP + k * (k + 1) / 2 > FB_PAIRS)) break;
                     S += n;
                     K += k;
+                    PP += k * (k + 1) / 2;
                     nl++;
                     cost[ci] += (double)(n + 1) * (n + 1) + 4.0 * n + 8.0;
                 }

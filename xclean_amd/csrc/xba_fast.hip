@@ -19,6 +19,8 @@ namespace xba {
 #define FB_HLP 64   // sum over a pass's landmarks of their touched slots
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
 #define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, masked-in flag
+#define FB_PAIRS 512  // sum over a pass's landmarks of k (k + 1) / 2 (their touched slot pairs)
+#define NFF 27        // fields of the compact observation record (lin_observations<..., true>)
 enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
@@ -35,7 +37,7 @@ struct FastLayout {
     int sc, sc64, wc, wd, tile, btile, ctl, wave, wave_stride, total;
     // offsets inside one wavefront's region
     int o_oblm, o_obt, o_obact, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
-        o_lrow;
+        o_lrow, o_lh, o_kown, o_ptab;
 };
 
 // rts: bytes per record entry (the observation record's storage type); everything else is FP64.
@@ -66,23 +68,13 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + FB_LM * FB_LREC * ts);
     L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
-    L.o_lrow = w;  w = align16(w + (trial ? 0 : NF * FB_OBS * rts));  // per observation: the whitened record
+    L.o_lrow = w;  w = align16(w + (trial ? 0 : NFF * FB_OBS * rts));  // per observation: the whitened record
+    L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
+    L.o_kown = w;  w = align16(w + (trial ? 0 : FB_HLP));                  // per touched-slot entry: its landmark
+    L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: landmark, si, sj
     L.wave_stride = w;
     L.total = o + nw * w;
     return L;
-}
-
-// Largest b in [0, B) with off[b] <= o (off non-decreasing, off[0] == 0).
-__device__ inline int owner_of(const int *off, int B, int o) {
-    int lo = 0, hi = B - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= o)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
 }
 
 __device__ inline int wave_incl_scan(int v, int lane) {
@@ -152,6 +144,15 @@ __device__ inline int next_pass(int *ctl, int lane) {
 }
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
+// Row r, column j of an observation's whitened Jacobian with respect to the host (host = true) or the target pose
+// block, read from the compact record: the host block's translation columns are the negated target ones.
+template <typename RT>
+__device__ inline double jpose(const RT *rec, bool host, int r, int j, int q) {
+    const int f = (host && j < 3) ? F_JH + 3 * r + j : F_JT + 6 * r + j;
+    const double v = (double)REC(f, q);
+    return (host && j >= 3) ? -v : v;
+}
+
 // RT is the type the per-observation record (whitened residual, weight and Jacobians) is stored in; every sum and
 // product is formed in FP64.  CLS names the work-item class the launch covers (0: narrow chunks, 1: wide chunks);
 // the code is the same, the two classes get separate symbols so that profiles report them apart.
@@ -174,6 +175,9 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     int *slots = (int *)(wb + FL.o_slots);
     T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
     RT *rec = (RT *)(wb + FL.o_lrow);
+    int *lh = (int *)(wb + FL.o_lh);
+    uint8_t *kown = (uint8_t *)(wb + FL.o_kown);
+    uint16_t *ptab = (uint16_t *)(wb + FL.o_ptab);
     int *ctl = (int *)(smem + FL.ctl);
 
     for (;;) {
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 const int li = A.lm_info[l];
                 const int h = (li & 0xfff) - ch.lo;
                 if ((li >> 12) & 1) {
-                    lin_observations<T, RT>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
+                    lin_observations<T, RT, true>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
                                         A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1], A.lmp[3 * (size_t)l + 2],
                                         lane, true);
                 } else {
@@ -226,6 +230,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 T *lr = lrec + lane * FB_LREC;
                 lr[LR_OK] = 0.0;
                 lr[LR_MASKED] = 0.0;
+                lh[lane] = (li & 0xfff) - ch.lo;
                 if ((li >> 12) & 1) {
                     const int h = (li & 0xfff) - ch.lo, q0 = lobs0[lane], n = lnobs[lane];
                     T s6[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -326,9 +331,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 int i = 0;
                 while (m) {
                     slots[koff + i] = __ffs(m) - 1;
+                    kown[koff + i] = (uint8_t)lane;
                     m &= m - 1u;
                     i++;
                 }
+                for (int si = 0, o = pof; si < kb; si++)
+                    for (int sj = 0; sj <= si; sj++, o++) ptab[o] = (uint16_t)((lane << 10) | (si << 5) | sj);
             }
             const double pcost = wave_sum(lcost);
             if (lane == 0) A.pass_cost[p] = pcost;
@@ -337,20 +345,20 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             //      Hlp[s][:, j] = sum_q w_q Jl_q^T J_{q,s}[:, j]  (J_{q,s} = Jt if t_q == s != h, Jh if s == h)
             for (int o = lane; o < ktot * 6; o += XBA_WAVE) {
                 const int kk = o / 6, j = o % 6;
-                const int bb = owner_of(loff, B, kk);
+                const int bb = kown[kk];
                 const T *lr = lrec + bb * FB_LREC;
-                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo, s = slots[kk];
+                const int h = lh[bb], s = slots[kk];
                 const int q0 = lobs0[bb], n = lnobs[bb];
                 T a0 = 0, a1 = 0, a2 = 0;
                 for (int q = q0; q < q0 + n; q++) {
                     const T w = REC(F_W, q);
                     const int t = rt[q];
                     if (w == 0.0 || t == h) continue;
-                    int f;
-                    if (s == h) f = F_JH;
-                    else if (t == s) f = F_JT;
+                    bool hb;
+                    if (s == h) hb = true;
+                    else if (t == s) hb = false;
                     else continue;
-                    const T y0 = REC(f + j, q), y1 = REC(f + 6 + j, q);
+                    const T y0 = jpose(rec, hb, 0, j, q), y1 = jpose(rec, hb, 1, j, q);
                     a0 += w * ((T)REC(F_JL, q) * y0 + (T)REC(F_JL + 3, q) * y1);
                     a1 += w * ((T)REC(F_JL + 1, q) * y0 + (T)REC(F_JL + 4, q) * y1);
                     a2 += w * ((T)REC(F_JL + 2, q) * y0 + (T)REC(F_JL + 5, q) * y1);
@@ -361,90 +369,87 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 U[12 + j] = lr[LR_L + 3] * a0 + lr[LR_L + 4] * a1 + lr[LR_L + 5] * a2;
             }
             WAVE_SYNC();
-            // ---- reduced system: one lane per (landmark, touched block pair), the 6x6 block in registers:
-            //      direct pose-pose terms of the observations that touch it minus U2^T U1, then 21 or 36 atomics
+            // ---- reduced system: one lane per (landmark, touched block pair), the 6x6 block in two halves of
+            //      three rows held in registers: direct pose-pose terms of the observations that touch it minus
+            //      U2^T U1, then the half's atomics (lower triangle only on the diagonal blocks)
             for (int o = lane; o < ptot; o += XBA_WAVE) {
-                const int bb = owner_of(poff, B, o);
+                const int pk = ptab[o];
+                const int bb = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
                 const T *lr = lrec + bb * FB_LREC;
-                const int p = o - poff[bb];
-                int si = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
-                while ((si + 1) * (si + 2) / 2 <= p) si++;
-                while (si * (si + 1) / 2 > p) si--;
-                const int sj = p - si * (si + 1) / 2;
                 const int base = loff[bb];
                 const int s2 = slots[base + si], s1 = slots[base + sj];
-                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo;
-                T acc[36];
-                if (lr[LR_OK] != 0.0) {
-                    const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
-                    T a2[18], a1[18];
-                    for (int e = 0; e < 18; e++) {
-                        a2[e] = U2[e];
-                        a1[e] = U1[e];
-                    }
-                    for (int i = 0; i < 6; i++)
-                        for (int j = 0; j < 6; j++)
-                            acc[6 * i + j] = -(a2[i] * a1[j] + a2[6 + i] * a1[6 + j] + a2[12 + i] * a1[12 + j]);
-                } else {
-                    for (int e = 0; e < 36; e++) acc[e] = (T)0;
-                }
-                if (s2 == s1 || s1 == h || s2 == h) {
-                    const int q0 = lobs0[bb], n = lnobs[bb];
-                    for (int q = q0; q < q0 + n; q++) {
-                        const T w = REC(F_W, q);
-                        const int t = rt[q];
-                        if (w == 0.0 || t == h) continue;
-                        int fx, fy;
-                        if (s2 == s1) {
-                            if (s2 == h) fx = fy = F_JH;
-                            else if (t == s2) fx = fy = F_JT;
-                            else continue;
-                        } else if (s1 == h) {
-                            if (t != s2) continue;
-                            fx = F_JT;
-                            fy = F_JH;
-                        } else {
-                            if (t != s1) continue;
-                            fx = F_JH;
-                            fy = F_JT;
-                        }
-                        T x0[6], x1[6], y0[6], y1[6];
-                        for (int e = 0; e < 6; e++) {
-                            x0[e] = w * REC(fx + e, q);
-                            x1[e] = w * REC(fx + 6 + e, q);
-                            y0[e] = REC(fy + e, q);
-                            y1[e] = REC(fy + 6 + e, q);
-                        }
-                        for (int i = 0; i < 6; i++)
-                            for (int j = 0; j < 6; j++) acc[6 * i + j] += x0[i] * y0[j] + x1[i] * y1[j];
-                    }
-                }
+                const int h = lh[bb];
+                const bool schur = lr[LR_OK] != 0.0;
+                const bool direct = s2 == s1 || s1 == h || s2 == h;
+                const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
+                T a1[18];
+                for (int e = 0; e < 18; e++) a1[e] = schur ? U1[e] : 0.0;
                 double *dst = tile + (s2 * (s2 + 1) / 2 + s1) * TBLK;
-                if (si == sj) {
-                    for (int i = 0; i < 6; i++)
-                        for (int j = 0; j <= i; j++) lds_add<double>(dst + 6 * i + j, (double)acc[6 * i + j]);
-                } else {
-                    for (int e = 0; e < 36; e++) lds_add<double>(dst + e, (double)acc[e]);
+                for (int hh = 0; hh < 2; hh++) {
+                    T acc[18];
+                    for (int ii = 0; ii < 3; ii++) {
+                        const int i = 3 * hh + ii;
+                        const T c0 = schur ? U2[i] : 0.0, c1 = schur ? U2[6 + i] : 0.0, c2 = schur ? U2[12 + i] : 0.0;
+                        for (int jj = 0; jj < 6; jj++)
+                            acc[6 * ii + jj] = -(c0 * a1[jj] + c1 * a1[6 + jj] + c2 * a1[12 + jj]);
+                    }
+                    if (direct) {
+                        const int q0 = lobs0[bb], n = lnobs[bb];
+                        for (int q = q0; q < q0 + n; q++) {
+                            const T w = REC(F_W, q);
+                            const int t = rt[q];
+                            if (w == 0.0 || t == h) continue;
+                            bool hx, hy;
+                            if (s2 == s1) {
+                                if (s2 == h) hx = hy = true;
+                                else if (t == s2) hx = hy = false;
+                                else continue;
+                            } else if (s1 == h) {
+                                if (t != s2) continue;
+                                hx = false;
+                                hy = true;
+                            } else {
+                                if (t != s1) continue;
+                                hx = true;
+                                hy = false;
+                            }
+                            T y0[6], y1[6];
+                            for (int e = 0; e < 6; e++) {
+                                y0[e] = jpose(rec, hy, 0, e, q);
+                                y1[e] = jpose(rec, hy, 1, e, q);
+                            }
+                            for (int ii = 0; ii < 3; ii++) {
+                                const T x0 = w * jpose(rec, hx, 0, 3 * hh + ii, q);
+                                const T x1 = w * jpose(rec, hx, 1, 3 * hh + ii, q);
+                                for (int jj = 0; jj < 6; jj++) acc[6 * ii + jj] += x0 * y0[jj] + x1 * y1[jj];
+                            }
+                        }
+                    }
+                    for (int ii = 0; ii < 3; ii++) {
+                        const int i = 3 * hh + ii;
+                        for (int jj = 0; jj < 6; jj++)
+                            if (si != sj || jj <= i) lds_add<double>(dst + 6 * i + jj, acc[6 * ii + jj]);
+                    }
                 }
             }
             // ---- gradient: one lane per (landmark, touched slot), 6 entries
             for (int o = lane; o < ktot; o += XBA_WAVE) {
-                const int bb = owner_of(loff, B, o);
+                const int bb = kown[o];
                 const T *lr = lrec + bb * FB_LREC;
                 const int s = slots[o];
-                const int l = lb + bb, h = (A.lm_info[l] & 0xfff) - ch.lo;
+                const int h = lh[bb];
                 const int q0 = lobs0[bb], n = lnobs[bb];
                 T acc[6] = {0, 0, 0, 0, 0, 0};
                 for (int q = q0; q < q0 + n; q++) {
                     const T w = REC(F_W, q);
                     const int t = rt[q];
                     if (w == 0.0 || t == h) continue;
-                    int f;
-                    if (s == h) f = F_JH;
-                    else if (t == s) f = F_JT;
+                    bool hb;
+                    if (s == h) hb = true;
+                    else if (t == s) hb = false;
                     else continue;
                     const T e0 = w * (T)REC(F_E, q), e1 = w * (T)REC(F_E + 1, q);
-                    for (int i = 0; i < 6; i++) acc[i] += REC(f + i, q) * e0 + REC(f + 6 + i, q) * e1;
+                    for (int i = 0; i < 6; i++) acc[i] += jpose(rec, hb, 0, i, q) * e0 + jpose(rec, hb, 1, i, q) * e1;
                 }
                 if (lr[LR_OK] != 0.0) {
                     const T *U = hlp + o * 18;

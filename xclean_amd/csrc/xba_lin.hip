@@ -231,7 +231,9 @@ __device__ inline void tile_add(T *acc, T *cmp, int i, T v) {
 // ----------------------------------------------------------------- one landmark: observations into LDS --
 // Lanes over the landmark's observations: whitened residual (FP64), Huber weight and cost (FP64), whitened
 // Jacobians (computed in T), stored per lane in the wave's record in the record type RT.
-template <typename T, typename RT = T>
+// COMPACT (the multi-wave kernels' record): no residual-dimension field, and of the host block only its rotation
+// columns (its translation columns are the negated target ones, see obs_eval), at F_JH + 3 r + j.
+template <typename T, typename RT = T, bool COMPACT = false>
 __device__ inline void lin_observations(const KArgs<T> &A, RT *rec, double *rc, int *rt, const double *sc64,
                                         const T *sc, const T *wc, const double *wd, int o0, int no, int lo, int h,
                                         double a, double b, double rho, int lane, bool jac) {
@@ -274,7 +276,7 @@ __device__ inline void lin_observations(const KArgs<T> &A, RT *rec, double *rc, 
         if (A.chi2_out) A.chi2_out[k] = act ? en2 : (double)INFINITY;
         rec[F_W * XBA_WAVE + lane] = act ? (RT)w : (RT)0;
         rc[lane] = act ? c : 0.0;
-        rec[F_ND * XBA_WAVE + lane] = (RT)nd;
+        if (!COMPACT) rec[F_ND * XBA_WAVE + lane] = (RT)nd;
         rt[lane] = t;
         if (act) {
             rec[(F_E + 0) * XBA_WAVE + lane] = (RT)e[0];
@@ -285,7 +287,10 @@ __device__ inline void lin_observations(const KArgs<T> &A, RT *rec, double *rc, 
                     for (int j = 0; j < 3; j++) rec[(F_JL + 3 * r + j) * XBA_WAVE + lane] = on ? (RT)Jl[r][j] : (RT)0;
                     for (int j = 0; j < 6; j++) {
                         rec[(F_JT + 6 * r + j) * XBA_WAVE + lane] = on ? (RT)Jt[r][j] : (RT)0;
-                        rec[(F_JH + 6 * r + j) * XBA_WAVE + lane] = on ? (RT)Jh[r][j] : (RT)0;
+                        if (!COMPACT)
+                            rec[(F_JH + 6 * r + j) * XBA_WAVE + lane] = on ? (RT)Jh[r][j] : (RT)0;
+                        else if (j < 3)
+                            rec[(F_JH + 3 * r + j) * XBA_WAVE + lane] = on ? (RT)Jh[r][j] : (RT)0;
                     }
                 }
         }

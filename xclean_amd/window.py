@@ -69,8 +69,9 @@ class _Packed:
             start = np.concatenate([[0], np.cumsum(cnt)])
         self.P, self.L0, self.M0, self.npf = P, L0, M0, npf
         self.L, self.M = L0 + npf, M0 + npf
-        # landmark order: those coupling at most NARROW_SLOTS consecutive slots first, then by host slot (the
-        # solver's work items are contiguous landmark ranges, so this keeps them short in slots)
+        # landmark order: those coupling at most NARROW_SLOTS consecutive slots first, then by the first and last
+        # slot they couple (the solver's work items are contiguous landmark ranges inside a slot tile, so this
+        # keeps neighbouring landmarks in the same tile and the work items long)
         owner = np.repeat(np.arange(self.L), cnt)
         lo = host.copy()
         hi = host.copy()
@@ -78,7 +79,7 @@ class _Packed:
             np.minimum.at(lo, owner, ob_pose)
             np.maximum.at(hi, owner, ob_pose)
         wide = (hi - lo + 1) > NARROW_SLOTS
-        perm = np.lexsort((host, wide))
+        perm = np.lexsort((hi, lo, wide))
         self.perm = perm
         cnt_s = cnt[perm]
         st_s = np.zeros(self.L + 1, np.int64)
