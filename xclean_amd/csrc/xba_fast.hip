@@ -213,6 +213,37 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     lin_observations<T, RT, true>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
                                         A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1], A.lmp[3 * (size_t)l + 2],
                                         lane, true);
+                    // the observation's own pose-pose blocks (t,t), (h,h), (t,h) and gradient entries, straight
+                    // into the tile (they need no landmark elimination)
+                    const double w = (double)REC(F_W, lane);
+                    const int t = rt[lane];
+                    if (w != 0.0 && t != h) {
+                        double xt[2][6], xh[2][6];
+                        for (int r = 0; r < 2; r++)
+                            for (int j = 0; j < 6; j++) {
+                                xt[r][j] = jpose(rec, false, r, j, lane);
+                                xh[r][j] = jpose(rec, true, r, j, lane);
+                            }
+                        const double e0 = w * (double)REC(F_E, lane), e1 = w * (double)REC(F_E + 1, lane);
+                        for (int j = 0; j < 6; j++) {
+                            lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
+                            lds_add<double>(btile + 6 * h + j, xh[0][j] * e0 + xh[1][j] * e1);
+                        }
+                        double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK, *dhh = tile + (h * (h + 1) / 2 + h) * TBLK;
+                        for (int i = 0; i < 6; i++)
+                            for (int j = 0; j <= i; j++) {
+                                lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]));
+                                lds_add<double>(dhh + 6 * i + j, w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]));
+                            }
+                        const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
+                        double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
+                        for (int i = 0; i < 6; i++)
+                            for (int j = 0; j < 6; j++) {
+                                const double v = tlo ? xh[0][i] * xt[0][j] + xh[1][i] * xt[1][j]
+                                                     : xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j];
+                                lds_add<double>(dth + 6 * i + j, w * v);
+                            }
+                    }
                 } else {
                     REC(F_W, lane) = (RT)0;
                     rcq[lane] = 0.0;
@@ -341,89 +372,60 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             const double pcost = wave_sum(lcost);
             if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
-            // ---- U = L^-1 Hlp, one lane per column of a (landmark, slot) block:
-            //      Hlp[s][:, j] = sum_q w_q Jl_q^T J_{q,s}[:, j]  (J_{q,s} = Jt if t_q == s != h, Jh if s == h)
-            for (int o = lane; o < ktot * 6; o += XBA_WAVE) {
-                const int kk = o / 6, j = o % 6;
-                const int bb = kown[kk];
+            // ---- U = L^-1 Hlp, one lane per observation: its two 3x6 terms G Jt and G Jh (G = w L^-1 Jl^T)
+            //      added into U of its target slot and of its host slot (the slot's entry is its rank in the
+            //      landmark's touched-slot mask)
+            for (int e = lane; e < ktot * 18; e += XBA_WAVE) hlp[e] = 0.0;
+            WAVE_SYNC();
+            if (lane < nob) {
+                const int bb = ob_lm[lane];
                 const T *lr = lrec + bb * FB_LREC;
-                const int h = lh[bb], s = slots[kk];
-                const int q0 = lobs0[bb], n = lnobs[bb];
-                T a0 = 0, a1 = 0, a2 = 0;
-                for (int q = q0; q < q0 + n; q++) {
-                    const T w = REC(F_W, q);
-                    const int t = rt[q];
-                    if (w == 0.0 || t == h) continue;
-                    bool hb;
-                    if (s == h) hb = true;
-                    else if (t == s) hb = false;
-                    else continue;
-                    const T y0 = jpose(rec, hb, 0, j, q), y1 = jpose(rec, hb, 1, j, q);
-                    a0 += w * ((T)REC(F_JL, q) * y0 + (T)REC(F_JL + 3, q) * y1);
-                    a1 += w * ((T)REC(F_JL + 1, q) * y0 + (T)REC(F_JL + 4, q) * y1);
-                    a2 += w * ((T)REC(F_JL + 2, q) * y0 + (T)REC(F_JL + 5, q) * y1);
+                const T w = REC(F_W, lane);
+                const int t = rt[lane], h = lh[bb];
+                if (lr[LR_OK] != 0.0 && w != 0.0 && t != h) {
+                    const unsigned m = lmask[bb];
+                    T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
+                    T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
+                    T G[3][2];
+                    for (int r = 0; r < 2; r++) {
+                        const T j0 = REC(F_JL + 3 * r, lane), j1 = REC(F_JL + 3 * r + 1, lane),
+                                j2 = REC(F_JL + 3 * r + 2, lane);
+                        G[0][r] = w * (lr[LR_L] * j0);
+                        G[1][r] = w * (lr[LR_L + 1] * j0 + lr[LR_L + 2] * j1);
+                        G[2][r] = w * (lr[LR_L + 3] * j0 + lr[LR_L + 4] * j1 + lr[LR_L + 5] * j2);
+                    }
+                    for (int j = 0; j < 6; j++) {
+                        const T t0 = jpose(rec, false, 0, j, lane), t1 = jpose(rec, false, 1, j, lane);
+                        const T h0 = jpose(rec, true, 0, j, lane), h1 = jpose(rec, true, 1, j, lane);
+                        for (int c = 0; c < 3; c++) {
+                            lds_add<double>(Ut + 6 * c + j, G[c][0] * t0 + G[c][1] * t1);
+                            lds_add<double>(Uh + 6 * c + j, G[c][0] * h0 + G[c][1] * h1);
+                        }
+                    }
                 }
-                T *U = hlp + kk * 18;
-                U[j] = lr[LR_L] * a0;
-                U[6 + j] = lr[LR_L + 1] * a0 + lr[LR_L + 2] * a1;
-                U[12 + j] = lr[LR_L + 3] * a0 + lr[LR_L + 4] * a1 + lr[LR_L + 5] * a2;
             }
             WAVE_SYNC();
-            // ---- reduced system: one lane per (landmark, touched block pair), the 6x6 block in two halves of
-            //      three rows held in registers: direct pose-pose terms of the observations that touch it minus
-            //      U2^T U1, then the half's atomics (lower triangle only on the diagonal blocks)
+            // ---- reduced system: one lane per (landmark, touched block pair), -U2^T U1 in two halves of three
+            //      rows held in registers, then the half's atomics (lower triangle only on the diagonal blocks)
             for (int o = lane; o < ptot; o += XBA_WAVE) {
                 const int pk = ptab[o];
                 const int bb = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
                 const T *lr = lrec + bb * FB_LREC;
                 const int base = loff[bb];
                 const int s2 = slots[base + si], s1 = slots[base + sj];
-                const int h = lh[bb];
                 const bool schur = lr[LR_OK] != 0.0;
-                const bool direct = s2 == s1 || s1 == h || s2 == h;
+                if (!schur) continue;
                 const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
                 T a1[18];
-                for (int e = 0; e < 18; e++) a1[e] = schur ? U1[e] : 0.0;
+                for (int e = 0; e < 18; e++) a1[e] = U1[e];
                 double *dst = tile + (s2 * (s2 + 1) / 2 + s1) * TBLK;
                 for (int hh = 0; hh < 2; hh++) {
                     T acc[18];
                     for (int ii = 0; ii < 3; ii++) {
                         const int i = 3 * hh + ii;
-                        const T c0 = schur ? U2[i] : 0.0, c1 = schur ? U2[6 + i] : 0.0, c2 = schur ? U2[12 + i] : 0.0;
+                        const T c0 = U2[i], c1 = U2[6 + i], c2 = U2[12 + i];
                         for (int jj = 0; jj < 6; jj++)
                             acc[6 * ii + jj] = -(c0 * a1[jj] + c1 * a1[6 + jj] + c2 * a1[12 + jj]);
-                    }
-                    if (direct) {
-                        const int q0 = lobs0[bb], n = lnobs[bb];
-                        for (int q = q0; q < q0 + n; q++) {
-                            const T w = REC(F_W, q);
-                            const int t = rt[q];
-                            if (w == 0.0 || t == h) continue;
-                            bool hx, hy;
-                            if (s2 == s1) {
-                                if (s2 == h) hx = hy = true;
-                                else if (t == s2) hx = hy = false;
-                                else continue;
-                            } else if (s1 == h) {
-                                if (t != s2) continue;
-                                hx = false;
-                                hy = true;
-                            } else {
-                                if (t != s1) continue;
-                                hx = true;
-                                hy = false;
-                            }
-                            T y0[6], y1[6];
-                            for (int e = 0; e < 6; e++) {
-                                y0[e] = jpose(rec, hy, 0, e, q);
-                                y1[e] = jpose(rec, hy, 1, e, q);
-                            }
-                            for (int ii = 0; ii < 3; ii++) {
-                                const T x0 = w * jpose(rec, hx, 0, 3 * hh + ii, q);
-                                const T x1 = w * jpose(rec, hx, 1, 3 * hh + ii, q);
-                                for (int jj = 0; jj < 6; jj++) acc[6 * ii + jj] += x0 * y0[jj] + x1 * y1[jj];
-                            }
-                        }
                     }
                     for (int ii = 0; ii < 3; ii++) {
                         const int i = 3 * hh + ii;
@@ -432,30 +434,16 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     }
                 }
             }
-            // ---- gradient: one lane per (landmark, touched slot), 6 entries
+            // ---- gradient: one lane per (landmark, touched slot), its Schur part -U^T g (6 entries)
             for (int o = lane; o < ktot; o += XBA_WAVE) {
                 const int bb = kown[o];
                 const T *lr = lrec + bb * FB_LREC;
+                if (lr[LR_OK] == 0.0) continue;
                 const int s = slots[o];
-                const int h = lh[bb];
-                const int q0 = lobs0[bb], n = lnobs[bb];
-                T acc[6] = {0, 0, 0, 0, 0, 0};
-                for (int q = q0; q < q0 + n; q++) {
-                    const T w = REC(F_W, q);
-                    const int t = rt[q];
-                    if (w == 0.0 || t == h) continue;
-                    bool hb;
-                    if (s == h) hb = true;
-                    else if (t == s) hb = false;
-                    else continue;
-                    const T e0 = w * (T)REC(F_E, q), e1 = w * (T)REC(F_E + 1, q);
-                    for (int i = 0; i < 6; i++) acc[i] += jpose(rec, hb, 0, i, q) * e0 + jpose(rec, hb, 1, i, q) * e1;
-                }
-                if (lr[LR_OK] != 0.0) {
-                    const T *U = hlp + o * 18;
-                    for (int i = 0; i < 6; i++)
-                        acc[i] -= U[i] * lr[LR_G] + U[6 + i] * lr[LR_G + 1] + U[12 + i] * lr[LR_G + 2];
-                }
+                const T *U = hlp + o * 18;
+                T acc[6];
+                for (int i = 0; i < 6; i++)
+                    acc[i] = -(U[i] * lr[LR_G] + U[6 + i] * lr[LR_G + 1] + U[12 + i] * lr[LR_G + 2]);
                 for (int i = 0; i < 6; i++) lds_add<double>(btile + 6 * s + i, (double)acc[i]);
             }
             WAVE_SYNC();
