@@ -476,51 +476,90 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
 }
 
 // ---------------------------------------------------------------------------------------- dense Cholesky --
-// In-place lower Cholesky of the n x n row-major A by one workgroup (right-looking, one column per step).
-// Returns false (uniformly) on a non-positive or non-finite pivot.
+// In-place lower Cholesky of the n x n row-major A by one workgroup (a multiple of 64 threads): per column the
+// pivot test (read by every thread, so the exit is uniform), the column scale, then the trailing update over a
+// 16 x 16 thread grid.  Returns false on a non-positive or non-finite pivot.
 template <typename T>
 __device__ bool chol_wg(T *A, int n, int *sflag) {
+    (void)sflag;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int r = tid >> 4, c = tid & 15, nr = nt >> 4;
     for (int j = 0; j < n; j++) {
-        if (tid == 0) {
-            T d = A[(size_t)j * n + j];
-            bool ok = d > (T)0 && isfinite(d);
-            A[(size_t)j * n + j] = ok ? sqrt(d) : (T)1;
-            *sflag = ok ? 1 : 0;
-        }
+        const T d = A[(size_t)j * n + j];
+        if (!(d > (T)0 && isfinite(d))) return false;
+        const T sd = sqrt(d), s = (T)1 / sd;
+        for (int i = j + 1 + tid; i < n; i += nt) A[(size_t)i * n + j] *= s;
         __syncthreads();
-        if (!*sflag) return false;
-        const T djj = A[(size_t)j * n + j];
-        for (int i = j + 1 + tid; i < n; i += nt) A[(size_t)i * n + j] /= djj;
-        __syncthreads();
-        const int m = n - j - 1;
-        for (int e = tid; e < m * m; e += nt) {
-            int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) A[(size_t)i * n + k] -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
+        if (tid == 0) A[(size_t)j * n + j] = sd;
+        for (int i = j + 1 + r; i < n; i += nr) {
+            T *Ai = A + (size_t)i * n;
+            const T aij = Ai[j];
+            for (int k = j + 1 + c; k <= i; k += 16) Ai[k] -= aij * A[(size_t)k * n + j];
         }
         __syncthreads();
     }
     return true;
 }
 
-// x <- (L L^T)^-1 x by forward and back substitution (column-oriented).
+// x <- (L L^T)^-1 x: forward then back substitution in blocks of 32 columns; the triangle of each block is solved
+// by the first wavefront with the values in registers, the rest of the vector updated by every thread.
 template <typename T>
 __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int j = 0; j < n; j++) {
-        if (tid == 0) x[j] /= Lm[(size_t)j * n + j];
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int BS = 32;
+    for (int j0 = 0; j0 < n; j0 += BS) {
+        const int nb = min(BS, n - j0);
+        if (wv == 0) {
+            T xi = lane < nb ? x[j0 + lane] : (T)0;
+            for (int jj = 0; jj < nb; jj++) {
+                const T xj = __shfl(xi, jj, 64) / Lm[(size_t)(j0 + jj) * n + j0 + jj];
+                if (lane == jj)
+                    xi = xj;
+                else if (lane > jj && lane < nb)
+                    xi -= Lm[(size_t)(j0 + lane) * n + j0 + jj] * xj;
+            }
+            if (lane < nb) x[j0 + lane] = xi;
+        }
         __syncthreads();
-        const T xj = x[j];
-        for (int i = j + 1 + tid; i < n; i += nt) x[i] -= Lm[(size_t)i * n + j] * xj;
+        for (int i = j0 + nb + tid; i < n; i += nt) {
+            const T *Li = Lm + (size_t)i * n + j0;
+            T acc = x[i];
+            for (int jj = 0; jj < nb; jj++) acc -= Li[jj] * x[j0 + jj];
+            x[i] = acc;
+        }
         __syncthreads();
     }
-    for (int j = n - 1; j >= 0; j--) {
-        if (tid == 0) x[j] /= Lm[(size_t)j * n + j];
+    for (int j1 = n; j1 > 0; j1 -= BS) {
+        const int j0 = max(0, j1 - BS), nb = j1 - j0;
+        if (wv == 0) {
+            T xi = lane < nb ? x[j0 + lane] : (T)0;
+            for (int jj = nb - 1; jj >= 0; jj--) {
+                const T xj = __shfl(xi, jj, 64) / Lm[(size_t)(j0 + jj) * n + j0 + jj];
+                if (lane == jj)
+                    xi = xj;
+                else if (lane < jj)
+                    xi -= Lm[(size_t)(j0 + jj) * n + j0 + lane] * xj;
+            }
+            if (lane < nb) x[j0 + lane] = xi;
+        }
         __syncthreads();
-        const T xj = x[j];
-        for (int i = tid; i < j; i += nt) x[i] -= Lm[(size_t)j * n + i] * xj;
+        for (int i = tid; i < j0; i += nt) {
+            T acc = x[i];
+            for (int jj = 0; jj < nb; jj++) acc -= Lm[(size_t)(j0 + jj) * n + i] * x[j0 + jj];
+            x[i] = acc;
+        }
         __syncthreads();
     }
+}
+
+// Sum and max over a wavefront (butterfly), the result in every lane.
+__device__ inline double dwave_sum(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ inline double dwave_max(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+    return v;
 }
 
 // ------------------------------------------------------------------------------------------- k_solve --
@@ -532,6 +571,7 @@ __global__ __launch_bounds__(256) void k_solve(DArgs D, int lds_cap) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
     __shared__ T sy[XBA_MAX_STATE];
+    __shared__ double sred[32];
     __shared__ int sflag;
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
@@ -547,12 +587,12 @@ __global__ __launch_bounds__(256) void k_solve(DArgs D, int lds_cap) {
         ss[i] = (sizeof(T) == 8) ? 1.0 : (hd > 0.0 ? 1.0 / sqrt(hd) : 1.0);
     }
     __syncthreads();
-    for (int e = tid; e < n * n; e += nt) {
-        int i = e / n, j = e % n;
-        double h = H[e];
-        if (i == j) h += lam * h + 1e-9;
-        F[e] = (T)(h * ss[i] * ss[j]);
-    }
+    for (int i = tid >> 6; i < n; i += nt >> 6)
+        for (int j = tid & 63; j < n; j += 64) {
+            double h = H[(size_t)i * n + j];
+            if (i == j) h += lam * h + 1e-9;
+            F[(size_t)i * n + j] = (T)(h * ss[i] * ss[j]);
+        }
     __syncthreads();
     bool ok = chol_wg<T>(F, n, &sflag);
     int nref = 0;
@@ -571,35 +611,33 @@ __global__ __launch_bounds__(256) void k_solve(DArgs D, int lds_cap) {
             __syncthreads();
             nref = it + 1;
             if (it + 1 == maxit) break;
-            // residual r = b - Hd x in FP64
-            for (int i = tid; i < n; i += nt) {
-                double acc = bv[i];
+            // residual r = b - Hd x in FP64: one wavefront per row, lanes along the row
+            const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+            double rmax = 0.0, bmax = 0.0;
+            for (int i = wv; i < n; i += nw) {
                 const double *Hi = H + (size_t)i * n;
-                for (int j = 0; j < n; j++) acc -= Hi[j] * sx[j];
-                acc -= (lam * Hi[i] + 1e-9) * sx[i];
-                sr[i] = acc;
+                double acc = 0.0;
+                for (int j = lane; j < n; j += 64) acc += Hi[j] * sx[j];
+                acc = dwave_sum(acc);
+                const double ri = bv[i] - acc - (lam * Hi[i] + 1e-9) * sx[i];
+                if (lane == 0) sr[i] = ri;
+                rmax = fmax(rmax, fabs(ri));
+                bmax = fmax(bmax, fabs(bv[i]));
+            }
+            if (lane == 0) {
+                sred[2 * wv] = rmax;
+                sred[2 * wv + 1] = bmax;
             }
             __syncthreads();
-            if (tid == 0) {
-                double rn = 0.0, bn = 0.0;
-                for (int i = 0; i < n; i++) {
-                    rn = fmax(rn, fabs(sr[i]));
-                    bn = fmax(bn, fabs(bv[i]));
-                }
-                sflag = (rn <= 1e-15 * bn) ? 1 : 0;
+            double rn = 0.0, bn = 0.0;
+            for (int q = 0; q < nw; q++) {
+                rn = fmax(rn, sred[2 * q]);
+                bn = fmax(bn, sred[2 * q + 1]);
             }
             __syncthreads();
-            if (sflag) break;
+            if (rn <= 1e-15 * bn) break;
         }
         for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
-        // keep the factor for the adjoint when it lives in LDS
-        if (in_lds)
-            for (int e = tid; e < n * n; e += nt) {
-                if (sizeof(T) == 8)
-                    D.fact64[W.h0 + e] = (double)F[e];
-                else
-                    D.fact32[W.h0 + e] = (float)F[e];
-            }
         __syncthreads();
         // trial state (vio/window.py:522-532, 945-955)
         for (int s = tid; s < W.P; s += nt) {
