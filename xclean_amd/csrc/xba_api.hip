@@ -282,7 +282,7 @@ void launch_iter(xba_ctx *c, hipStream_t s) {
     KArgs<T> A = kargs<T>(c);
     launch_lin<T>(c, A, s);
     DArgs D = dargs(c);
-    hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
+    hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
     launch_trial_kernel<T>(c, A, s);
     hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
     hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
@@ -851,7 +851,7 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
         (void)hipEventRecord(ev[3], s);
         hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
         (void)hipEventRecord(ev[4], s);
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(256), c->solve_lds_cap, s, D, c->solve_lds_cap);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
         (void)hipEventRecord(ev[5], s);
         launch_trial_kernel<T>(c, A, s, 0);
         (void)hipEventRecord(ev[6], s);

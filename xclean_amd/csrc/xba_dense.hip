@@ -476,33 +476,90 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
 }
 
 // ---------------------------------------------------------------------------------------- dense Cholesky --
-// In-place lower Cholesky of the n x n row-major A by one workgroup (a multiple of 64 threads): per column the
-// pivot test (read by every thread, so the exit is uniform), the column scale, then the trailing update over a
-// 16 x 16 thread grid.  Returns false on a non-positive or non-finite pivot.
+// In-place lower Cholesky of the n x n row-major A by one workgroup (a multiple of 64 threads), right-looking in
+// panels of 32 columns: the panel's diagonal block factored by the first wavefront (wavefront-synchronous), the
+// rows below it solved against that block one row per thread, then the trailing matrix updated over a 16 x 16
+// thread grid.  Every thread reads each pivot, so a failure exits uniformly.  Returns false on a non-positive or
+// non-finite pivot.
+// (the matrix may live in LDS or in global memory: wait for both before the lanes read each other's writes)
+#define CHOL_WAVE_SYNC()                                       \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
+        __builtin_amdgcn_s_waitcnt(0);                         \
+        __builtin_amdgcn_wave_barrier();                       \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); \
+    } while (0)
+
 template <typename T>
 __device__ bool chol_wg(T *A, int n, int *sflag) {
-    (void)sflag;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     const int r = tid >> 4, c = tid & 15, nr = nt >> 4;
-    for (int j = 0; j < n; j++) {
-        const T d = A[(size_t)j * n + j];
-        if (!(d > (T)0 && isfinite(d))) return false;
-        const T sd = sqrt(d), s = (T)1 / sd;
-        for (int i = j + 1 + tid; i < n; i += nt) A[(size_t)i * n + j] *= s;
+    constexpr int BS = 32;
+    for (int j0 = 0; j0 < n; j0 += BS) {
+        const int nb = min(BS, n - j0), j1 = j0 + nb;
+        // diagonal block
+        if (wv == 0) {
+            int ok = 1;
+            for (int jj = j0; jj < j1; jj++) {
+                const T d = A[(size_t)jj * n + jj];
+                if (!(d > (T)0 && isfinite(d))) {
+                    ok = 0;
+                    break;
+                }
+                const T sd = sqrt(d), s = (T)1 / sd;
+                const int i = jj + 1 + lane;
+                if (i < j1) A[(size_t)i * n + jj] *= s;
+                CHOL_WAVE_SYNC();
+                if (lane == 0) A[(size_t)jj * n + jj] = sd;
+                for (int ii = jj + 1 + (lane >> 3); ii < j1; ii += 8) {
+                    T *Ai = A + (size_t)ii * n;
+                    const T aij = Ai[jj];
+                    for (int k = jj + 1 + (lane & 7); k <= ii; k += 8) Ai[k] -= aij * A[(size_t)k * n + jj];
+                }
+                CHOL_WAVE_SYNC();
+            }
+            if (lane == 0) *sflag = ok;
+        }
         __syncthreads();
-        if (tid == 0) A[(size_t)j * n + j] = sd;
-        for (int i = j + 1 + r; i < n; i += nr) {
+        if (!*sflag) return false;
+        // rows below the block: L[i][j0:j1] = A[i][j0:j1] L_block^-T
+        for (int i = j1 + tid; i < n; i += nt) {
             T *Ai = A + (size_t)i * n;
-            const T aij = Ai[j];
-            for (int k = j + 1 + c; k <= i; k += 16) Ai[k] -= aij * A[(size_t)k * n + j];
+            for (int jj = j0; jj < j1; jj++) {
+                const T *Lj = A + (size_t)jj * n;
+                T v = Ai[jj];
+                for (int m = j0; m < jj; m++) v -= Ai[m] * Lj[m];
+                Ai[jj] = v / Lj[jj];
+            }
+        }
+        __syncthreads();
+        // trailing update
+        for (int i = j1 + r; i < n; i += nr) {
+            T *Ai = A + (size_t)i * n;
+            for (int k = j1 + c; k <= i; k += 16) {
+                const T *Ak = A + (size_t)k * n;
+                T v = Ai[k];
+                for (int m = j0; m < j1; m++) v -= Ai[m] * Ak[m];
+                Ai[k] = v;
+            }
         }
         __syncthreads();
     }
     return true;
 }
 
+// Broadcast of lane jj's value (jj uniform) through the scalar unit.
+__device__ inline float lane_bcast(float v, int jj) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), jj)); }
+__device__ inline double lane_bcast(double v, int jj) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, jj);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), jj);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // x <- (L L^T)^-1 x: forward then back substitution in blocks of 32 columns; the triangle of each block is solved
-// by the first wavefront with the values in registers, the rest of the vector updated by every thread.
+// by the first wavefront with its row (forward) or column (backward) of L and the reciprocal pivots in registers,
+// the rest of the vector is updated by every thread.
 template <typename T>
 __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
@@ -510,15 +567,21 @@ __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
     for (int j0 = 0; j0 < n; j0 += BS) {
         const int nb = min(BS, n - j0);
         if (wv == 0) {
-            T xi = lane < nb ? x[j0 + lane] : (T)0;
-            for (int jj = 0; jj < nb; jj++) {
-                const T xj = __shfl(xi, jj, 64) / Lm[(size_t)(j0 + jj) * n + j0 + jj];
-                if (lane == jj)
-                    xi = xj;
-                else if (lane > jj && lane < nb)
-                    xi -= Lm[(size_t)(j0 + lane) * n + j0 + jj] * xj;
+            const bool in = lane < nb;
+            T xi = in ? x[j0 + lane] : (T)0;
+            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * n + j0 + lane] : (T)0;
+            T row[BS];
+#pragma unroll
+            for (int jj = 0; jj < BS; jj++) row[jj] = (in && jj < lane) ? Lm[(size_t)(j0 + lane) * n + j0 + jj] : (T)0;
+#pragma unroll
+            for (int jj = 0; jj < BS; jj++) {
+                if (jj < nb) {
+                    const T xj = lane_bcast(xi, jj) * lane_bcast(dinv, jj);
+                    if (lane == jj) xi = xj;
+                    else xi -= row[jj] * xj;
+                }
             }
-            if (lane < nb) x[j0 + lane] = xi;
+            if (in) x[j0 + lane] = xi;
         }
         __syncthreads();
         for (int i = j0 + nb + tid; i < n; i += nt) {
@@ -532,15 +595,22 @@ __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
     for (int j1 = n; j1 > 0; j1 -= BS) {
         const int j0 = max(0, j1 - BS), nb = j1 - j0;
         if (wv == 0) {
-            T xi = lane < nb ? x[j0 + lane] : (T)0;
-            for (int jj = nb - 1; jj >= 0; jj--) {
-                const T xj = __shfl(xi, jj, 64) / Lm[(size_t)(j0 + jj) * n + j0 + jj];
-                if (lane == jj)
-                    xi = xj;
-                else if (lane < jj)
-                    xi -= Lm[(size_t)(j0 + jj) * n + j0 + lane] * xj;
+            const bool in = lane < nb;
+            T xi = in ? x[j0 + lane] : (T)0;
+            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * n + j0 + lane] : (T)0;
+            T col[BS];
+#pragma unroll
+            for (int jj = 0; jj < BS; jj++)
+                col[jj] = (in && jj > lane && jj < nb) ? Lm[(size_t)(j0 + jj) * n + j0 + lane] : (T)0;
+#pragma unroll
+            for (int jj = BS - 1; jj >= 0; jj--) {
+                if (jj < nb) {
+                    const T xj = lane_bcast(xi, jj) * lane_bcast(dinv, jj);
+                    if (lane == jj) xi = xj;
+                    else xi -= col[jj] * xj;
+                }
             }
-            if (lane < nb) x[j0 + lane] = xi;
+            if (in) x[j0 + lane] = xi;
         }
         __syncthreads();
         for (int i = tid; i < j0; i += nt) {
@@ -567,7 +637,7 @@ __device__ inline double dwave_max(double v) {
 // itself; MIXED (T = float) factors the Jacobi-scaled Hd in FP32 and refines x in FP64 against Hd.
 // `lam_mode`: 0 the window's lambda; the factor is kept in fact64/fact32 for the adjoint.
 template <typename T>
-__global__ __launch_bounds__(256) void k_solve(DArgs D, int lds_cap) {
+__global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
     __shared__ T sy[XBA_MAX_STATE];
