@@ -20,7 +20,7 @@ namespace xba {
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
 #define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, masked-in flag
 #define FB_PAIRS 512  // sum over a pass's landmarks of k (k + 1) / 2 (their touched slot pairs)
-#define NFF 27        // fields of the compact observation record (lin_observations<..., true>)
+enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_N = 10 };  // per-landmark accumulator: Hll (6), bl (3), active rows
 enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
@@ -55,7 +55,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
-    L.o_obt = w;   w = align16(w + FB_OBS * 4);
+    L.o_obt = w;   w = align16(w + (trial ? FB_OBS * 4 : 0));
     L.o_obact = w; w = align16(w + (trial ? FB_OBS * 4 : 0));
     L.o_rcq = w;   w = align16(w + FB_OBS * 8);
     L.o_u = w;     w = align16(w + (trial ? FB_OBS * 3 * ts : 0));
@@ -68,7 +68,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + FB_LM * FB_LREC * ts);
     L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
-    L.o_lrow = w;  w = align16(w + (trial ? 0 : NFF * FB_OBS * rts));  // per observation: the whitened record
+    L.o_lrow = w;  w = align16(w + (trial ? 0 : FB_LM * LA_N * 8));     // per landmark: Hll, bl sums, active count
     L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
     L.o_kown = w;  w = align16(w + (trial ? 0 : FB_HLP));                  // per touched-slot entry: its landmark
     L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: landmark, si, sj
@@ -99,16 +99,19 @@ __device__ inline void lds_add(T *p, T v) {
 // One pass (the B landmarks from lb, planned on the host so that their observations fit one lane each and their
 // touched slots fit the pass's U storage): per landmark its first record row and observation count, per record row
 // its landmark.  Returns the pass's observation count.
-__device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane, int *lobs0, int *lnobs, int *ob_lm) {
+__device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane, int *lobs0, int *lnobs, int *ob_lm,
+                                int *base) {
     const int l = lb + lane;
     const bool in = lane < B;
-    const int n = in ? (ob_start[l + 1] - ob_start[l]) : 0;
+    const int o0 = in ? ob_start[l] : 0;
+    const int n = in ? (ob_start[l + 1] - o0) : 0;
     const int S = wave_incl_scan(n, lane);
     if (in) {
         lobs0[lane] = S - n;
         lnobs[lane] = n;
         for (int q = 0; q < n; q++) ob_lm[S - n + q] = lane;
     }
+    *base = __shfl(o0, 0, XBA_WAVE);  // the pass's observation rows are contiguous from here
     return __shfl(S, B - 1, XBA_WAVE);
 }
 
@@ -144,13 +147,64 @@ __device__ inline int next_pass(int *ctl, int lane) {
 }
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
-// Row r, column j of an observation's whitened Jacobian with respect to the host (host = true) or the target pose
-// block, read from the compact record: the host block's translation columns are the negated target ones.
+// One observation row held in its lane's registers in the record type RT: Huber weight (times the row weight),
+// whitened residual, whitened Jacobians wrt the landmark, the target pose block and the host block's rotation
+// columns (its translation columns are the negated target ones, see obs_eval), and the target slot in the tile.
 template <typename RT>
-__device__ inline double jpose(const RT *rec, bool host, int r, int j, int q) {
-    const int f = (host && j < 3) ? F_JH + 3 * r + j : F_JT + 6 * r + j;
-    const double v = (double)REC(f, q);
-    return (host && j >= 3) ? -v : v;
+struct ObsRow {
+    RT w, e[2], jl[6], jt[12], jhr[6];
+    int t;
+    __device__ inline double jh(int r, int j) const { return j < 3 ? (double)jhr[3 * r + j] : -(double)jt[6 * r + j]; }
+};
+
+// Residual, Huber weight and cost (FP64) and whitened Jacobians (FP64, stored in RT) of observation row k of a
+// landmark hosted at tile slot h (vio/window.py:125-146, 164-263, 284-288).  Returns the robust cost (0 inactive).
+template <typename RT>
+__device__ inline double obs_row(const KArgs<double> &A, int k, int lo, int h, double a, double b, double rho,
+                                 const double *sc64, const double *sc, const double *wc, const double *wd,
+                                 ObsRow<RT> &o) {
+    const int info = A.ob_info[k];
+    const int t = (info & 0xfff) - lo, cam = (info >> 12) & 1, valid = (info >> 13) & 1;
+    const int nd = cam == 0 ? 2 : 1;
+    o.t = t;
+    double e[2] = {0.0, 0.0}, Jl[2][3], Jt[2][6], Jh[2][6];
+    bool act = false;
+    if (valid)
+        act = obs_eval<double>(sc64 + h * SC64_STRIDE, sc64 + t * SC64_STRIDE, sc + h * SC_STRIDE,
+                               sc + t * SC_STRIDE, wc, wd, a, b, rho, cam, A.z[2 * (size_t)k], A.z[2 * (size_t)k + 1],
+                               t == h, true, e, Jl, Jt, Jh);
+    double w = 0.0, c = 0.0, en2 = 0.0;
+    if (act) {
+        const double isg = A.isig[k];
+        for (int i = 0; i < nd; i++) {
+            e[i] *= isg;
+            en2 += e[i] * e[i];
+        }
+        huber(sqrt(en2), wd[5], &w, &c);
+        if (A.ob_w) {
+            w *= A.ob_w[k];
+            c *= A.ob_w[k];
+        }
+        for (int r = 0; r < nd; r++) {
+            for (int j = 0; j < 3; j++) Jl[r][j] *= isg;
+            for (int j = 0; j < 6; j++) {
+                Jt[r][j] *= isg;
+                Jh[r][j] *= isg;
+            }
+        }
+    }
+    if (A.chi2_out) A.chi2_out[k] = act ? en2 : (double)INFINITY;
+    o.w = act ? (RT)w : (RT)0;
+    for (int r = 0; r < 2; r++) {
+        const bool on = act && r < nd;
+        o.e[r] = on ? (RT)e[r] : (RT)0;
+        for (int j = 0; j < 3; j++) {
+            o.jl[3 * r + j] = on ? (RT)Jl[r][j] : (RT)0;
+            o.jhr[3 * r + j] = on ? (RT)Jh[r][j] : (RT)0;
+        }
+        for (int j = 0; j < 6; j++) o.jt[6 * r + j] = on ? (RT)Jt[r][j] : (RT)0;
+    }
+    return act ? c : 0.0;
 }
 
 // RT is the type the per-observation record (whitened residual, weight and Jacobians) is stored in; every sum and
@@ -167,14 +221,14 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     double *tile = (double *)(smem + FL.tile), *btile = (double *)(smem + FL.btile);
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
-    int *ob_lm = (int *)(wb + FL.o_oblm), *rt = (int *)(wb + FL.o_obt);
+    int *ob_lm = (int *)(wb + FL.o_oblm);
     double *rcq = (double *)(wb + FL.o_rcq);
     unsigned *lmask = (unsigned *)(wb + FL.o_lmask);
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs), *loff = (int *)(wb + FL.o_loff);
     int *poff = (int *)(wb + FL.o_poff);
     int *slots = (int *)(wb + FL.o_slots);
     T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
-    RT *rec = (RT *)(wb + FL.o_lrow);
+    double *lacc = (double *)(wb + FL.o_lrow);
     int *lh = (int *)(wb + FL.o_lh);
     uint8_t *kown = (uint8_t *)(wb + FL.o_kown);
     uint16_t *ptab = (uint16_t *)(wb + FL.o_ptab);
@@ -201,53 +255,84 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             const int p = next_pass(ctl, lane);
             if (p >= ch.p1) break;
             const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
-            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm);
+            int obase;
+            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm, &obase);
+            // the landmark lanes' own loads, issued now so that they arrive during the observation phase, and their
+            // accumulators cleared
+            int pl_li = 0;
+            double pl_rho = 0.0, pl_rm = 0.0, pl_ri = 0.0;
+            if (lane < B) {
+                pl_li = A.lm_info[lb + lane];
+                pl_rho = A.lmp[3 * (size_t)(lb + lane) + 2];
+                pl_rm = A.rho_meas[lb + lane];
+                pl_ri = (double)A.rho_isig[lb + lane];
+                for (int i = 0; i < LA_N; i++) lacc[LA_N * lane + i] = 0.0;
+                lmask[lane] = 0u;
+            }
             WAVE_SYNC();
-            // ---- observations: whitened residual, Huber weight and cost, whitened Jacobians into the record
+            // ---- observations (one lane each, kept in registers): whitened residual, Huber weight and cost,
+            //      whitened Jacobians; the landmark's Hll / bl sums and touched slots; the row's own pose-pose
+            //      blocks (t,t), (h,h), (t,h) and gradient entries straight into the tile (they need no elimination)
+            ObsRow<RT> ob;
+            ob.w = (RT)0;
+            ob.t = 0;
+            int ob_bb = 0, ob_h = 0;
             if (lane < nob) {
-                const int qb = ob_lm[lane], l = lb + qb;
-                const int k = A.ob_start[l] + (lane - lobs0[qb]);
+                ob_bb = ob_lm[lane];
+                const int l = lb + ob_bb, k = obase + lane;
                 const int li = A.lm_info[l];
-                const int h = (li & 0xfff) - ch.lo;
-                if ((li >> 12) & 1) {
-                    lin_observations<T, RT, true>(A, rec, rcq, rt, sc64, sc, wc, wd, k - lane, lane + 1, ch.lo, h,
-                                        A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1], A.lmp[3 * (size_t)l + 2],
-                                        lane, true);
-                    // the observation's own pose-pose blocks (t,t), (h,h), (t,h) and gradient entries, straight
-                    // into the tile (they need no landmark elimination)
-                    const double w = (double)REC(F_W, lane);
-                    const int t = rt[lane];
-                    if (w != 0.0 && t != h) {
-                        double xt[2][6], xh[2][6];
-                        for (int r = 0; r < 2; r++)
-                            for (int j = 0; j < 6; j++) {
-                                xt[r][j] = jpose(rec, false, r, j, lane);
-                                xh[r][j] = jpose(rec, true, r, j, lane);
-                            }
-                        const double e0 = w * (double)REC(F_E, lane), e1 = w * (double)REC(F_E + 1, lane);
+                ob_h = (li & 0xfff) - ch.lo;
+                double cq = 0.0;
+                if ((li >> 12) & 1)
+                    cq = obs_row<RT>(A, k, ch.lo, ob_h, A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1],
+                                     A.lmp[3 * (size_t)l + 2], sc64, sc, wc, wd, ob);
+                rcq[lane] = cq;
+                const double w = (double)ob.w;
+                const int t = ob.t, h = ob_h;
+                if (w != 0.0) {
+                    double *la = lacc + LA_N * ob_bb;
+                    const double l00 = ob.jl[0], l01 = ob.jl[1], l02 = ob.jl[2];
+                    const double l10 = ob.jl[3], l11 = ob.jl[4], l12 = ob.jl[5];
+                    const double e0 = ob.e[0], e1 = ob.e[1];
+                    lds_add<double>(la + 0, w * (l00 * l00 + l10 * l10));
+                    lds_add<double>(la + 1, w * (l00 * l01 + l10 * l11));
+                    lds_add<double>(la + 2, w * (l00 * l02 + l10 * l12));
+                    lds_add<double>(la + 3, w * (l01 * l01 + l11 * l11));
+                    lds_add<double>(la + 4, w * (l01 * l02 + l11 * l12));
+                    lds_add<double>(la + 5, w * (l02 * l02 + l12 * l12));
+                    lds_add<double>(la + 6, w * (l00 * e0 + l10 * e1));
+                    lds_add<double>(la + 7, w * (l01 * e0 + l11 * e1));
+                    lds_add<double>(la + 8, w * (l02 * e0 + l12 * e1));
+                    lds_add<double>(la + LA_NACT, 1.0);
+                }
+                if (w != 0.0 && t != h) {
+                    __hip_atomic_fetch_or(lmask + ob_bb, (1u << t) | (1u << h), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                    double xt[2][6], xh[2][6];
+                    for (int r = 0; r < 2; r++)
                         for (int j = 0; j < 6; j++) {
-                            lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
-                            lds_add<double>(btile + 6 * h + j, xh[0][j] * e0 + xh[1][j] * e1);
+                            xt[r][j] = (double)ob.jt[6 * r + j];
+                            xh[r][j] = ob.jh(r, j);
                         }
-                        double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK, *dhh = tile + (h * (h + 1) / 2 + h) * TBLK;
-                        for (int i = 0; i < 6; i++)
-                            for (int j = 0; j <= i; j++) {
-                                lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]));
-                                lds_add<double>(dhh + 6 * i + j, w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]));
-                            }
-                        const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
-                        double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
-                        for (int i = 0; i < 6; i++)
-                            for (int j = 0; j < 6; j++) {
-                                const double v = tlo ? xh[0][i] * xt[0][j] + xh[1][i] * xt[1][j]
-                                                     : xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j];
-                                lds_add<double>(dth + 6 * i + j, w * v);
-                            }
+                    const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
+                    for (int j = 0; j < 6; j++) {
+                        lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
+                        lds_add<double>(btile + 6 * h + j, xh[0][j] * e0 + xh[1][j] * e1);
                     }
-                } else {
-                    REC(F_W, lane) = (RT)0;
-                    rcq[lane] = 0.0;
-                    rt[lane] = 0;
+                    double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK, *dhh = tile + (h * (h + 1) / 2 + h) * TBLK;
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j <= i; j++) {
+                            lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]));
+                            lds_add<double>(dhh + 6 * i + j, w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]));
+                        }
+                    const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
+                    double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) {
+                            const double v = tlo ? xh[0][i] * xt[0][j] + xh[1][i] * xt[1][j]
+                                                 : xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j];
+                            lds_add<double>(dth + 6 * i + j, w * v);
+                        }
                 }
             }
             WAVE_SYNC();
@@ -257,43 +342,25 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             int kb = 0;
             unsigned msk = 0u;
             if (lane < B) {
-                const int l = lb + lane, li = A.lm_info[l];
+                const int l = lb + lane, li = pl_li;
                 T *lr = lrec + lane * FB_LREC;
                 lr[LR_OK] = 0.0;
                 lr[LR_MASKED] = 0.0;
                 lh[lane] = (li & 0xfff) - ch.lo;
                 if ((li >> 12) & 1) {
-                    const int h = (li & 0xfff) - ch.lo, q0 = lobs0[lane], n = lnobs[lane];
-                    T s6[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                    int nact = 0;
-                    for (int q = q0; q < q0 + n; q++) {
-                        const T w = (T)REC(F_W, q);
-                        if (w == 0.0) continue;
-                        nact++;
-                        lcost += rcq[q];
-                        const T l00 = REC(F_JL, q), l01 = REC(F_JL + 1, q), l02 = REC(F_JL + 2, q);
-                        const T l10 = REC(F_JL + 3, q), l11 = REC(F_JL + 4, q), l12 = REC(F_JL + 5, q);
-                        const T e0 = REC(F_E, q), e1 = REC(F_E + 1, q);
-                        s6[0] += w * (l00 * l00 + l10 * l10);
-                        s6[1] += w * (l00 * l01 + l10 * l11);
-                        s6[2] += w * (l00 * l02 + l10 * l12);
-                        s6[3] += w * (l01 * l01 + l11 * l11);
-                        s6[4] += w * (l01 * l02 + l11 * l12);
-                        s6[5] += w * (l02 * l02 + l12 * l12);
-                        s6[6] += w * (l00 * e0 + l10 * e1);
-                        s6[7] += w * (l01 * e0 + l11 * e1);
-                        s6[8] += w * (l02 * e0 + l12 * e1);
-                        const int t = rt[q];
-                        if (t != h) msk |= (1u << t) | (1u << h);
-                    }
+                    const int q0 = lobs0[lane], n = lnobs[lane];
+                    for (int q = q0; q < q0 + n; q++) lcost += rcq[q];
+                    const double *la = lacc + LA_N * lane;
+                    const int nact = (int)la[LA_NACT];
+                    msk = lmask[lane];
                     kb = __popc(msk);
                     double H6[6], bl[3];
-                    for (int i = 0; i < 6; i++) H6[i] = (double)s6[i];
-                    for (int i = 0; i < 3; i++) bl[i] = (double)s6[6 + i];
-                    const double rho_l = A.lmp[3 * (size_t)l + 2];
-                    const T ris = A.rho_isig[l];
+                    for (int i = 0; i < 6; i++) H6[i] = la[LA_H + i];
+                    for (int i = 0; i < 3; i++) bl[i] = la[LA_B + i];
+                    const double rho_l = pl_rho;
+                    const T ris = pl_ri;
                     if (ris > (T)0) {
-                        double risd = (double)ris, r0 = (rho_l - A.rho_meas[l]) * risd, w, cc;
+                        double risd = (double)ris, r0 = (rho_l - pl_rm) * risd, w, cc;
                         huber(fabs(r0), wd[5], &w, &cc);
                         lcost += cc;
                         H6[5] += w * risd * risd;
@@ -348,7 +415,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 } else {
                     A.lmok[l] = 0;
                 }
-                lmask[lane] = msk;
             }
             const int koff = wave_incl_scan(kb, lane) - kb;
             const int ktot = __shfl(koff + kb, B - 1, XBA_WAVE);
@@ -378,25 +444,24 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             for (int e = lane; e < ktot * 18; e += XBA_WAVE) hlp[e] = 0.0;
             WAVE_SYNC();
             if (lane < nob) {
-                const int bb = ob_lm[lane];
+                const int bb = ob_bb;
                 const T *lr = lrec + bb * FB_LREC;
-                const T w = REC(F_W, lane);
-                const int t = rt[lane], h = lh[bb];
+                const T w = (T)ob.w;
+                const int t = ob.t, h = ob_h;
                 if (lr[LR_OK] != 0.0 && w != 0.0 && t != h) {
                     const unsigned m = lmask[bb];
                     T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
                     T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
                     T G[3][2];
                     for (int r = 0; r < 2; r++) {
-                        const T j0 = REC(F_JL + 3 * r, lane), j1 = REC(F_JL + 3 * r + 1, lane),
-                                j2 = REC(F_JL + 3 * r + 2, lane);
+                        const T j0 = ob.jl[3 * r], j1 = ob.jl[3 * r + 1], j2 = ob.jl[3 * r + 2];
                         G[0][r] = w * (lr[LR_L] * j0);
                         G[1][r] = w * (lr[LR_L + 1] * j0 + lr[LR_L + 2] * j1);
                         G[2][r] = w * (lr[LR_L + 3] * j0 + lr[LR_L + 4] * j1 + lr[LR_L + 5] * j2);
                     }
                     for (int j = 0; j < 6; j++) {
-                        const T t0 = jpose(rec, false, 0, j, lane), t1 = jpose(rec, false, 1, j, lane);
-                        const T h0 = jpose(rec, true, 0, j, lane), h1 = jpose(rec, true, 1, j, lane);
+                        const T t0 = ob.jt[j], t1 = ob.jt[6 + j];
+                        const T h0 = ob.jh(0, j), h1 = ob.jh(1, j);
                         for (int c = 0; c < 3; c++) {
                             lds_add<double>(Ut + 6 * c + j, G[c][0] * t0 + G[c][1] * t1);
                             lds_add<double>(Uh + 6 * c + j, G[c][0] * h0 + G[c][1] * h1);
@@ -507,7 +572,8 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
             const int p = next_pass(ctl, lane);
             if (p >= ch.p1) break;
             const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
-            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm);
+            int obase;
+            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm, &obase);
             WAVE_SYNC();
             // ---- observations at the current state: u_q = w Jl^T (Jt dp_t + Jh dp_h)
             int q_l = 0, q_k = 0, q_t = 0, q_h = 0, q_cam = 0, q_valid = 0;
@@ -516,7 +582,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
             if (q_in) {
                 const int qb = ob_lm[lane];
                 q_l = lb + qb;
-                q_k = A.ob_start[q_l] + (lane - lobs0[qb]);
+                q_k = obase + lane;
                 const int li = A.lm_info[q_l];
                 q_mask = (li >> 12) & 1;
                 q_h = (li & 0xfff) - ch.lo;
