@@ -51,8 +51,9 @@ extern "C" {
 
 enum xba_precision {
     XBA_FP64 = 0,  /* assembly, Schur and Cholesky in FP64 (the plain path) */
-    XBA_MIXED = 1  /* assembly and Schur in FP32 with FP32 accumulation; FP32 Cholesky of the Jacobi-scaled
-                      system with FP64 iterative refinement; FP64 SE(3) update and state */
+    XBA_MIXED = 1  /* per-observation Jacobian record in FP32, assembly and Schur sums in FP64 (the multi-wavefront
+                      kernels; the deterministic family accumulates FP32 with Kahan compensation); FP32 Cholesky of
+                      the Jacobi-scaled system with FP64 iterative refinement; FP64 SE(3) update and state */
 };
 
 /* A batch of windows, all arrays concatenated over windows in window order.  Slots, landmarks, observations,
@@ -106,6 +107,9 @@ typedef struct xba_options {
                                    (bitwise reproducible; FP32 tiles use Kahan-compensated FP32 accumulation);
                                    0: multi-wavefront kernels accumulating the reduced system in FP64 LDS atomics
                                    (reproducible to rounding only) */
+    int32_t big_n;              /* windows whose reduced system is larger than big_n are factored by the tiled
+                                   multi-workgroup Cholesky (64 x 64 tiles, trailing updates on MFMA); smaller ones
+                                   by one workgroup each.  0 = automatic (192); -1 = never tiled */
 } xba_options;
 
 typedef struct xba_ctx xba_ctx;

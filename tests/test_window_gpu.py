@@ -150,3 +150,134 @@ def test_fast_graph_replay_matches_eager_to_rounding():
     _gpu_solve([b], 4, "fp64", use_graph=True)
     assert _rel(a.lmp, b.lmp) < 1e-12
     assert _rel(a.pwb, b.pwb) < 1e-12
+
+
+# ---- the tiled multi-workgroup Cholesky (64 x 64 tiles, MFMA trailing updates) of large reduced systems
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["vo_small", "cfg0", "vi_small", "ur", "posefree"])
+def test_tiled_cholesky_forced_on_small_windows_matches_oracle(name, precision):
+    """big_n = 1 sends every window through the tiled factorisation (one to two tiles per side here)."""
+    pb = _case(name)
+    ref = pb.copy()
+    st_ref, dx_ref, _ = O.window_solve(ref, iters=1, history=True)
+    bs, st = _gpu_solve([pb], 1, precision, use_graph=False, big_n=1)
+    e = _rel(bs.last_dx()[0], dx_ref[0])
+    print(name, precision, "tiled dx rel err", e, flush=True)
+    assert st[0]["chol_ok"]
+    assert e < (1e-9 if precision == "fp64" else 1e-6)
+    assert st[0]["accepted"] == st_ref["accepted"]
+
+
+def _large_windows():
+    # n = 300 (20 VI keyframes: 5 tiles of 64 per side) and n = 240 (40 VO keyframes: 4 tiles)
+    return [synth.make_window(P=20, L=2000, views=(2, 10), seed=[5, 0], vi=True)[0],
+            synth.make_window(P=40, L=3000, views=(2, 10), seed=[5, 1])[0]]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tiled_cholesky_large_windows_match_oracle(precision):
+    pbs = _large_windows()
+    refs = [pb.copy() for pb in pbs]
+    outs = [O.window_solve(r, iters=1, history=True) for r in refs]
+    print("large windows", [(pb.P, pb.n_state(), pb.M) for pb in pbs], precision, flush=True)
+    bs, st = _gpu_solve(pbs, 1, precision, use_graph=False)
+    assert bs.plan_info()["max_n"] == 300
+    for dx, (sr, dxr, _a), s in zip(bs.last_dx(), outs, st):
+        e = _rel(dx, dxr[0])
+        print("n", len(dx), "dx rel err", e, "refinement steps", s["refine_steps"], flush=True)
+        assert s["chol_ok"]
+        assert e < (1e-9 if precision == "fp64" else 1e-6)
+        assert s["accepted"] == sr["accepted"]
+
+
+def test_tiled_and_one_workgroup_windows_in_one_batch_with_graph_replay():
+    """Large and small windows solved together for three LM iterations from a captured graph."""
+    pbs = _large_windows() + [synth.make_window(P=P, L=L, views=(2, min(10, P)), seed=[6, i])[0]
+                              for i, (P, L) in enumerate([(5, 80), (12, 600)])]
+    refs = [pb.copy() for pb in pbs]
+    outs = [O.window_solve(r, iters=3, history=True) for r in refs]
+    bs, st = _gpu_solve(pbs, 3, "fp64", use_graph=True)
+    for pb, ref, (sr, _d, _a), s in zip(pbs, refs, outs, st):
+        assert s["accepted"] == sr["accepted"]
+        assert _rel(pb.pwb, ref.pwb) < 1e-9
+        assert _rel(pb.Rwb, ref.Rwb) < 1e-9
+        assert _rel(pb.lmp, ref.lmp) < 1e-9
+
+
+def test_config1_visual_inertial_window_mixed_and_fp64_match_oracle():
+    """Config 1: 20 visual-inertial keyframes (n = 300, tiled Cholesky), 20,000 landmarks, ~110,000 observation
+    rows, 19 IMU factors.  FP64 within 1e-9 and mixed precision within 1e-6 of the oracle's first update."""
+    pb = synth.make_config(1)[0]
+    ref = pb.copy()
+    st_ref, dx_ref, _ = O.window_solve(ref, iters=1, history=True)
+    print("config 1", pb.P, pb.n_state(), pb.L, pb.M, flush=True)
+    for precision, bar in (("fp64", 1e-9), ("mixed", 1e-6)):
+        bs, st = _gpu_solve([pb.copy()], 1, precision, use_graph=False)
+        e = _rel(bs.last_dx()[0], dx_ref[0])
+        print(precision, "dx rel err", e, "refinement steps", st[0]["refine_steps"], flush=True)
+        assert st[0]["chol_ok"]
+        assert e < bar
+        assert st[0]["accepted"] == st_ref["accepted"]
+
+
+def _wide_tracks_window():
+    """48 keyframes, 800 landmarks; where a landmark is visible from a keyframe 32 or more slots from its host, its last
+    observation is moved to such a keyframe (measurement re-projected from the truth plus 1 px noise), so about 200
+    landmarks span more slots than one work-item tile holds and get compacted tiles of the slots they touch."""
+    pb, tr = synth.make_window(P=48, L=800, views=(2, 8), seed=[7, 0])
+    Rwc = np.einsum("pij,jk->pik", tr["Rwb"], pb.Rbc)
+    pwc = tr["pwb"] + np.einsum("pij,j->pi", tr["Rwb"], pb.pbc)
+    rng = np.random.default_rng(3)
+    for l in range(pb.L):
+        h = pb.lm_host[l]
+        a, b, rho = tr["lmp"][l]
+        X = Rwc[h] @ np.array([a, b, 1.0]) / rho + pwc[h]
+        q = np.einsum("pji,pj->pi", Rwc, X[None] - pwc)
+        with np.errstate(all="ignore"):
+            u = pb.fx * q[:, 0] / q[:, 2] + pb.cx
+            v = pb.fy * q[:, 1] / q[:, 2] + pb.cy
+        vis = (q[:, 2] > 0.3) & (u >= 0) & (u < 752) & (v >= 0) & (v < 480)
+        far = np.nonzero(vis & (np.abs(np.arange(pb.P) - h) >= 32))[0]
+        k = pb.ob_start[l + 1] - 1
+        if len(far) == 0 or pb.ob_pose[k] == h:
+            continue
+        t = rng.choice(far)
+        pb.ob_pose[k] = t
+        pb.ob_z[k] = [u[t] + rng.normal(), v[t] + rng.normal()]
+    return pb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_landmarks_spanning_more_than_a_tile_match_oracle(precision):
+    from xclean_amd import roofline
+    pb = _wide_tracks_window()
+    span = roofline.landmark_span(pb.lm_host, pb.ob_start, pb.ob_pose)
+    assert (span > 32).sum() > 150
+    ref = pb.copy()
+    st_ref, dx_ref, _ = O.window_solve(ref, iters=3, history=True)
+    bs, st = _gpu_solve([pb], 3, precision, use_graph=True)
+    print(precision, "wide-span landmarks", int((span > 32).sum()), "plan", bs.plan_info(), flush=True)
+    assert st[0]["accepted"] == st_ref["accepted"]
+    bar = 1e-9 if precision == "fp64" else 1e-6
+    assert _rel(pb.pwb, ref.pwb) < bar
+    assert _rel(pb.lmp, ref.lmp) < bar
+    # the first update, from a fresh solve
+    pb2 = _wide_tracks_window()
+    bs2, _ = _gpu_solve([pb2], 1, precision, use_graph=False)
+    assert _rel(bs2.last_dx()[0], dx_ref[0]) < bar
+
+
+def test_config4_shaped_window_fp64_matches_oracle():
+    """60 visual-inertial keyframes (n = 900: tiled Cholesky), 20,000 landmarks with 2-13 views (some spanning more
+    than 32 keyframes), one LM iteration, FP64 and mixed precision against the oracle."""
+    pb = synth.make_window(P=60, L=20000, views=(2, 13), seed=[5, 0], vi=True)[0]
+    ref = pb.copy()
+    st_ref, dx_ref, _ = O.window_solve(ref, iters=1, history=True)
+    for precision, bar in (("fp64", 1e-9), ("mixed", 1e-6)):
+        bs, st = _gpu_solve([pb.copy()], 1, precision, use_graph=False)
+        e = _rel(bs.last_dx()[0], dx_ref[0])
+        print("n 900", precision, "dx rel err", e, "refinement steps", st[0]["refine_steps"], flush=True)
+        assert st[0]["chol_ok"]
+        assert e < bar
+        assert st[0]["accepted"] == st_ref["accepted"]

@@ -7,11 +7,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <vector>
 
 #include "xba_internal.h"
 #include "xba_lin.hip"
 #include "xba_dense.hip"
+#include "xba_bigchol.hip"
 #include "xba_fast.hip"
 #include "xba_np.hip"
 #include "xba_adj.hip"
@@ -42,14 +44,15 @@ T *dalloc(size_t n, std::vector<void *> &owned) {
 template <typename T>
 __global__ void k_upload(int nobs, int nlm, const int32_t *ob_pose, const uint8_t *ob_cam, const uint8_t *ob_valid,
                          const double *isig, int32_t *ob_info, T *isigT, const int32_t *lm_host,
-                         const uint8_t *lm_mask, const double *rho_isig, int32_t *lm_info, T *riT) {
+                         const uint8_t *lm_mask, const double *rho_isig, int32_t *lm_info, T *riT,
+                         const uint8_t *ob_tile, const uint8_t *lm_tile) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nobs || i < nlm; i += gridDim.x * blockDim.x) {
         if (i < nobs) {
-            ob_info[i] = pack_obs(ob_pose[i], ob_cam[i] ? 1 : 0, ob_valid[i] ? 1 : 0);
+            ob_info[i] = pack_obs(ob_pose[i], ob_cam[i] ? 1 : 0, ob_valid[i] ? 1 : 0, ob_tile ? ob_tile[i] : 0);
             isigT[i] = (T)isig[i];
         }
         if (i < nlm) {
-            lm_info[i] = lm_host[i] | ((lm_mask[i] ? 1 : 0) << 12);
+            lm_info[i] = lm_host[i] | ((lm_mask[i] ? 1 : 0) << 12) | ((lm_tile ? lm_tile[i] : 0) << 13);
             riT[i] = (T)rho_isig[i];
         }
     }
@@ -106,6 +109,8 @@ struct xba_ctx {
     int32_t *colmap = nullptr, *slot_off = nullptr, *slot_dim = nullptr, *ob_start = nullptr, *lm_host = nullptr,
             *ob_pose = nullptr, *imu_ij = nullptr, *prior_slot = nullptr, *prior_dim = nullptr, *lm_win = nullptr;
     int32_t *ob_info = nullptr, *lm_info = nullptr;
+    int32_t *chunk_slots = nullptr;                  // tile slot lists of the compacted work items
+    uint8_t *ob_tile = nullptr, *lm_tile = nullptr;  // tile slot of every observation / landmark host (fast family)
     void *isigT = nullptr, *lmpT = nullptr, *lmp_trial = nullptr, *riT = nullptr;
     void *partH = nullptr, *partb = nullptr, *partH_lo = nullptr, *partb_lo = nullptr, *lmHi = nullptr, *lmbl = nullptr;
     double *partH64 = nullptr, *partb64 = nullptr;
@@ -115,6 +120,11 @@ struct xba_ctx {
     double *R2 = nullptr, *p2 = nullptr, *v2 = nullptr, *g2 = nullptr, *a2 = nullptr;
     double *H = nullptr, *b = nullptr, *dx = nullptr, *fact64 = nullptr, *imu_scr = nullptr, *wstate = nullptr;
     float *fact32 = nullptr;
+    // windows factored by the tiled Cholesky
+    int nbig = 0, big_npmax = 0, big_nbmax = 0;
+    BigWin *bigs = nullptr;
+    float *bigF32 = nullptr;
+    double *bigF64 = nullptr, *bigS = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t gstream = nullptr;
@@ -166,6 +176,7 @@ KArgs<T> kargs(xba_ctx *c) {
     A.sched_ctr = c->sched_ctr;
     A.pass_cost = c->pass_cost;
     A.ob_w = c->B.ob_w;
+    A.chunk_slots = c->chunk_slots;
     return A;
 }
 
@@ -198,10 +209,25 @@ DArgs dargs(xba_ctx *c) {
     D.partc = c->partc;
     D.partc2 = c->partc2;
     D.nref_max = 6;
+    D.nbig = c->nbig;
+    D.bigs = c->bigs;
+    D.bigF32 = c->bigF32;
+    D.bigF64 = c->bigF64;
+    D.bigS = c->bigS;
+    D.chunk_slots = c->chunk_slots;
     return D;
 }
 
 bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
+
+// The distinct window slots landmark l touches (its host and the slots of its observations), ascending.
+std::vector<int> touched_slots(const xba_batch &B, int l) {
+    std::vector<int> v(1, B.lm_host[l]);
+    for (int k = B.ob_start[l]; k < B.ob_start[l + 1]; k++) v.push_back(B.ob_pose[k]);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
 
 // The multi-wave kernels compute in FP64 whatever the precision (only their observation record is stored in the
 // working type): their arguments take the FP64 whitening weights the caller uploaded and FP64 landmark blocks.
@@ -222,6 +248,7 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
     F.ok_out = A.ok_out; F.adj_gz = A.adj_gz; F.adj_gw = A.adj_gw; F.adj_y = A.adj_y;
     F.pass_l0 = A.pass_l0; F.sched = A.sched; F.sched_ctr = A.sched_ctr; F.pass_cost = A.pass_cost;
     F.ob_w = A.ob_w;
+    F.chunk_slots = A.chunk_slots;
     return F;
 }
 
@@ -277,12 +304,30 @@ void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 =
     hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
 }
 
+// The damped solve and trial state of every window: the tiled Cholesky sequence for the large windows, one
+// workgroup per window for the others.
+template <typename T>
+void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
+    if (c->nbig > 0) {
+        hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
+        for (int k = 0; k < c->big_nbmax; k++) {
+            const int below = c->big_nbmax - k - 1;
+            hipLaunchKernelGGL(k_big_panel<T>, dim3(std::max(1, (below + 3) / 4), c->nbig), dim3(256), 0, s, D, k);
+            if (below > 0)
+                hipLaunchKernelGGL(k_big_update<T>, dim3(below * (below + 1) / 2, c->nbig), dim3(256), 0, s, D, k);
+        }
+        hipLaunchKernelGGL(k_big_solve<T>, dim3(c->nbig), dim3(512), 0, s, D);
+    }
+    if (c->nbig < c->nwin)
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
+}
+
 template <typename T>
 void launch_iter(xba_ctx *c, hipStream_t s) {
     KArgs<T> A = kargs<T>(c);
     launch_lin<T>(c, A, s);
     DArgs D = dargs(c);
-    hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
+    launch_solve<T>(c, D, s);
     launch_trial_kernel<T>(c, A, s);
     hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
     hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
@@ -301,11 +346,11 @@ int pack_measurements(xba_ctx *c, hipStream_t s) {
     if (mixed(c))
         hipLaunchKernelGGL(k_upload<float>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
                            B.ob_valid, B.ob_isig, c->ob_info, (float *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
-                           c->lm_info, (float *)c->riT);
+                           c->lm_info, (float *)c->riT, (const uint8_t *)c->ob_tile, (const uint8_t *)c->lm_tile);
     else
         hipLaunchKernelGGL(k_upload<double>, dim3(g), dim3(256), 0, s, c->nobs, c->nlm, c->ob_pose, B.ob_cam,
                            B.ob_valid, B.ob_isig, c->ob_info, (double *)c->isigT, c->lm_host, B.lm_mask, B.rho_isig,
-                           c->lm_info, (double *)c->riT);
+                           c->lm_info, (double *)c->riT, (const uint8_t *)c->ob_tile, (const uint8_t *)c->lm_tile);
     HIPCK(hipGetLastError());
     return XBA_OK;
 }
@@ -424,6 +469,31 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         for (int l = W.lm0; l < W.lm0 + W.L; l++) lm_win[l] = w;
     }
     if (any_vi && (!B.vel || !B.bgs || !B.bas)) return fail(XBA_E_ARG);
+    // windows above big_n: the tiled Cholesky (padded to whole 64 x 64 tiles)
+    std::vector<BigWin> bigs;
+    int64_t bigF = 0, bigSv = 0;
+    {
+        const int big_n = opt->big_n == 0 ? 192 : opt->big_n;
+        for (int w = 0; w < nwin; w++) {
+            WinInfo &W = wins[w];
+            W.big = -1;
+            if (big_n < 0 || W.n <= big_n) continue;
+            BigWin bw;
+            bw.win = w;
+            bw.n = W.n;
+            bw.np = (W.n + XBA_TILE - 1) / XBA_TILE * XBA_TILE;
+            bw.nb = bw.np / XBA_TILE;
+            bw.f0 = bigF;
+            bw.s0 = bigSv;
+            bigF += big_matrix_entries(bw.np);
+            bigSv += bw.np;
+            W.big = (int32_t)bigs.size();
+            bigs.push_back(bw);
+            c->big_npmax = std::max(c->big_npmax, bw.np);
+            c->big_nbmax = std::max(c->big_nbmax, bw.nb);
+        }
+        c->nbig = (int)bigs.size();
+    }
     c->nstate = st;
     c->nh = h0;
     c->maxn = maxn;
@@ -464,6 +534,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             }
             llo[l] = lo;
             lhi[l] = hi;
+            if (c->fast && hi - lo + 1 > tile_cap) {
+                // spans more slots than a tile holds: its work item's tile is the set of slots it touches
+                if ((int)touched_slots(B, l).size() > tile_cap) return fail(XBA_E_RANGE);
+                continue;
+            }
             if (hi - lo + 1 > tile) tile = hi - lo + 1;
         }
     }
@@ -481,6 +556,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         for (int l = 0; l < c->nlm; l++) nwl += (lhi[l] - llo[l] + 1 > narrow) ? 1 : 0;
         lpc_wide = (int)std::max<int64_t>(16, std::min<int64_t>(lpc, nwl / std::max(ncu, 1)));
     }
+    std::vector<int32_t> chunk_slots;
     auto emit = [&](std::vector<Chunk> &dst, int w, int cs, int ce, int lo, int hi) {
         Chunk ch;
         memset(&ch, 0, sizeof(ch));
@@ -489,6 +565,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         ch.l1 = ce;
         ch.lo = lo;
         ch.R = hi - lo + 1;
+        ch.smap = -1;
         dst.push_back(ch);
     };
     std::vector<int> wc0(nwin), wc1(nwin), ww0(nwin), ww1(nwin);
@@ -498,6 +575,30 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         ww0[w] = (int)wide.size();
         int l = W.lm0, lend = W.lm0 + W.L;
         while (l < lend) {
+            if (c->fast && lhi[l] - llo[l] + 1 > tile_cap) {
+                // compacted work item: consecutive such landmarks while the union of their touched slots fits
+                std::vector<int> uni = touched_slots(B, l);
+                const int cs = l;
+                l++;
+                while (l < lend && l - cs < lpc_wide && lhi[l] - llo[l] + 1 > tile_cap) {
+                    std::vector<int> t = touched_slots(B, l), m;
+                    std::set_union(uni.begin(), uni.end(), t.begin(), t.end(), std::back_inserter(m));
+                    if ((int)m.size() > tile_cap) break;
+                    uni.swap(m);
+                    l++;
+                }
+                Chunk ch;
+                memset(&ch, 0, sizeof(ch));
+                ch.win = w;
+                ch.l0 = cs;
+                ch.l1 = l;
+                ch.lo = uni.front();
+                ch.R = (int)uni.size();
+                ch.smap = (int32_t)chunk_slots.size();
+                chunk_slots.insert(chunk_slots.end(), uni.begin(), uni.end());
+                wide.push_back(ch);
+                continue;
+            }
             const bool isw = lhi[l] - llo[l] + 1 > narrow;
             const int lim = isw ? tile_cap : narrow;
             int cs = l, lo = llo[l], hi = lhi[l];
@@ -566,6 +667,19 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         std::sort(sched.begin() + nn, sched.end(), by_cost);
     }
     c->npass = (int)pass_l0.size();
+    // tile slot of every observation and landmark host inside its work item (the multi-wave kernels read them from
+    // the packed words)
+    std::vector<uint8_t> ob_tile, lm_tile;
+    if (c->fast) {
+        ob_tile.assign(std::max(c->nobs, 1), 0);
+        lm_tile.assign(std::max(c->nlm, 1), 0);
+        for (const Chunk &ch : chunks)
+            for (int l = ch.l0; l < ch.l1; l++) {
+                lm_tile[l] = (uint8_t)chunk_tile(ch, chunk_slots.data(), B.lm_host[l]);
+                for (int k = B.ob_start[l]; k < B.ob_start[l + 1]; k++)
+                    ob_tile[k] = (uint8_t)chunk_tile(ch, chunk_slots.data(), B.ob_pose[k]);
+            }
+    }
     c->nnarrow = c->fast ? nn : (int)chunks.size();
     c->Rnarrow = Rn;
     c->Rwide = Rw;
@@ -615,6 +729,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->prior_slot = dalloc<int32_t>(c->npblk, O);
     c->prior_dim = dalloc<int32_t>(c->npblk, O);
     c->ob_info = dalloc<int32_t>(c->nobs, O);
+    c->chunk_slots = dalloc<int32_t>(chunk_slots.size(), O);
+    if (c->fast) {
+        c->ob_tile = dalloc<uint8_t>(c->nobs, O);
+        c->lm_tile = dalloc<uint8_t>(c->nlm, O);
+    }
     c->lm_info = dalloc<int32_t>(c->nlm, O);
     c->isigT = mx ? (void *)dalloc<float>(c->nobs, O) : (void *)dalloc<double>(c->nobs, O);
     c->lmpT = (void *)dalloc<double>(3 * (size_t)c->nlm, O);
@@ -651,6 +770,10 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->fact32 = mx ? dalloc<float>(h0, O) : dalloc<float>(1, O);
     c->imu_scr = dalloc<double>(465 * (size_t)std::max(c->nimu, 1), O);
     c->wstate = dalloc<double>((size_t)NWS * nwin, O);
+    c->bigs = dalloc<BigWin>(bigs.size(), O);
+    c->bigF32 = (mx && c->nbig) ? dalloc<float>(bigF, O) : dalloc<float>(1, O);
+    c->bigF64 = (!mx && c->nbig) ? dalloc<double>(bigF, O) : dalloc<double>(1, O);
+    c->bigS = dalloc<double>(bigSv, O);
     for (void *p : O)
         if (!p) return fail(XBA_E_HIP);
     auto up = [&](void *dst, const void *src, size_t bytes) -> bool {
@@ -661,6 +784,9 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
               (pass_l0.empty() || up(c->pass_l0, pass_l0.data(), pass_l0.size() * 4)) &&
               (sched.empty() || up(c->sched, sched.data(), sched.size() * 4)) &&
               up(c->wins, wins.data(), nwin * sizeof(WinInfo)) &&
+              (bigs.empty() || up(c->bigs, bigs.data(), bigs.size() * sizeof(BigWin))) &&
+              (chunk_slots.empty() || up(c->chunk_slots, chunk_slots.data(), chunk_slots.size() * 4)) &&
+              (!c->fast || (up(c->ob_tile, ob_tile.data(), (size_t)c->nobs) && up(c->lm_tile, lm_tile.data(), (size_t)c->nlm))) &&
               up(c->colmap, colmap.data(), colmap.size() * 4) && up(c->slot_off, slot_off.data(), c->nslot * 4) &&
               up(c->slot_dim, B.slot_dim, c->nslot * 4) && up(c->ob_start, B.ob_start, (c->nlm + 1) * 4) &&
               up(c->lm_host, B.lm_host, c->nlm * 4) && up(c->ob_pose, B.ob_pose, (size_t)c->nobs * 4) &&
@@ -680,7 +806,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
-    if (c->lin_lds > 64 * 1024) {
+    if (!c->fast && c->lin_lds > 64 * 1024) {  // the single-wavefront kernels' tile (their launch only)
+        if (c->lin_lds > 160 * 1024) return fail(XBA_E_RANGE);
         if (mx) {
             HIPCK(hipFuncSetAttribute((const void *)k_lin<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
             HIPCK(hipFuncSetAttribute((const void *)k_trial<float>, hipFuncAttributeMaxDynamicSharedMemorySize, c->lin_lds));
@@ -851,7 +978,7 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
         (void)hipEventRecord(ev[3], s);
         hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
         (void)hipEventRecord(ev[4], s);
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
+        launch_solve<T>(c, D, s);
         (void)hipEventRecord(ev[5], s);
         launch_trial_kernel<T>(c, A, s, 0);
         (void)hipEventRecord(ev[6], s);

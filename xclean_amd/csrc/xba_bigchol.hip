@@ -1,0 +1,343 @@
+// Tiled multi-workgroup Cholesky solve of large reduced systems (windows above the one-workgroup size limit),
+// right-looking over 64 x 64 tiles of the damped, padded matrix:
+//
+//   k_big_prep    Hd = H + diag(lam diag(H) + 1e-9) (vio/window.py:933-936), Jacobi-scaled in mixed precision,
+//                 copied into the padded layout (identity on the padding), lower tiles only
+//   k_big_panel   step k: the diagonal tile factored in registers by one wavefront (every workgroup of the launch
+//                 factors it; the first one stores it), then L(i,k) = A(i,k) L(k,k)^-T for the tiles below it, one
+//                 tile per wavefront with a row per lane
+//   k_big_update  step k: A(i,j) -= L(i,k) L(j,k)^T over the trailing lower tiles on the matrix cores (FP32 or
+//                 FP64 16x16x4 MFMA), one tile per workgroup, 32 x 32 per wavefront
+//   k_big_solve   the diagonal factor tiles put in place, then forward / back substitution and, in mixed precision,
+//                 FP64 iterative refinement against the FP64 damped matrix (vio/window.py:937-941), dx = -Hd^-1 b,
+//                 and the trial state
+//
+// The diagonal tiles are stored into a side buffer (np x 64 after the matrix) during the factorisation because the
+// other workgroups of the same panel launch still read A(k,k) from the matrix; k_big_solve copies them into place.
+#include "xba_internal.h"
+
+namespace xba {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+// D = A B + C on one wavefront: A 16x4 (lane l supplies A[l % 16][l / 16]), B 4x16 (lane l supplies
+// B[l / 16][l % 16]), C/D 16x16 with four elements per lane in column l % 16; row(l / 16, q) gives the row of element
+// q, which differs between the two instructions (FP32: four consecutive rows per lane group; FP64: rows interleaved
+// with stride 4).
+template <>
+struct Mfma<float> {
+    typedef f32x4 V;
+    __device__ static V fma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int g, int q) { return 4 * g + q; }
+};
+template <>
+struct Mfma<double> {
+    typedef f64x4 V;
+    __device__ static V fma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int g, int q) { return g + 4 * q; }
+};
+
+template <typename T>
+__device__ inline T *big_base(const DArgs &D) {
+    if constexpr (sizeof(T) == 4)
+        return (T *)D.bigF32;
+    else
+        return (T *)D.bigF64;
+}
+
+// Offset of the window's matrix and of its diagonal-tile side buffer inside the tiled-factor buffer.
+__host__ __device__ inline int64_t big_matrix_entries(int np) { return (int64_t)np * np + (int64_t)np * XBA_TILE; }
+
+// ---------------------------------------------------------------------------------------------- k_big_prep --
+// grid (max np, nbig): one padded row per workgroup.  gn = 1: lambda = 0 (the Gauss-Newton step).
+template <typename T>
+__global__ __launch_bounds__(256) void k_big_prep(DArgs D, int gn) {
+    __shared__ double ss[XBA_MAX_STATE];
+    const BigWin B = D.bigs[blockIdx.y];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    if (i >= B.np) return;
+    const WinInfo W = D.wins[B.win];
+    const int n = B.n, np = B.np;
+    double *ws = D.wstate + (size_t)NWS * B.win;
+    const double lam = gn ? 0.0 : ws[WS_LAM];
+    const double *H = D.H + W.h0;
+    for (int j = tid; j < n; j += blockDim.x) {
+        const double h = H[(size_t)j * n + j], hd = h + (lam * h + 1e-9);
+        ss[j] = (sizeof(T) == 8) ? 1.0 : (hd > 0.0 ? 1.0 / sqrt(hd) : 1.0);
+    }
+    __syncthreads();
+    T *F = big_base<T>(D) + B.f0 + (size_t)i * np;
+    const double si = i < n ? ss[i] : 1.0;
+    const int jend = (i / XBA_TILE + 1) * XBA_TILE;  // through the diagonal tile: nothing above it is read
+    for (int j = tid; j < jend; j += blockDim.x) {
+        double v = 0.0;
+        if (i < n && j < n) {
+            double h = H[(size_t)i * n + j];
+            if (i == j) h += lam * h + 1e-9;
+            v = h * si * ss[j];
+        } else if (i == j) {
+            v = 1.0;
+        }
+        F[j] = (T)v;
+    }
+    if (tid == 0) {
+        D.bigS[B.s0 + i] = si;
+        if (i == 0) ws[WS_BIGOK] = 1.0;
+    }
+}
+
+// Broadcast of lane jj's value (jj uniform).
+__device__ inline float big_bcast(float v, int jj) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), jj));
+}
+__device__ inline double big_bcast(double v, int jj) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, jj);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), jj);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+#define BIG_WAVE_SYNC()                                        \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
+        __builtin_amdgcn_s_waitcnt(0);                         \
+        __builtin_amdgcn_wave_barrier();                       \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); \
+    } while (0)
+
+// --------------------------------------------------------------------------------------------- k_big_panel --
+// grid (max(1, ceil((max nb - k - 1) / 4)), nbig), 256 threads: wavefront w of workgroup x solves tile row
+// i = k + 1 + 4 x + w.
+template <typename T>
+__global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
+    __shared__ T sLt[XBA_TILE][XBA_TILE];  // the diagonal factor transposed: sLt[j][m] = L[m][j] (0 for m < j)
+    __shared__ T scol[XBA_TILE], sinv[XBA_TILE];
+    __shared__ int sok;
+    const BigWin B = D.bigs[blockIdx.y];
+    if (k >= B.nb) return;
+    double *ws = D.wstate + (size_t)NWS * B.win;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, np = B.np;
+    // the flag is read once per workgroup: another workgroup of this launch may clear it meanwhile
+    if (tid == 0) sok = ws[WS_BIGOK] != 0.0;
+    __syncthreads();
+    if (!sok) return;
+    __syncthreads();  // every thread has read sok before wavefront 0 rewrites it
+    T *F = big_base<T>(D) + B.f0;
+    if (wv == 0) {
+        // lane r holds row r of the diagonal tile (lower part; 0 above the diagonal)
+        const T *Ar = F + (size_t)(k * XBA_TILE + lane) * np + k * XBA_TILE;
+        T a[XBA_TILE];
+#pragma unroll
+        for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? Ar[m] : (T)0;
+        int ok = 1;
+        T dinv = (T)0;
+#pragma unroll
+        for (int j = 0; j < XBA_TILE; j++) {
+            const T d = big_bcast(a[j], j);
+            if (!(d > (T)0) || !isfinite(d)) {  // uniform: every lane reads the same pivot
+                ok = 0;
+                break;
+            }
+            const T sd = sqrt(d), inv = (T)1 / sd;
+            if (lane == j) dinv = inv;
+            a[j] = (lane == j) ? sd : a[j] * inv;  // L[lane][j] below the diagonal (entries above it are not used)
+            scol[lane] = a[j];
+            BIG_WAVE_SYNC();
+            const T lij = (lane > j) ? a[j] : (T)0;
+#pragma unroll
+            for (int m = j + 1; m < XBA_TILE; m++) a[m] -= lij * scol[m];
+            BIG_WAVE_SYNC();
+        }
+        if (lane == 0) sok = ok;
+        if (ok) {
+#pragma unroll
+            for (int m = 0; m < XBA_TILE; m++) sLt[m][lane] = (m <= lane) ? a[m] : (T)0;
+            sinv[lane] = dinv;
+            if (blockIdx.x == 0) {  // the factor's diagonal tile goes to the side buffer (row-major 64 wide)
+                T *Dr = F + (size_t)np * np + (size_t)(k * XBA_TILE + lane) * XBA_TILE;
+#pragma unroll
+                for (int m = 0; m < XBA_TILE; m++) Dr[m] = (m <= lane) ? a[m] : (T)0;
+            }
+        }
+    }
+    __syncthreads();
+    if (!sok) {
+        if (tid == 0) ws[WS_BIGOK] = 0.0;
+        return;
+    }
+    // L(i,k) = A(i,k) L(k,k)^-T: lane r solves x L^T = a for row r of the tile by forward substitution
+    const int i = k + 1 + 4 * (int)blockIdx.x + wv;
+    if (i < B.nb) {
+        T *Xr = F + (size_t)(i * XBA_TILE + lane) * np + k * XBA_TILE;
+        T x[XBA_TILE];
+#pragma unroll
+        for (int m = 0; m < XBA_TILE; m++) x[m] = Xr[m];
+#pragma unroll
+        for (int j = 0; j < XBA_TILE; j++) {
+            const T xj = x[j] * sinv[j];
+            x[j] = xj;
+#pragma unroll
+            for (int m = j + 1; m < XBA_TILE; m++) x[m] -= xj * sLt[j][m];
+        }
+#pragma unroll
+        for (int m = 0; m < XBA_TILE; m++) Xr[m] = x[m];
+    }
+}
+
+// -------------------------------------------------------------------------------------------- k_big_update --
+// grid (m (m + 1) / 2 for m = max nb - k - 1, nbig): workgroup x updates the trailing lower tile (i, j),
+// i = k + 1 + ti, j = k + 1 + tj, x = ti (ti + 1) / 2 + tj.  Wavefront w computes the 32 x 32 quadrant
+// (rows 32 (w / 2), cols 32 (w % 2)) as 2 x 2 MFMA blocks of 16 x 16, over the tile's k-range in two halves staged
+// in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_big_update(DArgs D, int k) {
+    constexpr int KH = XBA_TILE / 2, LD = KH + (sizeof(T) == 4 ? 4 : 2);
+    __shared__ T sI[XBA_TILE][LD], sJ[XBA_TILE][LD];
+    typedef typename Mfma<T>::V V;
+    const BigWin B = D.bigs[blockIdx.y];
+    const int m = B.nb - k - 1, x = blockIdx.x;
+    int ti = (int)((sqrtf(8.0f * (float)x + 1.0f) - 1.0f) * 0.5f);
+    while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
+    while (ti * (ti + 1) / 2 > x) ti--;
+    const int tj = x - ti * (ti + 1) / 2;
+    if (ti >= m) return;
+    __shared__ int sgo;
+    if (threadIdx.x == 0) sgo = D.wstate[(size_t)NWS * B.win + WS_BIGOK] != 0.0;
+    __syncthreads();
+    if (!sgo) return;
+    const int i = k + 1 + ti, j = k + 1 + tj, np = B.np;
+    T *F = big_base<T>(D) + B.f0;
+    const T *Li = F + (size_t)(i * XBA_TILE) * np + k * XBA_TILE;
+    const T *Lj = F + (size_t)(j * XBA_TILE) * np + k * XBA_TILE;
+    T *C = F + (size_t)(i * XBA_TILE) * np + j * XBA_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r0 = 32 * (wv >> 1), c0 = 32 * (wv & 1), lr = lane & 15, lk = lane >> 4;
+    V acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                acc[a][b][q] = C[(size_t)(r0 + 16 * a + Mfma<T>::row(lk, q)) * np + c0 + 16 * b + lr];
+    for (int h = 0; h < 2; h++) {
+        if (h) __syncthreads();  // every wavefront has read the first half
+        for (int e = tid; e < XBA_TILE * KH; e += blockDim.x) {
+            const int r = e / KH, c = e % KH;
+            sI[r][c] = Li[(size_t)r * np + h * KH + c];
+            sJ[r][c] = Lj[(size_t)r * np + h * KH + c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KH / 4; kk++) {
+            T av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) av[a] = -sI[r0 + 16 * a + lr][4 * kk + lk];
+#pragma unroll
+            for (int b = 0; b < 2; b++) bv[b] = sJ[c0 + 16 * b + lr][4 * kk + lk];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::fma(av[a], bv[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = r0 + 16 * a + Mfma<T>::row(lk, q), col = c0 + 16 * b + lr;
+                if (i != j || row >= col) C[(size_t)row * np + col] = acc[a][b][q];
+            }
+}
+
+// --------------------------------------------------------------------------------------------- k_big_solve --
+// grid (nbig), 512 threads: dx = -Hd^-1 b with the tiled factor (FP64: one solve; mixed: FP32 solves of the
+// scaled system refined in FP64 against Hd until |r| <= 1e-15 |b|), then the trial state.
+template <typename T>
+__global__ __launch_bounds__(512) void k_big_solve(DArgs D) {
+    __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
+    __shared__ T sy[XBA_MAX_STATE];
+    __shared__ double sred[64];
+    const BigWin B = D.bigs[blockIdx.x];
+    const WinInfo W = D.wins[B.win];
+    const int n = B.n, np = B.np, tid = threadIdx.x, nt = blockDim.x;
+    double *ws = D.wstate + (size_t)NWS * B.win;
+    const double lam = ws[WS_LAM];
+    const double *H = D.H + W.h0, *bv = D.b + W.st0;
+    T *F = big_base<T>(D) + B.f0;
+    const bool ok = ws[WS_BIGOK] != 0.0;
+    int nref = 0;
+    if (ok) {
+        const T *Dg = F + (size_t)np * np;
+        for (int e = tid; e < np * XBA_TILE; e += nt) {
+            const int r = e / XBA_TILE, c = e % XBA_TILE;
+            F[(size_t)r * np + (r / XBA_TILE) * XBA_TILE + c] = Dg[e];
+        }
+        for (int i = tid; i < np; i += nt) {
+            ss[i] = D.bigS[B.s0 + i];
+            sx[i] = 0.0;
+            sr[i] = i < n ? bv[i] : 0.0;
+        }
+        __syncthreads();
+        const int maxit = (sizeof(T) == 8) ? 1 : D.nref_max;
+        const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+        for (int it = 0; it < maxit; it++) {
+            for (int i = tid; i < np; i += nt) sy[i] = (T)(sr[i] * ss[i]);
+            __syncthreads();
+            chol_solve_wg<T>(F, np, sy);
+            for (int i = tid; i < n; i += nt) sx[i] += (double)sy[i] * ss[i];
+            __syncthreads();
+            nref = it + 1;
+            if (it + 1 == maxit) break;
+            // r = b - Hd x in FP64: one wavefront per row, lanes along the row
+            double rmax = 0.0, bmax = 0.0;
+            for (int i = wv; i < n; i += nw) {
+                const double *Hi = H + (size_t)i * n;
+                double acc = 0.0;
+                for (int j = lane; j < n; j += 64) acc += Hi[j] * sx[j];
+                acc = dwave_sum(acc);
+                const double ri = bv[i] - acc - (lam * Hi[i] + 1e-9) * sx[i];
+                if (lane == 0) sr[i] = ri;
+                rmax = fmax(rmax, fabs(ri));
+                bmax = fmax(bmax, fabs(bv[i]));
+            }
+            if (lane == 0) {
+                sred[2 * wv] = rmax;
+                sred[2 * wv + 1] = bmax;
+            }
+            __syncthreads();
+            double rn = 0.0, bn = 0.0;
+            for (int q = 0; q < nw; q++) {
+                rn = fmax(rn, sred[2 * q]);
+                bn = fmax(bn, sred[2 * q + 1]);
+            }
+            __syncthreads();
+            if (rn <= 1e-15 * bn) break;
+        }
+        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
+        __syncthreads();
+        trial_state(D, W, tid, nt);
+    } else {
+        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
+    }
+    if (tid == 0) {
+        ws[WS_CHOLOK] = ok ? 1.0 : 0.0;
+        ws[WS_NREF] = nref;
+    }
+}
+
+template __global__ void k_big_prep<float>(DArgs, int);
+template __global__ void k_big_prep<double>(DArgs, int);
+template __global__ void k_big_panel<float>(DArgs, int);
+template __global__ void k_big_panel<double>(DArgs, int);
+template __global__ void k_big_update<float>(DArgs, int);
+template __global__ void k_big_update<double>(DArgs, int);
+template __global__ void k_big_solve<float>(DArgs);
+template __global__ void k_big_solve<double>(DArgs);
+
+}  // namespace xba

@@ -249,6 +249,12 @@ struct DArgs {
     const double *partc, *partc2;
     double *imu_chi2;                     // (nimu) or null: whitened chi2 of every IMU factor
     int32_t nref_max;
+    int32_t nbig;                         // windows solved by the tiled Cholesky
+    const BigWin *bigs;                   // (nbig)
+    float *bigF32;                        // tiled factors (sum np^2), mixed precision
+    double *bigF64;                       // tiled factors (sum np^2), FP64
+    double *bigS;                         // (sum np) Jacobi scaling of the tiled factors
+    const int32_t *chunk_slots;           // tile slot lists of the work items with smap >= 0
 };
 
 // ------------------------------------------------------------------------------------------- k_reduce --
@@ -291,8 +297,10 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
                 if (c == W.c1) c = W.c2;
                 if (c >= W.c3) break;
                 const Chunk ch = D.chunks[c];
-                int r2 = s2 - ch.lo, r1 = s1 - ch.lo;
-                if (r1 < 0 || r2 >= ch.R) continue;
+                const int r1 = chunk_tile(ch, D.chunk_slots, s1);
+                if (r1 < 0) continue;
+                const int r2 = chunk_tile(ch, D.chunk_slots, s2);
+                if (r2 < 0) continue;
                 int64_t e = ch.partH + (int64_t)(r2 * (r2 + 1) / 2 + r1) * 36 + 6 * i + j;
                 v.add(partH[e]);
                 if (kah) v.add(partH_lo[e]);
@@ -309,8 +317,8 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
                 if (c == W.c1) c = W.c2;
                 if (c >= W.c3) break;
                 const Chunk ch = D.chunks[c];
-                int r = s - ch.lo;
-                if (r < 0 || r >= ch.R) continue;
+                const int r = chunk_tile(ch, D.chunk_slots, s);
+                if (r < 0) continue;
                 v.add(partb[ch.partb + 6 * r + i]);
                 if (kah) v.add(partb_lo[ch.partb + 6 * r + i]);
             }
@@ -632,6 +640,27 @@ __device__ inline double dwave_max(double v) {
     return v;
 }
 
+// The trial state of a window from its update dx (vio/window.py:522-532, 945-955): R <- R Exp(dphi), p += dp, and
+// v, bg, ba additive on 15-dimensional slots.
+__device__ inline void trial_state(const DArgs &D, const WinInfo &W, int tid, int nt) {
+    for (int s = tid; s < W.P; s += nt) {
+        int gs = W.slot0 + s, o = D.slot_off[gs];
+        const double *dxs = D.dx + W.st0 + o;
+        double E[9];
+        so3_exp_nb(dxs, E);
+        mm3(D.Rwb + 9 * (size_t)gs, E, D.R2 + 9 * (size_t)gs);
+        for (int i = 0; i < 3; i++) D.p2[3 * (size_t)gs + i] = D.pwb[3 * (size_t)gs + i] + dxs[3 + i];
+        if (W.vi) {
+            bool full = D.slot_dim[gs] == 15;
+            for (int i = 0; i < 3; i++) {
+                D.v2[3 * (size_t)gs + i] = D.vel[3 * (size_t)gs + i] + (full ? dxs[6 + i] : 0.0);
+                D.g2[3 * (size_t)gs + i] = D.bgs[3 * (size_t)gs + i] + (full ? dxs[9 + i] : 0.0);
+                D.a2[3 * (size_t)gs + i] = D.bas[3 * (size_t)gs + i] + (full ? dxs[12 + i] : 0.0);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------- k_solve --
 // Damped Cholesky solve of every window, dx = -Hd^-1 b, then the trial state.  FP64 (T = double) factors Hd
 // itself; MIXED (T = float) factors the Jacobi-scaled Hd in FP32 and refines x in FP64 against Hd.
@@ -645,6 +674,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     __shared__ int sflag;
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
+    if (W.big >= 0) return;  // factored by the tiled kernels (k_big_*)
     const int n = W.n;
     double *ws = D.wstate + (size_t)NWS * w;
     const double lam = ws[WS_LAM];
@@ -709,23 +739,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
         }
         for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
         __syncthreads();
-        // trial state (vio/window.py:522-532, 945-955)
-        for (int s = tid; s < W.P; s += nt) {
-            int gs = W.slot0 + s, o = D.slot_off[gs];
-            const double *dxs = D.dx + W.st0 + o;
-            double E[9];
-            so3_exp_nb(dxs, E);
-            mm3(D.Rwb + 9 * (size_t)gs, E, D.R2 + 9 * (size_t)gs);
-            for (int i = 0; i < 3; i++) D.p2[3 * (size_t)gs + i] = D.pwb[3 * (size_t)gs + i] + dxs[3 + i];
-            if (W.vi) {
-                bool full = D.slot_dim[gs] == 15;
-                for (int i = 0; i < 3; i++) {
-                    D.v2[3 * (size_t)gs + i] = D.vel[3 * (size_t)gs + i] + (full ? dxs[6 + i] : 0.0);
-                    D.g2[3 * (size_t)gs + i] = D.bgs[3 * (size_t)gs + i] + (full ? dxs[9 + i] : 0.0);
-                    D.a2[3 * (size_t)gs + i] = D.bas[3 * (size_t)gs + i] + (full ? dxs[12 + i] : 0.0);
-                }
-            }
-        }
+        trial_state(D, W, tid, nt);
     } else {
         for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
     }

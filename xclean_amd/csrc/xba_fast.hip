@@ -160,11 +160,11 @@ struct ObsRow {
 // Residual, Huber weight and cost (FP64) and whitened Jacobians (FP64, stored in RT) of observation row k of a
 // landmark hosted at tile slot h (vio/window.py:125-146, 164-263, 284-288).  Returns the robust cost (0 inactive).
 template <typename RT>
-__device__ inline double obs_row(const KArgs<double> &A, int k, int lo, int h, double a, double b, double rho,
+__device__ inline double obs_row(const KArgs<double> &A, int k, int h, double a, double b, double rho,
                                  const double *sc64, const double *sc, const double *wc, const double *wd,
                                  ObsRow<RT> &o) {
     const int info = A.ob_info[k];
-    const int t = (info & 0xfff) - lo, cam = (info >> 12) & 1, valid = (info >> 13) & 1;
+    const int t = obs_tile(info), cam = (info >> 12) & 1, valid = (info >> 13) & 1;
     const int nd = cam == 0 ? 2 : 1;
     o.t = t;
     double e[2] = {0.0, 0.0}, Jl[2][3], Jt[2][6], Jh[2][6];
@@ -238,12 +238,13 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
         const int c = next_chunk(A, kind, cbeg, cend, ctl);
         if (c < 0) break;
         const Chunk ch = A.chunks[c];
+        const int32_t *smap = ch.smap >= 0 ? A.chunk_slots + ch.smap : nullptr;
         const WinInfo W = A.wins[ch.win];
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;
         const int R = ch.R, nblk = R * (R + 1) / 2;
         const double lamd = A.lam_override >= 0.0 ? A.lam_override : A.wstate[(size_t)NWS * ch.win + WS_LAM];
         if (wv == 0) {
-            load_slots<T>(sc, sc64, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane);
+            load_slots<T>(sc, sc64, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane, smap);
             load_wconst<T>(wc, wp, lane);
             load_wd(wd, wp, lane);
         }
@@ -281,10 +282,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 ob_bb = ob_lm[lane];
                 const int l = lb + ob_bb, k = obase + lane;
                 const int li = A.lm_info[l];
-                ob_h = (li & 0xfff) - ch.lo;
+                ob_h = lm_tile(li);
                 double cq = 0.0;
                 if ((li >> 12) & 1)
-                    cq = obs_row<RT>(A, k, ch.lo, ob_h, A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1],
+                    cq = obs_row<RT>(A, k, ob_h, A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1],
                                      A.lmp[3 * (size_t)l + 2], sc64, sc, wc, wd, ob);
                 rcq[lane] = cq;
                 const double w = (double)ob.w;
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 T *lr = lrec + lane * FB_LREC;
                 lr[LR_OK] = 0.0;
                 lr[LR_MASKED] = 0.0;
-                lh[lane] = (li & 0xfff) - ch.lo;
+                lh[lane] = lm_tile(li);
                 if ((li >> 12) & 1) {
                     const int q0 = lobs0[lane], n = lnobs[lane];
                     for (int q = q0; q < q0 + n; q++) lcost += rcq[q];
@@ -546,6 +547,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
         const int c = next_chunk(A, kind, cbeg, cend, ctl);
         if (c < 0) break;
         const Chunk ch = A.chunks[c];
+        const int32_t *smap = ch.smap >= 0 ? A.chunk_slots + ch.smap : nullptr;
         const WinInfo W = A.wins[ch.win];
         if (A.wstate[(size_t)NWS * ch.win + WS_CHOLOK] == 0.0) {
             if (tid == 0) A.partc2[c] = 0.0;
@@ -556,16 +558,16 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
         T *sc2 = sc + R * SC_STRIDE;
         double *sc642 = sc64 + R * SC64_STRIDE;
         if (wv == 0) {
-            load_slots<T>(sc, sc64, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane);
+            load_slots<T>(sc, sc64, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane, smap);
             load_wconst<T>(wc, wp, lane);
             load_wd(wd, wp, lane);
         } else if (wv == 1 % NW) {
-            load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane);
+            load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
         }
-        if (NW == 1) load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane);
+        if (NW == 1) load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
         for (int i = tid; i < 6 * R; i += blockDim.x) {
             int s = i / 6, j = i % 6;
-            dps[i] = (T)A.dx[W.st0 + A.slot_off[W.slot0 + ch.lo + s] + j];
+            dps[i] = (T)A.dx[W.st0 + A.slot_off[W.slot0 + chunk_slot(ch, A.chunk_slots, s)] + j];
         }
         __syncthreads();
         for (;;) {
@@ -585,9 +587,9 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                 q_k = obase + lane;
                 const int li = A.lm_info[q_l];
                 q_mask = (li >> 12) & 1;
-                q_h = (li & 0xfff) - ch.lo;
+                q_h = lm_tile(li);
                 const int info = A.ob_info[q_k];
-                q_t = (info & 0xfff) - ch.lo;
+                q_t = obs_tile(info);
                 q_cam = (info >> 12) & 1;
                 q_valid = (info >> 13) & 1;
                 T u0 = 0, u1 = 0, u2 = 0;

@@ -8,17 +8,36 @@
 #define XBA_WAVE 64
 
 // One work item of the landmark kernels: a contiguous landmark range of one window whose observations touch only
-// the slots [lo, lo + R).  Its reduced-system partial lives at partH (R(R+1)/2 lower-triangular 6x6 blocks) and
-// partb (6R entries) of the partial buffers.  The multi-wave kernels walk it in the passes [p0, p1) of the pass
-// list (each pass: the landmarks [pass_l0[p], pass_l0[p + 1])).
+// R slots, its tile slots: [lo, lo + R) when smap < 0, else the ascending window slots chunk_slots[smap .. smap + R)
+// (work items of landmarks spanning more slots than a tile holds, which touch few of them).  Its reduced-system
+// partial lives at partH (R(R+1)/2 lower-triangular 6x6 blocks over the tile slots) and partb (6R entries) of the
+// partial buffers.  The multi-wave kernels walk it in the passes [p0, p1) of the pass list (each pass: the
+// landmarks [pass_l0[p], pass_l0[p + 1])).
 struct Chunk {
-    int32_t win, l0, l1, lo, R, p0, p1, pad;
+    int32_t win, l0, l1, lo, R, p0, p1, smap;
     int64_t partH, partb;
 };
 
+// Window slot of tile slot t of a work item.
+__host__ __device__ inline int chunk_slot(const Chunk &ch, const int32_t *chunk_slots, int t) {
+    return ch.smap < 0 ? ch.lo + t : chunk_slots[ch.smap + t];
+}
+
+// Tile slot of window slot s in a work item, or -1 when the work item does not touch it.
+__host__ __device__ inline int chunk_tile(const Chunk &ch, const int32_t *chunk_slots, int s) {
+    if (ch.smap < 0) return (s >= ch.lo && s < ch.lo + ch.R) ? s - ch.lo : -1;
+    for (int t = 0; t < ch.R; t++) {
+        const int v = chunk_slots[ch.smap + t];
+        if (v == s) return t;
+        if (v > s) break;
+    }
+    return -1;
+}
+
 // Per-window record.
 struct WinInfo {
-    int32_t slot0, P, lm0, L, n, c0, c1, imu0, imu1, pblk0, pblk1, prow0, prow1, np, vi, pad;
+    int32_t slot0, P, lm0, L, n, c0, c1, imu0, imu1, pblk0, pblk1, prow0, prow1, np, vi;
+    int32_t big;     // index into the tiled-solve window list, -1 when the one-workgroup solve handles the window
     int32_t c2, c3;  // second work-item range of the window (landmarks coupling more slots than the narrow tile)
     int64_t st0;   // offset of the window's state vector (dx, b)
     int64_t h0;    // offset of the window's n x n matrices
@@ -41,14 +60,27 @@ enum {
     WS_CUR,
     WS_ITERS,
     WS_COSTTRIAL_REST,
+    WS_BIGOK,       // tiled Cholesky of the window: 1 while every pivot so far was positive and finite
     NWS = 16
 };
 
-// Packed observation word: slot (bits 0-11) | cam << 12 | valid << 13.  Packed landmark word: host slot (bits
-// 0-11) | mask << 12.
-__host__ __device__ inline int32_t pack_obs(int slot, int cam, int valid) {
-    return slot | (cam << 12) | (valid << 13);
+// A window whose reduced system is factored by the tiled multi-workgroup Cholesky: its matrix is padded to np, a
+// multiple of XBA_TILE, with an identity block on the padding, and stored row-major at f0 of the tiled-factor
+// buffer; its Jacobi scaling (or ones) at s0 of the scale buffer.
+#define XBA_TILE 64
+struct BigWin {
+    int32_t win, n, np, nb;  // window, state dimension, padded dimension, tiles per side (np / XBA_TILE)
+    int64_t f0, s0;
+};
+
+// Packed observation word: slot (bits 0-11) | cam << 12 | valid << 13 | tile slot << 14 (bits 14-19, the slot's
+// index in its landmark's work-item tile).  Packed landmark word: host slot (bits 0-11) | mask << 12 | host tile
+// slot << 13 (bits 13-18).
+__host__ __device__ inline int32_t pack_obs(int slot, int cam, int valid, int tile) {
+    return slot | (cam << 12) | (valid << 13) | (tile << 14);
 }
+__device__ inline int obs_tile(int info) { return (info >> 14) & 63; }
+__device__ inline int lm_tile(int li) { return (li >> 13) & 63; }
 
 template <typename T>
 struct KArgs {
@@ -86,4 +118,5 @@ struct KArgs {
     int32_t *sched_ctr;        // multi-wave kernels: per launch kind, a hand-out counter and a finished counter
     double *pass_cost;         // multi-wave kernels: robust cost of every pass (summed per work item in pass order)
     const double *ob_w;        // (nobs) per-observation weight multiplier, or null (1)
+    const int32_t *chunk_slots; // tile slot lists of the work items with smap >= 0
 };

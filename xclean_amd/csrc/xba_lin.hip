@@ -61,15 +61,16 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int ts) {
 }
 
 // --------------------------------------------------------------------------------------- per-slot caches --
-// Camera poses of the slots [lo, lo+R) of window w: Rwc = Rwb Rbc, pwc = pwb + Rwb pbc (vio/window.py:110-122),
-// computed in FP64 and stored in the working precision.
+// Camera poses of the R tile slots of a work item of window w (slots lo + t, or map[t] when map is given):
+// Rwc = Rwb Rbc, pwc = pwb + Rwb pbc (vio/window.py:110-122), computed in FP64 and stored in the working precision.
 template <typename T>
 __device__ void load_slots(T *sc, double *sc64, const double *Rwb, const double *pwb, const double *wp, int slot0,
-                           int lo, int R, int lane) {
+                           int lo, int R, int lane, const int32_t *map = nullptr) {
     const double *Rbc = wp + 8, *pbc = wp + 17;
     for (int s = lane; s < R; s += XBA_WAVE) {
-        const double *Rb = Rwb + 9 * (size_t)(slot0 + lo + s);
-        const double *pb = pwb + 3 * (size_t)(slot0 + lo + s);
+        const int ws = slot0 + (map ? map[s] : lo + s);
+        const double *Rb = Rwb + 9 * (size_t)ws;
+        const double *pb = pwb + 3 * (size_t)ws;
         T *c = sc + s * SC_STRIDE;
         double *c64 = sc64 + s * SC64_STRIDE;
         for (int i = 0; i < 3; i++) {

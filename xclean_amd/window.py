@@ -145,7 +145,7 @@ class BatchSolver:
     """Solve many windows at once on the GPU.  ``problems`` is a list of ``Problem``s (mutated by ``download``)."""
 
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
-                 device="cuda", deterministic=False, with_weights=False):
+                 device="cuda", deterministic=False, with_weights=False, big_n=0):
         import torch
         _lib.require_gpu()
         self.torch = torch
@@ -235,6 +235,7 @@ class BatchSolver:
         o.precision, o.lm_per_chunk, o.slot_tile, o.grid, o.use_graph = self.prec, lm_per_chunk, slot_tile, grid, int(
             bool(use_graph))
         o.deterministic = int(bool(deterministic))
+        o.big_n = int(big_n)
         self.opts = o
         ctx = C.c_void_p()
         _lib.check(self.L.xba_create(C.byref(ctx), C.byref(b), C.byref(o)), "xba_create")
