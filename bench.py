@@ -44,6 +44,47 @@ def _measured_traffic():
     return float(m.group(1)) if m else None
 
 
+def single_window_configs(torch, synth, BatchSolver, log):
+    """Configurations 1 (20 visual-inertial keyframes, n = 300) and 4 (60 visual-inertial keyframes, n = 900,
+    200,000 landmarks): one LM iteration from the initial state replayed from its graph, mixed precision and FP64,
+    median of 7; the mixed-vs-FP64 first-update error; the damped solve's share (tiled Cholesky + refinement)."""
+    out = {}
+    for cfg in (1, 4):
+        pb, _ = synth.make_config(cfg)
+        rec = {"keyframes": pb.P, "n": pb.n_state(), "landmarks": pb.L, "obs_rows": pb.M,
+               "imu_factors": len(pb.imu or [])}
+        dxs = {}
+        for prec in ("mixed", "fp64"):
+            b = BatchSolver([pb.copy()], precision=prec, use_graph=True)
+            b.upload(1e-4)
+            b.iterate(1)
+            dxs[prec] = b.last_dx()[0]
+            ts = []
+            for _ in range(7):
+                b.upload(1e-4)
+                torch.cuda.synchronize(b.dev)
+                a = time.perf_counter()
+                b.iterate(1)
+                torch.cuda.synchronize(b.dev)
+                ts.append(time.perf_counter() - a)
+            t = float(np.median(ts))
+            b.upload(1e-4)
+            k = b.profile_iteration()
+            st = b.stats()[0]
+            n = pb.n_state()
+            rec[prec] = {"value": pb.M / t, "unit": "obs/s per LM iteration", "ms_per_iter": t * 1e3,
+                         "solve_ms": k["chol_solve"], "refinement_solves": st["refine_steps"],
+                         "solve_effective_tflops": (n ** 3 / 3.0 + 2.0 * n * n) / (k["chol_solve"] * 1e-3) / 1e12,
+                         "kernel_ms": k}
+            del b
+        rec["dx_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(dxs["mixed"] - dxs["fp64"]) /
+                                                np.linalg.norm(dxs["fp64"]))
+        log("bench: config %d: mixed %.3e / fp64 %.3e obs/s per LM iteration, mixed-vs-fp64 dx err %.2e" % (
+            cfg, rec["mixed"]["value"], rec["fp64"]["value"], rec["dx_rel_err_mixed_vs_fp64"]))
+        out["config%d" % cfg] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +95,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (three launches) for HBM counter calibration")
+    ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner (solver defaults)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the single-window configurations 1 and 4")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (windows would shard across GPUs with "
@@ -76,8 +119,14 @@ def main():
                                                                                        time.time() - t0))
 
     bs = BatchSolver([pb.copy() for pb in pbs], precision="mixed", use_graph=True)
-    log("bench: plan", bs.plan_info())
     dev = bs.dev
+    tune = None
+    if not args.no_tune:  # plan options picked on this GPU before the timed region (not timed)
+        from xclean_amd import autotune
+        tune = autotune.tune(bs, repeats=3, log=log)
+        log("bench: autotune -> %s (%.3f ms per LM iteration; %s)" % (
+            tune["options"], tune["ms"], "cached" if tune["cached"] else "%.1f s" % tune["seconds"]))
+    log("bench: plan", bs.plan_info())
 
     def step():
         bs.upload(1e-4)
@@ -136,8 +185,9 @@ def main():
     # observations and landmarks that launch covers
     M_n = L_n = 0
     ks_n = []
+    narrow_slots = bs.plan_options["slot_tile"] or NARROW_SLOTS
     for w in bs.packed:
-        narrow = roofline.landmark_span(w.lm_host, w.ob_start, w.ob_pose) <= NARROW_SLOTS
+        narrow = roofline.landmark_span(w.lm_host, w.ob_start, w.ob_pose) <= narrow_slots
         cnt = np.diff(w.ob_start)
         M_n += int(cnt[narrow].sum())
         L_n += int(narrow.sum())
@@ -152,6 +202,8 @@ def main():
     ks = np.concatenate([roofline.touched_slots(w.lm_host, w.ob_start, w.ob_pose) for w in bs.packed])
     it_flops = roofline.iteration_flops(M, ks, ns)
     traffic = _measured_traffic()
+
+    extra = None if args.no_extra else single_window_configs(torch, synth, BatchSolver, log)
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +261,10 @@ def main():
                              "work runs on the VALU, whose FP64 rate equals the published FP64 matrix peak"
                              % (lin_flops / lin_bytes)},
         "iteration_flops": it_flops,
+        "autotune": None if tune is None else {"options": tune["options"], "ms_per_lm_iteration": tune["ms"],
+                                               "cached": tune["cached"], "seconds": tune["seconds"],
+                                               "trials": len(tune["trials"])},
+        "single_window_configs": extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,40 @@
+"""The autotuner's search and candidate space (host logic, no GPU): coordinate descent keeps the fastest value of
+each option in turn, skips plans the solver rejects, never re-times a plan, and the candidate space always holds the
+solver's defaults."""
+import numpy as np
+import pytest
+
+from xclean_amd import autotune as AT
+
+
+def test_coordinate_descent_finds_the_separable_optimum_and_skips_rejected_plans():
+    best = dict(slot_tile=12, lm_per_chunk=64, big_n=128, grid=0)
+    space = dict(slot_tile=[0, 8, 12, 20], lm_per_chunk=[0, 64, 256], big_n=[0, 128, -1], grid=[0, 512])
+    seen = []
+
+    def measure(o):
+        seen.append(tuple(sorted(o.items())))
+        if o["slot_tile"] == 20:
+            raise RuntimeError("tile does not fit")
+        return 1.0 + sum(0.1 * (o[k] != best[k]) for k in best)
+
+    opts, t, trials = AT.search(measure, space)
+    assert opts == best
+    assert t == pytest.approx(1.0)
+    assert len(seen) == len(set(seen))                       # no plan timed twice
+    assert any(tt is None for _o, tt in trials)              # the rejected plan is recorded as such
+
+
+def test_search_raises_when_the_default_plan_is_rejected():
+    with pytest.raises(RuntimeError):
+        AT.search(lambda o: None, dict(slot_tile=[8]))
+
+
+def test_candidate_space_holds_the_defaults_and_only_moving_thresholds():
+    sp = AT.candidate_space(nlm=2_000_000, ncu=256, ns=np.array([30, 90, 180]))
+    for k in AT.ORDER:
+        assert sp[k][0] == 0
+    assert sp["big_n"] == [0, 96, 128]                       # 256 / 384 / 512 / never change nothing here
+    sp2 = AT.candidate_space(nlm=200_000, ncu=256, ns=np.array([900]))
+    assert sp2["big_n"] == [0, -1]                           # the tiled solve or one workgroup
+    assert all(8 <= v <= 4096 for v in sp2["lm_per_chunk"][1:])

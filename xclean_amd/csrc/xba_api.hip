@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -110,6 +111,7 @@ struct xba_ctx {
             *ob_pose = nullptr, *imu_ij = nullptr, *prior_slot = nullptr, *prior_dim = nullptr, *lm_win = nullptr;
     int32_t *ob_info = nullptr, *lm_info = nullptr;
     int32_t *chunk_slots = nullptr;                  // tile slot lists of the compacted work items
+    int32_t *pair_start = nullptr, *pair_ent = nullptr, *bslot_start = nullptr, *bslot_ent = nullptr;
     uint8_t *ob_tile = nullptr, *lm_tile = nullptr;  // tile slot of every observation / landmark host (fast family)
     void *isigT = nullptr, *lmpT = nullptr, *lmp_trial = nullptr, *riT = nullptr;
     void *partH = nullptr, *partb = nullptr, *partH_lo = nullptr, *partb_lo = nullptr, *lmHi = nullptr, *lmbl = nullptr;
@@ -119,12 +121,14 @@ struct xba_ctx {
     double *Rwb = nullptr, *pwb = nullptr, *vel = nullptr, *bgs = nullptr, *bas = nullptr;
     double *R2 = nullptr, *p2 = nullptr, *v2 = nullptr, *g2 = nullptr, *a2 = nullptr;
     double *H = nullptr, *b = nullptr, *dx = nullptr, *fact64 = nullptr, *imu_scr = nullptr, *wstate = nullptr;
+    double *imu_out = nullptr;
+    int32_t *imu_win = nullptr, *imu_color = nullptr;
     float *fact32 = nullptr;
     // windows factored by the tiled Cholesky
-    int nbig = 0, big_npmax = 0, big_nbmax = 0;
+    int nbig = 0, big_npmax = 0, big_nbmax = 0, big_nmax = 0;
     BigWin *bigs = nullptr;
     float *bigF32 = nullptr;
-    double *bigF64 = nullptr, *bigS = nullptr;
+    double *bigF64 = nullptr, *bigS = nullptr, *bigV = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t gstream = nullptr;
@@ -205,6 +209,9 @@ DArgs dargs(xba_ctx *c) {
     D.fact64 = c->fact64;
     D.fact32 = c->fact32;
     D.imu_scr = c->imu_scr;
+    D.imu_out = c->imu_out;
+    D.imu_win = c->imu_win;
+    D.imu_color = c->imu_color;
     D.wstate = c->wstate;
     D.partc = c->partc;
     D.partc2 = c->partc2;
@@ -214,7 +221,12 @@ DArgs dargs(xba_ctx *c) {
     D.bigF32 = c->bigF32;
     D.bigF64 = c->bigF64;
     D.bigS = c->bigS;
+    D.bigV = c->bigV;
     D.chunk_slots = c->chunk_slots;
+    D.pair_start = c->pair_start;
+    D.pair_ent = c->pair_ent;
+    D.bslot_start = c->bslot_start;
+    D.bslot_ent = c->bslot_ent;
     return D;
 }
 
@@ -250,6 +262,12 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
     F.ob_w = A.ob_w;
     F.chunk_slots = A.chunk_slots;
     return F;
+}
+
+// Prior and IMU terms of every window: the IMU factors in parallel (k_imu), then one workgroup per window.
+void launch_terms(xba_ctx *c, const DArgs &D, hipStream_t s, int trial) {
+    if (c->nimu > 0) hipLaunchKernelGGL(k_imu, dim3(c->nimu), dim3(256), 0, s, D, trial);
+    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, trial);
 }
 
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
@@ -301,7 +319,7 @@ void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 =
     DArgs D = dargs(c);
     D.imu_chi2 = imu_chi2;
     launch_reduce<T>(c, D, s);
-    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
+    launch_terms(c, D, s, 0);
 }
 
 // The damped solve and trial state of every window: the tiled Cholesky sequence for the large windows, one
@@ -312,11 +330,18 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
         hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
         for (int k = 0; k < c->big_nbmax; k++) {
             const int below = c->big_nbmax - k - 1;
-            hipLaunchKernelGGL(k_big_panel<T>, dim3(std::max(1, (below + 3) / 4), c->nbig), dim3(256), 0, s, D, k);
+            hipLaunchKernelGGL(k_big_panel<T>, dim3(1 + (below + 3) / 4, c->nbig), dim3(256), 0, s, D, k);
             if (below > 0)
                 hipLaunchKernelGGL(k_big_update<T>, dim3(below * (below + 1) / 2, c->nbig), dim3(256), 0, s, D, k);
         }
-        hipLaunchKernelGGL(k_big_solve<T>, dim3(c->nbig), dim3(512), 0, s, D);
+        const int maxit = sizeof(T) == 8 ? 1 : D.nref_max;
+        for (int it = 0; it < maxit; it++) {
+            hipLaunchKernelGGL(k_big_trsv<T>, dim3(c->nbig), dim3(1024), 0, s, D, it);
+            if (it + 1 < maxit)
+                hipLaunchKernelGGL(k_big_resid, dim3((c->big_nmax + XBA_RES_ROWS - 1) / XBA_RES_ROWS, c->nbig),
+                                   dim3(256), 0, s, D);
+        }
+        hipLaunchKernelGGL(k_big_finish, dim3(c->nbig), dim3(256), 0, s, D);
     }
     if (c->nbig < c->nwin)
         hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
@@ -329,7 +354,7 @@ void launch_iter(xba_ctx *c, hipStream_t s) {
     DArgs D = dargs(c);
     launch_solve<T>(c, D, s);
     launch_trial_kernel<T>(c, A, s);
-    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
+    launch_terms(c, D, s, 1);
     hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
     int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
     hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
@@ -415,7 +440,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     };
     // ---- per-window records
     std::vector<WinInfo> wins(nwin);
-    std::vector<int32_t> slot_off(std::max(c->nslot, 1)), colmap, lm_win(std::max(c->nlm, 1));
+    std::vector<int32_t> slot_off(std::max(c->nslot, 1)), colmap, lm_win(std::max(c->nlm, 1)),
+        imu_win(std::max(c->nimu, 1)), imu_color(std::max(c->nimu, 1));
     int64_t st = 0, h0 = 0, prS = 0;
     int maxn = 0;
     bool any_vi = false;
@@ -467,6 +493,23 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
                 return fail(XBA_E_ARG);
         }
         for (int l = W.lm0; l < W.lm0 + W.L; l++) lm_win[l] = w;
+        for (int f = W.imu0; f < W.imu1; f++) imu_win[f] = w;
+        // colours: the lowest one not used by an earlier factor of the window sharing a slot
+        W.imu_ncol = 0;
+        for (int f = W.imu0; f < W.imu1; f++) {
+            int col = 0;
+            for (bool clash = true; clash; ) {
+                clash = false;
+                for (int g = W.imu0; g < f && !clash; g++)
+                    if (imu_color[g] == col && (B.imu_ij[2 * g] == B.imu_ij[2 * f] || B.imu_ij[2 * g] == B.imu_ij[2 * f + 1] ||
+                                                B.imu_ij[2 * g + 1] == B.imu_ij[2 * f] ||
+                                                B.imu_ij[2 * g + 1] == B.imu_ij[2 * f + 1]))
+                        clash = true;
+                if (clash) col++;
+            }
+            imu_color[f] = col;
+            W.imu_ncol = std::max(W.imu_ncol, col + 1);
+        }
     }
     if (any_vi && (!B.vel || !B.bgs || !B.bas)) return fail(XBA_E_ARG);
     // windows above big_n: the tiled Cholesky (padded to whole 64 x 64 tiles)
@@ -485,12 +528,14 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             bw.nb = bw.np / XBA_TILE;
             bw.f0 = bigF;
             bw.s0 = bigSv;
+            bw.v0 = 3 * bigSv;
             bigF += big_matrix_entries(bw.np);
             bigSv += bw.np;
             W.big = (int32_t)bigs.size();
             bigs.push_back(bw);
             c->big_npmax = std::max(c->big_npmax, bw.np);
             c->big_nbmax = std::max(c->big_nbmax, bw.nb);
+            c->big_nmax = std::max(c->big_nmax, bw.n);
         }
         c->nbig = (int)bigs.size();
     }
@@ -633,6 +678,54 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         pb += 6 * ch.R;
         Rmax = std::max(Rmax, ch.R);
     }
+    if (pH / 36 > INT32_MAX || pb / 6 > INT32_MAX) return fail(XBA_E_SIZE);
+    // ---- reduction lists (k_reduce): per slot pair of a window, the partial blocks holding it, in the window's
+    // work-item order (narrow items, then wide ones); per slot, its partial gradient segments
+    std::vector<int32_t> pair_start, pair_ent, bslot_start(c->nslot + 1, 0), bslot_ent;
+    {
+        int64_t npairs = 0;
+        for (int w = 0; w < nwin; w++) {
+            if (npairs > INT32_MAX) return fail(XBA_E_SIZE);
+            wins[w].pair0 = (int32_t)npairs;
+            npairs += (int64_t)wins[w].P * (wins[w].P + 1) / 2;
+        }
+        if (npairs + 1 > INT32_MAX) return fail(XBA_E_SIZE);
+        pair_start.assign(npairs + 1, 0);
+        auto each_block = [&](auto &&fn) {
+            for (int w = 0; w < nwin; w++) {
+                for (int pass = 0; pass < 2; pass++) {
+                    const int cb = pass ? wins[w].c2 : wins[w].c0, ce = pass ? wins[w].c3 : wins[w].c1;
+                    for (int ci = cb; ci < ce; ci++) {
+                        const Chunk &ch = chunks[ci];
+                        for (int r2 = 0; r2 < ch.R; r2++) {
+                            const int s2 = chunk_slot(ch, chunk_slots.data(), r2);
+                            fn(w, ch, r2, s2, -1, -1);
+                            for (int r1 = 0; r1 <= r2; r1++)
+                                fn(w, ch, r2, s2, r1, chunk_slot(ch, chunk_slots.data(), r1));
+                        }
+                    }
+                }
+            }
+        };
+        each_block([&](int w, const Chunk &, int, int s2, int r1, int s1) {
+            if (r1 < 0)
+                bslot_start[wins[w].slot0 + s2 + 1]++;
+            else
+                pair_start[wins[w].pair0 + s2 * (s2 + 1) / 2 + s1 + 1]++;
+        });
+        for (size_t q = 1; q < pair_start.size(); q++) pair_start[q] += pair_start[q - 1];
+        for (size_t q = 1; q < bslot_start.size(); q++) bslot_start[q] += bslot_start[q - 1];
+        pair_ent.resize(std::max<int32_t>(pair_start.back(), 1));
+        bslot_ent.resize(std::max<int32_t>(bslot_start.back(), 1));
+        std::vector<int32_t> pf(pair_start.begin(), pair_start.end() - 1), bf(bslot_start.begin(), bslot_start.end() - 1);
+        each_block([&](int w, const Chunk &ch, int r2, int s2, int r1, int s1) {
+            if (r1 < 0)
+                bslot_ent[bf[wins[w].slot0 + s2]++] = (int32_t)(ch.partb / 6 + r2);
+            else
+                pair_ent[pf[wins[w].pair0 + s2 * (s2 + 1) / 2 + s1]++] =
+                    (int32_t)(ch.partH / 36 + r2 * (r2 + 1) / 2 + r1);
+        });
+    }
     // ---- pass plan and hand-out order of the multi-wave kernels: every work item is cut into passes whose
     // observations fit one lane each and whose touched slots fit the pass's U storage; work items are handed out
     // largest first within each class
@@ -730,6 +823,10 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->prior_dim = dalloc<int32_t>(c->npblk, O);
     c->ob_info = dalloc<int32_t>(c->nobs, O);
     c->chunk_slots = dalloc<int32_t>(chunk_slots.size(), O);
+    c->pair_start = dalloc<int32_t>(pair_start.size(), O);
+    c->pair_ent = dalloc<int32_t>(pair_ent.size(), O);
+    c->bslot_start = dalloc<int32_t>(bslot_start.size(), O);
+    c->bslot_ent = dalloc<int32_t>(bslot_ent.size(), O);
     if (c->fast) {
         c->ob_tile = dalloc<uint8_t>(c->nobs, O);
         c->lm_tile = dalloc<uint8_t>(c->nlm, O);
@@ -769,11 +866,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->adj_y = dalloc<double>(c->nstate, O);
     c->fact32 = mx ? dalloc<float>(h0, O) : dalloc<float>(1, O);
     c->imu_scr = dalloc<double>(465 * (size_t)std::max(c->nimu, 1), O);
+    c->imu_out = dalloc<double>(IMU_OUT * (size_t)std::max(c->nimu, 1), O);
+    c->imu_win = dalloc<int32_t>(std::max(c->nimu, 1), O);
+    c->imu_color = dalloc<int32_t>(std::max(c->nimu, 1), O);
     c->wstate = dalloc<double>((size_t)NWS * nwin, O);
     c->bigs = dalloc<BigWin>(bigs.size(), O);
     c->bigF32 = (mx && c->nbig) ? dalloc<float>(bigF, O) : dalloc<float>(1, O);
     c->bigF64 = (!mx && c->nbig) ? dalloc<double>(bigF, O) : dalloc<double>(1, O);
     c->bigS = dalloc<double>(bigSv, O);
+    c->bigV = dalloc<double>(3 * bigSv, O);
     for (void *p : O)
         if (!p) return fail(XBA_E_HIP);
     auto up = [&](void *dst, const void *src, size_t bytes) -> bool {
@@ -786,11 +887,16 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
               up(c->wins, wins.data(), nwin * sizeof(WinInfo)) &&
               (bigs.empty() || up(c->bigs, bigs.data(), bigs.size() * sizeof(BigWin))) &&
               (chunk_slots.empty() || up(c->chunk_slots, chunk_slots.data(), chunk_slots.size() * 4)) &&
+              up(c->pair_start, pair_start.data(), pair_start.size() * 4) &&
+              up(c->pair_ent, pair_ent.data(), pair_ent.size() * 4) &&
+              up(c->bslot_start, bslot_start.data(), bslot_start.size() * 4) &&
+              up(c->bslot_ent, bslot_ent.data(), bslot_ent.size() * 4) &&
               (!c->fast || (up(c->ob_tile, ob_tile.data(), (size_t)c->nobs) && up(c->lm_tile, lm_tile.data(), (size_t)c->nlm))) &&
               up(c->colmap, colmap.data(), colmap.size() * 4) && up(c->slot_off, slot_off.data(), c->nslot * 4) &&
               up(c->slot_dim, B.slot_dim, c->nslot * 4) && up(c->ob_start, B.ob_start, (c->nlm + 1) * 4) &&
               up(c->lm_host, B.lm_host, c->nlm * 4) && up(c->ob_pose, B.ob_pose, (size_t)c->nobs * 4) &&
-              up(c->lm_win, lm_win.data(), c->nlm * 4) && up(c->imu_ij, B.imu_ij, 2 * (size_t)c->nimu * 4) &&
+              up(c->lm_win, lm_win.data(), c->nlm * 4) && up(c->imu_win, imu_win.data(), (size_t)c->nimu * 4) &&
+              up(c->imu_color, imu_color.data(), (size_t)c->nimu * 4) && up(c->imu_ij, B.imu_ij, 2 * (size_t)c->nimu * 4) &&
               up(c->prior_slot, B.prior_slot, c->npblk * 4) && up(c->prior_dim, B.prior_dim, c->npblk * 4);
     if (!ok) return fail(XBA_E_HIP);
     if (hipMemset(c->sched_ctr, 0, 8 * sizeof(int32_t)) != hipSuccess) return fail(XBA_E_HIP);
@@ -976,7 +1082,7 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
         (void)hipEventRecord(ev[2], s);
         launch_reduce<T>(c, D, s);
         (void)hipEventRecord(ev[3], s);
-        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 0);
+        launch_terms(c, D, s, 0);
         (void)hipEventRecord(ev[4], s);
         launch_solve<T>(c, D, s);
         (void)hipEventRecord(ev[5], s);
@@ -984,7 +1090,7 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
         (void)hipEventRecord(ev[6], s);
         launch_trial_kernel<T>(c, A, s, 1);
         (void)hipEventRecord(ev[7], s);
-        hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, 1);
+        launch_terms(c, D, s, 1);
         (void)hipEventRecord(ev[8], s);
         hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
         (void)hipEventRecord(ev[9], s);

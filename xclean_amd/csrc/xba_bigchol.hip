@@ -4,16 +4,18 @@
 //   k_big_prep    Hd = H + diag(lam diag(H) + 1e-9) (vio/window.py:933-936), Jacobi-scaled in mixed precision,
 //                 copied into the padded layout (identity on the padding), lower tiles only
 //   k_big_panel   step k: the diagonal tile factored in registers by one wavefront (every workgroup of the launch
-//                 factors it; the first one stores it), then L(i,k) = A(i,k) L(k,k)^-T for the tiles below it, one
-//                 tile per wavefront with a row per lane
+//                 factors it; the first one also inverts it into a side buffer), then L(i,k) = A(i,k) L(k,k)^-T for
+//                 the tiles below it, one tile per wavefront with a row per lane
 //   k_big_update  step k: A(i,j) -= L(i,k) L(j,k)^T over the trailing lower tiles on the matrix cores (FP32 or
 //                 FP64 16x16x4 MFMA), one tile per workgroup, 32 x 32 per wavefront
-//   k_big_solve   the diagonal factor tiles put in place, then forward / back substitution and, in mixed precision,
-//                 FP64 iterative refinement against the FP64 damped matrix (vio/window.py:937-941), dx = -Hd^-1 b,
-//                 and the trial state
+//   k_big_trsv    one solve L L^T y = s of the scaled residual with the tiled factor (one workgroup per window:
+//                 tile-row matrix-vector products and the diagonal-tile inverses, no serial column loop), x += y
+//   k_big_resid   r = b - Hd x in FP64 over many workgroups, with per-workgroup maxima for the stopping test
+//                 |r| <= 1e-15 |b| (mixed precision refines until it holds or the step budget ends)
+//   k_big_finish  dx = -x (vio/window.py:937-941) and the trial state
 //
-// The diagonal tiles are stored into a side buffer (np x 64 after the matrix) during the factorisation because the
-// other workgroups of the same panel launch still read A(k,k) from the matrix; k_big_solve copies them into place.
+// The diagonal tiles of the factor are only needed as their inverses, which the solves apply; the matrix keeps
+// A(k,k) (the other workgroups of the same panel launch still read it).
 #include "xba_internal.h"
 
 namespace xba {
@@ -85,7 +87,10 @@ __global__ __launch_bounds__(256) void k_big_prep(DArgs D, int gn) {
     }
     if (tid == 0) {
         D.bigS[B.s0 + i] = si;
-        if (i == 0) ws[WS_BIGOK] = 1.0;
+        if (i == 0) {
+            ws[WS_BIGOK] = 1.0;
+            ws[WS_BIGCONV] = 0.0;
+        }
     }
 }
 
@@ -109,15 +114,19 @@ __device__ inline double big_bcast(double v, int jj) {
     } while (0)
 
 // --------------------------------------------------------------------------------------------- k_big_panel --
-// grid (max(1, ceil((max nb - k - 1) / 4)), nbig), 256 threads: wavefront w of workgroup x solves tile row
-// i = k + 1 + 4 x + w.
+// grid (1 + ceil((max nb - k - 1) / 4), nbig), 256 threads.  Every workgroup factors the diagonal tile (k,k) in
+// registers (wavefront 0, lane r holding row r); workgroup 0 then inverts it into the side buffer, and wavefront w
+// of workgroup x >= 1 solves tile row i = k + 1 + 4 (x - 1) + w.
 template <typename T>
 __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
-    __shared__ T sLt[XBA_TILE][XBA_TILE];  // the diagonal factor transposed: sLt[j][m] = L[m][j] (0 for m < j)
-    __shared__ T scol[XBA_TILE], sinv[XBA_TILE];
+    __shared__ __align__(16) T sLt[XBA_TILE][XBA_TILE];  // the factor transposed: sLt[j][m] = L[m][j] (0, m < j)
+    __shared__ __align__(16) T scol[XBA_TILE];
+    __shared__ T sinv[XBA_TILE];
     __shared__ int sok;
     const BigWin B = D.bigs[blockIdx.y];
     if (k >= B.nb) return;
+    const int x = blockIdx.x, i = k + 1 + 4 * (x - 1) + (int)(threadIdx.x >> 6);
+    if (x > 0 && k + 1 + 4 * (x - 1) >= B.nb) return;  // no tile row of this window for this workgroup
     double *ws = D.wstate + (size_t)NWS * B.win;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, np = B.np;
     // the flag is read once per workgroup: another workgroup of this launch may clear it meanwhile
@@ -156,10 +165,21 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
 #pragma unroll
             for (int m = 0; m < XBA_TILE; m++) sLt[m][lane] = (m <= lane) ? a[m] : (T)0;
             sinv[lane] = dinv;
-            if (blockIdx.x == 0) {  // the factor's diagonal tile goes to the side buffer (row-major 64 wide)
+            if (x == 0) {
+                // row `lane` of L(k,k)^-1 (X L = I, X lower triangular, by back substitution over the columns) into
+                // the side buffer (row-major, 64 wide)
+                BIG_WAVE_SYNC();
+                T xr[XBA_TILE];
+#pragma unroll
+                for (int j = XBA_TILE - 1; j >= 0; j--) {
+                    T v = (lane == j) ? (T)1 : (T)0;
+#pragma unroll
+                    for (int m = j + 1; m < XBA_TILE; m++) v -= xr[m] * sLt[j][m];
+                    xr[j] = v * sinv[j];
+                }
                 T *Dr = F + (size_t)np * np + (size_t)(k * XBA_TILE + lane) * XBA_TILE;
 #pragma unroll
-                for (int m = 0; m < XBA_TILE; m++) Dr[m] = (m <= lane) ? a[m] : (T)0;
+                for (int m = 0; m < XBA_TILE; m++) Dr[m] = xr[m];
             }
         }
     }
@@ -169,21 +189,20 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
         return;
     }
     // L(i,k) = A(i,k) L(k,k)^-T: lane r solves x L^T = a for row r of the tile by forward substitution
-    const int i = k + 1 + 4 * (int)blockIdx.x + wv;
-    if (i < B.nb) {
+    if (x > 0 && i < B.nb) {
         T *Xr = F + (size_t)(i * XBA_TILE + lane) * np + k * XBA_TILE;
-        T x[XBA_TILE];
+        T xv[XBA_TILE];
 #pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) x[m] = Xr[m];
+        for (int m = 0; m < XBA_TILE; m++) xv[m] = Xr[m];
 #pragma unroll
         for (int j = 0; j < XBA_TILE; j++) {
-            const T xj = x[j] * sinv[j];
-            x[j] = xj;
+            const T xj = xv[j] * sinv[j];
+            xv[j] = xj;
 #pragma unroll
-            for (int m = j + 1; m < XBA_TILE; m++) x[m] -= xj * sLt[j][m];
+            for (int m = j + 1; m < XBA_TILE; m++) xv[m] -= xj * sLt[j][m];
         }
 #pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) Xr[m] = x[m];
+        for (int m = 0; m < XBA_TILE; m++) Xr[m] = xv[m];
     }
 }
 
@@ -255,79 +274,173 @@ __global__ __launch_bounds__(256) void k_big_update(DArgs D, int k) {
             }
 }
 
-// --------------------------------------------------------------------------------------------- k_big_solve --
-// grid (nbig), 512 threads: dx = -Hd^-1 b with the tiled factor (FP64: one solve; mixed: FP32 solves of the
-// scaled system refined in FP64 against Hd until |r| <= 1e-15 |b|), then the trial state.
+// ---------------------------------------------------------------------------------------------- k_big_trsv --
+// grid (nbig), 1024 threads.  Refinement step `it`: it = 0 solves for the right-hand side b from x = 0; a later step
+// first tests the residual of the previous one (|r| <= 1e-15 |b|: converged, nothing more to do), then solves for r.
+// The solve runs in the factor's precision on s = scale * rhs and adds scale * y to x (FP64).
+__device__ inline float big_red16(float v) {
+    for (int d = 8; d >= 1; d >>= 1) v += __shfl_xor(v, d, 16);
+    return v;
+}
+__device__ inline double big_red16(double v) {
+    for (int d = 8; d >= 1; d >>= 1) v += __shfl_xor(v, d, 16);
+    return v;
+}
+
 template <typename T>
-__global__ __launch_bounds__(512) void k_big_solve(DArgs D) {
-    __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
-    __shared__ T sy[XBA_MAX_STATE];
-    __shared__ double sred[64];
+__global__ __launch_bounds__(1024) void k_big_trsv(DArgs D, int it) {
+    __shared__ T sv[XBA_MAX_STATE];
+    __shared__ T st[XBA_TILE];
+    __shared__ T spart[16][XBA_TILE];
+    __shared__ double sred[32];
+    __shared__ int sgo;
     const BigWin B = D.bigs[blockIdx.x];
     const WinInfo W = D.wins[B.win];
-    const int n = B.n, np = B.np, tid = threadIdx.x, nt = blockDim.x;
+    const int n = B.n, np = B.np, nb = B.nb, tid = threadIdx.x, nt = blockDim.x;
     double *ws = D.wstate + (size_t)NWS * B.win;
+    double *x = D.bigV + B.v0, *r = x + np, *pm = r + np;
+    const double *ss = D.bigS + B.s0, *bv = D.b + W.st0;
+    if (tid == 0) sgo = ws[WS_BIGOK] != 0.0 && ws[WS_BIGCONV] == 0.0;
+    __syncthreads();
+    if (!sgo) return;
+    if (it > 0) {
+        const int nres = (n + XBA_RES_ROWS - 1) / XBA_RES_ROWS;
+        double rn = 0.0, bn = 0.0;
+        for (int q = tid; q < nres; q += nt) {
+            rn = fmax(rn, pm[2 * q]);
+            bn = fmax(bn, pm[2 * q + 1]);
+        }
+        rn = dwave_max(rn);
+        bn = dwave_max(bn);
+        if ((tid & 63) == 0) {
+            sred[2 * (tid >> 6)] = rn;
+            sred[2 * (tid >> 6) + 1] = bn;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, c = 0.0;
+            for (int q = 0; q < nt / 64; q++) {
+                a = fmax(a, sred[2 * q]);
+                c = fmax(c, sred[2 * q + 1]);
+            }
+            sgo = !(a <= 1e-15 * c);
+            if (!sgo) ws[WS_BIGCONV] = 1.0;
+        }
+        __syncthreads();
+        if (!sgo) return;
+    }
+    for (int i = tid; i < np; i += nt) {
+        const double rhs = i < n ? (it == 0 ? bv[i] : r[i]) : 0.0;
+        sv[i] = (T)(rhs * ss[i]);
+    }
+    __syncthreads();
+    const T *F = big_base<T>(D) + B.f0, *Dinv = F + (size_t)np * np;
+    // forward: y_I = L(I,I)^-1 (s_I - sum_{J<I} L(I,J) y_J), 16 threads per row
+    {
+        const int i = tid >> 4, q = tid & 15;
+        for (int I = 0; I < nb; I++) {
+            const T *row = F + (size_t)(I * XBA_TILE + i) * np;
+            T acc = (T)0;
+            for (int c = q; c < I * XBA_TILE; c += 16) acc += row[c] * sv[c];
+            acc = big_red16(acc);
+            if (q == 0) st[i] = sv[I * XBA_TILE + i] - acc;
+            __syncthreads();
+            const T *Dr = Dinv + (size_t)(I * XBA_TILE + i) * XBA_TILE;
+            T y = (T)0;
+            for (int c = q; c <= i; c += 16) y += Dr[c] * st[c];
+            y = big_red16(y);
+            if (q == 0) sv[I * XBA_TILE + i] = y;
+            __syncthreads();
+        }
+    }
+    // backward: x_I = L(I,I)^-T (y_I - sum_{J>I} L(J,I)^T x_J), a column per lane, 16 row groups
+    {
+        const int c = tid & 63, g = tid >> 6;
+        for (int I = nb - 1; I >= 0; I--) {
+            T acc = (T)0;
+            for (int rr = (I + 1) * XBA_TILE + g; rr < np; rr += 16) acc += F[(size_t)rr * np + I * XBA_TILE + c] * sv[rr];
+            spart[g][c] = acc;
+            __syncthreads();
+            if (g == 0) {
+                T t = sv[I * XBA_TILE + c];
+                for (int q = 0; q < 16; q++) t -= spart[q][c];
+                st[c] = t;
+            }
+            __syncthreads();
+            T xv = (T)0;
+            for (int i = g; i < XBA_TILE; i += 16)
+                if (i >= c) xv += Dinv[(size_t)(I * XBA_TILE + i) * XBA_TILE + c] * st[i];
+            spart[g][c] = xv;
+            __syncthreads();
+            if (g == 0) {
+                T t = (T)0;
+                for (int q = 0; q < 16; q++) t += spart[q][c];
+                sv[I * XBA_TILE + c] = t;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += nt) x[i] = (it == 0 ? 0.0 : x[i]) + (double)sv[i] * ss[i];
+    if (tid == 0) ws[WS_NREF] = it + 1;
+}
+
+// --------------------------------------------------------------------------------------------- k_big_resid --
+// grid (ceil(max n / XBA_RES_ROWS), nbig), 256 threads: one row per wavefront, r_i = b_i - (Hd x)_i in FP64, and
+// the workgroup's max |r_i| and max |b_i|.
+__global__ __launch_bounds__(256) void k_big_resid(DArgs D) {
+    __shared__ double sm[8];
+    const BigWin B = D.bigs[blockIdx.y];
+    const int n = B.n, np = B.np, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if ((int)blockIdx.x * XBA_RES_ROWS >= n) return;
+    const double *ws = D.wstate + (size_t)NWS * B.win;
+    if (ws[WS_BIGOK] == 0.0 || ws[WS_BIGCONV] != 0.0) return;  // constant during this launch
+    const WinInfo W = D.wins[B.win];
     const double lam = ws[WS_LAM];
     const double *H = D.H + W.h0, *bv = D.b + W.st0;
-    T *F = big_base<T>(D) + B.f0;
-    const bool ok = ws[WS_BIGOK] != 0.0;
-    int nref = 0;
-    if (ok) {
-        const T *Dg = F + (size_t)np * np;
-        for (int e = tid; e < np * XBA_TILE; e += nt) {
-            const int r = e / XBA_TILE, c = e % XBA_TILE;
-            F[(size_t)r * np + (r / XBA_TILE) * XBA_TILE + c] = Dg[e];
-        }
-        for (int i = tid; i < np; i += nt) {
-            ss[i] = D.bigS[B.s0 + i];
-            sx[i] = 0.0;
-            sr[i] = i < n ? bv[i] : 0.0;
-        }
-        __syncthreads();
-        const int maxit = (sizeof(T) == 8) ? 1 : D.nref_max;
-        const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
-        for (int it = 0; it < maxit; it++) {
-            for (int i = tid; i < np; i += nt) sy[i] = (T)(sr[i] * ss[i]);
-            __syncthreads();
-            chol_solve_wg<T>(F, np, sy);
-            for (int i = tid; i < n; i += nt) sx[i] += (double)sy[i] * ss[i];
-            __syncthreads();
-            nref = it + 1;
-            if (it + 1 == maxit) break;
-            // r = b - Hd x in FP64: one wavefront per row, lanes along the row
-            double rmax = 0.0, bmax = 0.0;
-            for (int i = wv; i < n; i += nw) {
-                const double *Hi = H + (size_t)i * n;
-                double acc = 0.0;
-                for (int j = lane; j < n; j += 64) acc += Hi[j] * sx[j];
-                acc = dwave_sum(acc);
-                const double ri = bv[i] - acc - (lam * Hi[i] + 1e-9) * sx[i];
-                if (lane == 0) sr[i] = ri;
-                rmax = fmax(rmax, fabs(ri));
-                bmax = fmax(bmax, fabs(bv[i]));
-            }
-            if (lane == 0) {
-                sred[2 * wv] = rmax;
-                sred[2 * wv + 1] = bmax;
-            }
-            __syncthreads();
-            double rn = 0.0, bn = 0.0;
-            for (int q = 0; q < nw; q++) {
-                rn = fmax(rn, sred[2 * q]);
-                bn = fmax(bn, sred[2 * q + 1]);
-            }
-            __syncthreads();
-            if (rn <= 1e-15 * bn) break;
-        }
-        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
-        __syncthreads();
-        trial_state(D, W, tid, nt);
-    } else {
-        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
+    double *x = D.bigV + B.v0, *r = x + np, *pm = r + np;
+    const int i = blockIdx.x * XBA_RES_ROWS + wv;
+    double ra = 0.0, ba = 0.0;
+    if (i < n) {
+        const double *Hi = H + (size_t)i * n;
+        double acc = 0.0;
+        for (int j = lane; j < n; j += 64) acc += Hi[j] * x[j];
+        acc = dwave_sum(acc);
+        const double ri = bv[i] - acc - (lam * Hi[i] + 1e-9) * x[i];
+        if (lane == 0) r[i] = ri;
+        ra = fabs(ri);
+        ba = fabs(bv[i]);
     }
+    if (lane == 0) {
+        sm[2 * wv] = ra;
+        sm[2 * wv + 1] = ba;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int q = 0; q < XBA_RES_ROWS; q++) {
+            a = fmax(a, sm[2 * q]);
+            c = fmax(c, sm[2 * q + 1]);
+        }
+        pm[2 * blockIdx.x] = a;
+        pm[2 * blockIdx.x + 1] = c;
+    }
+}
+
+// -------------------------------------------------------------------------------------------- k_big_finish --
+// grid (nbig), 256 threads: the update dx = -x and the trial state (NaN update when the factorisation failed).
+__global__ __launch_bounds__(256) void k_big_finish(DArgs D) {
+    const BigWin B = D.bigs[blockIdx.x];
+    const WinInfo W = D.wins[B.win];
+    const int n = B.n, tid = threadIdx.x, nt = blockDim.x;
+    double *ws = D.wstate + (size_t)NWS * B.win;
+    const bool ok = ws[WS_BIGOK] != 0.0;
+    const double *x = D.bigV + B.v0;
+    for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = ok ? -x[i] : NAN;
+    __syncthreads();
+    if (ok) trial_state(D, W, tid, nt);
     if (tid == 0) {
         ws[WS_CHOLOK] = ok ? 1.0 : 0.0;
-        ws[WS_NREF] = nref;
+        if (!ok) ws[WS_NREF] = 0.0;
     }
 }
 
@@ -337,7 +450,7 @@ template __global__ void k_big_panel<float>(DArgs, int);
 template __global__ void k_big_panel<double>(DArgs, int);
 template __global__ void k_big_update<float>(DArgs, int);
 template __global__ void k_big_update<double>(DArgs, int);
-template __global__ void k_big_solve<float>(DArgs);
-template __global__ void k_big_solve<double>(DArgs);
+template __global__ void k_big_trsv<float>(DArgs, int);
+template __global__ void k_big_trsv<double>(DArgs, int);
 
 }  // namespace xba

@@ -230,6 +230,11 @@ __device__ void imu_raw(const double *Ri, const double *pi, const double *vi, co
     }
 }
 
+// IMU factor output record: blocks Ji^T Ji, Jj^T Jj, Ji^T Jj (3 x 225), gradients (2 x 15), cost.
+#define IMU_O_B 675
+#define IMU_O_COST 705
+#define IMU_OUT 706
+
 struct DArgs {
     int32_t nwin;
     const Chunk *chunks;
@@ -245,6 +250,9 @@ struct DArgs {
     double *H, *b, *dx, *fact64;          // (sum n^2), (sum n), (sum n), (sum n^2)
     float *fact32;                        // (sum n^2)
     double *imu_scr;                      // (nimu, 465)
+    double *imu_out;                      // (nimu, IMU_OUT) per-factor weighted blocks, gradients and cost
+    const int32_t *imu_win;               // (nimu) window of every IMU factor
+    const int32_t *imu_color;             // (nimu) colour of every IMU factor (no shared slot within a colour)
     double *wstate;
     const double *partc, *partc2;
     double *imu_chi2;                     // (nimu) or null: whitened chi2 of every IMU factor
@@ -254,12 +262,18 @@ struct DArgs {
     float *bigF32;                        // tiled factors (sum np^2), mixed precision
     double *bigF64;                       // tiled factors (sum np^2), FP64
     double *bigS;                         // (sum np) Jacobi scaling of the tiled factors
+    double *bigV;                         // (sum 3 np) refinement vectors of the tiled solves
     const int32_t *chunk_slots;           // tile slot lists of the work items with smap >= 0
+    // reduction lists: for every slot pair (s2 >= s1) of a window, the partial blocks (index into the partial
+    // buffer in units of 36 entries) that hold it, in work-item order; for every slot, its partial gradient
+    // segments (units of 6 entries)
+    const int32_t *pair_start, *pair_ent, *bslot_start, *bslot_ent;
 };
 
 // ------------------------------------------------------------------------------------------- k_reduce --
 // H[w] (n x n, FP64 out) = sum over the window's chunks, in chunk order, of their lower-triangular partial tiles
-// (mirrored), with zeros in the non-pose columns; b likewise; the visual cost = sum of the chunk costs.
+// (mirrored), with zeros in the non-pose columns; b likewise; the visual cost = sum of the chunk costs.  Every entry
+// visits only the partial blocks that hold its slot pair (the reduction lists built at planning).
 template <typename T>
 struct Acc {  // FP64: plain sum; FP32: Kahan-compensated sum
     T s = 0, c = 0;
@@ -293,15 +307,26 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
                 int t = s2; s2 = s1; s1 = t;
                 t = i; i = j; j = t;
             }
-            for (int c = W.c0; c < W.c3; c++) {
-                if (c == W.c1) c = W.c2;
-                if (c >= W.c3) break;
-                const Chunk ch = D.chunks[c];
-                const int r1 = chunk_tile(ch, D.chunk_slots, s1);
-                if (r1 < 0) continue;
-                const int r2 = chunk_tile(ch, D.chunk_slots, s2);
-                if (r2 < 0) continue;
-                int64_t e = ch.partH + (int64_t)(r2 * (r2 + 1) / 2 + r1) * 36 + 6 * i + j;
+            const int pr = W.pair0 + s2 * (s2 + 1) / 2 + s1;
+            const int q0 = D.pair_start[pr], q1 = D.pair_start[pr + 1];
+            // eight list entries' loads issued together, then added in list order
+            int q = q0;
+            for (; q + 8 <= q1; q += 8) {
+                T h[8], hl[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int64_t e = (int64_t)D.pair_ent[q + u] * 36 + 6 * i + j;
+                    h[u] = partH[e];
+                    hl[u] = kah ? partH_lo[e] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    v.add(h[u]);
+                    if (kah) v.add(hl[u]);
+                }
+            }
+            for (; q < q1; q++) {
+                const int64_t e = (int64_t)D.pair_ent[q] * 36 + 6 * i + j;
                 v.add(partH[e]);
                 if (kah) v.add(partH_lo[e]);
             }
@@ -312,15 +337,11 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
         int ca = D.colmap[W.st0 + idx];
         Acc<T> v;
         if (ca >= 0) {
-            int s = ca >> 3, i = ca & 7;
-            for (int c = W.c0; c < W.c3; c++) {
-                if (c == W.c1) c = W.c2;
-                if (c >= W.c3) break;
-                const Chunk ch = D.chunks[c];
-                const int r = chunk_tile(ch, D.chunk_slots, s);
-                if (r < 0) continue;
-                v.add(partb[ch.partb + 6 * r + i]);
-                if (kah) v.add(partb_lo[ch.partb + 6 * r + i]);
+            const int s = ca >> 3, i = ca & 7, gs = W.slot0 + s;
+            for (int q = D.bslot_start[gs]; q < D.bslot_start[gs + 1]; q++) {
+                const int64_t e = (int64_t)D.bslot_ent[q] * 6 + i;
+                v.add(partb[e]);
+                if (kah) v.add(partb_lo[e]);
             }
         }
         D.b[W.st0 + idx] = v.value();
@@ -330,6 +351,72 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
         for (int c = W.c0; c < W.c1; c++) cs += D.partc[c];
         for (int c = W.c2; c < W.c3; c++) cs += D.partc[c];
         D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
+    }
+}
+
+// --------------------------------------------------------------------------------------------- k_imu --
+// One workgroup per IMU factor: raw residual and Jacobians (vio/imu_factor.py:771-875), whitening by the factor's
+// square-root information W, Huber at huber_k_dof(15) on the whitened norm (vio/window.py:831-862), and the factor's
+// weighted blocks w Ji^T Ji, w Jj^T Jj, w Ji^T Jj, gradients w J^T r and cost into its output record (trial == 1:
+// cost only, at the trial state).  k_terms adds them to the window's system in factor order.
+__global__ __launch_bounds__(256) void k_imu(DArgs D, int trial) {
+    __shared__ double sr[15], sJi[225], sJj[225];
+    __shared__ double se;
+    const int fi = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int w = D.imu_win[fi];
+    const WinInfo W = D.wins[w];
+    const double *R = trial ? D.R2 : D.Rwb, *p = trial ? D.p2 : D.pwb;
+    const double *v = trial ? D.v2 : D.vel, *bg = trial ? D.g2 : D.bgs, *ba = trial ? D.a2 : D.bas;
+    const double kd = D.win_par[(size_t)XBA_WPAR * w + 6];
+    const double *g = D.win_par + (size_t)XBA_WPAR * w + 20;
+    double *scr = D.imu_scr + 465 * (size_t)fi, *o = D.imu_out + (size_t)IMU_OUT * fi;
+    const double *Wm = D.imu_rec + (size_t)XBA_IMU_REC * fi + 76;
+    if (tid == 0) {
+        const int i = W.slot0 + D.imu_ij[2 * fi], j = W.slot0 + D.imu_ij[2 * fi + 1];
+        imu_raw(R + 9 * (size_t)i, p + 3 * (size_t)i, v + 3 * (size_t)i, bg + 3 * (size_t)i, ba + 3 * (size_t)i,
+                R + 9 * (size_t)j, p + 3 * (size_t)j, v + 3 * (size_t)j, bg + 3 * (size_t)j, ba + 3 * (size_t)j, g,
+                D.imu_rec + (size_t)XBA_IMU_REC * fi, scr, trial ? nullptr : scr + 15, trial ? nullptr : scr + 240);
+    }
+    __syncthreads();
+    // whiten: X <- W X (W lower triangular)
+    for (int e = tid; e < 15; e += nt) {
+        double acc = 0.0;
+        for (int k = 0; k <= e; k++) acc += Wm[15 * e + k] * scr[k];
+        sr[e] = acc;
+    }
+    if (!trial)
+        for (int e = tid; e < 450; e += nt) {
+            const int which = e / 225, ic = e % 225, i = ic / 15, c = ic % 15;
+            const double *X = scr + 15 + 225 * which;
+            double acc = 0.0;
+            for (int k = 0; k <= i; k++) acc += Wm[15 * i + k] * X[15 * k + c];
+            (which ? sJj : sJi)[ic] = acc;
+        }
+    __syncthreads();
+    if (tid == 0) {
+        double e2 = 0.0;
+        for (int k = 0; k < 15; k++) e2 += sr[k] * sr[k];
+        se = e2;
+        const double e = sqrt(e2);
+        o[IMU_O_COST] = (e <= kd) ? 0.5 * e2 : kd * (e - 0.5 * kd);
+        if (D.imu_chi2 && !trial) D.imu_chi2[fi] = e2;
+    }
+    __syncthreads();
+    if (trial) return;
+    const double e = sqrt(se), wgt = (e <= kd) ? 1.0 : kd / e;
+    for (int q = tid; q < 3 * 225; q += nt) {
+        const int which = q / 225, ac = q % 225, a = ac / 15, c = ac % 15;
+        const double *X = which == 1 ? sJj : sJi, *Y = which == 0 ? sJi : sJj;
+        double acc = 0.0;
+        for (int k = 0; k < 15; k++) acc += X[15 * k + a] * Y[15 * k + c];
+        o[q] = wgt * acc;
+    }
+    for (int q = tid; q < 30; q += nt) {
+        const double *X = q < 15 ? sJi : sJj;
+        const int a = q % 15;
+        double acc = 0.0;
+        for (int k = 0; k < 15; k++) acc += X[15 * k + a] * sr[k];
+        o[IMU_O_B + q] = wgt * acc;
     }
 }
 
@@ -397,75 +484,36 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
         }
         __syncthreads();
     }
-    // ---- IMU factors (vio/window.py:831-862)
+    // ---- IMU factors (vio/window.py:831-862): their weighted whitened blocks come from k_imu.  The cost is summed
+    // in factor order; the blocks are added one colour at a time (factors of one colour share no slot, colours
+    // assigned in factor order at planning), so every H entry receives its terms in a fixed order.
     const int nf = W.imu1 - W.imu0;
-    if (nf > 0) {
-        const double kd = D.win_par[(size_t)XBA_WPAR * w + 6];
-        const double *g = D.win_par + (size_t)XBA_WPAR * w + 20;
-        // raw residuals and Jacobians, one thread per factor
-        for (int f = tid; f < nf; f += nt) {
-            int fi = W.imu0 + f, i = W.slot0 + D.imu_ij[2 * fi], j = W.slot0 + D.imu_ij[2 * fi + 1];
-            double *scr = D.imu_scr + 465 * (size_t)fi;
-            imu_raw(R + 9 * (size_t)i, p + 3 * (size_t)i, v + 3 * (size_t)i, bg + 3 * (size_t)i, ba + 3 * (size_t)i,
-                    R + 9 * (size_t)j, p + 3 * (size_t)j, v + 3 * (size_t)j, bg + 3 * (size_t)j, ba + 3 * (size_t)j,
-                    g, D.imu_rec + (size_t)XBA_IMU_REC * fi, scr, trial ? nullptr : scr + 15,
-                    trial ? nullptr : scr + 240);
-        }
-        __syncthreads();
-        for (int f = 0; f < nf; f++) {
-            int fi = W.imu0 + f;
-            const double *scr = D.imu_scr + 465 * (size_t)fi, *Wm = D.imu_rec + (size_t)XBA_IMU_REC * fi + 76;
-            // whiten: X <- W X (W lower triangular)
-            for (int e = tid; e < 15; e += nt) {
-                double s = 0.0;
-                for (int k = 0; k <= e; k++) s += Wm[15 * e + k] * scr[k];
-                sr[e] = s;
-            }
-            if (!trial)
-                for (int e = tid; e < 450; e += nt) {
-                    int which = e / 225, ic = e % 225, i = ic / 15, c = ic % 15;
-                    const double *X = scr + 15 + 225 * which;
-                    double s = 0.0;
-                    for (int k = 0; k <= i; k++) s += Wm[15 * i + k] * X[15 * k + c];
-                    (which ? sJj : sJi)[ic] = s;
+    if (tid == 0)
+        for (int f = 0; f < nf; f++) cost += D.imu_out[(size_t)IMU_OUT * (W.imu0 + f) + IMU_O_COST];
+    if (!trial)
+        for (int col = 0; col < W.imu_ncol; col++) {
+            for (int q = tid; q < nf * 705; q += nt) {
+                const int f = q / 705, e = q % 705, fi = W.imu0 + f;
+                if (D.imu_color[fi] != col) continue;
+                const double v = D.imu_out[(size_t)IMU_OUT * fi + e];
+                const int i = D.imu_ij[2 * fi], j = D.imu_ij[2 * fi + 1];
+                const int ci = D.slot_off[W.slot0 + i], cj = D.slot_off[W.slot0 + j];
+                if (e >= IMU_O_B) {
+                    b[(e - IMU_O_B < 15 ? ci : cj) + (e - IMU_O_B) % 15] += v;
+                    continue;
                 }
-            __syncthreads();
-            double e2 = 0.0;
-            for (int k = 0; k < 15; k++) e2 += sr[k] * sr[k];
-            double e = sqrt(e2), wgt = (e <= kd) ? 1.0 : kd / e;
-            if (tid == 0) {
-                cost += (e <= kd) ? 0.5 * e2 : kd * (e - 0.5 * kd);
-                if (D.imu_chi2 && !trial) D.imu_chi2[fi] = e2;
-            }
-            if (!trial) {
-                int i = D.imu_ij[2 * fi], j = D.imu_ij[2 * fi + 1];
-                int ci = D.slot_off[W.slot0 + i], cj = D.slot_off[W.slot0 + j];
-                for (int q = tid; q < 3 * 225; q += nt) {
-                    int which = q / 225, ac = q % 225, a = ac / 15, c = ac % 15;
-                    double s = 0.0;
-                    if (which == 0) {
-                        for (int k = 0; k < 15; k++) s += sJi[15 * k + a] * sJi[15 * k + c];
-                        H[(size_t)(ci + a) * n + ci + c] += wgt * s;
-                    } else if (which == 1) {
-                        for (int k = 0; k < 15; k++) s += sJj[15 * k + a] * sJj[15 * k + c];
-                        H[(size_t)(cj + a) * n + cj + c] += wgt * s;
-                    } else {
-                        for (int k = 0; k < 15; k++) s += sJi[15 * k + a] * sJj[15 * k + c];
-                        H[(size_t)(ci + a) * n + cj + c] += wgt * s;
-                        H[(size_t)(cj + c) * n + ci + a] += wgt * s;
-                    }
-                }
-                for (int q = tid; q < 30; q += nt) {
-                    int which = q / 15, a = q % 15;
-                    double s = 0.0;
-                    const double *X = which ? sJj : sJi;
-                    for (int k = 0; k < 15; k++) s += X[15 * k + a] * sr[k];
-                    b[(which ? cj : ci) + a] += wgt * s;
+                const int which = e / 225, ac = e % 225, a = ac / 15, c = ac % 15;
+                if (which == 0) {
+                    H[(size_t)(ci + a) * n + ci + c] += v;
+                } else if (which == 1) {
+                    H[(size_t)(cj + a) * n + cj + c] += v;
+                } else {
+                    H[(size_t)(ci + a) * n + cj + c] += v;
+                    H[(size_t)(cj + c) * n + ci + a] += v;
                 }
             }
             __syncthreads();
         }
-    }
     if (tid == 0) {
         double *ws = D.wstate + (size_t)NWS * w;
         if (trial) {

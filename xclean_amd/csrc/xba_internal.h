@@ -39,6 +39,8 @@ struct WinInfo {
     int32_t slot0, P, lm0, L, n, c0, c1, imu0, imu1, pblk0, pblk1, prow0, prow1, np, vi;
     int32_t big;     // index into the tiled-solve window list, -1 when the one-workgroup solve handles the window
     int32_t c2, c3;  // second work-item range of the window (landmarks coupling more slots than the narrow tile)
+    int32_t pair0;   // first slot pair of the window in the reduction lists (pairs s2 >= s1, s2 (s2 + 1) / 2 + s1)
+    int32_t imu_ncol;  // colours of the window's IMU factors
     int64_t st0;   // offset of the window's state vector (dx, b)
     int64_t h0;    // offset of the window's n x n matrices
     int64_t prS0;  // offset of the window's prior S
@@ -61,16 +63,20 @@ enum {
     WS_ITERS,
     WS_COSTTRIAL_REST,
     WS_BIGOK,       // tiled Cholesky of the window: 1 while every pivot so far was positive and finite
+    WS_BIGCONV,     // tiled solve: 1 once the refinement has converged in this LM iteration
     NWS = 16
 };
 
 // A window whose reduced system is factored by the tiled multi-workgroup Cholesky: its matrix is padded to np, a
 // multiple of XBA_TILE, with an identity block on the padding, and stored row-major at f0 of the tiled-factor
-// buffer; its Jacobi scaling (or ones) at s0 of the scale buffer.
+// buffer, followed by the inverses of the factor's diagonal tiles (np x XBA_TILE, row-major); its Jacobi scaling
+// (or ones) at s0 of the scale buffer; its refinement vectors (solution x, residual r, per-workgroup residual and
+// right-hand-side maxima) at v0 of the refinement buffer (3 np entries).
 #define XBA_TILE 64
+#define XBA_RES_ROWS 4  // rows per workgroup of the refinement residual
 struct BigWin {
     int32_t win, n, np, nb;  // window, state dimension, padded dimension, tiles per side (np / XBA_TILE)
-    int64_t f0, s0;
+    int64_t f0, s0, v0;
 };
 
 // Packed observation word: slot (bits 0-11) | cam << 12 | valid << 13 | tile slot << 14 (bits 14-19, the slot's

@@ -145,7 +145,9 @@ class BatchSolver:
     """Solve many windows at once on the GPU.  ``problems`` is a list of ``Problem``s (mutated by ``download``)."""
 
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
-                 device="cuda", deterministic=False, with_weights=False, big_n=0):
+                 device="cuda", deterministic=False, with_weights=False, big_n=0, tuned=None):
+        """``tuned``: a dict of plan options (lm_per_chunk, slot_tile, grid, big_n) chosen by
+        ``xclean_amd.autotune``, overriding the individual arguments."""
         import torch
         _lib.require_gpu()
         self.torch = torch
@@ -231,30 +233,44 @@ class BatchSolver:
         if not vi:
             b.vel = b.bgs = b.bas = None
         self.batch = b
-        o = _lib.XbaOptions()
-        o.precision, o.lm_per_chunk, o.slot_tile, o.grid, o.use_graph = self.prec, lm_per_chunk, slot_tile, grid, int(
-            bool(use_graph))
-        o.deterministic = int(bool(deterministic))
-        o.big_n = int(big_n)
-        self.opts = o
-        ctx = C.c_void_p()
-        _lib.check(self.L.xba_create(C.byref(ctx), C.byref(b), C.byref(o)), "xba_create")
-        self.ctx = ctx
+        self.use_graph = bool(use_graph)
+        self.deterministic = bool(deterministic)
+        self.ctx = None
+        plan = dict(lm_per_chunk=lm_per_chunk, slot_tile=slot_tile, grid=grid, big_n=big_n)
+        plan.update(tuned or {})
+        self.replan(**plan)
         ns, nh = C.c_int64(), C.c_int64()
-        self.L.xba_sizes(ctx, C.byref(ns), C.byref(nh))
+        self.L.xba_sizes(self.ctx, C.byref(ns), C.byref(nh))
         self.n_state, self.n_h = ns.value, nh.value
         self.nobs = int(self.win_obs0[-1])
         self.nlm = int(self.win_lm0[-1])
 
-    def __del__(self):
+    def replan(self, lm_per_chunk=0, slot_tile=0, grid=0, big_n=0):
+        """(Re)create the solver context with these plan options over the same device data (the state is
+        reloaded by the next ``upload``)."""
+        self._destroy()
+        o = _lib.XbaOptions()
+        o.precision, o.use_graph, o.deterministic = self.prec, int(self.use_graph), int(self.deterministic)
+        o.lm_per_chunk, o.slot_tile, o.grid, o.big_n = int(lm_per_chunk), int(slot_tile), int(grid), int(big_n)
+        self.opts = o
+        self.plan_options = dict(lm_per_chunk=int(lm_per_chunk), slot_tile=int(slot_tile), grid=int(grid),
+                                 big_n=int(big_n))
+        ctx = C.c_void_p()
+        _lib.check(self.L.xba_create(C.byref(ctx), C.byref(self.batch), C.byref(o)), "xba_create")
+        self.ctx = ctx
+
+    def _destroy(self):
         ctx = getattr(self, "ctx", None)
         if ctx is not None and ctx.value:
-            try:
-                self.torch.cuda.synchronize()
-                self.L.xba_destroy(ctx)
-            except Exception:
-                pass
-            self.ctx = None
+            self.torch.cuda.synchronize()
+            self.L.xba_destroy(ctx)
+        self.ctx = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
 
     @property
     def stream(self):
