@@ -85,6 +85,27 @@ def single_window_configs(torch, synth, BatchSolver, log):
     return out
 
 
+def differentiation_config(synth, log, nwin=64, per_window=200):
+    """Configuration 3: 64 windows of 15 keyframes x 5,000 landmarks; mixed-precision LM to convergence (10
+    iterations), the converged step's implicit-function adjoint w.r.t. pixels and row weights (torch.autograd),
+    checked against FP64 central differences (h = 1e-6) at 200 coordinates per window."""
+    from xclean_amd import gradcheck
+    pbs = [synth.make_config(3, seed=i)[0] for i in range(nwin)]
+    r = gradcheck.differentiation_check(pbs, iters=10, per_window=per_window, seed=0)
+    M = sum(pb.M for pb in pbs)
+    out = {"windows": nwin, "obs_rows": M, "lm_iters": r["lm_iters"], "forward_ms": r["forward_ms"],
+           "backward_ms": r["backward_ms"], "forward_obs_per_s_per_lm_iteration": M * r["lm_iters"] /
+           (r["forward_ms"] * 1e-3), "coords_per_window": int(np.min(r["coords_per_window"])),
+           "max_rel_err_vs_fd": float(np.max(r["rel_err_vs_fd"])),
+           "median_rel_err_vs_fd": float(np.median(r["rel_err_vs_fd"])),
+           "max_rel_err_fp64_adjoint_vs_fd": float(np.max(r["rel_err_fp64_adjoint_vs_fd"])),
+           "max_rel_err_vs_fp64_adjoint": float(np.max(r["rel_err_vs_fp64_adjoint"])),
+           "accepted_mean": r["accepted_mean"]}
+    log("bench: config 3: forward %.2f ms, backward %.3f ms, adjoint vs FD max rel err %.2e" % (
+        out["forward_ms"], out["backward_ms"], out["max_rel_err_vs_fd"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +225,7 @@ def main():
     traffic = _measured_traffic()
 
     extra = None if args.no_extra else single_window_configs(torch, synth, BatchSolver, log)
+    diff = None if args.no_extra else differentiation_config(synth, log)
 
     cpu = None
     if not args.no_cpu:
@@ -265,6 +287,7 @@ def main():
                                                "cached": tune["cached"], "seconds": tune["seconds"],
                                                "trials": len(tune["trials"])},
         "single_window_configs": extra,
+        "differentiation": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

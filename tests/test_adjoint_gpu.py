@@ -122,3 +122,18 @@ def test_backward_needs_a_fresh_step():
     solver.iterate(1)                          # the state moved: the kept factor no longer matches
     with pytest.raises(_lib.XbaError):
         solver.adjoint(np.zeros(pb.n_state()))
+
+
+def test_differentiation_workload_shape_gradients_match_fd():
+    """Eight windows of the differentiation configuration's shape (15 keyframes, 5,000 landmarks, ~30,000 rows):
+    mixed-precision LM (10 iterations), adjoint of the converged step checked against FP64 central differences
+    (h = 1e-6) at 60 coordinates per window within 1e-5, and against the FP64 path's adjoint."""
+    from xclean_amd import gradcheck
+    pbs = [synth.make_config(3, seed=i)[0] for i in range(8)]
+    print("windows", [(pb.P, pb.L, pb.M) for pb in pbs], flush=True)
+    r = gradcheck.differentiation_check(pbs, iters=10, per_window=60, seed=1)
+    print({k: (np.max(v) if isinstance(v, list) else v) for k, v in r.items()}, flush=True)
+    assert min(r["coords_per_window"]) == 60
+    assert max(r["rel_err_vs_fd"]) < 1e-5
+    assert max(r["rel_err_fp64_adjoint_vs_fd"]) < 1e-5
+    assert max(r["rel_err_vs_fp64_adjoint"]) < 1e-5
