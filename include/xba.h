@@ -14,6 +14,10 @@
  *   xba_np_solve
  *       backend/ba.py:713-869 `_solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0, prior, rotf)` (and its
  *       FP32 twin `_solve_gpu`, ba.py:930-1089): the local-BA drop-in that tests/test_ba_solver.py calls.
+ *   xba_np_id_solve
+ *       backend/ba.py:1212-1342 `_solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters,
+ *       lam0, prior, rotf, rho_floor)`: the inverse-depth local BA (landmarks hosted by a keyframe or by a constant
+ *       out-of-window pose).
  *   xba_backward
  *       no reference counterpart (the reference has no differentiable solve): the implicit-function adjoint of
  *       the last solved update with respect to observed pixels and per-observation weights.
@@ -189,6 +193,32 @@ typedef struct xba_np_problem {
 
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                  double *dx_hist_out, void *stream);
+
+/* ---- the inverse-depth local-BA drop-in (backend/ba.py:1212-1342 `_solve_np_id`) --------------------------------
+ * poses (device, nk x 4 x 4, T_wc, in/out); rho (device, nl, in/out) inverse depth and mbear (device, nl x 3, in/out)
+ * bearing (x_n, y_n, 1) of every landmark in its host camera frame; hidx (device int32, nl) the host keyframe or -1,
+ * in which case the host T_wc is the constant row hpose_ext[l] = [R | t] (device, nl x 12; may be NULL when no
+ * hidx is -1); okf/olm/ouv/free_mask as xba_np_problem; rho_floor the lower bound of an updated inverse depth;
+ * optional prior (pr_m0 (nl,3) bearing mean, pr_rho0, pr_wbear, pr_invsig, pr_valid (nl); pr_m0 NULL = none) and
+ * rotation factors (their Hessian blocks enter the reduced system here, ba.py:1293-1296).  stats_out (device, 4):
+ * chi2_0, chi2_n, rms_0, rms_n.  Synchronous. */
+typedef struct xba_npid_problem {
+    int32_t nk, nl, N;
+    double *poses, *rho, *mbear;
+    const int32_t *hidx;
+    const double *hpose_ext;
+    const int32_t *okf, *olm;
+    const double *ouv;
+    const uint8_t *free_mask;
+    double fx, cx, cy, huber_k, rho_floor;
+    const double *pr_m0, *pr_rho0, *pr_wbear, *pr_invsig, *pr_valid;
+    int32_t n_rot;
+    const int32_t *rot_ab;
+    const double *rot_M, *rot_sig;
+} xba_npid_problem;
+
+int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                    double *dx_hist_out, void *stream);
 
 /* Diagnostics (no reference counterpart): a plain device-to-device copy kernel of `bytes` (a multiple of 16;
  * 16-byte aligned pointers) on `stream`, so HBM counters (FETCH_SIZE, WRITE_SIZE) can be calibrated against a

@@ -55,6 +55,16 @@ class OnpProblem(C.Structure):
                 ("pr_valid", f64p), ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p)]
 
 
+class OnpidProblem(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32),
+                ("poses", f64p), ("rho", f64p), ("mbear", f64p), ("hidx", i32p), ("hpose_ext", f64p),
+                ("okf", i32p), ("olm", i32p), ("ouv", f64p),
+                ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
+                ("rho_floor", C.c_double), ("free_mask", u8p),
+                ("pr_m0", f64p), ("pr_rho0", f64p), ("pr_wbear", f64p), ("pr_invsig", f64p), ("pr_valid", f64p),
+                ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -79,6 +89,8 @@ def lib():
         L.o_obs_residual_jac.restype = C.c_int
         L.o_imu_factor.argtypes = [f64p] * 12 + [C.c_int, f64p, f64p, f64p, f64p]
         L.onp_solve.argtypes = [C.POINTER(OnpProblem), C.c_int, C.c_double, f64p, f64p, u8p]
+        L.onpid_solve.argtypes = [C.POINTER(OnpidProblem), C.c_int, C.c_double, f64p, f64p, u8p]
+        L.o_id_blocks.argtypes = [f64p, f64p, C.c_double, f64p, f64p, C.c_int, C.c_double, f64p, C.c_double, f64p]
         for name in ("o_so3_exp_nb", "o_so3_log_nb", "o_log_quat", "o_jr_inv", "o_se3_exp", "o_so3_project"):
             getattr(L, name).argtypes = [f64p, f64p]
         L.o_exp_jr.argtypes = [f64p, f64p, f64p]
@@ -301,6 +313,67 @@ def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
     if history:
         return poses, pts, st, dxh, acc.astype(bool)
     return poses, pts, st
+
+
+def solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
+                rotf=None, rho_floor=1e-4, history=False):
+    """backend/ba.py:1212-1342 (_solve_np_id): returns (poses T_wc, rho, mbear, stats) like the reference.
+    prior = (m0 (nl, >=2), rho0, wbear, invsig, valid) with scalars broadcast; rotf = [(a, b, R, sigma), ...]."""
+    nk, nl, N = len(poses), len(rho), len(okf)
+    poses = np.ascontiguousarray(poses, np.float64).copy()
+    rho = np.ascontiguousarray(rho, np.float64).copy()
+    mbear = np.ascontiguousarray(mbear, np.float64).reshape(nl, 3).copy()
+    hidx = np.ascontiguousarray(hidx, np.int32)
+    hext = np.ascontiguousarray(np.broadcast_to(np.asarray(hpose_ext, np.float64).reshape(-1, 12), (nl, 12)))
+    okf = np.ascontiguousarray(okf, np.int32)
+    olm = np.ascontiguousarray(olm, np.int32)
+    ouv = np.ascontiguousarray(ouv, np.float64)
+    if np.ndim(free) == 0:
+        fm = np.ones(nk, np.uint8)
+        fm[int(free)] = 0
+    else:
+        fm = np.asarray(free, bool).astype(np.uint8)
+    s = OnpidProblem()
+    s.nk, s.nl, s.N = nk, nl, N
+    s.poses, s.rho, s.mbear, s.hidx, s.hpose_ext = (_p(poses, f64p), _p(rho, f64p), _p(mbear, f64p),
+                                                   _p(hidx, i32p), _p(hext, f64p))
+    s.okf, s.olm, s.ouv = _p(okf, i32p), _p(olm, i32p), _p(ouv, f64p)
+    s.fx, s.cx, s.cy, s.huber_k, s.rho_floor = rig.fx_rect, rig.cx, rig.cy, 1.345, rho_floor
+    s.free_mask = _p(fm, u8p)
+    keep = [hext]
+    if prior is not None:
+        m0, rho0, wbear, invsig, valid = prior
+        m0 = np.asarray(m0, np.float64).reshape(nl, -1)
+        pm = np.zeros((nl, 3))
+        pm[:, :min(3, m0.shape[1])] = m0[:, :3]
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (nl,))) for a in (rho0, wbear, invsig,
+                                                                                              valid)]
+        keep += [pm] + arrs
+        s.pr_m0 = _p(pm, f64p)
+        s.pr_rho0, s.pr_wbear, s.pr_invsig, s.pr_valid = [_p(a, f64p) for a in arrs]
+    if rotf:
+        ab = np.ascontiguousarray([[f[0], f[1]] for f in rotf], np.int32)
+        Ms = np.ascontiguousarray(np.stack([f[2] for f in rotf]), np.float64)
+        sg = np.ascontiguousarray([f[3] for f in rotf], np.float64)
+        keep += [ab, Ms, sg]
+        s.n_rot, s.rot_ab, s.rot_M, s.rot_sig = len(rotf), _p(ab, i32p), _p(Ms, f64p), _p(sg, f64p)
+    stats = np.zeros(4)
+    dxh = np.zeros((iters, 6 * nk)) if history else None
+    acc = np.zeros(iters, np.uint8) if history else None
+    lib().onpid_solve(C.byref(s), iters, lam0, _p(stats, f64p), _p(dxh, f64p), _p(acc, u8p))
+    st = {"chi2_0": stats[0], "chi2_n": stats[1], "rms_0": stats[2], "rms_n": stats[3]}
+    if history:
+        return poses, rho, mbear, st, dxh, acc.astype(bool)
+    return poses, rho, mbear, st
+
+
+def id_blocks(Tcw_k, Rh, rho, Pc, Ph, host_in, fx, r, sw=1.0):
+    """backend/ba.py:1138-1209 for one observation: (Jobs (2,6), Jhost (2,6), Jlm (2,3), rw (2,))."""
+    J = np.zeros(32)
+    a = [np.ascontiguousarray(x, np.float64) for x in (Tcw_k, Rh, Pc, Ph, r)]
+    lib().o_id_blocks(_p(a[0], f64p), _p(a[1], f64p), float(rho), _p(a[2], f64p), _p(a[3], f64p), int(host_in),
+                      float(fx), _p(a[4], f64p), float(sw), _p(J, f64p))
+    return J[:12].reshape(2, 6), J[12:24].reshape(2, 6), J[24:30].reshape(2, 3), J[30:32]
 
 
 def _unary(name, x, shape_out):

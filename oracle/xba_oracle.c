@@ -1467,3 +1467,338 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
     free(Hlli); free(bl); free(Hpl); free(T3); free(dxl); free(order); free(bounds);
     return 0;
 }
+
+/* ======================================================= backend/ba.py _solve_np_id (inverse depth, T_cw) == */
+typedef struct {
+    int32_t nk, nl, N;
+    double *poses;              /* (nk,4,4) T_wc in, T_wc out */
+    double *rho;                /* (nl) in/out */
+    double *mbear;              /* (nl,3) host-frame bearing (x_n, y_n, 1), in/out */
+    const int32_t *hidx;        /* (nl) host keyframe, or -1 (host pose from hpose_ext) */
+    const double *hpose_ext;    /* (nl,12) T_wc rows [R | t] of hosts outside the window */
+    const int32_t *okf, *olm;   /* (N) */
+    const double *ouv;          /* (N,2) */
+    double fx, cx, cy, huber_k, rho_floor;
+    const uint8_t *free_mask;   /* (nk) */
+    /* optional prior (ba.py:1250-1264): m0 (nl,3), rho0, wbear, invsig, valid (nl) */
+    const double *pr_m0, *pr_rho0, *pr_wbear, *pr_invsig, *pr_valid;
+    int32_t n_rot;
+    const int32_t *rot_ab;
+    const double *rot_M, *rot_sig;
+} onpid_problem;
+
+/* backend/ba.py:1116-1135 _host_frames: (R_wc, t_wc) of landmark l's host */
+static void id_host(const onpid_problem *q, const double *Twc, int l, double *R, double *t) {
+    int h = q->hidx[l];
+    if (h >= 0) {
+        const double *T = Twc + 16 * h;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) R[3 * i + j] = T[4 * i + j];
+            t[i] = T[4 * i + 3];
+        }
+    } else {
+        const double *E = q->hpose_ext + 12 * l;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) R[3 * i + j] = E[4 * i + j];
+            t[i] = E[4 * i + 3];
+        }
+    }
+}
+
+/* backend/ba.py:1226-1234 geom: residuals r (N,2), camera points Pc (N,3), host points Ph (N,3); returns sum r^2 */
+static double id_residuals(const onpid_problem *q, const double *Twc, const double *Tcw, const double *rho,
+                           const double *mb, double *r, double *Pc, double *Ph) {
+    double s = 0.0;
+    for (int k = 0; k < q->N; k++) {
+        int l = q->olm[k];
+        double R[9], t[3], P[3], Pw[3], Pcc[3];
+        id_host(q, Twc, l, R, t);
+        double ir = fmax(rho[l], 1e-12);
+        for (int i = 0; i < 3; i++) P[i] = mb[3 * l + i] / ir;
+        for (int i = 0; i < 3; i++) Pw[i] = (R[3 * i] * P[0] + R[3 * i + 1] * P[1] + R[3 * i + 2] * P[2]) + t[i];
+        const double *T = Tcw + 16 * q->okf[k];
+        for (int i = 0; i < 3; i++) Pcc[i] = (T[4 * i] * Pw[0] + T[4 * i + 1] * Pw[1] + T[4 * i + 2] * Pw[2]) + T[4 * i + 3];
+        double z = fmax(Pcc[2], 1e-6);
+        r[2 * k] = q->fx * Pcc[0] / z + q->cx - q->ouv[2 * k];
+        r[2 * k + 1] = q->fx * Pcc[1] / z + q->cy - q->ouv[2 * k + 1];
+        if (Pc) memcpy(Pc + 3 * k, Pcc, sizeof(Pcc));
+        if (Ph) memcpy(Ph + 3 * k, P, sizeof(P));
+        s += r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1];
+    }
+    return s;
+}
+
+/* backend/ba.py:1138-1209 _analytic_blocks_id for one observation: J (32) = Jobs (2x6), Jhost (2x6), Jlm (2x3),
+ * all times sw, and rw = r sw (2).  Jhost = 0 when the host is outside the window. */
+void o_id_blocks(const double *Tcw_k, const double *Rh, double rho, const double *Pc, const double *Ph, int host_in,
+                 double fx, const double *r, double sw, double *J) {
+    double x = Pc[0], y = Pc[1], zz = fmax(Pc[2], 1e-6), iz = 1.0 / zz;
+    double A[2][3] = {{fx * iz, 0.0, -fx * x * iz * iz}, {0.0, fx * iz, -fx * y * iz * iz}};
+    double dPo[3][6] = {{1, 0, 0, 0, zz, -y}, {0, 1, 0, -zz, 0, x}, {0, 0, 1, y, -x, 0}};
+    double Rcw[9], Rch[9], ARch[2][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Rcw[3 * i + j] = Tcw_k[4 * i + j];
+    mm3(Rcw, Rh, Rch);
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 3; c++) ARch[a][c] = A[a][0] * Rch[c] + A[a][1] * Rch[3 + c] + A[a][2] * Rch[6 + c];
+    double dPh[3][6] = {{-1, 0, 0, 0, -Ph[2], Ph[1]}, {0, -1, 0, Ph[2], 0, -Ph[0]}, {0, 0, -1, -Ph[1], Ph[0], 0}};
+    double irho = 1.0 / fmax(rho, 1e-12);
+    for (int a = 0; a < 2; a++) {
+        for (int c = 0; c < 6; c++) {
+            J[6 * a + c] = (A[a][0] * dPo[0][c] + A[a][1] * dPo[1][c] + A[a][2] * dPo[2][c]) * sw;
+            J[12 + 6 * a + c] =
+                host_in ? (ARch[a][0] * dPh[0][c] + ARch[a][1] * dPh[1][c] + ARch[a][2] * dPh[2][c]) * sw : 0.0;
+        }
+        J[24 + 3 * a] = ARch[a][0] * irho * sw;
+        J[24 + 3 * a + 1] = ARch[a][1] * irho * sw;
+        J[24 + 3 * a + 2] = -(ARch[a][0] * Ph[0] + ARch[a][1] * Ph[1] + ARch[a][2] * Ph[2]) * irho * sw;
+        J[30 + a] = r[a] * sw;
+    }
+}
+
+/* backend/ba.py:644-697 with the Hessian blocks: _solve_np_id adds both to Hpp and grad (ba.py:1293-1296) */
+static void rot_factor_blocks_id(const onpid_problem *q, const double *Tcw, int n, double *H, double *g) {
+    for (int f = 0; f < q->n_rot; f++) {
+        int a = q->rot_ab[2 * f], b = q->rot_ab[2 * f + 1];
+        double w = 1.0 / fmax(q->rot_sig[f], 1e-9), Ri[9], Rj[9], A[9], E[9], e[3], Jr[9], Ja[9], Jb[9], r[3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                Ri[3 * i + j] = Tcw[16 * a + 4 * i + j];
+                Rj[3 * i + j] = Tcw[16 * b + 4 * i + j];
+            }
+        mmt3(Ri, Rj, A);
+        mtm3(q->rot_M + 9 * f, A, E);
+        o_log_quat(E, e);
+        o_jr_inv(e, Jr);
+        mmt3(Jr, A, Ja);
+        for (int i = 0; i < 9; i++) {
+            Ja[i] *= w;
+            Jb[i] = -w * Jr[i];
+        }
+        for (int i = 0; i < 3; i++) r[i] = w * e[i];
+        int ia = 6 * a + 3, ib = 6 * b + 3;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) {
+                double haa = 0.0, hab = 0.0, hbb = 0.0;
+                for (int m = 0; m < 3; m++) {
+                    haa += Ja[3 * m + i] * Ja[3 * m + j];
+                    hab += Ja[3 * m + i] * Jb[3 * m + j];
+                    hbb += Jb[3 * m + i] * Jb[3 * m + j];
+                }
+                H[(size_t)(ia + i) * n + ia + j] += haa;
+                H[(size_t)(ia + i) * n + ib + j] += hab;
+                H[(size_t)(ib + j) * n + ia + i] += hab;
+                H[(size_t)(ib + i) * n + ib + j] += hbb;
+            }
+            double ga = 0.0, gb = 0.0;
+            for (int j = 0; j < 3; j++) {
+                ga += Ja[3 * j + i] * r[j];
+                gb += Jb[3 * j + i] * r[j];
+            }
+            g[ia + i] += -ga;
+            g[ib + i] += -gb;
+        }
+    }
+}
+
+/* backend/ba.py:1212-1342 _solve_np_id.  stats: chi2_0, chi2_n, rms_0, rms_n.  dx_hist (iters, 6 nk) optional. */
+int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double *dx_hist, uint8_t *acc_hist) {
+    int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
+    double K = q->huber_k, lam = lam0;
+    size_t Nn = N ? N : 1, nln = nl ? nl : 1;
+    double *Tcw = malloc(16 * nk * sizeof(double)), *Tn = malloc(16 * nk * sizeof(double));
+    double *Twc = malloc(16 * nk * sizeof(double)), *Twn = malloc(16 * nk * sizeof(double));
+    memcpy(Twc, q->poses, 16 * nk * sizeof(double));
+    for (int i = 0; i < nk; i++) se3_inv(Twc + 16 * i, Tcw + 16 * i);
+    double *r = malloc(2 * Nn * sizeof(double)), *Pc = malloc(3 * Nn * sizeof(double));
+    double *Ph = malloc(3 * Nn * sizeof(double)), *r2 = malloc(2 * Nn * sizeof(double));
+    double *J = malloc(32 * Nn * sizeof(double)), *Hpl = malloc(36 * Nn * sizeof(double));
+    double *rho = q->rho, *mb = q->mbear;
+    double *rhon = malloc(nln * sizeof(double)), *mbn = malloc(3 * nln * sizeof(double));
+    double *Hpp = malloc((size_t)n * n * sizeof(double)), *Hred = malloc((size_t)n * n * sizeof(double));
+    double *bp = malloc(n * sizeof(double)), *Hll = malloc(9 * nln * sizeof(double));
+    double *Hlli = malloc(9 * nln * sizeof(double)), *bl = malloc(3 * nln * sizeof(double));
+    double *dxl = malloc(3 * nln * sizeof(double));
+    int *order = malloc(Nn * sizeof(int)), *bounds = malloc((nl + 1) * sizeof(int));
+    memset(bounds, 0, (nl + 1) * sizeof(int));
+    for (int k = 0; k < N; k++) bounds[q->olm[k] + 1]++;
+    for (int j = 0; j < nl; j++) bounds[j + 1] += bounds[j];
+    {
+        int *fill = malloc((size_t)(nl > 0 ? nl : 1) * sizeof(int));
+        for (int j = 0; j < nl; j++) fill[j] = bounds[j];
+        for (int k = 0; k < N; k++) order[fill[q->olm[k]]++] = k;
+        free(fill);
+    }
+    double chi0 = id_residuals(q, Twc, Tcw, rho, mb, r, NULL, NULL);
+    double rms0 = N ? sqrt(chi0 / N) : NAN;
+    for (int it = 0; it < iters; it++) {
+        if (acc_hist) acc_hist[it] = 0;
+        if (dx_hist)
+            for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = NAN;
+        double chi_cur = id_residuals(q, Twc, Tcw, rho, mb, r, Pc, Ph);
+        memset(Hpp, 0, (size_t)n * n * sizeof(double));
+        memset(bp, 0, n * sizeof(double));
+        memset(Hll, 0, 9 * nln * sizeof(double));
+        memset(bl, 0, 3 * nln * sizeof(double));
+        for (int k = 0; k < N; k++) {
+            int l = q->olm[k], ko = q->okf[k], kh = q->hidx[l];
+            double e = sqrt(r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1]);
+            double w = (e <= K) ? 1.0 : K / fmax(e, 1e-9), sw = sqrt(w), Rh[9], th[3];
+            id_host(q, Twc, l, Rh, th);
+            double *Jk = J + 32 * k;
+            o_id_blocks(Tcw + 16 * ko, Rh, rho[l], Pc + 3 * k, Ph + 3 * k, kh >= 0, q->fx, r + 2 * k, sw, Jk);
+            const double *Jl = Jk + 24, *rw = Jk + 30;
+            for (int i = 0; i < 3; i++) {
+                for (int j = 0; j < 3; j++) Hll[9 * l + 3 * i + j] += Jl[i] * Jl[j] + Jl[3 + i] * Jl[3 + j];
+                bl[3 * l + i] += -(Jl[i] * rw[0] + Jl[3 + i] * rw[1]);
+            }
+            for (int sa = 0; sa < 2; sa++) {
+                const double *Ja = Jk + 12 * sa;
+                for (int i = 0; i < 6; i++)
+                    for (int c = 0; c < 3; c++) Hpl[36 * k + 18 * sa + 3 * i + c] = Ja[i] * Jl[c] + Ja[6 + i] * Jl[3 + c];
+            }
+            int slot[2] = {ko, kh};
+            for (int sa = 0; sa < 2; sa++) {
+                if (slot[sa] < 0) continue;
+                const double *Ja = Jk + 12 * sa;
+                for (int a = 0; a < 6; a++) bp[6 * slot[sa] + a] += -(Ja[a] * rw[0] + Ja[6 + a] * rw[1]);
+                for (int sb = 0; sb < 2; sb++) {
+                    if (slot[sb] < 0) continue;
+                    const double *Jb = Jk + 12 * sb;
+                    for (int a = 0; a < 6; a++)
+                        for (int b = 0; b < 6; b++)
+                            Hpp[(size_t)(6 * slot[sa] + a) * n + 6 * slot[sb] + b] += Ja[a] * Jb[b] + Ja[6 + a] * Jb[6 + b];
+                }
+            }
+        }
+        if (q->pr_m0) { /* ba.py:1250-1264: bearing prior unrobust, rho prior Huber */
+            for (int j = 0; j < nl; j++) {
+                if (!(q->pr_valid[j] > 0.5)) continue;
+                double rp[3] = {(mb[3 * j] - q->pr_m0[3 * j]) * q->pr_wbear[j],
+                                (mb[3 * j + 1] - q->pr_m0[3 * j + 1]) * q->pr_wbear[j], (rho[j] - q->pr_rho0[j]) * q->pr_invsig[j]};
+                double J0[3] = {q->pr_wbear[j], q->pr_wbear[j], q->pr_invsig[j]};
+                for (int c = 0; c < 3; c++) {
+                    double wp = fabs(rp[c]) <= K ? 1.0 : K / fmax(fabs(rp[c]), 1e-9);
+                    double swp = c < 2 ? 1.0 : sqrt(wp), Jpr = J0[c] * swp;
+                    Hll[9 * j + 4 * c] += Jpr * Jpr;
+                    bl[3 * j + c] -= Jpr * rp[c] * swp;
+                }
+            }
+        }
+        if (q->n_rot) rot_factor_blocks_id(q, Tcw, n, Hpp, bp);
+        int singular = 0;
+        for (int j = 0; j < nl; j++) {
+            double tr = Hll[9 * j] + Hll[9 * j + 4] + Hll[9 * j + 8], d = lam * fmax(tr / 3.0, 1e-9);
+            for (int i = 0; i < 3; i++) Hll[9 * j + 4 * i] += d;
+            if (!inv3(Hll + 9 * j, Hlli + 9 * j)) singular = 1;
+        }
+        if (singular) break; /* np.linalg.inv raises LinAlgError out of the solver */
+        memcpy(Hred, Hpp, (size_t)n * n * sizeof(double));
+        for (int j = 0; j < nl; j++) {
+            const double *Hi = Hlli + 9 * j;
+            for (int ai = bounds[j]; ai < bounds[j + 1]; ai++) {
+                int ka = order[ai], sla[2] = {q->okf[ka], q->hidx[j]};
+                for (int sa = 0; sa < 2; sa++) {
+                    if (sla[sa] < 0) continue;
+                    const double *Ha = Hpl + 36 * ka + 18 * sa;
+                    double T3[18];
+                    for (int i = 0; i < 6; i++)
+                        for (int c = 0; c < 3; c++) T3[3 * i + c] = Ha[3 * i] * Hi[c] + Ha[3 * i + 1] * Hi[3 + c] + Ha[3 * i + 2] * Hi[6 + c];
+                    for (int i = 0; i < 6; i++)
+                        bp[6 * sla[sa] + i] -= T3[3 * i] * bl[3 * j] + T3[3 * i + 1] * bl[3 * j + 1] + T3[3 * i + 2] * bl[3 * j + 2];
+                    for (int bi = bounds[j]; bi < bounds[j + 1]; bi++) {
+                        int kb = order[bi], slb[2] = {q->okf[kb], q->hidx[j]};
+                        for (int sb = 0; sb < 2; sb++) {
+                            if (slb[sb] < 0) continue;
+                            const double *Hb = Hpl + 36 * kb + 18 * sb;
+                            for (int i = 0; i < 6; i++)
+                                for (int c = 0; c < 6; c++)
+                                    Hred[(size_t)(6 * sla[sa] + i) * n + 6 * slb[sb] + c] -=
+                                        T3[3 * i] * Hb[3 * c] + T3[3 * i + 1] * Hb[3 * c + 1] + T3[3 * i + 2] * Hb[3 * c + 2];
+                        }
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < nk; i++) {
+            if (q->free_mask[i]) continue;
+            int f0 = 6 * i;
+            for (int a = 0; a < 6; a++)
+                for (int c = 0; c < n; c++) {
+                    Hred[(size_t)(f0 + a) * n + c] = 0.0;
+                    Hred[(size_t)c * n + f0 + a] = 0.0;
+                }
+            for (int a = 0; a < 6; a++) {
+                Hred[(size_t)(f0 + a) * n + f0 + a] = 1.0;
+                bp[f0 + a] = 0.0;
+            }
+        }
+        for (int i = 0; i < n; i++) Hred[(size_t)i * n + i] += lam * fmax(Hred[(size_t)i * n + i], 1e-9);
+        if (!lu_solve(Hred, n, bp)) break;
+        double *dxp = bp;
+        if (dx_hist)
+            for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = dxp[i];
+        for (int j = 0; j < nl; j++) {
+            double rhs[3] = {bl[3 * j], bl[3 * j + 1], bl[3 * j + 2]};
+            for (int ai = bounds[j]; ai < bounds[j + 1]; ai++) {
+                int k = order[ai], sl[2] = {q->okf[k], q->hidx[j]};
+                for (int sa = 0; sa < 2; sa++) {
+                    if (sl[sa] < 0) continue;
+                    const double *Hk = Hpl + 36 * k + 18 * sa, *dk = dxp + 6 * sl[sa];
+                    for (int c = 0; c < 3; c++) {
+                        double s = 0.0;
+                        for (int i = 0; i < 6; i++) s += Hk[3 * i + c] * dk[i];
+                        rhs[c] += -s;
+                    }
+                }
+            }
+            const double *Hi = Hlli + 9 * j;
+            for (int i = 0; i < 3; i++) dxl[3 * j + i] = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+        }
+        memcpy(Tn, Tcw, 16 * nk * sizeof(double));
+        for (int i = 0; i < nk; i++) {
+            if (!q->free_mask[i]) continue;
+            double E[16], R[9], Rp[9];
+            o_se3_exp(dxp + 6 * i, E);
+            for (int a = 0; a < 4; a++)
+                for (int c = 0; c < 4; c++) {
+                    double s = 0.0;
+                    for (int m = 0; m < 4; m++) s += E[4 * a + m] * Tcw[16 * i + 4 * m + c];
+                    Tn[16 * i + 4 * a + c] = s;
+                }
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) R[3 * a + c] = Tn[16 * i + 4 * a + c];
+            o_so3_project(R, Rp);
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) Tn[16 * i + 4 * a + c] = Rp[3 * a + c];
+        }
+        for (int i = 0; i < nk; i++) se3_inv(Tn + 16 * i, Twn + 16 * i);
+        for (int j = 0; j < nl; j++) {
+            mbn[3 * j] = mb[3 * j] + dxl[3 * j];
+            mbn[3 * j + 1] = mb[3 * j + 1] + dxl[3 * j + 1];
+            mbn[3 * j + 2] = mb[3 * j + 2];
+            rhon[j] = fmax(rho[j] + dxl[3 * j + 2], q->rho_floor);
+        }
+        double chi_new = id_residuals(q, Twn, Tn, rhon, mbn, r2, NULL, NULL);
+        if (chi_new <= chi_cur) {
+            memcpy(Tcw, Tn, 16 * nk * sizeof(double));
+            memcpy(Twc, Twn, 16 * nk * sizeof(double));
+            memcpy(rho, rhon, nl * sizeof(double));
+            memcpy(mb, mbn, 3 * nl * sizeof(double));
+            lam = fmax(lam * 0.5, 1e-7);
+            if (acc_hist) acc_hist[it] = 1;
+        } else {
+            lam = fmin(lam * 4.0, 1e2);
+        }
+    }
+    double chin = id_residuals(q, Twc, Tcw, rho, mb, r, NULL, NULL);
+    stats[0] = chi0;
+    stats[1] = chin;
+    stats[2] = rms0;
+    stats[3] = N ? sqrt(chin / N) : NAN;
+    memcpy(q->poses, Twc, 16 * nk * sizeof(double));
+    free(Tcw); free(Tn); free(Twc); free(Twn); free(r); free(Pc); free(Ph); free(r2); free(J); free(Hpl);
+    free(rhon); free(mbn); free(Hpp); free(Hred); free(bp); free(Hll); free(Hlli); free(bl); free(dxl);
+    free(order); free(bounds);
+    return 0;
+}

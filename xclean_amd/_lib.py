@@ -45,6 +45,16 @@ class XbaNpProblem(C.Structure):
                 ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
 
 
+class XbaNpidProblem(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32),
+                ("poses", vp), ("rho", vp), ("mbear", vp), ("hidx", vp), ("hpose_ext", vp),
+                ("okf", vp), ("olm", vp), ("ouv", vp), ("free_mask", vp),
+                ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
+                ("rho_floor", C.c_double),
+                ("pr_m0", vp), ("pr_rho0", vp), ("pr_wbear", vp), ("pr_invsig", vp), ("pr_valid", vp),
+                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
+
+
 class XbaError(RuntimeError):
     pass
 
@@ -52,7 +62,7 @@ class XbaError(RuntimeError):
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
-           "xba_np_solve",
+           "xba_np_solve", "xba_np_id_solve",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
 
@@ -79,6 +89,7 @@ def load():
     L.xba_gn_step.argtypes = [vp, vp, vp]
     L.xba_backward.argtypes = [vp, vp, vp, vp, vp]
     L.xba_np_solve.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
+    L.xba_np_id_solve.argtypes = [C.POINTER(XbaNpidProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_copy_kernel.argtypes = [vp, vp, C.c_int64, vp]
     L.xba_strerror.restype = C.c_char_p
     L.xba_strerror.argtypes = [C.c_int]

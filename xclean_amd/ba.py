@@ -9,6 +9,11 @@ the reference's forms.
 
 The whole loop runs in FP64 on the device as one stream of kernels (libxba ``xba_np_solve``); the host only plans
 the landmark grouping.  There is no CPU path: without the built library or a GPU this raises.
+
+``_solve_np_id`` is the inverse-depth twin (backend/ba.py:1212-1342, libxba ``xba_np_id_solve``): landmarks are a
+bearing (x_n, y_n, 1) and inverse depth in the camera frame of a host keyframe, or of a constant out-of-window host
+pose; every observation has an observer and a host pose block; the bearing prior is unrobust, the inverse-range
+prior Huber; the rotation factors' Hessian enters the reduced system; updated inverse depths are floored.
 """
 from __future__ import annotations
 
@@ -97,6 +102,81 @@ def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None
     s = stats.cpu().numpy()
     st = {"chi2_0": float(s[0]), "chi2_n": float(s[1]), "rms_0": float(s[2]), "rms_n": float(s[3])}
     out = (d_poses.cpu().numpy(), d_pts.cpu().numpy(), st)
+    del keep
+    if history:
+        return out + (dxh.cpu().numpy(),)
+    return out
+
+
+def _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
+                 rotf=None, rho_floor=1e-4, device=None, history=False):
+    """backend/ba.py:1212 ``_solve_np_id``.  Returns (poses T_wc (nk,4,4), rho (nl,), mbear (nl,3), stats dict) as
+    numpy arrays, stats = {chi2_0, chi2_n, rms_0, rms_n}; ``history=True`` also returns the (iters, 6 nk) pose
+    updates.  ``prior`` = (m0 (nl, >=2), rho0, wbear, invsig, valid), arrays or scalars; ``rotf`` = [(a, b, R (3,3),
+    sigma), ...]; ``hpose_ext`` (nl, 12) or (nl, 3, 4): host T_wc rows for landmarks with hidx < 0."""
+    import torch
+    _lib.require_gpu()
+    L = _lib.load()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    poses = np.asarray(poses, np.float64).reshape(-1, 4, 4)
+    rho = np.asarray(rho, np.float64).ravel()
+    nk, nl = len(poses), len(rho)
+    mbear = np.asarray(mbear, np.float64).reshape(nl, 3)
+    hidx = np.asarray(hidx).astype(np.int32).ravel()
+    hext = np.asarray(hpose_ext, np.float64).reshape(-1, 12) if hpose_ext is not None else np.zeros((0, 12))
+    if len(hext) != nl:
+        hext = np.zeros((nl, 12)) if not (hidx < 0).any() else np.broadcast_to(hext, (nl, 12))
+    okf = np.asarray(okf).astype(np.int32).ravel()
+    olm = np.asarray(olm).astype(np.int32).ravel()
+    ouv = np.asarray(ouv, np.float64).reshape(-1, 2)
+    N = len(okf)
+    if len(olm) != N or len(ouv) != N or len(hidx) != nl:
+        raise ValueError("okf, olm and ouv need one row per observation and hidx one entry per landmark")
+    fm = _free_mask(free, nk)
+    if fm.shape != (nk,):
+        raise ValueError("free mask must have one entry per keyframe")
+
+    def dt(x, dtype=torch.float64):
+        return torch.as_tensor(np.array(x, copy=True), dtype=dtype).to(dev).contiguous()
+
+    d_poses, d_rho, d_mb = dt(poses), dt(rho), dt(mbear)
+    d_hidx, d_hext = dt(hidx, torch.int32), dt(np.ascontiguousarray(hext))
+    d_okf, d_olm, d_ouv = dt(okf, torch.int32), dt(olm, torch.int32), dt(ouv)
+    d_free = dt(fm.astype(np.uint8), torch.uint8)
+    keep = [d_poses, d_rho, d_mb, d_hidx, d_hext, d_okf, d_olm, d_ouv, d_free]
+    q = _lib.XbaNpidProblem()
+    q.nk, q.nl, q.N = nk, nl, N
+    q.poses, q.rho, q.mbear, q.hidx, q.hpose_ext = (d_poses.data_ptr(), d_rho.data_ptr(), d_mb.data_ptr(),
+                                                    d_hidx.data_ptr(), d_hext.data_ptr())
+    q.okf, q.olm, q.ouv, q.free_mask = d_okf.data_ptr(), d_olm.data_ptr(), d_ouv.data_ptr(), d_free.data_ptr()
+    q.fx, q.cx, q.cy = _rig_intrinsics(rig)
+    q.huber_k, q.rho_floor = HUBER_K, float(rho_floor)
+    if prior is not None:
+        m0, rho0, wbear, invsig, valid = prior
+        m0 = np.asarray(m0, np.float64).reshape(nl, -1)
+        pm = np.zeros((nl, 3))
+        pm[:, :min(3, m0.shape[1])] = m0[:, :3]
+        arrs = [dt(np.broadcast_to(np.asarray(a, np.float64), (nl,))) for a in (rho0, wbear, invsig, valid)]
+        d_pm = dt(pm)
+        keep += [d_pm] + arrs
+        q.pr_m0 = d_pm.data_ptr()
+        q.pr_rho0, q.pr_wbear, q.pr_invsig, q.pr_valid = [a.data_ptr() for a in arrs]
+    if rotf:
+        ab = dt(np.array([[f[0], f[1]] for f in rotf], np.int32), torch.int32)
+        Ms = dt(np.stack([np.asarray(f[2], np.float64) for f in rotf]))
+        sg = dt(np.array([float(f[3]) for f in rotf], np.float64))
+        keep += [ab, Ms, sg]
+        q.n_rot, q.rot_ab, q.rot_M, q.rot_sig = len(rotf), ab.data_ptr(), Ms.data_ptr(), sg.data_ptr()
+    stats = torch.zeros(4, dtype=torch.float64, device=dev)
+    dxh = torch.zeros((max(iters, 0), D_POSE * nk), dtype=torch.float64, device=dev) if history else None
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        rc = L.xba_np_id_solve(C.byref(q), int(iters), float(lam0), _lib.XBA_FP64, C.c_void_p(stats.data_ptr()),
+                               C.c_void_p(dxh.data_ptr()) if dxh is not None else None, C.c_void_p(stream))
+    _lib.check(rc, "xba_np_id_solve")
+    s = stats.cpu().numpy()
+    st = {"chi2_0": float(s[0]), "chi2_n": float(s[1]), "rms_0": float(s[2]), "rms_n": float(s[3])}
+    out = (d_poses.cpu().numpy(), d_rho.cpu().numpy(), d_mb.cpu().numpy(), st)
     del keep
     if history:
         return out + (dxh.cpu().numpy(),)

@@ -1253,4 +1253,119 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
     return rc;
 }
 
+
+int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                    double *dx_hist_out, void *stream) {
+    if (!q || iters < 0 || q->nk < 1 || q->nl < 0 || q->N < 0 || !q->poses || !stats_out) return XBA_E_ARG;
+    if (precision != XBA_FP64) return XBA_E_ARG;  // the reference solve is FP64 throughout
+    if ((q->nl > 0 && (!q->rho || !q->mbear || !q->hidx)) || (q->N > 0 && (!q->okf || !q->olm || !q->ouv)) ||
+        !q->free_mask)
+        return XBA_E_ARG;
+    if (q->pr_m0 && (!q->pr_rho0 || !q->pr_wbear || !q->pr_invsig || !q->pr_valid)) return XBA_E_ARG;
+    if (q->n_rot < 0 || (q->n_rot > 0 && (!q->rot_ab || !q->rot_M || !q->rot_sig))) return XBA_E_ARG;
+    const int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
+    if (n > XBA_MAX_STATE) return XBA_E_SIZE;
+    hipStream_t s = (hipStream_t)stream;
+
+    // host planning: observations grouped by landmark (stable), index validation
+    std::vector<int32_t> okf(N), olm(N), order(N), lstart(nl + 1, 0), rab(2 * (size_t)q->n_rot), hidx(nl);
+    if (N) {
+        HIPCK(hipMemcpyAsync(okf.data(), q->okf, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(olm.data(), q->olm, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+    }
+    if (nl) HIPCK(hipMemcpyAsync(hidx.data(), q->hidx, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, s));
+    if (q->n_rot)
+        HIPCK(hipMemcpyAsync(rab.data(), q->rot_ab, sizeof(int32_t) * rab.size(), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (int k = 0; k < N; k++) {
+        if (okf[k] < 0 || okf[k] >= nk || olm[k] < 0 || olm[k] >= nl) return XBA_E_RANGE;
+        lstart[olm[k] + 1]++;
+    }
+    bool any_ext = false;
+    for (int j = 0; j < nl; j++) {
+        if (hidx[j] >= nk || hidx[j] < -1) return XBA_E_RANGE;
+        any_ext |= hidx[j] < 0;
+    }
+    if (any_ext && !q->hpose_ext) return XBA_E_ARG;
+    for (int f = 0; f < 2 * q->n_rot; f++)
+        if (rab[f] < 0 || rab[f] >= nk) return XBA_E_RANGE;
+    for (int j = 0; j < nl; j++) lstart[j + 1] += lstart[j];
+    {
+        std::vector<int32_t> fill(lstart.begin(), lstart.end() - 1);
+        for (int k = 0; k < N; k++) order[fill[olm[k]]++] = k;
+    }
+
+    std::vector<void *> owned;
+    auto release = [&]() {
+        for (void *p : owned) (void)hipFree(p);
+    };
+    NpArgs A{};
+    A.nk = nk; A.nl = nl; A.N = N; A.n_rot = q->n_rot; A.has_prior = q->pr_m0 ? 1 : 0; A.rot_h = 1;
+    A.fx = q->fx; A.cx = q->cx; A.cy = q->cy; A.K = q->huber_k; A.rho_floor = q->rho_floor;
+    A.okf = q->okf; A.olm = q->olm; A.ouv = q->ouv; A.free_mask = q->free_mask;
+    A.hidx = q->hidx; A.hext = q->hpose_ext; A.rho = q->rho; A.mb = q->mbear;
+    A.pr_m0 = q->pr_m0; A.pr_rho0 = q->pr_rho0; A.pr_wbear = q->pr_wbear; A.pr_invsig = q->pr_invsig;
+    A.pr_valid = q->pr_valid;
+    A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
+    A.dx_hist = dx_hist_out;
+    A.Tcw = dalloc<double>(16 * (size_t)nk, owned);
+    A.Tnew = dalloc<double>(16 * (size_t)nk, owned);
+    A.Twc = dalloc<double>(16 * (size_t)nk, owned);
+    A.Twn = dalloc<double>(16 * (size_t)nk, owned);
+    A.rhon = dalloc<double>(nl, owned);
+    A.mbn = dalloc<double>(3 * (size_t)nl, owned);
+    A.J = dalloc<double>(32 * (size_t)N, owned);
+    A.r2 = dalloc<double>(N, owned);
+    A.Hlli = dalloc<double>(9 * (size_t)nl, owned);
+    A.bl = dalloc<double>(3 * (size_t)nl, owned);
+    A.H = dalloc<double>((size_t)n * n, owned);
+    A.bp = dalloc<double>(n, owned);
+    A.dx = dalloc<double>(n, owned);
+    A.st = dalloc<double>(8, owned);
+    int32_t *d_order = dalloc<int32_t>(N, owned), *d_lstart = dalloc<int32_t>(nl + 1, owned);
+    if (owned.size() != 16) {
+        release();
+        return XBA_E_HIP;
+    }
+    A.order = d_order;
+    A.lstart = d_lstart;
+    const double st0[8] = {lam0, 0, 0, 0, 1, 0, 0, 0};
+    auto run = [&]() -> int {
+        if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(A.st, st0, sizeof(st0), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(A.Twc, q->poses, 16 * sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+        if (dx_hist_out && iters)
+            HIPCK(hipMemsetAsync(dx_hist_out, 0, sizeof(double) * (size_t)iters * n, s));
+        const int gk = (nk + 255) / 256, gN = (N + 255) / 256, gl = (nl + 255) / 256;
+        const int gz = std::min((n * n + 255) / 256, 1024);
+        const int gc = (int)((std::max<int64_t>(16 * (int64_t)nk, 3 * (int64_t)nl) + 255) / 256);
+        k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Twc, A.Tcw);
+        if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 0);
+        k_np_chi<<<1, 256, 0, s>>>(A, 5);
+        for (int it = 0; it < iters; it++) {
+            if (gN) k_id_obs<<<gN, 256, 0, s>>>(A);
+            if (gl) k_id_lm<<<gl, 256, 0, s>>>(A);
+            k_np_zero<<<gz, 256, 0, s>>>(A, n);
+            if (gl) k_id_schur<<<gl, 256, 0, s>>>(A, n);
+            k_np_solve<<<1, 256, 0, s>>>(A, n, it);
+            k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tnew, A.Twn);
+            if (gl) k_id_backsub<<<gl, 256, 0, s>>>(A);
+            if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 1);
+            k_np_accept<<<1, 256, 0, s>>>(A);
+            k_id_commit<<<gc, 256, 0, s>>>(A);
+        }
+        if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 0);
+        k_np_chi<<<1, 256, 0, s>>>(A, 6);
+        HIPCK(hipMemcpyAsync(q->poses, A.Twc, 16 * sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+        k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(s));
+        return XBA_OK;
+    };
+    const int rc = run();
+    release();
+    return rc;
+}
+
 }  // extern "C"

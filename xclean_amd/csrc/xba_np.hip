@@ -26,6 +26,14 @@ struct NpArgs {
     double *H, *bp, *dx;           // (n,n), (n), (n)
     double *st;                    // device scalars: [0] lam [1] chi_cur [2] chi_new [3] accept [4] ok [5] chi0 [6] chi_n
     double *dx_hist;               // (iters, n) or null
+    // the inverse-depth solve (backend/ba.py:1212-1342); J is then (N,32): Jobs 12, Jhost 12, Jlm 6, rw 2
+    int32_t rot_h;                 // 1: the rotation factors' Hessian blocks enter H as well as their gradient
+    const int32_t *hidx;           // (nl) host keyframe, or -1 (host pose from hext)
+    const double *hext;            // (nl,12) host T_wc rows [R | t] when hidx < 0
+    double *Twc, *Twn;             // (nk,16) T_wc of the current and the trial poses
+    double *rho, *mb, *rhon, *mbn; // (nl), (nl,3) inverse depth and host-frame bearing, current and trial
+    double rho_floor;
+    const double *pr_m0, *pr_wbear; // bearing prior: (nl,3) mean, (nl) weight
 };
 
 __device__ inline void np_residual(const NpArgs &A, const double *T, const double *X, const double *uv, double *P,
@@ -185,6 +193,22 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
                 }
                 A.bp[6 * a + 3 + i] += -ga;
                 A.bp[6 * b + 3 + i] += -gb;
+            }
+            if (A.rot_h) {  // the inverse-depth solve adds the Hessian blocks too (ba.py:1293-1296)
+                const int ia = 6 * a + 3, ib = 6 * b + 3;
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) {
+                        double haa = 0.0, hab = 0.0, hbb = 0.0;
+                        for (int m = 0; m < 3; m++) {
+                            haa += Ja[3 * m + i] * Ja[3 * m + c];
+                            hab += Ja[3 * m + i] * Jb[3 * m + c];
+                            hbb += Jb[3 * m + i] * Jb[3 * m + c];
+                        }
+                        A.H[(size_t)(ia + i) * n + ia + c] += haa;
+                        A.H[(size_t)(ia + i) * n + ib + c] += hab;
+                        A.H[(size_t)(ib + c) * n + ia + i] += hab;
+                        A.H[(size_t)(ib + i) * n + ib + c] += hbb;
+                    }
             }
         }
     __syncthreads();
@@ -368,6 +392,217 @@ __global__ void k_np_stats(const double *st, int N, double *stats) {
     stats[1] = st[6];
     stats[2] = N ? sqrt(st[5] / N) : NAN;
     stats[3] = N ? sqrt(st[6] / N) : NAN;
+}
+
+
+// ============================================================= the inverse-depth local BA (ba.py:1212-1342) ==
+// Host frame (R_wc, t_wc) of landmark l: the window pose Twc[hidx] or the constant hext row (ba.py:1116-1135).
+__device__ inline void id_host(const NpArgs &A, const double *Twc, int l, double *R, double *t) {
+    const int h = A.hidx[l];
+    const double *T = h >= 0 ? Twc + 16 * (size_t)h : nullptr, *E = A.hext + 12 * (size_t)l;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) R[3 * i + j] = T ? T[4 * i + j] : E[4 * i + j];
+        t[i] = T ? T[4 * i + 3] : E[4 * i + 3];
+    }
+}
+
+// ba.py:1226-1234 geom for observation k: host point Ph, camera point Pc, residual r.
+__device__ inline void id_residual(const NpArgs &A, int k, const double *Twc, const double *Tcw, const double *rho,
+                                   const double *mb, double *R, double *Ph, double *Pc, double *r) {
+    const int l = A.olm[k];
+    double t[3], Pw[3];
+    id_host(A, Twc, l, R, t);
+    const double ir = fmax(rho[l], 1e-12);
+    for (int i = 0; i < 3; i++) Ph[i] = mb[3 * (size_t)l + i] / ir;
+    for (int i = 0; i < 3; i++) Pw[i] = (R[3 * i] * Ph[0] + R[3 * i + 1] * Ph[1] + R[3 * i + 2] * Ph[2]) + t[i];
+    const double *T = Tcw + 16 * (size_t)A.okf[k];
+    for (int i = 0; i < 3; i++) Pc[i] = (T[4 * i] * Pw[0] + T[4 * i + 1] * Pw[1] + T[4 * i + 2] * Pw[2]) + T[4 * i + 3];
+    const double z = fmax(Pc[2], 1e-6);
+    r[0] = A.fx * Pc[0] / z + A.cx - A.ouv[2 * (size_t)k];
+    r[1] = A.fx * Pc[1] / z + A.cy - A.ouv[2 * (size_t)k + 1];
+}
+
+// Residual, Huber weight and the sqrt(w)-scaled blocks of _analytic_blocks_id (ba.py:1138-1209): observer
+// [I | -hat(Pc)], host [-I | hat(Ph)] in the host frame (0 for hosts outside the window), landmark (x_n, y_n, rho).
+__global__ void k_id_obs(NpArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.N) return;
+    double R[9], Ph[3], Pc[3], r[2];
+    id_residual(A, k, A.Twc, A.Tcw, A.rho, A.mb, R, Ph, Pc, r);
+    A.r2[k] = r[0] * r[0] + r[1] * r[1];
+    const double e = sqrt(r[0] * r[0] + r[1] * r[1]);
+    const double w = (e <= A.K) ? 1.0 : A.K / fmax(e, 1e-9), sw = sqrt(w);
+    const double x = Pc[0], y = Pc[1], zz = fmax(Pc[2], 1e-6), iz = 1.0 / zz, fx = A.fx;
+    const double Am[2][3] = {{fx * iz, 0.0, -fx * x * iz * iz}, {0.0, fx * iz, -fx * y * iz * iz}};
+    const double dPo[3][6] = {{1, 0, 0, 0, zz, -y}, {0, 1, 0, -zz, 0, x}, {0, 0, 1, y, -x, 0}};
+    const double dPh[3][6] = {{-1, 0, 0, 0, -Ph[2], Ph[1]}, {0, -1, 0, Ph[2], 0, -Ph[0]}, {0, 0, -1, -Ph[1], Ph[0], 0}};
+    const double *T = A.Tcw + 16 * (size_t)A.okf[k];
+    double Rcw[9], Rch[9], ARch[2][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Rcw[3 * i + j] = T[4 * i + j];
+    mm3(Rcw, R, Rch);
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 3; c++) ARch[a][c] = Am[a][0] * Rch[c] + Am[a][1] * Rch[3 + c] + Am[a][2] * Rch[6 + c];
+    const bool host_in = A.hidx[A.olm[k]] >= 0;
+    const double irho = 1.0 / fmax(A.rho[A.olm[k]], 1e-12);
+    double *J = A.J + 32 * (size_t)k;
+    for (int a = 0; a < 2; a++) {
+        for (int c = 0; c < 6; c++) {
+            J[6 * a + c] = (Am[a][0] * dPo[0][c] + Am[a][1] * dPo[1][c] + Am[a][2] * dPo[2][c]) * sw;
+            J[12 + 6 * a + c] =
+                host_in ? (ARch[a][0] * dPh[0][c] + ARch[a][1] * dPh[1][c] + ARch[a][2] * dPh[2][c]) * sw : 0.0;
+        }
+        J[24 + 3 * a] = ARch[a][0] * irho * sw;
+        J[24 + 3 * a + 1] = ARch[a][1] * irho * sw;
+        J[24 + 3 * a + 2] = -(ARch[a][0] * Ph[0] + ARch[a][1] * Ph[1] + ARch[a][2] * Ph[2]) * irho * sw;
+        J[30 + a] = r[a] * sw;
+    }
+}
+
+// Per landmark: Hll, bl, the bearing prior (unrobust) and inverse-range prior (Huber), damping by
+// lam * max(tr/3, 1e-9), inverse (ba.py:1244-1264, 1298-1301).
+__global__ void k_id_lm(NpArgs A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl) return;
+    const double lam = A.st[0];
+    double Hll[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bl[3] = {0, 0, 0};
+    for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
+        const double *J = A.J + 32 * (size_t)A.order[q], *Jl = J + 24, *rw = J + 30;
+        for (int i = 0; i < 3; i++) {
+            for (int c = 0; c < 3; c++) Hll[3 * i + c] += Jl[i] * Jl[c] + Jl[3 + i] * Jl[3 + c];
+            bl[i] += -(Jl[i] * rw[0] + Jl[3 + i] * rw[1]);
+        }
+    }
+    if (A.has_prior && A.pr_valid[j] > 0.5) {
+        const double *mb = A.mb + 3 * (size_t)j, *m0 = A.pr_m0 + 3 * (size_t)j;
+        const double rp[3] = {(mb[0] - m0[0]) * A.pr_wbear[j], (mb[1] - m0[1]) * A.pr_wbear[j],
+                              (A.rho[j] - A.pr_rho0[j]) * A.pr_invsig[j]};
+        const double J0[3] = {A.pr_wbear[j], A.pr_wbear[j], A.pr_invsig[j]};
+        for (int c = 0; c < 3; c++) {
+            const double wp = fabs(rp[c]) <= A.K ? 1.0 : A.K / fmax(fabs(rp[c]), 1e-9);
+            const double swp = c < 2 ? 1.0 : sqrt(wp), Jpr = J0[c] * swp;
+            Hll[4 * c] += Jpr * Jpr;
+            bl[c] -= Jpr * rp[c] * swp;
+        }
+    }
+    const double tr = Hll[0] + Hll[4] + Hll[8], d = lam * fmax(tr / 3.0, 1e-9);
+    for (int i = 0; i < 3; i++) Hll[4 * i] += d;
+    const double c00 = Hll[4] * Hll[8] - Hll[5] * Hll[7], c01 = Hll[5] * Hll[6] - Hll[3] * Hll[8],
+                 c02 = Hll[3] * Hll[7] - Hll[4] * Hll[6];
+    const double det = Hll[0] * c00 + Hll[1] * c01 + Hll[2] * c02, id = 1.0 / det;
+    double *Hi = A.Hlli + 9 * (size_t)j;
+    Hi[0] = c00 * id; Hi[1] = (Hll[2] * Hll[7] - Hll[1] * Hll[8]) * id; Hi[2] = (Hll[1] * Hll[5] - Hll[2] * Hll[4]) * id;
+    Hi[3] = c01 * id; Hi[4] = (Hll[0] * Hll[8] - Hll[2] * Hll[6]) * id; Hi[5] = (Hll[2] * Hll[3] - Hll[0] * Hll[5]) * id;
+    Hi[6] = c02 * id; Hi[7] = (Hll[1] * Hll[6] - Hll[0] * Hll[7]) * id; Hi[8] = (Hll[0] * Hll[4] - Hll[1] * Hll[3]) * id;
+    for (int i = 0; i < 3; i++) A.bl[3 * (size_t)j + i] = bl[i];
+}
+
+__device__ inline void id_hpl(const double *J, int sa, double *Hpl) {  // (6x3) Jpose_sa^T Jlm
+    const double *Jp = J + 12 * sa, *Jl = J + 24;
+    for (int i = 0; i < 6; i++)
+        for (int c = 0; c < 3; c++) Hpl[3 * i + c] = Jp[i] * Jl[c] + Jp[6 + i] * Jl[3 + c];
+}
+
+// Reduced system, one thread per landmark, FP64 atomics (ba.py:1266-1305): every observation contributes its
+// observer and (in-window) host pose blocks to Hpp and the gradient; H -= (Hpl_a Hlli) Hpl_b^T over the landmark's
+// (observation, slot) entries; bp -= (Hpl_a Hlli) bl.
+__global__ void k_id_schur(NpArgs A, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl) return;
+    const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
+    const int hs = A.hidx[j];
+    for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
+        const int ka = A.order[qa];
+        const double *Ja = A.J + 32 * (size_t)ka, *rw = Ja + 30;
+        const int sla[2] = {A.okf[ka], hs};
+        for (int sa = 0; sa < 2; sa++) {
+            if (sla[sa] < 0) continue;
+            const double *Jp = Ja + 12 * sa;
+            const int ki = 6 * sla[sa];
+            for (int sb = 0; sb < 2; sb++) {
+                if (sla[sb] < 0) continue;
+                const double *Jq = Ja + 12 * sb;
+                const int kk = 6 * sla[sb];
+                for (int i = 0; i < 6; i++)
+                    for (int c = 0; c < 6; c++)
+                        atomicAdd(A.H + (size_t)(ki + i) * n + kk + c, Jp[i] * Jq[c] + Jp[6 + i] * Jq[6 + c]);
+            }
+            double Hpa[18], T3[18];
+            id_hpl(Ja, sa, Hpa);
+            for (int i = 0; i < 6; i++)
+                for (int c = 0; c < 3; c++)
+                    T3[3 * i + c] = Hpa[3 * i] * Hi[c] + Hpa[3 * i + 1] * Hi[3 + c] + Hpa[3 * i + 2] * Hi[6 + c];
+            for (int i = 0; i < 6; i++)
+                atomicAdd(A.bp + ki + i, -(Jp[i] * rw[0] + Jp[6 + i] * rw[1]) -
+                                             (T3[3 * i] * bl[0] + T3[3 * i + 1] * bl[1] + T3[3 * i + 2] * bl[2]));
+            for (int qb = A.lstart[j]; qb < A.lstart[j + 1]; qb++) {
+                const int kb = A.order[qb];
+                const int slb[2] = {A.okf[kb], hs};
+                for (int sb = 0; sb < 2; sb++) {
+                    if (slb[sb] < 0) continue;
+                    double Hpb[18];
+                    id_hpl(A.J + 32 * (size_t)kb, sb, Hpb);
+                    const int kk = 6 * slb[sb];
+                    for (int i = 0; i < 6; i++)
+                        for (int c = 0; c < 6; c++)
+                            atomicAdd(A.H + (size_t)(ki + i) * n + kk + c,
+                                      -(T3[3 * i] * Hpb[3 * c] + T3[3 * i + 1] * Hpb[3 * c + 1] + T3[3 * i + 2] * Hpb[3 * c + 2]));
+                }
+            }
+        }
+    }
+}
+
+// dxl = Hlli (bl - sum Hpl^T dxp), trial bearing (x_n, y_n += dxl) and inverse depth max(rho + dxl, floor)
+// (ba.py:1318-1330).
+__global__ void k_id_backsub(NpArgs A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nl || A.st[4] == 0.0) return;
+    double rhs[3] = {A.bl[3 * (size_t)j], A.bl[3 * (size_t)j + 1], A.bl[3 * (size_t)j + 2]};
+    const int hs = A.hidx[j];
+    for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
+        const int k = A.order[q];
+        const int sl[2] = {A.okf[k], hs};
+        for (int sa = 0; sa < 2; sa++) {
+            if (sl[sa] < 0) continue;
+            double Hp[18];
+            id_hpl(A.J + 32 * (size_t)k, sa, Hp);
+            const double *d = A.dx + 6 * sl[sa];
+            for (int c = 0; c < 3; c++) {
+                double s = 0.0;
+                for (int i = 0; i < 6; i++) s += Hp[3 * i + c] * d[i];
+                rhs[c] += -s;
+            }
+        }
+    }
+    const double *Hi = A.Hlli + 9 * (size_t)j;
+    double dl[3];
+    for (int i = 0; i < 3; i++) dl[i] = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+    A.mbn[3 * (size_t)j] = A.mb[3 * (size_t)j] + dl[0];
+    A.mbn[3 * (size_t)j + 1] = A.mb[3 * (size_t)j + 1] + dl[1];
+    A.mbn[3 * (size_t)j + 2] = A.mb[3 * (size_t)j + 2];
+    A.rhon[j] = fmax(A.rho[j] + dl[2], A.rho_floor);
+}
+
+// r2 at the trial (trial = 1) or the current state.
+__global__ void k_id_r2(NpArgs A, int trial) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.N || (trial && A.st[4] == 0.0)) return;
+    double R[9], Ph[3], Pc[3], r[2];
+    if (trial)
+        id_residual(A, k, A.Twn, A.Tnew, A.rhon, A.mbn, R, Ph, Pc, r);
+    else
+        id_residual(A, k, A.Twc, A.Tcw, A.rho, A.mb, R, Ph, Pc, r);
+    A.r2[k] = r[0] * r[0] + r[1] * r[1];
+}
+
+// On an accepted step: the host frames, inverse depths and bearings follow the poses k_np_accept committed.
+__global__ void k_id_commit(NpArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A.st[3] == 0.0) return;
+    if (i < 16 * A.nk) A.Twc[i] = A.Twn[i];
+    if (i < A.nl) A.rho[i] = A.rhon[i];
+    if (i < 3 * A.nl) A.mb[i] = A.mbn[i];
 }
 
 }  // namespace xba

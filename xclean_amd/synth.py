@@ -143,6 +143,37 @@ def make_problem(nk=15, nl=600, obs_per=4, seed=5, pose_noise=0.02, pt_noise=0.0
                 olm=olm.astype(np.int32), ouv=ouv)
 
 
+def to_inverse_depth(pb, ext_every=0, ext_shift=None):
+    """The inverse-depth form of a ``make_problem`` problem for ``_solve_np_id`` (backend/ba.py:1212): every
+    landmark is hosted by the keyframe of its first observation, with its bearing (x/z, y/z, 1) and inverse depth
+    1/z in that keyframe's camera frame at the initial poses and points.  With ``ext_every`` = m > 0, every m-th
+    landmark's host is a constant pose outside the window (hidx = -1; the first observer's initial pose, moved by
+    the twist ``ext_shift`` when given), as the reference's cached out-of-window hosts.  Returns dict(poses0, rho0,
+    mbear0, hidx, hpose_ext (nl, 12), okf, olm, ouv, rig)."""
+    poses0, pts0 = pb["poses0"], pb["pts0"]
+    okf, olm = np.asarray(pb["okf"], np.int64), np.asarray(pb["olm"], np.int64)
+    nl = len(pts0)
+    first = np.full(nl, -1, np.int64)
+    for k in range(len(okf) - 1, -1, -1):
+        first[olm[k]] = okf[k]
+    host_T = poses0[first].copy()
+    hidx = first.astype(np.int32)
+    if ext_every > 0:
+        ext = np.arange(nl) % ext_every == ext_every - 1
+        hidx[ext] = -1
+        if ext_shift is not None:
+            host_T[ext] = np.einsum("nij,jk->nik", host_T[ext], se3_exp(ext_shift))
+    Tcw_h = np.stack([se3_inv(T) for T in host_T])
+    Xh = np.einsum("nij,nj->ni", Tcw_h[:, :3, :3], pts0) + Tcw_h[:, :3, 3]
+    mbear = np.stack([Xh[:, 0] / Xh[:, 2], Xh[:, 1] / Xh[:, 2], np.ones(nl)], 1)
+    rho = 1.0 / Xh[:, 2]
+    hext = np.zeros((nl, 12))
+    ext = hidx < 0
+    hext[ext] = host_T[ext, :3, :4].reshape(-1, 12)
+    return dict(poses0=poses0.copy(), rho0=rho, mbear0=mbear, hidx=hidx, hpose_ext=hext, okf=pb["okf"],
+                olm=pb["olm"], ouv=pb["ouv"], rig=pb["rig"], poses_true=pb["poses_true"])
+
+
 # ------------------------------------------------------------------------------------ inverse-depth windows --
 R_CAM_IN_BODY = np.array([[0.0, 0.0, 1.0], [-1.0, 0.0, 0.0], [0.0, -1.0, 0.0]])  # camera z forward = body x
 
