@@ -12,12 +12,13 @@
  *       vio/window.py:985-1008 `final_linearisation(pb)` (and one build_system at any damping): the reduced
  *       system H, b of every window, plus per-observation whitened chi2 and the landmark marginals.
  *   xba_np_solve
- *       backend/ba.py:713-869 `_solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0, prior, rotf)` (and its
- *       FP32 twin `_solve_gpu`, ba.py:930-1089): the local-BA drop-in that tests/test_ba_solver.py calls.
+ *       backend/ba.py:713-869 `_solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0, prior, rotf)` and, with
+ *       lm_policy = 1, its GPU twin `_solve_gpu` (ba.py:930-1089, the same solve under that driver's LM rule, here in
+ *       FP64): the local-BA drop-ins that tests/test_ba_solver.py and LocalBA.solve call.
  *   xba_np_id_solve
  *       backend/ba.py:1212-1342 `_solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters,
  *       lam0, prior, rotf, rho_floor)`: the inverse-depth local BA (landmarks hosted by a keyframe or by a constant
- *       out-of-window pose).
+ *       out-of-window pose); with lm_policy = 1 its GPU twin `_solve_gpu_id` (ba.py:1345-1497).
  *   xba_backward
  *       no reference counterpart (the reference has no differentiable solve): the implicit-function adjoint of
  *       the last solved update with respect to observed pixels and per-observation weights.
@@ -189,6 +190,10 @@ typedef struct xba_np_problem {
     int32_t n_rot;
     const int32_t *rot_ab;
     const double *rot_M, *rot_sig;
+    int32_t lm_policy;  /* 0: _solve_np's LM rule; 1: the GPU driver _solve_gpu's (ba.py:930-1089): every step applied,
+                           rejected by the next iteration when the Huber-weighted chi2 grew (state restored, lambda
+                           x4), non-finite steps lambda x10, the rotation factors' Hessian in the reduced system,
+                           SO(3) projected once at the end; stats are then its Huber-weighted chi2 / rms */
 } xba_np_problem;
 
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
@@ -215,6 +220,7 @@ typedef struct xba_npid_problem {
     int32_t n_rot;
     const int32_t *rot_ab;
     const double *rot_M, *rot_sig;
+    int32_t lm_policy;  /* as xba_np_problem.lm_policy; 1: _solve_gpu_id (ba.py:1345-1497) */
 } xba_npid_problem;
 
 int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,

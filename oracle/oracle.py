@@ -52,7 +52,8 @@ class OnpProblem(C.Structure):
                 ("poses", f64p), ("pts", f64p), ("okf", i32p), ("olm", i32p), ("ouv", f64p),
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
                 ("free_mask", u8p), ("pr_c", f64p), ("pr_rho0", f64p), ("pr_invsig", f64p),
-                ("pr_valid", f64p), ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p)]
+                ("pr_valid", f64p), ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p),
+                ("lm_policy", C.c_int32)]
 
 
 class OnpidProblem(C.Structure):
@@ -62,7 +63,8 @@ class OnpidProblem(C.Structure):
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
                 ("rho_floor", C.c_double), ("free_mask", u8p),
                 ("pr_m0", f64p), ("pr_rho0", f64p), ("pr_wbear", f64p), ("pr_invsig", f64p), ("pr_valid", f64p),
-                ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p)]
+                ("n_rot", C.c_int32), ("rot_ab", i32p), ("rot_M", f64p), ("rot_sig", f64p),
+                ("lm_policy", C.c_int32)]
 
 
 def build():
@@ -272,8 +274,10 @@ def imu_factor(si, sj, pre, g, W=None, whiten=True):
     return r, Ji, Jj, Jg
 
 
-def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None, rotf=None, history=False):
-    """backend/ba.py:713-869 (_solve_np): returns (poses T_wc, pts, stats) like the reference."""
+def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None, rotf=None, history=False,
+             policy="np"):
+    """backend/ba.py:713-869 (_solve_np): returns (poses T_wc, pts, stats) like the reference.  policy="gpu": the LM
+    policy of the reference's GPU driver _solve_gpu (ba.py:930-1089), computed in FP64."""
     nk, nl, N = len(poses), len(pts), len(okf)
     poses = np.ascontiguousarray(poses, np.float64).copy()
     pts = np.ascontiguousarray(pts, np.float64).copy()
@@ -290,6 +294,7 @@ def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
     s.poses, s.pts, s.okf, s.olm, s.ouv = _p(poses, f64p), _p(pts, f64p), _p(okf, i32p), _p(olm, i32p), _p(ouv, f64p)
     s.fx, s.cx, s.cy, s.huber_k = rig.fx_rect, rig.cx, rig.cy, 1.345
     s.free_mask = _p(fm, u8p)
+    s.lm_policy = 1 if policy == "gpu" else 0
     keep = []
     if prior is not None:
         c_, rho0, invsig, valid = prior
@@ -316,8 +321,9 @@ def solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
 
 
 def solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
-                rotf=None, rho_floor=1e-4, history=False):
-    """backend/ba.py:1212-1342 (_solve_np_id): returns (poses T_wc, rho, mbear, stats) like the reference.
+                rotf=None, rho_floor=1e-4, history=False, policy="np"):
+    """backend/ba.py:1212-1342 (_solve_np_id): returns (poses T_wc, rho, mbear, stats) like the reference
+    (policy="gpu": the LM policy of the GPU driver _solve_gpu_id, ba.py:1345-1497, computed in FP64).
     prior = (m0 (nl, >=2), rho0, wbear, invsig, valid) with scalars broadcast; rotf = [(a, b, R, sigma), ...]."""
     nk, nl, N = len(poses), len(rho), len(okf)
     poses = np.ascontiguousarray(poses, np.float64).copy()
@@ -340,6 +346,7 @@ def solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, it
     s.okf, s.olm, s.ouv = _p(okf, i32p), _p(olm, i32p), _p(ouv, f64p)
     s.fx, s.cx, s.cy, s.huber_k, s.rho_floor = rig.fx_rect, rig.cx, rig.cy, 1.345, rho_floor
     s.free_mask = _p(fm, u8p)
+    s.lm_policy = 1 if policy == "gpu" else 0
     keep = [hext]
     if prior is not None:
         m0, rho0, wbear, invsig, valid = prior

@@ -1194,6 +1194,11 @@ typedef struct {
     const int32_t *rot_ab;      /* (n_rot,2) */
     const double *rot_M;        /* (n_rot,3,3) */
     const double *rot_sig;      /* (n_rot) */
+    /* LM policy: 0 _solve_np (trial cost, accept iff the unrobust chi2 does not increase); 1 the GPU drivers
+     * _solve_gpu / _solve_gpu_id (ba.py:1015-1076, 1436-1497): a step is always applied, the next iteration rejects it
+     * when the Huber-weighted chi2 grew (restore the last accepted state, lam x4), a non-finite step sets lam x10,
+     * the rotation factors' Hessian enters, SO(3) is projected once at the end, Cholesky instead of LU */
+    int32_t lm_policy;
 } onp_problem;
 
 static double np_residuals(const onp_problem *q, const double *Tcw, const double *pts, double *r, double *Pc) {
@@ -1243,6 +1248,10 @@ static int lu_solve(double *A, int n, double *b) {
     return 1;
 }
 
+/* backend/ba.py:644-697 with the Hessian blocks (H may be NULL: gradient only); defined below */
+static void rot_blocks(int n_rot, const int32_t *rot_ab, const double *rot_M, const double *rot_sig, const double *Tcw,
+                       int n, double *H, double *g);
+
 /* backend/ba.py:644-697: only the gradient is used by _solve_np (ba.py:825-828) */
 static void rot_factor_grad(const onp_problem *q, const double *Tcw, double *g) {
     for (int f = 0; f < q->n_rot; f++) {
@@ -1279,6 +1288,8 @@ static void rot_factor_grad(const onp_problem *q, const double *Tcw, double *g) 
 int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_hist, uint8_t *acc_hist) {
     int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
     double fx = q->fx, K = q->huber_k, lam = lam0;
+    const int pol = q->lm_policy;
+    double chi_prev = -1.0, best_chi = -1.0, rms0_w = NAN;  /* the GPU drivers' policy state */
     double *Tcw = malloc(16 * nk * sizeof(double)), *Tn = malloc(16 * nk * sizeof(double));
     for (int i = 0; i < nk; i++) se3_inv(q->poses + 16 * i, Tcw + 16 * i);
     double *r = malloc(2 * (N ? N : 1) * sizeof(double)), *Pc = malloc(3 * (N ? N : 1) * sizeof(double));
@@ -1292,6 +1303,7 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
     double *Hpl = malloc(18 * (N ? N : 1) * sizeof(double)), *T3 = malloc(18 * (N ? N : 1) * sizeof(double));
     double *dxl = malloc(3 * (nl ? nl : 1) * sizeof(double));
     int *order = malloc((N ? N : 1) * sizeof(int)), *bounds = malloc((nl + 1) * sizeof(int));
+    double *Tbest = malloc(16 * nk * sizeof(double)), *pbest = malloc(3 * (nl ? nl : 1) * sizeof(double));
     /* stable argsort by landmark: counting sort */
     memset(bounds, 0, (nl + 1) * sizeof(int));
     for (int k = 0; k < N; k++) bounds[q->olm[k] + 1]++;
@@ -1311,6 +1323,7 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
         memset(bp, 0, n * sizeof(double));
         memset(Hll, 0, 9 * nl * sizeof(double));
         memset(bl, 0, 3 * nl * sizeof(double));
+        double chiw = 0.0;
         for (int k = 0; k < N; k++) {
             double e = sqrt(r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1]);
             double w = (e <= K) ? 1.0 : K / fmax(e, 1e-9), sw = sqrt(w);
@@ -1326,6 +1339,7 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
                     Jlm[a][c] = (Jp3[a][0] * T[c] + Jp3[a][1] * T[4 + c] + Jp3[a][2] * T[8 + c]) * sw;
                 rw[a] = r[2 * k + a] * sw;
             }
+            chiw += rw[0] * rw[0] + rw[1] * rw[1];
             int l = q->olm[k], kf = q->okf[k];
             for (int i = 0; i < 3; i++) {
                 for (int j = 0; j < 3; j++) Hll[9 * l + 3 * i + j] += Jlm[0][i] * Jlm[0][j] + Jlm[1][i] * Jlm[1][j];
@@ -1338,6 +1352,19 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
                 for (int b = 0; b < 6; b++)
                     Hpp[(size_t)(kf * 6 + a) * n + kf * 6 + b] += Jpose[0][a] * Jpose[0][b] + Jpose[1][a] * Jpose[1][b];
             }
+        }
+        if (pol) { /* ba.py:1015-1026: reject the previous step when the weighted chi2 grew */
+            if (isnan(rms0_w)) rms0_w = sqrt(chiw / N);
+            if (chi_prev >= 0.0 && chiw > chi_prev) {
+                memcpy(Tcw, Tbest, 16 * nk * sizeof(double));
+                memcpy(pts, pbest, 3 * nl * sizeof(double));
+                lam = fmin(lam * 4.0, 1e2);
+                continue;
+            }
+            chi_prev = chiw;
+            best_chi = chiw;
+            memcpy(Tbest, Tcw, 16 * nk * sizeof(double));
+            memcpy(pbest, pts, 3 * nl * sizeof(double));
         }
         if (q->pr_c) { /* ba.py:774-794 */
             int any = 0;
@@ -1395,7 +1422,12 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
                 }
             }
         }
-        if (q->n_rot) rot_factor_grad(q, Tcw, bp);
+        if (q->n_rot) {
+            if (pol)
+                rot_blocks(q->n_rot, q->rot_ab, q->rot_M, q->rot_sig, Tcw, n, Hred, bp); /* ba.py:1041-1047 */
+            else
+                rot_factor_grad(q, Tcw, bp);
+        }
         for (int i = 0; i < nk; i++) {
             if (q->free_mask[i]) continue;
             int f0 = 6 * i;
@@ -1411,7 +1443,19 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
             }
         }
         for (int i = 0; i < n; i++) Hred[(size_t)i * n + i] += lam * fmax(Hred[(size_t)i * n + i], 1e-9);
-        if (!lu_solve(Hred, n, bp)) break;
+        int solved;
+        if (pol) { /* the GPU Cholesky; its status is ignored there, a failure shows as a non-finite step */
+            solved = chol(Hred, n);
+            if (solved) {
+                double *x = malloc(n * sizeof(double));
+                chol_solve(Hred, n, bp, x);
+                memcpy(bp, x, n * sizeof(double));
+                free(x);
+            }
+        } else {
+            solved = lu_solve(Hred, n, bp);
+            if (!solved) break;
+        }
         double *dxp = bp;
         if (dx_hist)
             for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = dxp[i];
@@ -1428,6 +1472,33 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
             }
             const double *Hi = Hlli + 9 * j;
             for (int i = 0; i < 3; i++) dxl[3 * j + i] = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+        }
+        if (pol) { /* ba.py:1058-1073: non-finite -> lam x10; else apply (no projection), lam / 2 */
+            int fin = solved;
+            for (int i = 0; i < n && fin; i++) fin = isfinite(dxp[i]);
+            for (int i = 0; i < 3 * nl && fin; i++) fin = isfinite(dxl[i]);
+            if (!fin) {
+                if (dx_hist)
+                    for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = NAN;
+                lam = fmin(lam * 10.0, 1e3);
+                continue;
+            }
+            for (int i = 0; i < nk; i++) {
+                if (!q->free_mask[i]) continue;
+                double E[16], T[16];
+                o_se3_exp(dxp + 6 * i, E);
+                for (int a = 0; a < 4; a++)
+                    for (int c = 0; c < 4; c++) {
+                        double sum = 0.0;
+                        for (int m = 0; m < 4; m++) sum += E[4 * a + m] * Tcw[16 * i + 4 * m + c];
+                        T[4 * a + c] = sum;
+                    }
+                memcpy(Tcw + 16 * i, T, sizeof(T));
+            }
+            for (int j = 0; j < 3 * nl; j++) pts[j] += dxl[j];
+            lam = fmax(lam * 0.5, 1e-7);
+            if (acc_hist) acc_hist[it] = 1;
+            continue;
         }
         memcpy(Tn, Tcw, 16 * nk * sizeof(double));
         for (int i = 0; i < nk; i++) {
@@ -1462,9 +1533,30 @@ int onp_solve(onp_problem *q, int iters, double lam0, double *stats, double *dx_
     stats[1] = chin;
     stats[2] = rms0;
     stats[3] = N ? sqrt(chin / N) : NAN;
+    if (pol) { /* ba.py:1075-1089: one SO(3) projection, then the Huber-weighted chi2 of the result */
+        for (int i = 0; i < nk; i++) {
+            double R[9], Rp[9];
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) R[3 * a + c] = Tcw[16 * i + 4 * a + c];
+            o_so3_project(R, Rp);
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) Tcw[16 * i + 4 * a + c] = Rp[3 * a + c];
+        }
+        np_residuals(q, Tcw, pts, r, NULL);
+        double cw = 0.0;
+        for (int k = 0; k < N; k++) {
+            double e2 = r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1], e = sqrt(e2);
+            double w = (e <= K) ? 1.0 : K / fmax(e, 1e-9);
+            cw += w * e2;
+        }
+        stats[0] = best_chi > 0.0 ? best_chi : cw;
+        stats[1] = cw;
+        stats[2] = rms0_w;
+        stats[3] = sqrt(cw / N);
+    }
     for (int i = 0; i < nk; i++) se3_inv(Tcw + 16 * i, q->poses + 16 * i);
     free(Tcw); free(Tn); free(r); free(Pc); free(r2); free(pn); free(Hpp); free(Hred); free(bp); free(Hll);
-    free(Hlli); free(bl); free(Hpl); free(T3); free(dxl); free(order); free(bounds);
+    free(Hlli); free(bl); free(Hpl); free(T3); free(dxl); free(order); free(bounds); free(Tbest); free(pbest);
     return 0;
 }
 
@@ -1485,6 +1577,7 @@ typedef struct {
     int32_t n_rot;
     const int32_t *rot_ab;
     const double *rot_M, *rot_sig;
+    int32_t lm_policy;          /* as onp_problem.lm_policy */
 } onpid_problem;
 
 /* backend/ba.py:1116-1135 _host_frames: (R_wc, t_wc) of landmark l's host */
@@ -1557,17 +1650,18 @@ void o_id_blocks(const double *Tcw_k, const double *Rh, double rho, const double
 }
 
 /* backend/ba.py:644-697 with the Hessian blocks: _solve_np_id adds both to Hpp and grad (ba.py:1293-1296) */
-static void rot_factor_blocks_id(const onpid_problem *q, const double *Tcw, int n, double *H, double *g) {
-    for (int f = 0; f < q->n_rot; f++) {
-        int a = q->rot_ab[2 * f], b = q->rot_ab[2 * f + 1];
-        double w = 1.0 / fmax(q->rot_sig[f], 1e-9), Ri[9], Rj[9], A[9], E[9], e[3], Jr[9], Ja[9], Jb[9], r[3];
+static void rot_blocks(int n_rot, const int32_t *rot_ab, const double *rot_M, const double *rot_sig, const double *Tcw,
+                       int n, double *H, double *g) {
+    for (int f = 0; f < n_rot; f++) {
+        int a = rot_ab[2 * f], b = rot_ab[2 * f + 1];
+        double w = 1.0 / fmax(rot_sig[f], 1e-9), Ri[9], Rj[9], A[9], E[9], e[3], Jr[9], Ja[9], Jb[9], r[3];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
                 Ri[3 * i + j] = Tcw[16 * a + 4 * i + j];
                 Rj[3 * i + j] = Tcw[16 * b + 4 * i + j];
             }
         mmt3(Ri, Rj, A);
-        mtm3(q->rot_M + 9 * f, A, E);
+        mtm3(rot_M + 9 * f, A, E);
         o_log_quat(E, e);
         o_jr_inv(e, Jr);
         mmt3(Jr, A, Ja);
@@ -1577,7 +1671,7 @@ static void rot_factor_blocks_id(const onpid_problem *q, const double *Tcw, int 
         }
         for (int i = 0; i < 3; i++) r[i] = w * e[i];
         int ia = 6 * a + 3, ib = 6 * b + 3;
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < 3 && H; i++) {
             for (int j = 0; j < 3; j++) {
                 double haa = 0.0, hab = 0.0, hbb = 0.0;
                 for (int m = 0; m < 3; m++) {
@@ -1590,6 +1684,8 @@ static void rot_factor_blocks_id(const onpid_problem *q, const double *Tcw, int 
                 H[(size_t)(ib + j) * n + ia + i] += hab;
                 H[(size_t)(ib + i) * n + ib + j] += hbb;
             }
+        }
+        for (int i = 0; i < 3; i++) {
             double ga = 0.0, gb = 0.0;
             for (int j = 0; j < 3; j++) {
                 ga += Ja[3 * j + i] * r[j];
@@ -1605,6 +1701,8 @@ static void rot_factor_blocks_id(const onpid_problem *q, const double *Tcw, int 
 int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double *dx_hist, uint8_t *acc_hist) {
     int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
     double K = q->huber_k, lam = lam0;
+    const int pol = q->lm_policy;
+    double chi_prev = -1.0, best_chi = -1.0, rms0_w = NAN;  /* the GPU driver's policy state */
     size_t Nn = N ? N : 1, nln = nl ? nl : 1;
     double *Tcw = malloc(16 * nk * sizeof(double)), *Tn = malloc(16 * nk * sizeof(double));
     double *Twc = malloc(16 * nk * sizeof(double)), *Twn = malloc(16 * nk * sizeof(double));
@@ -1620,6 +1718,8 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
     double *Hlli = malloc(9 * nln * sizeof(double)), *bl = malloc(3 * nln * sizeof(double));
     double *dxl = malloc(3 * nln * sizeof(double));
     int *order = malloc(Nn * sizeof(int)), *bounds = malloc((nl + 1) * sizeof(int));
+    double *Tbest = malloc(16 * nk * sizeof(double)), *Wbest = malloc(16 * nk * sizeof(double));
+    double *rbest = malloc(nln * sizeof(double)), *mbest = malloc(3 * nln * sizeof(double));
     memset(bounds, 0, (nl + 1) * sizeof(int));
     for (int k = 0; k < N; k++) bounds[q->olm[k] + 1]++;
     for (int j = 0; j < nl; j++) bounds[j + 1] += bounds[j];
@@ -1640,6 +1740,7 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
         memset(bp, 0, n * sizeof(double));
         memset(Hll, 0, 9 * nln * sizeof(double));
         memset(bl, 0, 3 * nln * sizeof(double));
+        double chiw = 0.0;
         for (int k = 0; k < N; k++) {
             int l = q->olm[k], ko = q->okf[k], kh = q->hidx[l];
             double e = sqrt(r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1]);
@@ -1648,6 +1749,7 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
             double *Jk = J + 32 * k;
             o_id_blocks(Tcw + 16 * ko, Rh, rho[l], Pc + 3 * k, Ph + 3 * k, kh >= 0, q->fx, r + 2 * k, sw, Jk);
             const double *Jl = Jk + 24, *rw = Jk + 30;
+            chiw += rw[0] * rw[0] + rw[1] * rw[1];
             for (int i = 0; i < 3; i++) {
                 for (int j = 0; j < 3; j++) Hll[9 * l + 3 * i + j] += Jl[i] * Jl[j] + Jl[3 + i] * Jl[3 + j];
                 bl[3 * l + i] += -(Jl[i] * rw[0] + Jl[3 + i] * rw[1]);
@@ -1671,6 +1773,23 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
                 }
             }
         }
+        if (pol) { /* ba.py:1436-1445 */
+            if (isnan(rms0_w)) rms0_w = sqrt(chiw / (N > 1 ? N : 1));
+            if (chi_prev >= 0.0 && chiw > chi_prev) {
+                memcpy(Tcw, Tbest, 16 * nk * sizeof(double));
+                memcpy(Twc, Wbest, 16 * nk * sizeof(double));
+                memcpy(rho, rbest, nl * sizeof(double));
+                memcpy(mb, mbest, 3 * nl * sizeof(double));
+                lam = fmin(lam * 4.0, 1e2);
+                continue;
+            }
+            chi_prev = chiw;
+            best_chi = chiw;
+            memcpy(Tbest, Tcw, 16 * nk * sizeof(double));
+            memcpy(Wbest, Twc, 16 * nk * sizeof(double));
+            memcpy(rbest, rho, nl * sizeof(double));
+            memcpy(mbest, mb, 3 * nl * sizeof(double));
+        }
         if (q->pr_m0) { /* ba.py:1250-1264: bearing prior unrobust, rho prior Huber */
             for (int j = 0; j < nl; j++) {
                 if (!(q->pr_valid[j] > 0.5)) continue;
@@ -1685,7 +1804,7 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
                 }
             }
         }
-        if (q->n_rot) rot_factor_blocks_id(q, Tcw, n, Hpp, bp);
+        if (q->n_rot) rot_blocks(q->n_rot, q->rot_ab, q->rot_M, q->rot_sig, Tcw, n, Hpp, bp);
         int singular = 0;
         for (int j = 0; j < nl; j++) {
             double tr = Hll[9 * j] + Hll[9 * j + 4] + Hll[9 * j + 8], d = lam * fmax(tr / 3.0, 1e-9);
@@ -1734,7 +1853,19 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
             }
         }
         for (int i = 0; i < n; i++) Hred[(size_t)i * n + i] += lam * fmax(Hred[(size_t)i * n + i], 1e-9);
-        if (!lu_solve(Hred, n, bp)) break;
+        int solved;
+        if (pol) { /* the GPU Cholesky; its status is ignored there, a failure shows as a non-finite step */
+            solved = chol(Hred, n);
+            if (solved) {
+                double *x = malloc(n * sizeof(double));
+                chol_solve(Hred, n, bp, x);
+                memcpy(bp, x, n * sizeof(double));
+                free(x);
+            }
+        } else {
+            solved = lu_solve(Hred, n, bp);
+            if (!solved) break;
+        }
         double *dxp = bp;
         if (dx_hist)
             for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = dxp[i];
@@ -1754,6 +1885,38 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
             }
             const double *Hi = Hlli + 9 * j;
             for (int i = 0; i < 3; i++) dxl[3 * j + i] = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+        }
+        if (pol) { /* ba.py:1468-1478: non-finite -> lam x10; else apply (no projection), lam / 2 */
+            int fin = solved;
+            for (int i = 0; i < n && fin; i++) fin = isfinite(dxp[i]);
+            for (int i = 0; i < 3 * nl && fin; i++) fin = isfinite(dxl[i]);
+            if (!fin) {
+                if (dx_hist)
+                    for (int i = 0; i < n; i++) dx_hist[(size_t)it * n + i] = NAN;
+                lam = fmin(lam * 10.0, 1e3);
+                continue;
+            }
+            for (int i = 0; i < nk; i++) {
+                if (!q->free_mask[i]) continue;
+                double E[16], T[16];
+                o_se3_exp(dxp + 6 * i, E);
+                for (int a = 0; a < 4; a++)
+                    for (int c = 0; c < 4; c++) {
+                        double sum = 0.0;
+                        for (int m = 0; m < 4; m++) sum += E[4 * a + m] * Tcw[16 * i + 4 * m + c];
+                        T[4 * a + c] = sum;
+                    }
+                memcpy(Tcw + 16 * i, T, sizeof(T));
+                se3_inv(Tcw + 16 * i, Twc + 16 * i);
+            }
+            for (int j = 0; j < nl; j++) {
+                mb[3 * j] += dxl[3 * j];
+                mb[3 * j + 1] += dxl[3 * j + 1];
+                rho[j] = fmax(rho[j] + dxl[3 * j + 2], q->rho_floor);
+            }
+            lam = fmax(lam * 0.5, 1e-7);
+            if (acc_hist) acc_hist[it] = 1;
+            continue;
         }
         memcpy(Tn, Tcw, 16 * nk * sizeof(double));
         for (int i = 0; i < nk; i++) {
@@ -1796,9 +1959,31 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
     stats[1] = chin;
     stats[2] = rms0;
     stats[3] = N ? sqrt(chin / N) : NAN;
+    if (pol) { /* ba.py:1480-1497: one SO(3) projection, then the Huber-weighted chi2 of the result */
+        for (int i = 0; i < nk; i++) {
+            double R[9], Rp[9];
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) R[3 * a + c] = Tcw[16 * i + 4 * a + c];
+            o_so3_project(R, Rp);
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) Tcw[16 * i + 4 * a + c] = Rp[3 * a + c];
+            se3_inv(Tcw + 16 * i, Twc + 16 * i);
+        }
+        id_residuals(q, Twc, Tcw, rho, mb, r, NULL, NULL);
+        double cw = 0.0;
+        for (int k = 0; k < N; k++) {
+            double e2 = r[2 * k] * r[2 * k] + r[2 * k + 1] * r[2 * k + 1], e = sqrt(e2);
+            double w = (e <= K) ? 1.0 : K / fmax(e, 1e-9);
+            cw += w * e2;
+        }
+        stats[0] = best_chi > 0.0 ? best_chi : cw;
+        stats[1] = cw;
+        stats[2] = rms0_w;
+        stats[3] = sqrt(cw / (N > 1 ? N : 1));
+    }
     memcpy(q->poses, Twc, 16 * nk * sizeof(double));
     free(Tcw); free(Tn); free(Twc); free(Twn); free(r); free(Pc); free(Ph); free(r2); free(J); free(Hpl);
     free(rhon); free(mbn); free(Hpp); free(Hred); free(bp); free(Hll); free(Hlli); free(bl); free(dxl);
-    free(order); free(bounds);
+    free(order); free(bounds); free(Tbest); free(Wbest); free(rbest); free(mbest);
     return 0;
 }

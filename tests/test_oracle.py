@@ -333,3 +333,17 @@ def test_posefree_block_equals_the_full_solve():
     ss = O.window_solve(split, iters=3)
     assert abs(sf["cost"] - ss["cost"]) <= 1e-12 * abs(sf["cost"])
     assert _rel(split.pwb, full.pwb) < 1e-12
+
+
+def test_oracle_gpu_rule_meets_the_reference_agreement_bar():
+    """The oracle's restatement of the GPU drivers' LM rule (backend/ba.py:1015-1076) against _solve_np on
+    make_problem(): the reference's own GPU-vs-numpy bar (src/xclean/tests/test_ba.py:56-70)."""
+    pb = synth.make_problem()
+    args = (pb["poses0"], pb["pts0"], pb["okf"], pb["olm"], pb["ouv"], pb["rig"], 0, 5)
+    p_ref, q_ref, st_ref = O.solve_np(*args)
+    p, q, st = O.solve_np(*args, policy="gpu")
+    assert float(np.abs(p[:, :3, 3] - p_ref[:, :3, 3]).max()) < 5e-3
+    assert abs(st["rms_n"] - st_ref["rms_n"]) < 0.05
+    e0 = np.sqrt(np.mean(np.sum((pb["poses0"][:, :3, 3] - pb["poses_true"][:, :3, 3]) ** 2, 1)))
+    e1 = np.sqrt(np.mean(np.sum((p[:, :3, 3] - pb["poses_true"][:, :3, 3]) ** 2, 1)))
+    assert e1 < 0.6 * e0

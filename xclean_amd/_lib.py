@@ -42,7 +42,7 @@ class XbaNpProblem(C.Structure):
                 ("poses", vp), ("pts", vp), ("okf", vp), ("olm", vp), ("ouv", vp), ("free_mask", vp),
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
                 ("pr_c", vp), ("pr_rho0", vp), ("pr_invsig", vp), ("pr_valid", vp),
-                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
+                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp), ("lm_policy", C.c_int32)]
 
 
 class XbaNpidProblem(C.Structure):
@@ -52,7 +52,7 @@ class XbaNpidProblem(C.Structure):
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("huber_k", C.c_double),
                 ("rho_floor", C.c_double),
                 ("pr_m0", vp), ("pr_rho0", vp), ("pr_wbear", vp), ("pr_invsig", vp), ("pr_valid", vp),
-                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp)]
+                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp), ("lm_policy", C.c_int32)]
 
 
 class XbaError(RuntimeError):

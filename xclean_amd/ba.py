@@ -10,6 +10,12 @@ the reference's forms.
 The whole loop runs in FP64 on the device as one stream of kernels (libxba ``xba_np_solve``); the host only plans
 the landmark grouping.  There is no CPU path: without the built library or a GPU this raises.
 
+``_solve_gpu`` / ``_solve_gpu_id`` (backend/ba.py:930-1089, 1345-1497) are the reference's GPU drivers of the same two
+solves: same signatures and math, but their own LM rule -- every step is applied, the next iteration rejects it when
+the Huber-weighted chi2 grew (restoring the last accepted state, lambda x4), a non-finite step sets lambda x10, the
+rotation factors' Hessian enters the reduced system, SO(3) is projected once at the end, and the stats are the
+Huber-weighted chi2 with the pair and observation counts.  Here they run in FP64 on the same kernels.
+
 ``_solve_np_id`` is the inverse-depth twin (backend/ba.py:1212-1342, libxba ``xba_np_id_solve``): landmarks are a
 bearing (x_n, y_n, 1) and inverse depth in the camera frame of a host keyframe, or of a constant out-of-window host
 pose; every observation has an observer and a host pose block; the bearing prior is unrobust, the inverse-range
@@ -41,7 +47,7 @@ def _rig_intrinsics(rig):
 
 
 def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None, rotf=None, device=None,
-              history=False):
+              history=False, _policy=0):
     """backend/ba.py:713 ``_solve_np``.  Returns (poses T_wc (nk,4,4), pts (nl,3), stats dict) as numpy arrays,
     stats = {chi2_0, chi2_n, rms_0, rms_n}.  ``history=True`` also returns the (iters, 6 nk) pose updates.
 
@@ -78,6 +84,7 @@ def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None
     q.free_mask = d_free.data_ptr()
     q.fx, q.cx, q.cy = _rig_intrinsics(rig)
     q.huber_k = HUBER_K
+    q.lm_policy = int(_policy)
     if prior is not None:
         c_, rho0, invsig, valid = prior
         pc = dt(np.asarray(c_, np.float64).reshape(nl, 3))
@@ -109,7 +116,7 @@ def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None
 
 
 def _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
-                 rotf=None, rho_floor=1e-4, device=None, history=False):
+                 rotf=None, rho_floor=1e-4, device=None, history=False, _policy=0):
     """backend/ba.py:1212 ``_solve_np_id``.  Returns (poses T_wc (nk,4,4), rho (nl,), mbear (nl,3), stats dict) as
     numpy arrays, stats = {chi2_0, chi2_n, rms_0, rms_n}; ``history=True`` also returns the (iters, 6 nk) pose
     updates.  ``prior`` = (m0 (nl, >=2), rho0, wbear, invsig, valid), arrays or scalars; ``rotf`` = [(a, b, R (3,3),
@@ -151,6 +158,7 @@ def _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, i
     q.okf, q.olm, q.ouv, q.free_mask = d_okf.data_ptr(), d_olm.data_ptr(), d_ouv.data_ptr(), d_free.data_ptr()
     q.fx, q.cx, q.cy = _rig_intrinsics(rig)
     q.huber_k, q.rho_floor = HUBER_K, float(rho_floor)
+    q.lm_policy = int(_policy)
     if prior is not None:
         m0, rho0, wbear, invsig, valid = prior
         m0 = np.asarray(m0, np.float64).reshape(nl, -1)
@@ -181,3 +189,32 @@ def _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, i
     if history:
         return out + (dxh.cpu().numpy(),)
     return out
+
+
+def _npair(olm, nl):
+    """backend/ba.py:902-927 build_pairs' count: every ordered observation pair inside a landmark, sum c_l^2."""
+    c = np.bincount(np.asarray(olm, np.int64).ravel(), minlength=nl) if len(olm) else np.zeros(0, np.int64)
+    return int(np.sum(c.astype(np.int64) ** 2))
+
+
+def _solve_gpu(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, led=None, kf_id=None, prior=None, rotf=None,
+               device=None):
+    """backend/ba.py:930 ``_solve_gpu``: (poses T_wc, pts, stats) with stats = {chi2_0, chi2_n, rms_0, rms_n, npair,
+    n_obs}; ``led`` / ``kf_id`` (the reference's timing ledger hooks) are accepted and unused."""
+    p, q, st = _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0, prior=prior, rotf=rotf, device=device,
+                         _policy=1)
+    if int(iters) == 0:
+        st["rms_0"] = None  # the reference reports no initial rms without an iteration
+    st.update(npair=_npair(olm, len(np.asarray(pts).reshape(-1, 3))), n_obs=int(len(np.asarray(okf).ravel())))
+    return p, q, st
+
+
+def _solve_gpu_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None,
+                  rotf=None, rho_floor=1e-4, device=None):
+    """backend/ba.py:1345 ``_solve_gpu_id``: (poses T_wc, rho, mbear, stats) with the stats of ``_solve_gpu``."""
+    p, r, m, st = _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, iters, lam0,
+                               prior=prior, rotf=rotf, rho_floor=rho_floor, device=device, _policy=1)
+    if int(iters) == 0:
+        st["rms_0"] = None
+    st.update(npair=_npair(olm, len(np.asarray(rho).ravel())), n_obs=int(len(np.asarray(okf).ravel())))
+    return p, r, m, st

@@ -1199,6 +1199,10 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
     A.pr_c = q->pr_c; A.pr_rho0 = q->pr_rho0; A.pr_invsig = q->pr_invsig; A.pr_valid = q->pr_valid;
     A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
     A.dx_hist = dx_hist_out;
+    A.policy = q->lm_policy ? 1 : 0;
+    A.rot_h = A.policy;
+    A.jstride = 20;
+    A.rwoff = 18;
     A.Tcw = dalloc<double>(16 * (size_t)nk, owned);
     A.Tnew = dalloc<double>(16 * (size_t)nk, owned);
     A.pnew = dalloc<double>(3 * (size_t)nl, owned);
@@ -1209,15 +1213,17 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
     A.H = dalloc<double>((size_t)n * n, owned);
     A.bp = dalloc<double>(n, owned);
     A.dx = dalloc<double>(n, owned);
-    A.st = dalloc<double>(8, owned);
+    A.st = dalloc<double>(NST, owned);
+    A.Tbest = dalloc<double>(16 * (size_t)nk, owned);
+    A.pbest = dalloc<double>(3 * (size_t)nl, owned);
     int32_t *d_order = dalloc<int32_t>(N, owned), *d_lstart = dalloc<int32_t>(nl + 1, owned);
-    if (owned.size() != 13) {
+    if (owned.size() != 15) {
         release();
         return XBA_E_HIP;
     }
     A.order = d_order;
     A.lstart = d_lstart;
-    const double st0[8] = {lam0, 0, 0, 0, 1, 0, 0, 0};
+    const double st0[NST] = {lam0, 0, 0, 0, 1, 0, 0, 0, 0, 0, -1, -1, -1, 0, 0, 0};
     auto run = [&]() -> int {
         if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
@@ -1231,19 +1237,30 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
         k_np_chi<<<1, 256, 0, s>>>(A, 5);
         for (int it = 0; it < iters; it++) {
             if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
+            if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 0);
             if (gl) k_np_lm<<<gl, 256, 0, s>>>(A);
             k_np_zero<<<gz, 256, 0, s>>>(A, n);
             if (gl) k_np_schur<<<gl, 256, 0, s>>>(A, n);
             k_np_solve<<<1, 256, 0, s>>>(A, n, it);
             if (gl) k_np_backsub<<<gl, 256, 0, s>>>(A);
-            if (gN) k_np_trial_r2<<<gN, 256, 0, s>>>(A);
-            k_np_accept<<<1, 256, 0, s>>>(A);
-            if (gp) k_np_commit_pts<<<gp, 256, 0, s>>>(A);
+            if (A.policy) {
+                k_np_apply<<<1, 256, 0, s>>>(A, n, 0);
+            } else {
+                if (gN) k_np_trial_r2<<<gN, 256, 0, s>>>(A);
+                k_np_accept<<<1, 256, 0, s>>>(A);
+                if (gp) k_np_commit_pts<<<gp, 256, 0, s>>>(A);
+            }
         }
-        if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
-        k_np_chi<<<1, 256, 0, s>>>(A, 6);
+        if (A.policy) {
+            k_np_project<<<gk, 256, 0, s>>>(nk, A.Tcw);
+            if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
+            k_np_wstats<<<1, 256, 0, s>>>(A, stats_out);
+        } else {
+            if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
+            k_np_chi<<<1, 256, 0, s>>>(A, 6);
+            k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
+        }
         k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tcw, q->poses);
-        k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(s));
         return XBA_OK;
@@ -1308,6 +1325,9 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
     A.pr_valid = q->pr_valid;
     A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
     A.dx_hist = dx_hist_out;
+    A.policy = q->lm_policy ? 1 : 0;
+    A.jstride = 32;
+    A.rwoff = 30;
     A.Tcw = dalloc<double>(16 * (size_t)nk, owned);
     A.Tnew = dalloc<double>(16 * (size_t)nk, owned);
     A.Twc = dalloc<double>(16 * (size_t)nk, owned);
@@ -1321,15 +1341,19 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
     A.H = dalloc<double>((size_t)n * n, owned);
     A.bp = dalloc<double>(n, owned);
     A.dx = dalloc<double>(n, owned);
-    A.st = dalloc<double>(8, owned);
+    A.st = dalloc<double>(NST, owned);
+    A.Tbest = dalloc<double>(16 * (size_t)nk, owned);
+    A.Wbest = dalloc<double>(16 * (size_t)nk, owned);
+    A.rbest = dalloc<double>(nl, owned);
+    A.mbest = dalloc<double>(3 * (size_t)nl, owned);
     int32_t *d_order = dalloc<int32_t>(N, owned), *d_lstart = dalloc<int32_t>(nl + 1, owned);
-    if (owned.size() != 16) {
+    if (owned.size() != 20) {
         release();
         return XBA_E_HIP;
     }
     A.order = d_order;
     A.lstart = d_lstart;
-    const double st0[8] = {lam0, 0, 0, 0, 1, 0, 0, 0};
+    const double st0[NST] = {lam0, 0, 0, 0, 1, 0, 0, 0, 0, 0, -1, -1, -1, 0, 0, 0};
     auto run = [&]() -> int {
         if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
@@ -1345,20 +1369,32 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
         k_np_chi<<<1, 256, 0, s>>>(A, 5);
         for (int it = 0; it < iters; it++) {
             if (gN) k_id_obs<<<gN, 256, 0, s>>>(A);
+            if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 1);
             if (gl) k_id_lm<<<gl, 256, 0, s>>>(A);
             k_np_zero<<<gz, 256, 0, s>>>(A, n);
             if (gl) k_id_schur<<<gl, 256, 0, s>>>(A, n);
             k_np_solve<<<1, 256, 0, s>>>(A, n, it);
             k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tnew, A.Twn);
             if (gl) k_id_backsub<<<gl, 256, 0, s>>>(A);
-            if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 1);
-            k_np_accept<<<1, 256, 0, s>>>(A);
-            k_id_commit<<<gc, 256, 0, s>>>(A);
+            if (A.policy) {
+                k_np_apply<<<1, 256, 0, s>>>(A, n, 1);
+            } else {
+                if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 1);
+                k_np_accept<<<1, 256, 0, s>>>(A);
+                k_id_commit<<<gc, 256, 0, s>>>(A);
+            }
         }
-        if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 0);
-        k_np_chi<<<1, 256, 0, s>>>(A, 6);
+        if (A.policy) {
+            k_np_project<<<gk, 256, 0, s>>>(nk, A.Tcw);
+            k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tcw, A.Twc);
+            if (gN) k_id_obs<<<gN, 256, 0, s>>>(A);
+            k_np_wstats<<<1, 256, 0, s>>>(A, stats_out);
+        } else {
+            if (gN) k_id_r2<<<gN, 256, 0, s>>>(A, 0);
+            k_np_chi<<<1, 256, 0, s>>>(A, 6);
+            k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
+        }
         HIPCK(hipMemcpyAsync(q->poses, A.Twc, 16 * sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
-        k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(s));
         return XBA_OK;

@@ -34,7 +34,17 @@ struct NpArgs {
     double *rho, *mb, *rhon, *mbn; // (nl), (nl,3) inverse depth and host-frame bearing, current and trial
     double rho_floor;
     const double *pr_m0, *pr_wbear; // bearing prior: (nl,3) mean, (nl) weight
+    // LM policy: 0 the numpy solves (trial cost, accept iff the unrobust chi2 did not grow); 1 the GPU drivers
+    // _solve_gpu / _solve_gpu_id (ba.py:1015-1076, 1436-1483): the step is applied at once, the next iteration rejects
+    // it when the Huber-weighted chi2 grew (restoring the last accepted state, lam x4), a non-finite step sets lam
+    // x10, SO(3) is projected once at the end
+    int32_t policy, jstride, rwoff;  // J row stride and offset of the weighted residual (the chi2 of the gate)
+    double *Tbest, *pbest, *Wbest, *rbest, *mbest;  // the last accepted state (policy 1)
 };
+
+// device scalars beyond [0] lam [1] chi_cur [2] chi_new [3] accept [4] ok [5] chi0 [6] chi_n
+enum { ST_SKIP = 8, ST_NONFIN = 9, ST_CHIPREV = 10, ST_BESTCHI = 11, ST_RMS0 = 12, NST = 16 };
+__device__ inline bool np_skip(const NpArgs &A) { return A.policy && A.st[ST_SKIP] != 0.0; }
 
 __device__ inline void np_residual(const NpArgs &A, const double *T, const double *X, const double *uv, double *P,
                                    double *r) {
@@ -68,7 +78,7 @@ __global__ void k_np_obs(NpArgs A) {
 // Per landmark (observations in original order): Hll, bl, the inverse-range prior, damping, inverse (ba.py:759-797).
 __global__ void k_np_lm(NpArgs A) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl) return;
+    if (j >= A.nl || np_skip(A)) return;
     const double lam = A.st[0];
     double Hll[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bl[3] = {0, 0, 0};
     for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
@@ -121,7 +131,7 @@ __global__ void k_np_zero(NpArgs A, int n) {
 // One thread per landmark; FP64 atomics into the small n x n system.
 __global__ void k_np_schur(NpArgs A, int n) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl) return;
+    if (j >= A.nl || np_skip(A)) return;
     const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
     for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
         const int ka = A.order[qa], ki = 6 * A.okf[ka];
@@ -154,6 +164,7 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
     __shared__ double sred[256];
     __shared__ int sflag;
     const int tid = threadIdx.x, nt = blockDim.x;
+    if (np_skip(A)) return;  // (uniform: no kernel of this launch writes the flag)
     // chi2 of the current residuals, fixed-order reduction
     double s = 0.0;
     for (int k = tid; k < A.N; k += nt) s += A.r2[k];
@@ -264,7 +275,7 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
             double X[9];
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) X[3 * i + j] = Tn[4 * i + j];
-            for (int itp = 0; itp < 30; itp++) {  // X <- (X + X^-T) / 2 converges to the orthogonal polar factor
+            for (int itp = 0; itp < (A.policy ? 0 : 30); itp++) {  // the GPU drivers project once, at the end  // X <- (X + X^-T) / 2 converges to the orthogonal polar factor
                 double c00 = X[4] * X[8] - X[5] * X[7], c01 = X[5] * X[6] - X[3] * X[8], c02 = X[3] * X[7] - X[4] * X[6];
                 double det = X[0] * c00 + X[1] * c01 + X[2] * c02, id = 1.0 / det;
                 double Xi[9] = {c00 * id, (X[2] * X[7] - X[1] * X[8]) * id, (X[1] * X[5] - X[2] * X[4]) * id,
@@ -289,7 +300,7 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
 // Landmark back-substitution dxl = Hlli (bl - sum Hpl^T dxp_k) and the trial points (ba.py:844-856).
 __global__ void k_np_backsub(NpArgs A) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl || A.st[4] == 0.0) return;
+    if (j >= A.nl || A.st[4] == 0.0 || np_skip(A)) return;
     double rhs[3] = {A.bl[3 * (size_t)j], A.bl[3 * (size_t)j + 1], A.bl[3 * (size_t)j + 2]};
     for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
         const int k = A.order[q];
@@ -303,8 +314,11 @@ __global__ void k_np_backsub(NpArgs A) {
         }
     }
     const double *Hi = A.Hlli + 9 * (size_t)j;
-    for (int i = 0; i < 3; i++)
-        A.pnew[3 * (size_t)j + i] = A.pts[3 * (size_t)j + i] + (Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2]);
+    for (int i = 0; i < 3; i++) {
+        const double dl = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+        if (!isfinite(dl)) A.st[ST_NONFIN] = 1.0;
+        A.pnew[3 * (size_t)j + i] = A.pts[3 * (size_t)j + i] + dl;
+    }
 }
 
 __global__ void k_np_trial_r2(NpArgs A) {
@@ -463,7 +477,7 @@ __global__ void k_id_obs(NpArgs A) {
 // lam * max(tr/3, 1e-9), inverse (ba.py:1244-1264, 1298-1301).
 __global__ void k_id_lm(NpArgs A) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl) return;
+    if (j >= A.nl || np_skip(A)) return;
     const double lam = A.st[0];
     double Hll[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, bl[3] = {0, 0, 0};
     for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
@@ -508,7 +522,7 @@ __device__ inline void id_hpl(const double *J, int sa, double *Hpl) {  // (6x3) 
 // (observation, slot) entries; bp -= (Hpl_a Hlli) bl.
 __global__ void k_id_schur(NpArgs A, int n) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl) return;
+    if (j >= A.nl || np_skip(A)) return;
     const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
     const int hs = A.hidx[j];
     for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
@@ -557,7 +571,7 @@ __global__ void k_id_schur(NpArgs A, int n) {
 // (ba.py:1318-1330).
 __global__ void k_id_backsub(NpArgs A) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl || A.st[4] == 0.0) return;
+    if (j >= A.nl || A.st[4] == 0.0 || np_skip(A)) return;
     double rhs[3] = {A.bl[3 * (size_t)j], A.bl[3 * (size_t)j + 1], A.bl[3 * (size_t)j + 2]};
     const int hs = A.hidx[j];
     for (int q = A.lstart[j]; q < A.lstart[j + 1]; q++) {
@@ -578,6 +592,7 @@ __global__ void k_id_backsub(NpArgs A) {
     const double *Hi = A.Hlli + 9 * (size_t)j;
     double dl[3];
     for (int i = 0; i < 3; i++) dl[i] = Hi[3 * i] * rhs[0] + Hi[3 * i + 1] * rhs[1] + Hi[3 * i + 2] * rhs[2];
+    if (!(isfinite(dl[0]) && isfinite(dl[1]) && isfinite(dl[2]))) A.st[ST_NONFIN] = 1.0;
     A.mbn[3 * (size_t)j] = A.mb[3 * (size_t)j] + dl[0];
     A.mbn[3 * (size_t)j + 1] = A.mb[3 * (size_t)j + 1] + dl[1];
     A.mbn[3 * (size_t)j + 2] = A.mb[3 * (size_t)j + 2];
@@ -603,6 +618,133 @@ __global__ void k_id_commit(NpArgs A) {
     if (i < 16 * A.nk) A.Twc[i] = A.Twn[i];
     if (i < A.nl) A.rho[i] = A.rhon[i];
     if (i < 3 * A.nl) A.mb[i] = A.mbn[i];
+}
+
+
+// ================================================================== the GPU drivers' LM policy (policy 1) ==
+// Fixed-order sum of the Huber-weighted squared residuals (rw of the J rows) into st[slot].
+__device__ inline double np_wchi_sum(const NpArgs &A, double *sred) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double s = 0.0;
+    for (int k = tid; k < A.N; k += nt) {
+        const double *rw = A.J + (size_t)A.jstride * k + A.rwoff;
+        s += rw[0] * rw[0] + rw[1] * rw[1];
+    }
+    sred[tid] = s;
+    __syncthreads();
+    for (int d = nt / 2; d > 0; d >>= 1) {
+        if (tid < d) sred[tid] += sred[tid + d];
+        __syncthreads();
+    }
+    return sred[0];
+}
+
+// ba.py:1015-1026 / 1436-1445, after the linearisation: chi = weighted chi2 at the current state; if it grew past the
+// last accepted one, restore that state, lam = min(4 lam, 1e2) and skip the rest of the iteration; else remember the
+// state as the accepted one.
+__global__ __launch_bounds__(256) void k_np_gate(NpArgs A, int id) {
+    __shared__ double sred[256];
+    __shared__ int srej;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double chi = np_wchi_sum(A, sred);
+    if (tid == 0) {
+        if (A.st[ST_RMS0] < 0.0) A.st[ST_RMS0] = sqrt(chi / (A.N > 1 ? A.N : 1));
+        srej = A.st[ST_CHIPREV] >= 0.0 && chi > A.st[ST_CHIPREV];
+        A.st[ST_SKIP] = srej ? 1.0 : 0.0;
+        A.st[ST_NONFIN] = 0.0;
+        A.st[3] = 0.0;
+        if (srej) {
+            A.st[0] = fmin(A.st[0] * 4.0, 1e2);
+        } else {
+            A.st[ST_CHIPREV] = chi;
+            A.st[ST_BESTCHI] = chi;
+        }
+    }
+    __syncthreads();
+    double *dT = srej ? A.Tcw : A.Tbest;
+    const double *sT = srej ? A.Tbest : A.Tcw;
+    for (int i = tid; i < 16 * A.nk; i += nt) dT[i] = sT[i];
+    if (id) {
+        double *dW = srej ? A.Twc : A.Wbest, *dr = srej ? A.rho : A.rbest, *dm = srej ? A.mb : A.mbest;
+        const double *sW = srej ? A.Wbest : A.Twc, *sr = srej ? A.rbest : A.rho, *sm = srej ? A.mbest : A.mb;
+        for (int i = tid; i < 16 * A.nk; i += nt) dW[i] = sW[i];
+        for (int i = tid; i < A.nl; i += nt) dr[i] = sr[i];
+        for (int i = tid; i < 3 * A.nl; i += nt) dm[i] = sm[i];
+    } else {
+        double *dp = srej ? A.pts : A.pbest;
+        const double *sp = srej ? A.pbest : A.pts;
+        for (int i = tid; i < 3 * A.nl; i += nt) dp[i] = sp[i];
+    }
+}
+
+// ba.py:1058-1073 / 1468-1478: a failed or non-finite step sets lam = min(10 lam, 1e3); otherwise the step is
+// applied (poses exp(dx) T for free keyframes, landmarks + dl, inverse depth floored) and lam = max(lam / 2, 1e-7).
+__global__ __launch_bounds__(256) void k_np_apply(NpArgs A, int n, int id) {
+    __shared__ int sfin;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (np_skip(A)) return;
+    if (tid == 0) sfin = A.st[4] != 0.0 && A.st[ST_NONFIN] == 0.0;
+    __syncthreads();
+    if (sfin) {
+        int bad = 0;
+        for (int i = tid; i < n; i += nt) bad |= !isfinite(A.dx[i]);
+        if (bad) sfin = 0;  // benign race: every writer stores 0
+    }
+    __syncthreads();
+    if (!sfin) {
+        if (tid == 0) A.st[0] = fmin(A.st[0] * 10.0, 1e3);
+        return;
+    }
+    for (int i = tid; i < 16 * A.nk; i += nt) A.Tcw[i] = A.Tnew[i];
+    if (id) {
+        for (int i = tid; i < 16 * A.nk; i += nt) A.Twc[i] = A.Twn[i];
+        for (int i = tid; i < A.nl; i += nt) A.rho[i] = A.rhon[i];
+        for (int i = tid; i < 3 * A.nl; i += nt) A.mb[i] = A.mbn[i];
+    } else {
+        for (int i = tid; i < 3 * A.nl; i += nt) A.pts[i] = A.pnew[i];
+    }
+    if (tid == 0) {
+        A.st[0] = fmax(A.st[0] * 0.5, 1e-7);
+        A.st[3] = 1.0;
+    }
+}
+
+// The one SO(3) projection of the GPU drivers (ba.py:1076, 1480): the orthogonal polar factor of every rotation.
+__global__ void k_np_project(int nk, double *Tcw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double *T = Tcw + 16 * (size_t)k, X[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) X[3 * i + j] = T[4 * i + j];
+    for (int itp = 0; itp < 30; itp++) {  // X <- (X + X^-T) / 2
+        double c00 = X[4] * X[8] - X[5] * X[7], c01 = X[5] * X[6] - X[3] * X[8], c02 = X[3] * X[7] - X[4] * X[6];
+        double det = X[0] * c00 + X[1] * c01 + X[2] * c02, id = 1.0 / det;
+        double Xi[9] = {c00 * id, (X[2] * X[7] - X[1] * X[8]) * id, (X[1] * X[5] - X[2] * X[4]) * id,
+                        c01 * id, (X[0] * X[8] - X[2] * X[6]) * id, (X[2] * X[3] - X[0] * X[5]) * id,
+                        c02 * id, (X[1] * X[6] - X[0] * X[7]) * id, (X[0] * X[4] - X[1] * X[3]) * id};
+        double dmax = 0.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double nv = 0.5 * (X[3 * i + j] + Xi[3 * j + i]);
+                dmax = fmax(dmax, fabs(nv - X[3 * i + j]));
+                X[3 * i + j] = nv;
+            }
+        if (dmax < 1e-16) break;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T[4 * i + j] = X[3 * i + j];
+}
+
+// stats of the GPU drivers (ba.py:1086-1089, 1494-1497): chi2_0 is the weighted chi2 of the last accepted state (the
+// final one when there is none), rms_0 that of the first iteration, chi2_n / rms_n the weighted chi2 of the result.
+__global__ __launch_bounds__(256) void k_np_wstats(NpArgs A, double *stats) {
+    __shared__ double sred[256];
+    const double chi = np_wchi_sum(A, sred);
+    if (threadIdx.x) return;
+    stats[0] = A.st[ST_BESTCHI] > 0.0 ? A.st[ST_BESTCHI] : chi;
+    stats[1] = chi;
+    stats[2] = A.st[ST_RMS0] >= 0.0 ? A.st[ST_RMS0] : NAN;
+    stats[3] = sqrt(chi / (A.N > 1 ? A.N : 1));
 }
 
 }  // namespace xba
