@@ -117,7 +117,10 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (three launches) for HBM counter calibration")
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner (solver defaults)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the single-window configurations 1 and 4")
+    ap.add_argument("--no-extra", action="store_true", help="skip configurations 1, 3 and 4")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="diagnostic: run only configurations 1 and 4 (for profiling the tiled factorisation) and "
+                         "print their JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (windows would shard across GPUs with "
@@ -130,6 +133,10 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.configs_only:
+        print(json.dumps(single_window_configs(torch, synth, BatchSolver, log)))
+        return
 
     t0 = time.time()
     pbs = synth.make_batch(args.windows, seed=0)
