@@ -115,6 +115,57 @@ __device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane
     return __shfl(S, B - 1, XBA_WAVE);
 }
 
+// Segmented sums over consecutive lanes (a segment: one landmark's observation rows), accumulated into the
+// segment's last lane with DPP lane moves (VALU only, no LDS traffic): lane i adds lane i - d for d = 1, 2, 4, 8
+// inside its row of 16 lanes, then the row-15 and lane-31 broadcasts carry row sums across rows, each move taken
+// only when its source lane lies in lane i's segment [ss, i].  Only the last lane's result is the segment sum; a
+// lane with ss > lane (ss = 64) belongs to no segment.  A step no lane takes is skipped.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL, int ROWS, int N>
+__device__ inline void seg_step(double (&v)[N], bool take) {
+    if (__ballot(take) == 0) return;
+    // a select, not a multiply by 0/1: lanes outside any segment may hold non-finite values
+    for (int i = 0; i < N; i++) {
+        const double o = dpp_d<CTRL, ROWS>(v[i]);
+        v[i] = take ? v[i] + o : v[i];
+    }
+}
+
+template <int N>
+__device__ inline void seg_sum(double (&v)[N], int lane, int ss) {
+    const int r = lane & 15, row = lane >> 4;
+    seg_step<0x111, 0xf>(v, r >= 1 && lane - 1 >= ss);  // row_shr:1
+    seg_step<0x112, 0xf>(v, r >= 2 && lane - 2 >= ss);  // row_shr:2
+    seg_step<0x114, 0xf>(v, r >= 4 && lane - 4 >= ss);  // row_shr:4
+    seg_step<0x118, 0xf>(v, r >= 8 && lane - 8 >= ss);  // row_shr:8
+    seg_step<0x142, 0xa>(v, (row & 1) && lane - r - 1 >= ss);  // row_bcast:15 into rows 1, 3
+    seg_step<0x143, 0xc>(v, row >= 2 && 31 >= ss);              // row_bcast:31 into rows 2, 3
+}
+
+template <int CTRL, int ROWS>
+__device__ inline unsigned seg_or_step(unsigned v, bool take) {
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, true);
+    return take ? (v | o) : v;
+}
+
+__device__ inline unsigned seg_or(unsigned v, int lane, int ss) {
+    const int r = lane & 15, row = lane >> 4;
+    v = seg_or_step<0x111, 0xf>(v, r >= 1 && lane - 1 >= ss);
+    v = seg_or_step<0x112, 0xf>(v, r >= 2 && lane - 2 >= ss);
+    v = seg_or_step<0x114, 0xf>(v, r >= 4 && lane - 4 >= ss);
+    v = seg_or_step<0x118, 0xf>(v, r >= 8 && lane - 8 >= ss);
+    v = seg_or_step<0x142, 0xa>(v, (row & 1) && lane - r - 1 >= ss);
+    v = seg_or_step<0x143, 0xc>(v, row >= 2 && 31 >= ss);
+    return v;
+}
+
 // Work items are handed out dynamically, largest first (the host orders `sched`): thread 0 takes the next index of
 // sched[cbeg, cend) from the launch kind's counter, and the last workgroup to find the list exhausted resets both
 // counters for the next launch of that kind.  ctl[1] is the chunk's next pass, taken by the wavefronts in turn.
@@ -272,14 +323,18 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             WAVE_SYNC();
             // ---- observations (one lane each, kept in registers): whitened residual, Huber weight and cost,
-            //      whitened Jacobians; the landmark's Hll / bl sums and touched slots; the row's own pose-pose
-            //      blocks (t,t), (h,h), (t,h) and gradient entries straight into the tile (they need no elimination)
+            //      whitened Jacobians; the row's own target blocks (t,t), (t,h) and target gradient straight into
+            //      the tile.  The per-landmark sums (Hll, bl, active rows, touched slots) and the host blocks (h,h)
+            //      and host gradient (one host slot per landmark) are summed over the landmark's lanes first and
+            //      written once per landmark by its last lane.
             ObsRow<RT> ob;
             ob.w = (RT)0;
             ob.t = 0;
-            int ob_bb = 0, ob_h = 0;
+            int ob_bb = 0, ob_h = 0, seg_ss = XBA_WAVE, seg_last = 0;
             if (lane < nob) {
                 ob_bb = ob_lm[lane];
+                seg_ss = lobs0[ob_bb];
+                seg_last = lane == seg_ss + lnobs[ob_bb] - 1;
                 const int l = lb + ob_bb, k = obase + lane;
                 const int li = A.lm_info[l];
                 ob_h = lm_tile(li);
@@ -290,25 +345,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 rcq[lane] = cq;
                 const double w = (double)ob.w;
                 const int t = ob.t, h = ob_h;
-                if (w != 0.0) {
-                    double *la = lacc + LA_N * ob_bb;
-                    const double l00 = ob.jl[0], l01 = ob.jl[1], l02 = ob.jl[2];
-                    const double l10 = ob.jl[3], l11 = ob.jl[4], l12 = ob.jl[5];
-                    const double e0 = ob.e[0], e1 = ob.e[1];
-                    lds_add<double>(la + 0, w * (l00 * l00 + l10 * l10));
-                    lds_add<double>(la + 1, w * (l00 * l01 + l10 * l11));
-                    lds_add<double>(la + 2, w * (l00 * l02 + l10 * l12));
-                    lds_add<double>(la + 3, w * (l01 * l01 + l11 * l11));
-                    lds_add<double>(la + 4, w * (l01 * l02 + l11 * l12));
-                    lds_add<double>(la + 5, w * (l02 * l02 + l12 * l12));
-                    lds_add<double>(la + 6, w * (l00 * e0 + l10 * e1));
-                    lds_add<double>(la + 7, w * (l01 * e0 + l11 * e1));
-                    lds_add<double>(la + 8, w * (l02 * e0 + l12 * e1));
-                    lds_add<double>(la + LA_NACT, 1.0);
-                }
                 if (w != 0.0 && t != h) {
-                    __hip_atomic_fetch_or(lmask + ob_bb, (1u << t) | (1u << h), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
                     double xt[2][6], xh[2][6];
                     for (int r = 0; r < 2; r++)
                         for (int j = 0; j < 6; j++) {
@@ -316,16 +353,11 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                             xh[r][j] = ob.jh(r, j);
                         }
                     const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
-                    for (int j = 0; j < 6; j++) {
-                        lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
-                        lds_add<double>(btile + 6 * h + j, xh[0][j] * e0 + xh[1][j] * e1);
-                    }
-                    double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK, *dhh = tile + (h * (h + 1) / 2 + h) * TBLK;
+                    for (int j = 0; j < 6; j++) lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
+                    double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK;
                     for (int i = 0; i < 6; i++)
-                        for (int j = 0; j <= i; j++) {
+                        for (int j = 0; j <= i; j++)
                             lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]));
-                            lds_add<double>(dhh + 6 * i + j, w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]));
-                        }
                     const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
                     double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
                     for (int i = 0; i < 6; i++)
@@ -334,6 +366,52 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                                                  : xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j];
                             lds_add<double>(dth + 6 * i + j, w * v);
                         }
+                }
+            }
+            {
+                const double w = (double)ob.w;
+                const bool cross = w != 0.0 && ob.t != ob_h;
+                const bool head = seg_last;
+                double s[LA_N];
+                {
+                    const double l00 = ob.jl[0], l01 = ob.jl[1], l02 = ob.jl[2];
+                    const double l10 = ob.jl[3], l11 = ob.jl[4], l12 = ob.jl[5];
+                    const double e0 = ob.e[0], e1 = ob.e[1];
+                    s[LA_H + 0] = w * (l00 * l00 + l10 * l10);
+                    s[LA_H + 1] = w * (l00 * l01 + l10 * l11);
+                    s[LA_H + 2] = w * (l00 * l02 + l10 * l12);
+                    s[LA_H + 3] = w * (l01 * l01 + l11 * l11);
+                    s[LA_H + 4] = w * (l01 * l02 + l11 * l12);
+                    s[LA_H + 5] = w * (l02 * l02 + l12 * l12);
+                    s[LA_B + 0] = w * (l00 * e0 + l10 * e1);
+                    s[LA_B + 1] = w * (l01 * e0 + l11 * e1);
+                    s[LA_B + 2] = w * (l02 * e0 + l12 * e1);
+                    s[LA_NACT] = w != 0.0 ? 1.0 : 0.0;
+                }
+                seg_sum(s, lane, seg_ss);
+                const unsigned m = seg_or(cross ? (1u << ob.t) | (1u << ob_h) : 0u, lane, seg_ss);
+                if (head) {
+                    double *la = lacc + LA_N * ob_bb;
+                    for (int i = 0; i < LA_N; i++) la[i] = s[i];
+                    lmask[ob_bb] = m;
+                }
+                double xh[2][6];
+                for (int r = 0; r < 2; r++)
+                    for (int j = 0; j < 6; j++) xh[r][j] = cross ? ob.jh(r, j) : 0.0;
+                double g[6];
+                const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
+                for (int j = 0; j < 6; j++) g[j] = xh[0][j] * e0 + xh[1][j] * e1;
+                seg_sum(g, lane, seg_ss);
+                if (head && m)
+                    for (int j = 0; j < 6; j++) lds_add<double>(btile + 6 * ob_h + j, g[j]);
+                double hh[21];
+                for (int i = 0, o = 0; i < 6; i++)
+                    for (int j = 0; j <= i; j++, o++) hh[o] = w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]);
+                seg_sum(hh, lane, seg_ss);
+                if (head && m) {
+                    double *dhh = tile + (ob_h * (ob_h + 1) / 2 + ob_h) * TBLK;
+                    for (int i = 0, o = 0; i < 6; i++)
+                        for (int j = 0; j <= i; j++, o++) lds_add<double>(dhh + 6 * i + j, hh[o]);
                 }
             }
             WAVE_SYNC();
@@ -439,35 +517,43 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             const double pcost = wave_sum(lcost);
             if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
-            // ---- U = L^-1 Hlp, one lane per observation: its two 3x6 terms G Jt and G Jh (G = w L^-1 Jl^T)
-            //      added into U of its target slot and of its host slot (the slot's entry is its rank in the
-            //      landmark's touched-slot mask)
+            // ---- U = L^-1 Hlp, one lane per observation: its two 3x6 terms G Jt and G Jh (G = w L^-1 Jl^T); the
+            //      target term is added into U of its target slot, the host terms are summed over the landmark's
+            //      lanes and stored once into U of the host slot by the landmark's last lane (the slot's entry is
+            //      its rank in the landmark's touched-slot mask)
             for (int e = lane; e < ktot * 18; e += XBA_WAVE) hlp[e] = 0.0;
             WAVE_SYNC();
-            if (lane < nob) {
-                const int bb = ob_bb;
+            {
+                double uh[18];
+                for (int i = 0; i < 18; i++) uh[i] = 0.0;
+                const int bb = ob_bb, t = ob.t, h = ob_h;
                 const T *lr = lrec + bb * FB_LREC;
-                const T w = (T)ob.w;
-                const int t = ob.t, h = ob_h;
-                if (lr[LR_OK] != 0.0 && w != 0.0 && t != h) {
-                    const unsigned m = lmask[bb];
-                    T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
-                    T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
-                    T G[3][2];
-                    for (int r = 0; r < 2; r++) {
-                        const T j0 = ob.jl[3 * r], j1 = ob.jl[3 * r + 1], j2 = ob.jl[3 * r + 2];
-                        G[0][r] = w * (lr[LR_L] * j0);
-                        G[1][r] = w * (lr[LR_L + 1] * j0 + lr[LR_L + 2] * j1);
-                        G[2][r] = w * (lr[LR_L + 3] * j0 + lr[LR_L + 4] * j1 + lr[LR_L + 5] * j2);
-                    }
-                    for (int j = 0; j < 6; j++) {
-                        const T t0 = ob.jt[j], t1 = ob.jt[6 + j];
-                        const T h0 = ob.jh(0, j), h1 = ob.jh(1, j);
-                        for (int c = 0; c < 3; c++) {
-                            lds_add<double>(Ut + 6 * c + j, G[c][0] * t0 + G[c][1] * t1);
-                            lds_add<double>(Uh + 6 * c + j, G[c][0] * h0 + G[c][1] * h1);
+                const unsigned m = lane < nob ? lmask[bb] : 0u;
+                if (lane < nob) {
+                    const T w = (T)ob.w;
+                    if (lr[LR_OK] != 0.0 && w != 0.0 && t != h) {
+                        T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
+                        T G[3][2];
+                        for (int r = 0; r < 2; r++) {
+                            const T j0 = ob.jl[3 * r], j1 = ob.jl[3 * r + 1], j2 = ob.jl[3 * r + 2];
+                            G[0][r] = w * (lr[LR_L] * j0);
+                            G[1][r] = w * (lr[LR_L + 1] * j0 + lr[LR_L + 2] * j1);
+                            G[2][r] = w * (lr[LR_L + 3] * j0 + lr[LR_L + 4] * j1 + lr[LR_L + 5] * j2);
+                        }
+                        for (int j = 0; j < 6; j++) {
+                            const T t0 = ob.jt[j], t1 = ob.jt[6 + j];
+                            const T h0 = ob.jh(0, j), h1 = ob.jh(1, j);
+                            for (int c = 0; c < 3; c++) {
+                                lds_add<double>(Ut + 6 * c + j, G[c][0] * t0 + G[c][1] * t1);
+                                uh[6 * c + j] = G[c][0] * h0 + G[c][1] * h1;
+                            }
                         }
                     }
+                }
+                seg_sum(uh, lane, seg_ss);
+                if (seg_last && lr[LR_OK] != 0.0 && ((m >> h) & 1u)) {
+                    T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
+                    for (int i = 0; i < 18; i++) Uh[i] = uh[i];
                 }
             }
             WAVE_SYNC();
