@@ -67,6 +67,36 @@ def test_linearize_matches_oracle(name, det):
 
 
 @pytest.mark.parametrize("det", [False, True])
+@pytest.mark.parametrize("row", ["host_frame", "other_frame"])
+def test_nonfinite_observation_matches_oracle(row, det):
+    """A NaN pixel in a host-frame row only drops its landmark (its Hll fails the determinant guard,
+    vio/window.py:421): the reduced system stays finite and equal to the oracle's.  In a row of another keyframe it
+    reaches the pose blocks, and the reduced system is non-finite in both."""
+    from xclean_amd.window import BatchSolver
+    pb = _case("vo_small")
+    st, host, pose = np.asarray(pb.ob_start), np.asarray(pb.lm_host), np.asarray(pb.ob_pose)
+    want_host = row == "host_frame"
+    k = [k for l in range(pb.L) for k in range(st[l], st[l + 1])
+         if (pose[k] == host[l]) == want_host and st[l + 1] - st[l] > 3][3]
+    l = int(np.searchsorted(st, k, side="right") - 1)
+    z = np.asarray(pb.ob_z, np.float64).reshape(-1, 2).copy()
+    z[k, 0] = np.nan
+    pb.ob_z = z
+    lam = 1e-4
+    c, H6, b6, Hi, bl, Hlp, ok, chi = O.window_build(pb.copy(), lam)
+    H, b, _ = O.window_full_system(pb.copy(), H6, b6)
+    bs = BatchSolver([pb], precision="fp64", use_graph=False, deterministic=det)
+    bs.upload(lam)
+    r = bs.linearize(lam)[0]
+    assert not ok[l]
+    np.testing.assert_array_equal(r["lm_ok"][:pb.L], ok)
+    assert np.isfinite(r["H"]).all() == np.isfinite(H).all() == want_host
+    if want_host:
+        assert _rel(r["H"], H) < 1e-12
+        assert _rel(r["b"], b) < 1e-12
+
+
+@pytest.mark.parametrize("det", [False, True])
 @pytest.mark.parametrize("name", ["vo_small", "cfg0", "vi_small", "ur", "posefree"])
 def test_fp64_first_update_matches_oracle(name, det):
     pb = _case(name)

@@ -119,7 +119,8 @@ __device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane
 // segment's last lane with DPP lane moves (VALU only, no LDS traffic): lane i adds lane i - d for d = 1, 2, 4, 8
 // inside its row of 16 lanes, then the row-15 and lane-31 broadcasts carry row sums across rows, each move taken
 // only when its source lane lies in lane i's segment [ss, i].  Only the last lane's result is the segment sum; a
-// lane with ss > lane (ss = 64) belongs to no segment.  A step no lane takes is skipped.
+// lane with ss > lane (ss = 64) belongs to no segment.  A step no lane takes is skipped.  Every lane's values must be
+// finite: a move not taken is a multiply by 0.
 template <int CTRL, int ROWS>
 __device__ inline double dpp_d(double v) {
     const long long b = __double_as_longlong(v);
@@ -131,11 +132,8 @@ __device__ inline double dpp_d(double v) {
 template <int CTRL, int ROWS, int N>
 __device__ inline void seg_step(double (&v)[N], bool take) {
     if (__ballot(take) == 0) return;
-    // a select, not a multiply by 0/1: lanes outside any segment may hold non-finite values
-    for (int i = 0; i < N; i++) {
-        const double o = dpp_d<CTRL, ROWS>(v[i]);
-        v[i] = take ? v[i] + o : v[i];
-    }
+    const double f = take ? 1.0 : 0.0;  // exact: v + 1 * o rounds once, v + 0 * o is v (o finite)
+    for (int i = 0; i < N; i++) v[i] = fma(dpp_d<CTRL, ROWS>(v[i]), f, v[i]);
 }
 
 template <int N>
@@ -206,6 +204,19 @@ struct ObsRow {
     RT w, e[2], jl[6], jt[12], jhr[6];
     int t;
     __device__ inline double jh(int r, int j) const { return j < 3 ? (double)jhr[3 * r + j] : -(double)jt[6 * r + j]; }
+    __device__ inline void zero() {
+        w = (RT)0;
+        for (int i = 0; i < 2; i++) e[i] = (RT)0;
+        for (int i = 0; i < 6; i++) jl[i] = jhr[i] = (RT)0;
+        for (int i = 0; i < 12; i++) jt[i] = (RT)0;
+    }
+    // false when any entry is Inf or NaN (their sum is then not finite)
+    __device__ inline bool finite() const {
+        RT s = w + e[0] + e[1];
+        for (int i = 0; i < 6; i++) s += jl[i] + jhr[i];
+        for (int i = 0; i < 12; i++) s += jt[i];
+        return isfinite(s);
+    }
 };
 
 // Residual, Huber weight and cost (FP64) and whitened Jacobians (FP64, stored in RT) of observation row k of a
@@ -328,7 +339,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             //      and host gradient (one host slot per landmark) are summed over the landmark's lanes first and
             //      written once per landmark by its last lane.
             ObsRow<RT> ob;
-            ob.w = (RT)0;
+            ob.zero();
             ob.t = 0;
             int ob_bb = 0, ob_h = 0, seg_ss = XBA_WAVE, seg_last = 0;
             if (lane < nob) {
@@ -369,10 +380,16 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 }
             }
             {
+                // The lane sums below need finite values in every lane (lanes with nothing to add hold zeros).  A
+                // row with non-finite entries (from non-finite inputs) is left out of them and marks its landmark's
+                // Hll non-finite instead, so that the landmark fails the determinant guard as it would with the
+                // row summed in; its target terms above already carry the non-finite values into the tile.
+                const bool bad = ob.w != (RT)0 && !ob.finite();
+                if (bad) ob.zero();
                 const double w = (double)ob.w;
                 const bool cross = w != 0.0 && ob.t != ob_h;
                 const bool head = seg_last;
-                double s[LA_N];
+                double s[LA_N + 1];
                 {
                     const double l00 = ob.jl[0], l01 = ob.jl[1], l02 = ob.jl[2];
                     const double l10 = ob.jl[3], l11 = ob.jl[4], l12 = ob.jl[5];
@@ -387,8 +404,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     s[LA_B + 1] = w * (l01 * e0 + l11 * e1);
                     s[LA_B + 2] = w * (l02 * e0 + l12 * e1);
                     s[LA_NACT] = w != 0.0 ? 1.0 : 0.0;
+                    s[LA_N] = bad ? 1.0 : 0.0;
                 }
                 seg_sum(s, lane, seg_ss);
+                if (s[LA_N] != 0.0) s[LA_H] = (double)NAN;
                 const unsigned m = seg_or(cross ? (1u << ob.t) | (1u << ob_h) : 0u, lane, seg_ss);
                 if (head) {
                     double *la = lacc + LA_N * ob_bb;
