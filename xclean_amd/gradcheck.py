@@ -103,11 +103,14 @@ def differentiation_check(problems, iters=10, per_window=200, seed=0, h=1e-6, pr
     w = torch.tensor(np.concatenate([pb.ob_w for pb in problems]), dtype=torch.float64, device=dev,
                      requires_grad=True)
     g = torch.as_tensor(rng.standard_normal(solver.n_state), dtype=torch.float64, device=dev)
-    # warm-up of the same launches (graph capture, first replays), then the timed forward and backward
+    # warm-up of the same launches (graph capture, first replays) and of the autograd engine's device thread (started
+    # by the first backward of the process), then the timed forward and backward
     solver.set_measurements(z.detach(), w.detach())
     solver.upload(1e-4)
     solver.iterate(1)
-    solver.adjoint(solver.gn_step() * 0 + g)
+    (g * WindowStep.apply(z, w, solver)).sum().backward()
+    z.grad = None
+    w.grad = None
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     solver.upload(1e-4)
