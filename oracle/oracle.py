@@ -11,6 +11,8 @@ Functions and the reference code they restate (paths relative to the reference's
   visual_cost           vio/window.py:466-493
   obs_residual_jac      vio/window.py:125-263  (obs_residual + obs_jacobians_ws)
   imu_factor            vio/imu_factor.py:771-875 (factor_core)
+  preintegrate          vio/imu_factor.py:392-704 (_step, preint_core)
+  sqrt_info             vio/imu_factor.py:707-746 (sqrt_info_core)
   solve_np              backend/ba.py:713-869  (_solve_np, the drop-in entry point)
   so3_exp_nb/so3_log_nb vio/window.py:43-105; log_quat/jr_inv/exp_jr vio/imu_factor.py:195-330
   se3_exp/so3_project   core/lie.py:301-314, 228-242
@@ -97,6 +99,11 @@ def lib():
             getattr(L, name).argtypes = [f64p, f64p]
         L.o_exp_jr.argtypes = [f64p, f64p, f64p]
         L.o_chol_solve.argtypes = [f64p, C.c_int, f64p, f64p]
+        L.o_preint.argtypes = [f64p, f64p, f64p, C.c_int, C.c_double, C.c_double, C.c_double, f64p, f64p,
+                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               f64p, f64p, f64p, f64p, f64p, f64p]
+        L.o_sqrt_info.argtypes = [f64p, C.c_double, C.c_double, C.c_double, f64p, f64p]
+        L.o_sqrt_info.restype = C.c_int
         L.o_chol_solve.restype = C.c_int
         _LIB = L
     return _LIB
@@ -428,3 +435,32 @@ def chol_solve(A, b):
     x = np.zeros(len(b))
     ok = lib().o_chol_solve(_p(A, f64p), len(b), _p(b, f64p), _p(x, f64p))
     return x if ok else None
+
+
+PI_STATUS, PI_NSUB, PI_NINNER, PI_EXTRAP, PI_TEND, PI_GAPMAX, PI_NREL, PI_TNOM, PI_SMAX = range(9)
+
+
+def preintegrate(t, gyr, acc, t0, t1, horizon, bg, ba, sg2, sa2, swg2, swa2, tnom):
+    """vio/imu_factor.py:545-704 (preint_core): dict(dR, dv, dp, J (9,6), C (15,15), info (9,)); horizon=None means
+    every sample is released."""
+    t = np.ascontiguousarray(t, np.float64)
+    gyr = np.ascontiguousarray(gyr, np.float64).reshape(-1, 3)
+    acc = np.ascontiguousarray(acc, np.float64).reshape(-1, 3)
+    bg = np.ascontiguousarray(bg, np.float64)
+    ba = np.ascontiguousarray(ba, np.float64)
+    out = dict(dR=np.zeros((3, 3)), dv=np.zeros(3), dp=np.zeros(3), J=np.zeros((9, 6)), C=np.zeros((15, 15)),
+               info=np.zeros(9))
+    hz = np.inf if horizon is None else float(horizon)
+    lib().o_preint(_p(t, f64p), _p(gyr, f64p), _p(acc, f64p), len(t), float(t0), float(t1), hz, _p(bg, f64p),
+                   _p(ba, f64p), float(sg2), float(sa2), float(swg2), float(swa2), float(tnom),
+                   *[_p(out[k], f64p) for k in ("dR", "dv", "dp", "J", "C", "info")])
+    return out
+
+
+def sqrt_info(Cm, kappa, sbg2, sba2, J):
+    """vio/imu_factor.py:707-746 (sqrt_info_core): W (15,15) lower, or None when not positive definite."""
+    Cm = np.ascontiguousarray(Cm, np.float64)
+    J = np.ascontiguousarray(J, np.float64)
+    W = np.zeros((15, 15))
+    ok = lib().o_sqrt_info(_p(Cm, f64p), float(kappa), float(sbg2), float(sba2), _p(J, f64p), _p(W, f64p))
+    return W if ok else None

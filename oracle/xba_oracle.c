@@ -22,6 +22,9 @@
  *   - vio/window.py:985-1008 final_linearisation       -> ow_final_linearisation
  *   - vio/imu_factor.py:195-330 _abc, _exp_jr, _jr_inv, _log -> abc, exp_jr, jr_inv, log_quat
  *   - vio/imu_factor.py:771-875 factor_core            -> o_imu_factor
+ *   - vio/imu_factor.py:366-542 _F_left, _step         -> pre_fleft, pre_step
+ *   - vio/imu_factor.py:545-704 preint_core            -> o_preint
+ *   - vio/imu_factor.py:707-746 sqrt_info_core         -> o_sqrt_info
  *   - vio/marg.py:47-57, 102-127 _boxminus, Prior._d/terms/cost -> prior_d, prior_terms
  *   - core/lie.py:47-79, 108-141, 201-242, 301-335     series thresholds, coefficients, so3_project, se3_exp, se3_inv
  *   - backend/ba.py:644-697  _rot_factor_blocks        -> rot_factor_blocks
@@ -1986,4 +1989,280 @@ int onpid_solve(onpid_problem *q, int iters, double lam0, double *stats, double 
     free(rhon); free(mbn); free(Hpp); free(Hred); free(bp); free(Hll); free(Hlli); free(bl); free(dxl);
     free(order); free(bounds); free(Tbest); free(Wbest); free(rbest); free(mbest);
     return 0;
+}
+
+/* ================================================================ IMU preintegration (vio/imu_factor.py) ===== */
+/* info vector and status codes of vio/imu_factor.py:360-372 */
+enum { PI_STATUS, PI_NSUB, PI_NINNER, PI_EXTRAP, PI_TEND, PI_GAPMAX, PI_NREL, PI_TNOM, PI_SMAX, PI_NINFO };
+enum { PI_OK = 0, PI_BAD_INTERVAL = 1, PI_NO_DATA = 2, PI_BEFORE_FIRST = 3, PI_NONFINITE = 4, PI_NO_RATE = 5 };
+
+/* Y = F X for the sub-interval transition F (vio/imu_factor.py:366-389), X and Y 15x15 row-major, order
+   [phi, v, p, eps_g, eps_a]; B[0..7] = Rt, F1, F2, Bg0, Bg1, Bg2, Ba1, Ba2 (3x3 row-major each). */
+static void pre_fleft(double *const B[8], double h, const double *X, double *Y) {
+    const double *Rt = B[0], *F1 = B[1], *F2 = B[2], *G0 = B[3], *G1 = B[4], *G2 = B[5], *A1 = B[6], *A2 = B[7];
+    for (int c = 0; c < 15; c++)
+        for (int i = 0; i < 3; i++) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int k = 0; k < 3; k++) {
+                double xph = X[k * 15 + c], xg = X[(9 + k) * 15 + c], xa = X[(12 + k) * 15 + c];
+                s0 += Rt[3 * i + k] * xph + G0[3 * i + k] * xg;
+                s1 += F1[3 * i + k] * xph + G1[3 * i + k] * xg + A1[3 * i + k] * xa;
+                s2 += F2[3 * i + k] * xph + G2[3 * i + k] * xg + A2[3 * i + k] * xa;
+            }
+            Y[i * 15 + c] = s0;
+            Y[(3 + i) * 15 + c] = s1 + X[(3 + i) * 15 + c];
+            Y[(6 + i) * 15 + c] = s2 + h * X[(3 + i) * 15 + c] + X[(6 + i) * 15 + c];
+            Y[(9 + i) * 15 + c] = X[(9 + i) * 15 + c];
+            Y[(12 + i) * 15 + c] = X[(12 + i) * 15 + c];
+        }
+}
+
+/* vio/imu_factor.py:392-542 _step: advance (dR, dv, dp), the bias Jacobians J (9x6 row-major: rows phi v p, cols
+   bg ba) and the internal covariance P (15x15, errors computed minus true, eps = bias deviation) over one
+   sub-interval of length h with the bias-corrected rates w, a and the (already scaled) white densities. */
+static void pre_step(double h, const double *w, const double *a, double sg2, double sa2, double swg2, double swa2,
+                     double *dR, double *dv, double *dp, double *J, double *P) {
+    double wh[3] = {w[0] * h, w[1] * h, w[2] * h}, wm[3] = {0.5 * wh[0], 0.5 * wh[1], 0.5 * wh[2]};
+    double dRk[9], Jrf[9], Rh[9], Jrh[9], dRm[9], am[3], M[9], N[9], T[9], MMt[9];
+    o_exp_jr(wh, dRk, Jrf);
+    o_exp_jr(wm, Rh, Jrh);
+    mm3(dR, Rh, dRm); /* the midpoint attitude */
+    for (int i = 0; i < 3; i++) {
+        const double *r = dRm + 3 * i;
+        am[i] = r[0] * a[0] + r[1] * a[1] + r[2] * a[2];
+        M[3 * i + 0] = r[1] * a[2] - r[2] * a[1]; /* M = dRm [a]x */
+        M[3 * i + 1] = -r[0] * a[2] + r[2] * a[0];
+        M[3 * i + 2] = r[0] * a[1] - r[1] * a[0];
+    }
+    mmt3(M, Rh, N);
+    mm3(M, Jrh, T);
+    double h2 = h * h, h3 = h2 * h, h4 = h3 * h, h5 = h4 * h, h6 = h5 * h, h7 = h6 * h;
+    double Rt[9], F1[9], F2[9], G0[9], G1[9], G2[9], A1[9], A2[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            int e = 3 * i + j;
+            Rt[e] = dRk[3 * j + i];
+            F1[e] = -N[e] * h;
+            F2[e] = -0.5 * N[e] * h2;
+            G0[e] = Jrf[e] * h;
+            G1[e] = -T[e] * (0.5 * h2);
+            G2[e] = -T[e] * (0.25 * h3);
+            A1[e] = dRm[e] * h;
+            A2[e] = 0.5 * dRm[e] * h2;
+        }
+    double *const B[8] = {Rt, F1, F2, G0, G1, G2, A1, A2};
+    double Y[225], Z[225];
+    pre_fleft(B, h, P, Y); /* F P */
+    for (int i = 0; i < 15; i++)
+        for (int j = 0; j < 15; j++) Z[i * 15 + j] = Y[j * 15 + i];
+    pre_fleft(B, h, Z, Y); /* F P F^T */
+    for (int i = 0; i < 15; i++) {
+        P[i * 15 + i] = Y[i * 15 + i];
+        for (int j = i + 1; j < 15; j++) P[i * 15 + j] = P[j * 15 + i] = 0.5 * (Y[i * 15 + j] + Y[j * 15 + i]);
+    }
+    mmt3(M, M, MMt);
+    /* continuous white noise and bias drift through the sub-interval, closed forms (imu_factor.py:459-500): every
+       entry of P receives exactly one term */
+    double qpp = sg2 * h + swg2 * h3 / 3.0, qvp1 = sg2 * h2 / 2.0 + swg2 * h4 / 8.0;
+    double qpp1 = sg2 * h3 / 6.0 + swg2 * h5 / 30.0;
+    double qvv_i = sa2 * h + swa2 * h3 / 3.0, qvv_m = sg2 * h3 / 3.0 + swg2 * h5 / 20.0;
+    double qpv_i = sa2 * h2 / 2.0 + swa2 * h4 / 8.0, qpv_m = sg2 * h4 / 8.0 + swg2 * h6 / 72.0;
+    double qpp_i = sa2 * h3 / 3.0 + swa2 * h5 / 20.0, qpp_m = sg2 * h5 / 20.0 + swg2 * h7 / 252.0;
+#define PADD(r, c, v) P[(r) * 15 + (c)] += (v)
+#define PADD2(r, c, v) do { double q_ = (v); PADD(r, c, q_); PADD(c, r, q_); } while (0)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double d = i == j ? 1.0 : 0.0, mm = MMt[3 * i + j], mij = M[3 * i + j], rij = dRm[3 * i + j];
+            PADD(i, j, qpp * d);
+            PADD2(3 + i, j, -mij * qvp1);
+            PADD2(6 + i, j, -mij * qpp1);
+            PADD(3 + i, 3 + j, qvv_i * d + qvv_m * mm);
+            PADD2(6 + i, 3 + j, qpv_i * d + qpv_m * mm);
+            PADD(6 + i, 6 + j, qpp_i * d + qpp_m * mm);
+            PADD2(i, 9 + j, swg2 * h2 / 2.0 * d);
+            PADD2(3 + i, 9 + j, -mij * swg2 * h3 / 6.0);
+            PADD2(6 + i, 9 + j, -mij * swg2 * h4 / 24.0);
+            PADD2(3 + i, 12 + j, rij * swa2 * h2 / 2.0);
+            PADD2(6 + i, 12 + j, rij * swa2 * h3 / 6.0);
+            PADD(9 + i, 9 + j, swg2 * h * d);
+            PADD(12 + i, 12 + j, swa2 * h * d);
+        }
+#undef PADD
+#undef PADD2
+    /* bias Jacobians, the exact derivatives of the discrete scheme: J <- A9 J - B (imu_factor.py:502-520) */
+    double Jn[54];
+    for (int c = 0; c < 6; c++)
+        for (int i = 0; i < 3; i++) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int k = 0; k < 3; k++) {
+                double jp = J[k * 6 + c];
+                s0 += Rt[3 * i + k] * jp;
+                s1 += F1[3 * i + k] * jp;
+                s2 += F2[3 * i + k] * jp;
+            }
+            Jn[i * 6 + c] = s0;
+            Jn[(3 + i) * 6 + c] = s1 + J[(3 + i) * 6 + c];
+            Jn[(6 + i) * 6 + c] = s2 + h * J[(3 + i) * 6 + c] + J[(6 + i) * 6 + c];
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Jn[i * 6 + j] -= G0[3 * i + j];
+            Jn[(3 + i) * 6 + j] -= G1[3 * i + j];
+            Jn[(6 + i) * 6 + j] -= G2[3 * i + j];
+            Jn[(3 + i) * 6 + 3 + j] -= A1[3 * i + j];
+            Jn[(6 + i) * 6 + 3 + j] -= A2[3 * i + j];
+        }
+    memcpy(J, Jn, sizeof(Jn));
+    /* the mean */
+    for (int i = 0; i < 3; i++) {
+        dp[i] = dp[i] + dv[i] * h + 0.5 * am[i] * h2;
+        dv[i] = dv[i] + am[i] * h;
+    }
+    double Rn[9];
+    mm3(dR, dRk, Rn);
+    memcpy(dR, Rn, sizeof(Rn));
+}
+
+static int finite6(const double *w, const double *a) {
+    return isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]);
+}
+
+/* vio/imu_factor.py:545-704 preint_core: preintegrate [t0, t1] from the samples with t <= horizon (stamps t (n,),
+   gyr / acc (n,3) row-major; bg, ba the linearisation biases; squared densities; tnom the nominal sample period).
+   Outputs dR (9), dv, dp (3), J (9x6), C (15x15 residual order [rR rv rp rbg rba]) and info (PI_NINFO); nothing but
+   info is meaningful unless info[PI_STATUS] == PI_OK. */
+void o_preint(const double *t, const double *gyr, const double *acc, int n, double t0, double t1, double horizon,
+              const double *bg, const double *ba, double sg2, double sa2, double swg2, double swa2, double tnom,
+              double *dR, double *dv, double *dp, double *J, double *C, double *info) {
+    for (int i = 0; i < 9; i++) dR[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    for (int i = 0; i < 3; i++) dv[i] = dp[i] = 0.0;
+    memset(J, 0, 54 * sizeof(double));
+    memset(C, 0, 225 * sizeof(double));
+    memset(info, 0, PI_NINFO * sizeof(double));
+    int lo = 0, hi = n;
+    while (lo < hi) { /* released count, from the stamps alone */
+        int mid = (lo + hi) / 2;
+        if (t[mid] <= horizon) lo = mid + 1; else hi = mid;
+    }
+    const int nrel = lo;
+    info[PI_NREL] = nrel;
+    if (!(t1 > t0) || !isfinite(t0) || !isfinite(t1)) { info[PI_STATUS] = PI_BAD_INTERVAL; return; }
+    if (nrel == 0) { info[PI_STATUS] = PI_NO_DATA; return; }
+    if (t0 < t[0]) { info[PI_STATUS] = PI_BEFORE_FIRST; return; }
+    const int rate_ok = tnom > 0.0 && isfinite(tnom);
+    if ((sg2 > 0.0 || sa2 > 0.0) && !rate_ok) { info[PI_STATUS] = PI_NO_RATE; return; }
+    info[PI_TNOM] = rate_ok ? tnom : 0.0;
+    const double te = t[nrel - 1];
+    info[PI_TEND] = te;
+    double P[225], w[3], a[3], smax = 1.0, gapmax = 0.0;
+    memset(P, 0, sizeof(P));
+    int nsub = 0, ninner = 0;
+    if (te > t0) {
+        const double tie = t1 < te ? t1 : te;
+        lo = 0;
+        hi = nrel;
+        while (lo < hi) { /* first released stamp > t0 */
+            int mid = (lo + hi) / 2;
+            if (t[mid] <= t0) lo = mid + 1; else hi = mid;
+        }
+        int k = lo, left = k - 1;
+        double e0 = t0;
+        for (;;) {
+            const int samp = k < nrel && t[k] < tie;
+            const double e1 = samp ? t[k] : tie, h = e1 - e0;
+            if (h > 0.0) {
+                const double m = 0.5 * (e0 + e1);
+                while (left + 2 < nrel && t[left + 1] <= m) left++; /* bracket, bounded at the last released pair */
+                const double ta = t[left], gap = t[left + 1] - ta;
+                if (gap > gapmax) gapmax = gap;
+                for (int c = 0; c < 3; c++) {
+                    double s = (gyr[3 * (left + 1) + c] - gyr[3 * left + c]) / gap;
+                    w[c] = s * (m - ta) + gyr[3 * left + c] - bg[c];
+                    s = (acc[3 * (left + 1) + c] - acc[3 * left + c]) / gap;
+                    a[c] = s * (m - ta) + acc[3 * left + c] - ba[c];
+                }
+                if (!finite6(w, a)) { info[PI_STATUS] = PI_NONFINITE; return; }
+                double sw = 1.0; /* sample-support rule: max(1, nu h / T_nom) */
+                if (rate_ok) {
+                    const double u = (m - ta) / gap;
+                    sw = 2.0 * ((1.0 - u) * (1.0 - u) + u * u) * h / tnom;
+                    if (sw < 1.0) sw = 1.0;
+                    if (sw > smax) smax = sw;
+                }
+                pre_step(h, w, a, sg2 * sw, sa2 * sw, swg2, swa2, dR, dv, dp, J, P);
+                nsub++;
+            }
+            if (!samp) break;
+            ninner++;
+            e0 = e1;
+            k++;
+        }
+    }
+    if (t1 > te) { /* hold the last released sample over (t_end, t1] */
+        const double ts = t0 > te ? t0 : te, h = t1 - ts;
+        for (int c = 0; c < 3; c++) {
+            w[c] = gyr[3 * (nrel - 1) + c] - bg[c];
+            a[c] = acc[3 * (nrel - 1) + c] - ba[c];
+        }
+        if (!finite6(w, a)) { info[PI_STATUS] = PI_NONFINITE; return; }
+        double sw = 1.0;
+        if (rate_ok) {
+            sw = (te > t0 ? 2.0 : 1.0) * h / tnom;
+            if (sw < 1.0) sw = 1.0;
+            if (sw > smax) smax = sw;
+        }
+        pre_step(h, w, a, sg2 * sw, sa2 * sw, swg2, swa2, dR, dv, dp, J, P);
+        nsub++;
+        if (te > t0) ninner++;
+        info[PI_EXTRAP] = h;
+    }
+    double Tm[9], T2[9], Rn[9]; /* one Newton-Schulz polar step */
+    mtm3(dR, dR, Tm);
+    for (int i = 0; i < 9; i++) T2[i] = (i % 4 == 0 ? 1.5 : 0.0) - 0.5 * Tm[i];
+    mm3(dR, T2, Rn);
+    memcpy(dR, Rn, sizeof(Rn));
+    for (int i = 0; i < 15; i++) /* residual order: r9 = -(computed - true), r_b = +eps: the cross block flips */
+        for (int j = 0; j < 15; j++) C[i * 15 + j] = ((i < 9) != (j < 9)) ? -P[i * 15 + j] : P[i * 15 + j];
+    info[PI_NSUB] = nsub;
+    info[PI_NINNER] = ninner;
+    info[PI_GAPMAX] = gapmax;
+    info[PI_SMAX] = smax;
+    info[PI_STATUS] = PI_OK;
+}
+
+/* vio/imu_factor.py:707-746 sqrt_info_core: W = L^-1 (lower), L L^T = kappa C + J_bg sbg2 J_bg^T + J_ba sba2 J_ba^T
+   on the 9 preintegration rows.  Returns 0 (W undefined) when the matrix is not positive definite. */
+int o_sqrt_info(const double *C, double kappa, double sbg2, double sba2, const double *J, double *W) {
+    double A[225], L[225];
+    memset(L, 0, sizeof(L));
+    for (int i = 0; i < 225; i++) A[i] = kappa * C[i];
+    if (sbg2 > 0.0 || sba2 > 0.0)
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 3; k++) s += sbg2 * J[i * 6 + k] * J[j * 6 + k] + sba2 * J[i * 6 + 3 + k] * J[j * 6 + 3 + k];
+                A[i * 15 + j] += s;
+            }
+    for (int j = 0; j < 15; j++) {
+        double s = A[j * 15 + j];
+        for (int k = 0; k < j; k++) s -= L[j * 15 + k] * L[j * 15 + k];
+        if (!(s > 0.0)) return 0;
+        const double dj = sqrt(s);
+        L[j * 15 + j] = dj;
+        for (int i = j + 1; i < 15; i++) {
+            s = A[i * 15 + j];
+            for (int k = 0; k < j; k++) s -= L[i * 15 + k] * L[j * 15 + k];
+            L[i * 15 + j] = s / dj;
+        }
+    }
+    memset(W, 0, 225 * sizeof(double));
+    for (int j = 0; j < 15; j++) {
+        W[j * 15 + j] = 1.0 / L[j * 15 + j];
+        for (int i = j + 1; i < 15; i++) {
+            double s = 0.0;
+            for (int k = j; k < i; k++) s += L[i * 15 + k] * W[k * 15 + j];
+            W[i * 15 + j] = -s / L[i * 15 + i];
+        }
+    }
+    return 1;
 }
