@@ -226,6 +226,42 @@ typedef struct xba_npid_problem {
 int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                     double *dx_hist_out, void *stream);
 
+/* ---- IMU preintegration of many intervals at once --------------------------------------------------------------
+ * Replaces vio/imu_factor.py:1008-1048 `preintegrate` (the numba kernel preint_core, :545-704, with its
+ * sub-interval step _step, :392-542) called once per interval, and :707-746 `sqrt_info_core` for the whitening.
+ * Interval f preintegrates [t0, t1] over the samples of its stream with t <= horizon (the causal rule of
+ * :46-66: later samples are never read, the last released one is held past its stamp), at the linearisation biases
+ * bg_lin, ba_lin, with the squared continuous densities sg2 (white gyro), sa2 (white accel), swg2, swa2 (bias walks)
+ * and the nominal sample period t_nom of the sample-support rule (:67-95; the caller resolves it as the reference's
+ * _t_nom does: 1 / the declared rate, else the median spacing of the released stamps, else 0).  Every pointer is a
+ * device pointer; streams are given as offsets into shared sample arrays, several intervals may share a stream.
+ * Outputs follow the reference's Preint: dR, dv, dp, J (rows phi v p, columns bg ba), the UNSCALED covariance C in
+ * residual order [rR rv rp rbg rba] and the info vector; nothing but info is meaningful unless info[0] == 0 (the
+ * reference's status codes: 1 t1 <= t0 or non-finite, 2 no sample released, 3 t0 before the first sample,
+ * 4 non-finite sample used, 5 no sample period for a covariance).  Optional: the whitening W = L^-1 with
+ * L L^T = kappa C + J_bg sbg2 J_bg^T + J_ba sba2 J_ba^T (w_ok 0 when not positive definite, W then zero), and the
+ * window solver's IMU record (XBA_IMU_REC).  Asynchronous on `stream`. */
+#define XBA_PRE_NPAR 17  /* per interval: t0, t1, horizon, bg_lin (3), ba_lin (3), sg2, sa2, swg2, swa2, t_nom, kappa, sbg2, sba2 */
+#define XBA_PRE_NINFO 9  /* status, n_sub, n_inner, extrap_span, t_data_end, gap_max, n_released, t_nom, white_scale_max */
+typedef struct {
+    int32_t nf;            /* intervals */
+    const double *t;       /* sample stamps of all streams, concatenated; non-decreasing within a stream */
+    const double *gyr;     /* (samples, 3) */
+    const double *acc;     /* (samples, 3) */
+    const int64_t *s0;     /* (nf,) first sample of interval f's stream */
+    const int32_t *ns;     /* (nf,) samples in that stream */
+    const double *par;     /* (nf, XBA_PRE_NPAR) */
+    double *dR, *dv, *dp;  /* (nf, 9) row-major, (nf, 3), (nf, 3) */
+    double *J;             /* (nf, 54) row-major 9 x 6 */
+    double *C;             /* (nf, 225) row-major */
+    double *info;          /* (nf, XBA_PRE_NINFO) */
+    double *W;             /* (nf, 225) or NULL */
+    int32_t *w_ok;         /* (nf,) or NULL */
+    double *rec;           /* (nf, XBA_IMU_REC) or NULL */
+} xba_preint_batch;
+
+int xba_preintegrate(const xba_preint_batch *b, void *stream);
+
 /* Diagnostics (no reference counterpart): a plain device-to-device copy kernel of `bytes` (a multiple of 16;
  * 16-byte aligned pointers) on `stream`, so HBM counters (FETCH_SIZE, WRITE_SIZE) can be calibrated against a
  * kernel whose traffic is known exactly.  Asynchronous. */

@@ -18,6 +18,7 @@
 #include "xba_fast.hip"
 #include "xba_np.hip"
 #include "xba_adj.hip"
+#include "xba_preint.hip"
 
 using namespace xba;
 
@@ -382,6 +383,20 @@ int pack_measurements(xba_ctx *c, hipStream_t s) {
 }  // namespace
 
 extern "C" {
+
+int xba_preintegrate(const xba_preint_batch *b, void *stream) {
+    if (!b || b->nf < 0) return XBA_E_ARG;
+    if (b->nf == 0) return 0;
+    if (!b->t || !b->gyr || !b->acc || !b->s0 || !b->ns || !b->par || !b->dR || !b->dv || !b->dp || !b->J ||
+        !b->C || !b->info)
+        return XBA_E_ARG;
+    static_assert(PP_N == XBA_PRE_NPAR, "interval parameter record");
+    const int grid = (b->nf + PRE_NG - 1) / PRE_NG;
+    hipLaunchKernelGGL(k_preint, dim3(grid), dim3(PRE_NG * PRE_G), 0, (hipStream_t)stream, b->t, b->gyr, b->acc,
+                       b->s0, b->ns, b->par, b->nf, b->dR, b->dv, b->dp, b->J, b->C, b->info, b->W, b->w_ok, b->rec);
+    HIPCK(hipGetLastError());
+    return 0;
+}
 
 int xba_copy_kernel(const void *src, void *dst, int64_t bytes, void *stream) {
     if (!src || !dst || bytes < 0 || bytes % 16 || ((uintptr_t)src | (uintptr_t)dst) % 16) return XBA_E_ARG;
