@@ -106,6 +106,80 @@ def differentiation_config(synth, log, nwin=64, per_window=200):
     return out
 
 
+def preintegration_config(torch, synth, log, cpu_budget=3.0, nwin=256, nkf=20, spacing=0.5, reps=20):
+    """IMU preintegration, batched: the configuration-1 shape (20 keyframes 0.5 s apart, 19 intervals of ~100
+    sub-intervals at 200 Hz) for 256 windows at once (4,864 intervals), EuRoC densities, with the whitening and the
+    window-solver records; HIP events over `reps` launches.  CPU baseline: the oracle's preint_core + sqrt_info_core
+    on one host thread over the first intervals of the same workload."""
+    from oracle import oracle as O
+    from xclean_amd import imu, roofline
+    streams = synth.imu_streams(nwin, spacing * (nkf - 1), seed=3)
+    noise = imu.ImuNoise(synth.IMU_NOISE["sigma_g"], synth.IMU_NOISE["sigma_a"], synth.IMU_NOISE["rw_g"],
+                         synth.IMU_NOISE["rw_a"], synth.IMU_NOISE["rate_hz"])
+    off = 0.0012  # keyframe stamps fall between IMU samples
+    iv = [(w, off + spacing * i, off + spacing * (i + 1) if i + 1 < nkf - 1 else spacing * (i + 1), None,
+           np.zeros(3), np.zeros(3)) for w in range(nwin) for i in range(nkf - 1)]
+    t, g, a, s0, ns, par = imu.pack(streams, iv, noise)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    T = lambda x, dt: torch.as_tensor(x, dtype=dt, device=dev).contiguous()
+    args = (T(t, torch.float64), T(g, torch.float64), T(a, torch.float64), T(s0, torch.int64), T(ns, torch.int32),
+            T(par, torch.float64))
+    for _ in range(2):
+        out = imu.launch(*args, want_rec=True)
+    torch.cuda.synchronize(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        out = imu.launch(*args, want_rec=True)
+    e1.record()
+    torch.cuda.synchronize(dev)
+    ms = e0.elapsed_time(e1) / reps
+    info = out["info"].cpu().numpy()
+    nsub = info[:, 1]
+    nf = len(iv)
+    flops = roofline.preint_flops(nsub)
+    samples_read = float(np.sum(nsub))  # one new sample per sub-interval (the bracket walk), plus the edge ones
+    tfl = flops / (ms * 1e-3) / 1e12
+    # spot check against the oracle, and the CPU baseline on the same intervals
+    q = noise.squares()
+    host = {k: out[k].cpu().numpy() for k in ("dR", "C", "W")}
+    err = 0.0
+    for f in range(min(64, nf)):
+        si, t0, t1, hz, bg, ba = iv[f]
+        r = O.preintegrate(*streams[si], t0, t1, hz, bg, ba, *q, par[f, 13])
+        W = O.sqrt_info(r["C"], 1.0, 0.0, 0.0, r["J"])
+        err = max(err, float(np.abs(host["C"][f] - r["C"]).max() / np.abs(r["C"]).max()),
+                  float(np.abs(host["W"][f] - W).max() / np.abs(W).max()), float(np.abs(host["dR"][f] - r["dR"]).max()))
+    tc0 = time.perf_counter()
+    done = 0
+    while done < nf and time.perf_counter() - tc0 < cpu_budget:
+        si, t0, t1, hz, bg, ba = iv[done]
+        r = O.preintegrate(*streams[si], t0, t1, hz, bg, ba, *q, par[done, 13])
+        O.sqrt_info(r["C"], 1.0, 0.0, 0.0, r["J"])
+        done += 1
+    tc = time.perf_counter() - tc0
+    rec = {"workload": "%d windows x %d keyframes %.1f s apart: %d intervals of ~%d sub-intervals at %.0f Hz, "
+                       "covariance + whitening + window records" % (nwin, nkf, spacing, nf, int(np.median(nsub)),
+                                                                    synth.IMU_NOISE["rate_hz"]),
+           "intervals": nf, "sub_intervals": int(np.sum(nsub)), "ms_per_launch": ms,
+           "intervals_per_s": nf / (ms * 1e-3), "sub_intervals_per_s": float(np.sum(nsub)) / (ms * 1e-3),
+           "all_ok": bool((info[:, 0] == 0).all()),
+           "max_rel_err_vs_oracle_first64": err,
+           "roofline": {"bound": "fp64 valu (serial sub-interval chain per interval)", "achieved": tfl,
+                        "peak": roofline.FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tfl / roofline.FP64_PEAK_TFLOPS,
+                        "algorithmic_flops": flops,
+                        "algorithmic_bytes": roofline.preint_bytes(samples_read, nf),
+                        "kernel": "k_preint"},
+           "cpu_baseline": None if done == 0 else {
+               "value": done / tc, "unit": "intervals/s", "cores": 1, "kind": "port",
+               "sample": "first %d of the %d intervals, oracle preint_core + sqrt_info_core (C), one host thread"
+                         % (done, nf)}}
+    log("bench: preintegration: %d intervals in %.3f ms (%.3e intervals/s, %.2f TFLOP/s); cpu oracle %s "
+        "intervals/s; max rel err vs oracle %.1e" % (nf, ms, rec["intervals_per_s"], tfl,
+                                                     "%.3e" % (done / tc) if done else "-", err))
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +192,8 @@ def main():
                     help="also run a plain 1 GiB device copy (three launches) for HBM counter calibration")
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner (solver defaults)")
     ap.add_argument("--no-extra", action="store_true", help="skip configurations 1, 3 and 4")
+    ap.add_argument("--preint-only", action="store_true",
+                    help="diagnostic: run only the batched IMU preintegration and print its JSON")
     ap.add_argument("--configs-only", action="store_true",
                     help="diagnostic: run only configurations 1 and 4 (for profiling the tiled factorisation) and "
                          "print their JSON")
@@ -136,6 +212,9 @@ def main():
 
     if args.configs_only:
         print(json.dumps(single_window_configs(torch, synth, BatchSolver, log)))
+        return
+    if args.preint_only:
+        print(json.dumps(preintegration_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)))
         return
 
     t0 = time.time()
@@ -233,6 +312,8 @@ def main():
 
     extra = None if args.no_extra else single_window_configs(torch, synth, BatchSolver, log)
     diff = None if args.no_extra else differentiation_config(synth, log)
+    preint = None if args.no_extra else preintegration_config(torch, synth, log,
+                                                              cpu_budget=0.0 if args.no_cpu else 3.0)
 
     cpu = None
     if not args.no_cpu:
@@ -295,6 +376,7 @@ def main():
                                                "trials": len(tune["trials"])},
         "single_window_configs": extra,
         "differentiation": diff,
+        "preintegration": preint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

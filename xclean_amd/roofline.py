@@ -77,3 +77,25 @@ def lin_kernel_flops(M, k):
     k = np.asarray(k, np.float64)
     lm = k > 0
     return 810.0 * M + float(np.sum(60 + 144 * k[lm] + 216 * k[lm] ** 2))
+
+
+# ---- IMU preintegration (vio/imu_factor.py:392-704), per sub-interval and per interval ------------------------------
+# FLOPs per sub-interval (an FMA counts 2): the covariance propagation F P F^T dominates: two passes of 15 columns,
+# 153 each (rows phi 18 FMA, v 27 FMA + 3, p 27 FMA + 6) = 4,590; symmetrisation 450; the closed-form noise ~725 and
+# its addition 225; bias Jacobians 6 columns x 69 = 414; two Exp / Jr evaluations ~120; the midpoint attitude, M, N,
+# T, M M^T and the eight blocks ~360; the mean ~75.  Total 6,960.
+PREINT_FLOPS_PER_SUB = 6960.0
+# Bytes per interval: its samples (stamp 8 + gyro 24 + accel 24 = 56 B each) read once, the 17 parameters, and the
+# outputs dR 72, dv 24, dp 24, J 432, C 1,800, info 72 (+ W 1,800 + w_ok 4 + the 301-entry record 2,408 when asked).
+PREINT_BYTES_SAMPLE = 56.0
+PREINT_BYTES_OUT = 136.0 + 2424.0
+PREINT_BYTES_OUT_W_REC = PREINT_BYTES_OUT + 1800.0 + 4.0 + 2408.0
+
+
+def preint_flops(n_sub):
+    return PREINT_FLOPS_PER_SUB * float(np.sum(n_sub))
+
+
+def preint_bytes(n_samples_read, n_intervals, whiten_and_records=True):
+    return PREINT_BYTES_SAMPLE * float(n_samples_read) + n_intervals * (
+        PREINT_BYTES_OUT_W_REC if whiten_and_records else PREINT_BYTES_OUT)

@@ -220,6 +220,26 @@ def _imu_signals(rng, T, rate):
     return w_b, a_w
 
 
+def imu_streams(n, duration, rate=None, seed=0):
+    """``n`` synthetic IMU streams of ``duration`` seconds at ``rate`` Hz (the configured IMU by default): the
+    analytic body rates and accelerations of ``_imu_signals`` plus gravity along the body z axis, constant biases and
+    white noise at the configured densities.  Returns [(t, gyro, acc)]; used as preintegration workloads."""
+    rate = IMU_NOISE["rate_hz"] if rate is None else float(rate)
+    rng = np.random.default_rng(seed)
+    ns = int(round(duration * rate)) + 1
+    t = np.arange(ns) / rate
+    out = []
+    for _ in range(n):
+        w_b, a_w = _imu_signals(rng, duration, rate)
+        bg = rng.normal(0, 2e-3, 3)
+        ba = rng.normal(0, 2e-2, 3)
+        gyro = w_b(t) + bg + rng.normal(0, IMU_NOISE["sigma_g"] * math.sqrt(rate), (ns, 3))
+        acc = a_w(t) + np.array([0.0, 0.0, 9.81]) + ba + rng.normal(0, IMU_NOISE["sigma_a"] * math.sqrt(rate),
+                                                                        (ns, 3))
+        out.append((t.copy(), gyro, acc))
+    return out
+
+
 def _integrate_truth(w_b, a_w, T, rate, v0, sub=8):
     """Truth (R_wb, p, v) at the sample times by fine midpoint integration of the analytic signals."""
     n = int(round(T * rate)) + 1
