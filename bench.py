@@ -125,12 +125,12 @@ def preintegration_config(torch, synth, log, cpu_budget=3.0, nwin=256, nkf=20, s
     args = (T(t, torch.float64), T(g, torch.float64), T(a, torch.float64), T(s0, torch.int64), T(ns, torch.int32),
             T(par, torch.float64))
     for _ in range(2):
-        out = imu.launch(*args, want_rec=True)
+        out = imu.launch(*args, want_rec=True)  # checks the streams against the sample arrays once
     torch.cuda.synchronize(dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        out = imu.launch(*args, want_rec=True)
+        out = imu.launch(*args, want_rec=True, check=False)
     e1.record()
     torch.cuda.synchronize(dev)
     ms = e0.elapsed_time(e1) / reps

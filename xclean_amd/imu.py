@@ -126,11 +126,13 @@ class XbaPreintBatch(C.Structure):
                 ("rec", vp)]
 
 
-def launch(t, gyr, acc, s0, ns, par, want_w=False, want_rec=False, stream=None):
+def launch(t, gyr, acc, s0, ns, par, want_w=False, want_rec=False, stream=None, check=True):
     """One `xba_preintegrate` launch over device tensors: t (S,), gyr / acc (S, 3) FP64, s0 (nf,) int64, ns (nf,)
     int32, par (nf, 17) FP64 (t0, t1, horizon, bg_lin, ba_lin, sg2, sa2, swg2, swa2, t_nom, kappa, sbg2, sba2).
     Returns a dict of device tensors (dR (nf,3,3), dv, dp, J (nf,9,6), C (nf,15,15), info (nf,9) [, W, w_ok]
-    [, rec (nf, 301)]).  Asynchronous on `stream` (default: torch's current stream)."""
+    [, rec (nf, 301)]).  Asynchronous on `stream` (default: torch's current stream), except for the bounds check
+    of the streams against the sample arrays (a device reduction read back on the host), which `check=False` skips
+    for inputs already checked."""
     import torch
     L = _lib.load()
     nf = int(par.shape[0])
@@ -141,7 +143,7 @@ def launch(t, gyr, acc, s0, ns, par, want_w=False, want_rec=False, stream=None):
             raise _lib.XbaError("preintegration inputs: contiguous device tensors of the stated dtypes and shapes")
     if s0.shape[0] != nf or ns.shape[0] != nf or gyr.shape[0] != t.shape[0] or acc.shape[0] != t.shape[0]:
         raise _lib.XbaError("preintegration inputs: inconsistent lengths")
-    if nf and bool(((s0 < 0) | (ns < 0) | (s0 + ns.to(torch.int64) > t.shape[0])).any()):
+    if check and nf and bool(((s0 < 0) | (ns < 0) | (s0 + ns.to(torch.int64) > t.shape[0])).any()):
         raise _lib.XbaError("preintegration inputs: a stream runs outside the sample arrays")
     out = dict(dR=torch.empty((nf, 3, 3), dtype=torch.float64, device=dev),
                dv=torch.empty((nf, 3), dtype=torch.float64, device=dev),
