@@ -262,6 +262,48 @@ typedef struct {
 
 int xba_preintegrate(const xba_preint_batch *b, void *stream);
 
+/* ---- Marginalisation algebra of the square-root prior, batched ----------------------------------------------
+ * Replaces vio/marg.py:165-177 `schur` and :73-84 `Prior.from_info` (with :212-242 `combine`'s re-expression of b
+ * at the blocks' first linearisation points), which the reference evaluates with numpy's eigh one matrix at a time.
+ * Matrix w of a batch is stored in an ld x ld slab (row-major, leading dimension ld <= XBA_MAX_STATE); every pointer
+ * is a device pointer; `work` holds XBA_MARG_WORK(ld) doubles per matrix.  Asynchronous on `stream`. */
+#define XBA_MARG_WORK(ld) (3 * (int64_t)(ld) * (ld) + 2 * (int64_t)(ld))
+#define XBA_MARG_FLOOR 1e-12  /* eigenvalues at or below this times the largest are dropped (marg.py:32 EIG_FLOOR) */
+
+/* H2 = H_kk - H_km H_mm^+ H_mk, b2 = b_k - H_km H_mm^+ b_m over keep's order (symmetrised), H_mm^+ through the
+ * eigen-decomposition of the symmetrised H_mm with the floor.  keep and marg: index lists into matrix w (nkeep[w],
+ * nmarg[w] entries, nkeep + nmarg <= ld). */
+typedef struct {
+    int32_t nb, ld;
+    const int32_t *nkeep, *nmarg;  /* (nb,) */
+    const int32_t *keep, *marg;    /* (nb, ld) */
+    const double *H, *b;           /* (nb, ld, ld), (nb, ld) */
+    double *H2, *b2;               /* (nb, ld, ld), (nb, ld) */
+    double *work;
+} xba_schur_batch;
+
+int xba_marg_schur(const xba_schur_batch *q, void *stream);
+
+/* S = diag(sqrt l) V^T over the eigenvalues l of (H + H^T) / 2 above the floor (rows in ascending l, as numpy's eigh
+ * orders them; the rows past `rank` are zero), r = diag(1 / sqrt l) V^T b'.  b' = b, or, when blk0 is given,
+ * b - H d with d = x_cur [-] x0 over the matrix's blocks [blk0[w], blk0[w+1]) (block k at offset boff[k], bdim[k]
+ * 6 or 15 dims; states as R 9, p 3, v 3, bg 3, ba 3), d = [Log(R0^T R), p - p0, v - v0, bg - bg0, ba - ba0]. */
+typedef struct {
+    int32_t nb, ld;
+    const int32_t *n;              /* (nb,) */
+    const double *H, *b;           /* (nb, ld, ld), (nb, ld) */
+    const int32_t *blk0;           /* (nb + 1,) or NULL */
+    const int32_t *boff, *bdim;    /* (blocks,) */
+    const double *bcur, *bx0;      /* (blocks, 21) */
+    double *S, *r;                 /* (nb, ld, ld), (nb, ld) */
+    int32_t *rank;                 /* (nb,) */
+    double *lam;                   /* (nb, ld) or NULL: every eigenvalue, ascending */
+    double *bx;                    /* (nb, ld) or NULL: b' (b re-expressed at x0) */
+    double *work;
+} xba_prior_batch;
+
+int xba_prior_from_info(const xba_prior_batch *q, void *stream);
+
 /* Diagnostics (no reference counterpart): a plain device-to-device copy kernel of `bytes` (a multiple of 16;
  * 16-byte aligned pointers) on `stream`, so HBM counters (FETCH_SIZE, WRITE_SIZE) can be calibrated against a
  * kernel whose traffic is known exactly.  Asynchronous. */

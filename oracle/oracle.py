@@ -464,3 +464,63 @@ def sqrt_info(Cm, kappa, sbg2, sba2, J):
     W = np.zeros((15, 15))
     ok = lib().o_sqrt_info(_p(Cm, f64p), float(kappa), float(sbg2), float(sba2), _p(J, f64p), _p(W, f64p))
     return W if ok else None
+
+
+# ---- vio/marg.py, restated in numpy (the marginalisation algebra of the square-root prior) ------------------------
+MARG_FLOOR = 1e-12  # vio/marg.py:32
+
+
+def _eig_floor(A):
+    """Eigen-decomposition of the symmetrised A and the mask of eigenvalues above the floor (vio/marg.py:75-79)."""
+    lam, V = np.linalg.eigh(0.5 * (A + A.T))
+    top = float(lam.max()) if len(lam) else 0.0
+    return lam, V, lam > MARG_FLOOR * max(top, 1e-300)
+
+
+def marg_schur(H, b, keep, marg):
+    """vio/marg.py:165-177: the Schur complement of the kept indices, H_mm inverted over its eigenvalues above the
+    floor."""
+    keep, marg = np.asarray(keep, np.int64), np.asarray(marg, np.int64)
+    lam, V, ok = _eig_floor(H[np.ix_(marg, marg)])
+    Hinv = (V[:, ok] / lam[ok]) @ V[:, ok].T
+    Hkm = H[np.ix_(keep, marg)]
+    H2 = H[np.ix_(keep, keep)] - Hkm @ Hinv @ Hkm.T
+    b2 = b[keep] - Hkm @ Hinv @ b[marg]
+    return 0.5 * (H2 + H2.T), b2
+
+
+def prior_from_info(H, b):
+    """vio/marg.py:73-84: S (rank x n, rows in ascending eigenvalue), r."""
+    lam, V, ok = _eig_floor(H)
+    sl = np.sqrt(lam[ok])
+    return sl[:, None] * V[:, ok].T, (V[:, ok].T @ b) / sl
+
+
+def boxminus(cur, x0, dim):
+    """vio/marg.py:47-56: [Log(R0^T R), p - p0 (, v - v0, bg - bg0, ba - ba0)]; states (R, p, v, bg, ba)."""
+    out = np.zeros(dim)
+    out[0:3] = so3_log_nb(np.asarray(x0[0]).T @ np.asarray(cur[0]))
+    out[3:6] = np.asarray(cur[1]) - np.asarray(x0[1])
+    if dim == 15:
+        for k, q in enumerate((2, 3, 4)):
+            out[6 + 3 * k:9 + 3 * k] = np.asarray(cur[q]) - np.asarray(x0[q])
+    return out
+
+
+def combine(old, sids_new, dims_new, states_cur, H_cur, b_cur):
+    """vio/marg.py:212-242.  old: None or (sids, dims, x0 list); states as (R, p, v, bg, ba) with zeros for absent
+    parts.  Returns (x0 list, S, r): blocks of the old prior keep their x0 (a pose block that grew keeps its pose x0
+    and takes the current v, bg, ba); b is re-expressed at x0 before the square root."""
+    x0, d, o = [], np.zeros(int(sum(dims_new))), 0
+    for s, dim, st in zip(sids_new, dims_new, states_cur):
+        if old is not None and s in old[0]:
+            j = old[0].index(s)
+            xo = old[2][j]
+            xb = xo if (old[1][j] == 15 or dim == 6) else (xo[0], xo[1], st[2], st[3], st[4])
+            x0.append(xb)
+            d[o:o + dim] = boxminus(st, xb, dim)
+        else:
+            x0.append(st)
+        o += dim
+    S, r = prior_from_info(H_cur, b_cur - H_cur @ d)
+    return x0, S, r

@@ -13,7 +13,7 @@ SRC = os.path.join(HERE, "csrc", "xba_api.hip")
 OUT = os.path.join(HERE, "libxba.so")
 DEPS = [os.path.join(HERE, "csrc", f) for f in ("xba_api.hip", "xba_lin.hip", "xba_dense.hip", "xba_internal.h",
                                                "xba_np.hip", "xba_bigchol.hip", "xba_fast.hip", "xba_adj.hip",
-                                               "xba_preint.hip")]
+                                               "xba_preint.hip", "xba_marg.hip")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions stay off (v_pk_* FP32 forms are not used anywhere in this library)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -19,6 +19,7 @@
 #include "xba_np.hip"
 #include "xba_adj.hip"
 #include "xba_preint.hip"
+#include "xba_marg.hip"
 
 using namespace xba;
 
@@ -394,6 +395,29 @@ int xba_preintegrate(const xba_preint_batch *b, void *stream) {
     const int grid = (b->nf + PRE_NG - 1) / PRE_NG;
     hipLaunchKernelGGL(k_preint, dim3(grid), dim3(PRE_NG * PRE_G), 0, (hipStream_t)stream, b->t, b->gyr, b->acc,
                        b->s0, b->ns, b->par, b->nf, b->dR, b->dv, b->dp, b->J, b->C, b->info, b->W, b->w_ok, b->rec);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+int xba_marg_schur(const xba_schur_batch *q, void *stream) {
+    if (!q || q->nb < 0 || q->ld <= 0 || q->ld > XBA_MAX_STATE) return XBA_E_ARG;
+    if (q->nb == 0) return 0;
+    if (!q->nkeep || !q->nmarg || !q->keep || !q->marg || !q->H || !q->b || !q->H2 || !q->b2 || !q->work)
+        return XBA_E_ARG;
+    static_assert(MG_EIG_FLOOR == XBA_MARG_FLOOR, "floor");
+    hipLaunchKernelGGL(k_marg_schur, dim3(q->nb), dim3(MG_THREADS), 0, (hipStream_t)stream, q->ld, q->nkeep,
+                       q->nmarg, q->keep, q->marg, q->H, q->b, q->H2, q->b2, q->work);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+int xba_prior_from_info(const xba_prior_batch *q, void *stream) {
+    if (!q || q->nb < 0 || q->ld <= 0 || q->ld > XBA_MAX_STATE) return XBA_E_ARG;
+    if (q->nb == 0) return 0;
+    if (!q->n || !q->H || !q->b || !q->S || !q->r || !q->rank || !q->work) return XBA_E_ARG;
+    if (q->blk0 && (!q->boff || !q->bdim || !q->bcur || !q->bx0)) return XBA_E_ARG;
+    hipLaunchKernelGGL(k_prior_from_info, dim3(q->nb), dim3(MG_THREADS), 0, (hipStream_t)stream, q->ld, q->n, q->H,
+                       q->b, q->blk0, q->boff, q->bdim, q->bcur, q->bx0, q->S, q->r, q->rank, q->lam, q->bx, q->work);
     HIPCK(hipGetLastError());
     return 0;
 }

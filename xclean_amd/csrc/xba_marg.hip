@@ -1,0 +1,252 @@
+// Marginalisation algebra of the square-root prior, FP64, batched over matrices (vio/marg.py; paths relative to the
+// reference's src/xclean/):
+//   xba_marg_schur      marg.py:165-177 schur: the kept block's Schur complement, H_mm inverted through its
+//                       eigen-decomposition with the rank floor (a rank-deficient block marginalises its observable
+//                       part)
+//   xba_prior_from_info marg.py:73-84 Prior.from_info (S = diag(sqrt l) V^T over eigenvalues above the floor,
+//                       r = diag(1 / sqrt l) V^T b), with, optionally, combine's re-expression of b at the blocks'
+//                       first linearisation points (marg.py:212-242: b_x0 = b_cur - H d, d = x_cur [-] x0)
+//
+// One workgroup owns one matrix.  The symmetric eigen-decomposition is a two-sided cyclic Jacobi with the
+// round-robin parallel ordering: each round rotates n/2 disjoint index pairs at once (rotation angles from the
+// current 2x2 blocks, then the row pass, then the column pass and the eigenvector update, each spread over the whole
+// workgroup); sweeps repeat until the off-diagonal mass is below 1e-30 of the matrix's (squared Frobenius) mass, or
+// 40 sweeps.  Matrices and eigenvectors live in a global workspace (n^2 FP64 each, L2-resident at these sizes).
+#include "xba_internal.h"
+
+namespace xba {
+
+#define MG_THREADS 256
+#define MG_EIG_FLOOR 1e-12  // marg.py:32 EIG_FLOOR, relative to the largest eigenvalue
+#define MG_MAX_SWEEPS 40
+
+__device__ inline double mg_block_sum(double v, double *red) {
+    const int tid = threadIdx.x;
+    for (int d = XBA_WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, XBA_WAVE);
+    __syncthreads();
+    if (tid % XBA_WAVE == 0) red[tid / XBA_WAVE] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)blockDim.x / XBA_WAVE; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Eigen-decomposition of the symmetric n x n matrix A (leading dimension ld, overwritten: its diagonal ends as the
+// eigenvalues), eigenvectors into the columns of V.  cs: LDS for n/2 + 1 (c, s) pairs; red: LDS for the sums.
+__device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, double *red) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int e = tid; e < n * n; e += nt) V[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
+    const int N = n + (n & 1);  // players of the round-robin; N - 1 is a dummy when n is odd
+    const int np = N / 2;
+    __syncthreads();
+    for (int sweep = 0; sweep < MG_MAX_SWEEPS; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < n * n; e += nt) {
+            const int i = e / n, j = e % n;
+            const double v = A[i * ld + j];
+            tot += v * v;
+            if (i != j) off += v * v;
+        }
+        off = mg_block_sum(off, red);
+        tot = mg_block_sum(tot, red);
+        if (!(off > 1e-30 * tot)) break;
+        for (int rd = 0; rd < N - 1; rd++) {
+            // pairs of this round: (N - 1, rd) and ((rd + k) mod (N - 1), (rd - k) mod (N - 1)), k = 1 .. np - 1
+            for (int k = tid; k < np; k += nt) {
+                int p = k == 0 ? N - 1 : (rd + k) % (N - 1), q = k == 0 ? rd : (rd - k + N - 1) % (N - 1);
+                if (p > q) { const int x = p; p = q; q = x; }
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    const double apq = A[p * ld + q];
+                    if (apq != 0.0) {
+                        const double th = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
+                        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                cs[3 * k] = c;
+                cs[3 * k + 1] = s;
+                cs[3 * k + 2] = (double)(p * 65536 + q);
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = tid; e < np * n; e += nt) {
+                const int k = e / n, j = e % n;
+                const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
+                if (q >= n) continue;
+                const double c = cs[3 * k], s = cs[3 * k + 1];
+                const double ap = A[p * ld + j], aq = A[q * ld + j];
+                A[p * ld + j] = c * ap - s * aq;
+                A[q * ld + j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int e = tid; e < np * n; e += nt) {
+                const int k = e / n, i = e % n;
+                const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
+                if (q >= n) continue;
+                const double c = cs[3 * k], s = cs[3 * k + 1];
+                double ap = A[i * ld + p], aq = A[i * ld + q];
+                A[i * ld + p] = c * ap - s * aq;
+                A[i * ld + q] = s * ap + c * aq;
+                ap = V[i * ld + p];
+                aq = V[i * ld + q];
+                V[i * ld + p] = c * ap - s * aq;
+                V[i * ld + q] = s * ap + c * aq;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// The rank of eigenvalue i in ascending order (ties by index).
+__device__ inline int mg_rank(const double *A, int ld, int n, int i) {
+    const double li = A[i * ld + i];
+    int r = 0;
+    for (int j = 0; j < n; j++) {
+        const double lj = A[j * ld + j];
+        r += (lj < li || (lj == li && j < i)) ? 1 : 0;
+    }
+    return r;
+}
+
+// marg.py:165-177.  keep (nk) and marg (m) index lists; H2 (nk x nk, leading dimension ld) and b2 over keep's order.
+// Workspace per matrix (MG_WORK): slab 0 holds H_mm (rows 0..m-1) and T = H_km V (rows m..m+nk-1), slab 1 the
+// eigenvectors V, slab 2 Wm = T diag(1/l) (0 for the dropped directions), then V^T b_m.
+#define MG_WORK(ld) (3 * (size_t)(ld) * (ld) + 2 * (size_t)(ld))
+__global__ __launch_bounds__(MG_THREADS) void k_marg_schur(int ld, const int32_t *nkeep, const int32_t *nmarg,
+                                                            const int32_t *keep, const int32_t *marg, const double *H,
+                                                            const double *b, double *H2, double *b2, double *work) {
+    __shared__ double cs[3 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ double red[MG_THREADS / XBA_WAVE];
+    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int nk = nkeep[w], m = nmarg[w];
+    const size_t L2 = (size_t)ld * ld;
+    const double *Hw = H + L2 * w, *bw = b + (size_t)ld * w;
+    const int32_t *kp = keep + (size_t)ld * w, *mp = marg + (size_t)ld * w;
+    double *A = work + MG_WORK(ld) * w, *V = A + L2, *Wm = V + L2, *vb = Wm + L2, *T = A + (size_t)m * ld;
+    double *Ho = H2 + L2 * w, *bo = b2 + (size_t)ld * w;
+    for (int e = tid; e < m * m; e += nt) {  // H_mm, symmetrised
+        const int i = e / m, j = e % m;
+        A[i * ld + j] = 0.5 * (Hw[mp[i] * ld + mp[j]] + Hw[mp[j] * ld + mp[i]]);
+    }
+    __syncthreads();
+    if (m > 0) mg_jacobi(A, V, m, ld, cs, red);
+    // Hinv = V_ok diag(1 / l) V_ok^T over the eigenvalues above the floor relative to the largest one
+    double lmax = -INFINITY;
+    for (int i = 0; i < m; i++) lmax = fmax(lmax, A[i * ld + i]);
+    const double floor_ = MG_EIG_FLOOR * fmax(lmax, 1e-300);
+    for (int e = tid; e < nk * m; e += nt) {
+        const int i = e / m, j = e % m;
+        double s = 0.0;
+        for (int q = 0; q < m; q++) s += Hw[kp[i] * ld + mp[q]] * V[q * ld + j];
+        T[i * ld + j] = s;
+    }
+    for (int i = tid; i < m; i += nt) {
+        double s = 0.0;
+        for (int q = 0; q < m; q++) s += V[q * ld + i] * bw[mp[q]];
+        vb[i] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < nk * m; e += nt) {
+        const int i = e / m, j = e % m;
+        const double l = A[j * ld + j];
+        Wm[i * ld + j] = (l > floor_) ? T[i * ld + j] / l : 0.0;
+    }
+    __syncthreads();
+    // H2 = H_kk - Wm T^T ; b2 = b_k - Wm (V^T b_m)
+    for (int e = tid; e < nk * nk; e += nt) {
+        const int i = e / nk, j = e % nk;
+        double s = 0.0;
+        for (int q = 0; q < m; q++) s += Wm[i * ld + q] * T[j * ld + q];
+        Ho[i * ld + j] = Hw[kp[i] * ld + kp[j]] - s;
+    }
+    for (int i = tid; i < nk; i += nt) {
+        double s = 0.0;
+        for (int q = 0; q < m; q++) s += Wm[i * ld + q] * vb[q];
+        bo[i] = bw[kp[i]] - s;
+    }
+    __syncthreads();
+    for (int e = tid; e < nk * nk; e += nt) {  // symmetrise (each pair once)
+        const int i = e / nk, j = e % nk;
+        if (i < j) {
+            const double v = 0.5 * (Ho[i * ld + j] + Ho[j * ld + i]);
+            Ho[i * ld + j] = v;
+            Ho[j * ld + i] = v;
+        }
+    }
+}
+
+// marg.py:73-84 (and 212-242 when blocks are given): S, r of (H, b) over eigenvalues above the floor.
+// blocks (optional): per matrix a range [blk0[w], blk0[w+1]) of blocks, each with its offset and dims in the matrix
+// and the current state and first linearisation point (R 9, p, v, bg, ba: 21 FP64 each).
+__global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const int32_t *nn, const double *H,
+                                                                 const double *b, const int32_t *blk0,
+                                                                 const int32_t *boff, const int32_t *bdim,
+                                                                 const double *bcur, const double *bx0, double *S,
+                                                                 double *r, int32_t *rank, double *lam,
+                                                                 double *bx_out, double *work) {
+    __shared__ double cs[3 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ double red[MG_THREADS / XBA_WAVE];
+    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int n = nn[w];
+    const size_t L2 = (size_t)ld * ld;
+    const double *Hw = H + L2 * w, *bw = b + (size_t)ld * w;
+    double *A = work + MG_WORK(ld) * w, *V = A + L2, *bx = V + 2 * L2, *d = bx + ld;  // bx: b at x0
+    double *Sw = S + L2 * w, *rw = r + (size_t)ld * w;
+    for (int e = tid; e < n * n; e += nt) {
+        const int i = e / n, j = e % n;
+        A[i * ld + j] = 0.5 * (Hw[i * ld + j] + Hw[j * ld + i]);
+    }
+    // b at the first linearisation points: b - H d, d = x_cur [-] x0 per block (Log(R0^T R), p - p0, ...)
+    for (int i = tid; i < n; i += nt) d[i] = 0.0;
+    __syncthreads();
+    if (blk0)
+        for (int k = blk0[w] + tid; k < blk0[w + 1]; k += nt) {
+            const double *xc = bcur + 21 * (size_t)k, *x0 = bx0 + 21 * (size_t)k;
+            double M[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    M[3 * i + j] = x0[i] * xc[j] + x0[3 + i] * xc[3 + j] + x0[6 + i] * xc[6 + j];  // R0^T R
+            double *dk = d + boff[k];
+            so3_log_nb(M, dk);
+            for (int q = 3; q < bdim[k]; q++) dk[q] = xc[6 + q] - x0[6 + q];  // p, v, bg, ba follow R in the record
+        }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += Hw[i * ld + j] * d[j];
+        bx[i] = bw[i] - s;
+        if (bx_out) bx_out[(size_t)ld * w + i] = bx[i];
+    }
+    __syncthreads();
+    if (n > 0) mg_jacobi(A, V, n, ld, cs, red);
+    double lmax = -INFINITY;
+    for (int i = 0; i < n; i++) lmax = fmax(lmax, A[i * ld + i]);
+    const double floor_ = MG_EIG_FLOOR * fmax(lmax, 1e-300);
+    int nkeep = 0;
+    for (int i = 0; i < n; i++) nkeep += A[i * ld + i] > floor_ ? 1 : 0;
+    // rows of S in ascending eigenvalue order among the kept ones (eigh's order); the rest of S and r zero
+    for (int e = tid; e < n * n; e += nt) Sw[(e / n) * ld + e % n] = 0.0;
+    for (int i = tid; i < n; i += nt) rw[i] = 0.0;
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        const double l = A[i * ld + i];
+        const int rk = mg_rank(A, ld, n, i);
+        if (lam) lam[(size_t)ld * w + rk] = l;
+        if (!(l > floor_)) continue;
+        const int row = rk - (n - nkeep);  // the kept eigenvalues are the nkeep largest
+        const double sl = sqrt(l);
+        double vb = 0.0;
+        for (int j = 0; j < n; j++) {
+            Sw[row * ld + j] = sl * V[j * ld + i];
+            vb += V[j * ld + i] * bx[j];
+        }
+        rw[row] = vb / sl;
+    }
+    if (tid == 0) rank[w] = nkeep;
+}
+
+}  // namespace xba
