@@ -180,6 +180,96 @@ def preintegration_config(torch, synth, log, cpu_budget=3.0, nwin=256, nkf=20, s
     return rec
 
 
+def marginalisation_config(torch, pbs, log, cpu_budget=3.0, reps=10):
+    """Marginalisation of every window's oldest keyframe at once (vio/engine_live.py:690-739 per window): the
+    visual rows of its hosted landmarks and the prior linearised at lambda = 0 (one batched linearisation), its 6
+    columns Schur'd out, the square-root prior rebuilt over the remaining blocks.  HIP events around the Schur and
+    square-root launches (the linearisation is the solver's own kernel, timed separately).  CPU baseline: the numpy
+    oracle of vio/marg.py (schur + from_info) with one BLAS thread on the first windows."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import oracle as O
+    from xclean_amd import marg
+    from xclean_amd.window import BatchSolver
+    qs = []
+    for pb in pbs:
+        q = pb.copy()
+        q.lm_mask = np.asarray(pb.lm_host) == 0
+        qs.append(q)
+    bs = BatchSolver(qs, precision="fp64", use_graph=False)
+    bs.upload(0.0)
+    torch.cuda.synchronize(bs.dev)
+    t0 = time.perf_counter()
+    lin = bs.linearize(0.0, want_chi2=False, want_marginals=False)
+    torch.cuda.synchronize(bs.dev)
+    lin_ms = (time.perf_counter() - t0) * 1e3  # the batched linearisation and the copy of every H to the host
+    Hs, bvs, keeps, margs, ns = [], [], [], [], []
+    for q, r in zip(qs, lin):
+        H, b = r["H"], r["b"]
+        P = q.P
+        # the engine keeps the blocks with information (vio/engine_live.py:700-705)
+        keep = np.array([c for i in range(1, P) if np.abs(np.diag(H)[6 * i:6 * i + 6]).sum() > 0
+                         for c in range(6 * i, 6 * i + 6)], np.int64)
+        Hs.append(H)
+        bvs.append(b)
+        keeps.append(keep)
+        margs.append(np.arange(6))
+        ns.append(len(keep))
+    dev = bs.dev
+    ld = max(H.shape[0] for H in Hs)
+    Hd = marg._slab(Hs, ld, torch, dev)
+    bd = marg._vecs(bvs, ld, torch, dev)
+    kp = marg._vecs(keeps, ld, torch, dev, np.int32)
+    mp = marg._vecs(margs, ld, torch, dev, np.int32)
+    nk = torch.as_tensor(np.array(ns, np.int32), device=dev)
+    nm = torch.full((len(Hs),), 6, dtype=torch.int32, device=dev)
+    work = torch.empty(len(Hs) * (3 * ld * ld + 2 * ld), dtype=torch.float64, device=dev)
+    for _ in range(2):
+        H2, b2 = marg.schur_dev(Hd, bd, kp, mp, nk, nm, work)
+        S, r, rank, lam, _ = marg.from_info_dev(H2, b2, nk, work=work)
+    torch.cuda.synchronize(dev)
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ts, tf = 0.0, 0.0
+    for _ in range(reps):
+        e[0].record()
+        H2, b2 = marg.schur_dev(Hd, bd, kp, mp, nk, nm, work)
+        e[1].record()
+        S, r, rank, lam, _ = marg.from_info_dev(H2, b2, nk, work=work)
+        e[2].record()
+        torch.cuda.synchronize(dev)
+        ts += e[0].elapsed_time(e[1])
+        tf += e[1].elapsed_time(e[2])
+    ts /= reps
+    tf /= reps
+    # parity spot check and the CPU baseline
+    H2h, b2h, Sh, rh, kh = (x.cpu().numpy() for x in (H2, b2, S, r, rank))
+    err = 0.0
+    tc0 = time.perf_counter()
+    done = 0
+    with threadpool_limits(limits=1):
+        while done < len(Hs) and (done < 8 or time.perf_counter() - tc0 < cpu_budget):
+            W2, w2 = O.marg_schur(Hs[done], bvs[done], keeps[done], margs[done])
+            Sr, rr = O.prior_from_info(W2, w2)
+            if done < 8:
+                n = ns[done]
+                Sg, rg = Sh[done, :kh[done], :n], rh[done, :kh[done]]
+                err = max(err, float(np.abs(Sg.T @ Sg - Sr.T @ Sr).max() / np.abs(Sr.T @ Sr).max()),
+                          float(np.abs(H2h[done, :n, :n] - W2).max() / np.abs(W2).max()))
+            done += 1
+    tc = time.perf_counter() - tc0
+    rec = {"workload": "oldest keyframe of each of the %d batch windows: hosted landmarks' rows + prior at lambda = 0, "
+                       "6 columns Schur'd out, square-root prior over the blocks with information (n %d..%d)"
+                       % (len(Hs), min(ns), max(ns)),
+           "windows": len(Hs), "n_max": int(max(ns)), "linearise_and_copy_ms": lin_ms, "schur_ms": ts, "sqrt_ms": tf,
+           "windows_per_s": len(Hs) / ((ts + tf) * 1e-3), "max_rel_err_vs_oracle_first8": err,
+           "cpu_baseline": {"value": done / tc, "unit": "windows/s", "cores": 1, "kind": "port",
+                            "sample": "first %d windows, numpy oracle of vio/marg.py (schur + from_info), one BLAS "
+                                      "thread" % done}}
+    log("bench: marginalisation: %d windows, Schur %.3f ms + square root %.3f ms (%.3e windows/s); cpu oracle %.3e "
+        "windows/s; max rel err %.1e" % (len(Hs), ts, tf, rec["windows_per_s"], rec["cpu_baseline"]["value"], err))
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +282,8 @@ def main():
                     help="also run a plain 1 GiB device copy (three launches) for HBM counter calibration")
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner (solver defaults)")
     ap.add_argument("--no-extra", action="store_true", help="skip configurations 1, 3 and 4")
+    ap.add_argument("--marg-only", action="store_true",
+                    help="diagnostic: run only the batched marginalisation and print its JSON")
     ap.add_argument("--preint-only", action="store_true",
                     help="diagnostic: run only the batched IMU preintegration and print its JSON")
     ap.add_argument("--configs-only", action="store_true",
@@ -215,6 +307,10 @@ def main():
         return
     if args.preint_only:
         print(json.dumps(preintegration_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)))
+        return
+    if args.marg_only:
+        print(json.dumps(marginalisation_config(torch, synth.make_batch(args.windows, seed=0), log,
+                                                cpu_budget=0.0 if args.no_cpu else 3.0)))
         return
 
     t0 = time.time()
@@ -314,6 +410,8 @@ def main():
     diff = None if args.no_extra else differentiation_config(synth, log)
     preint = None if args.no_extra else preintegration_config(torch, synth, log,
                                                               cpu_budget=0.0 if args.no_cpu else 3.0)
+    margin = None if args.no_extra else marginalisation_config(torch, pbs, log,
+                                                               cpu_budget=0.0 if args.no_cpu else 3.0)
 
     cpu = None
     if not args.no_cpu:
@@ -377,6 +475,7 @@ def main():
         "single_window_configs": extra,
         "differentiation": diff,
         "preintegration": preint,
+        "marginalisation": margin,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -10,15 +10,17 @@
 // One workgroup owns one matrix.  The symmetric eigen-decomposition is a two-sided cyclic Jacobi with the
 // round-robin parallel ordering: each round rotates n/2 disjoint index pairs at once (rotation angles from the
 // current 2x2 blocks, then the row pass, then the column pass and the eigenvector update, each spread over the whole
-// workgroup); sweeps repeat until the off-diagonal mass is below 1e-30 of the matrix's (squared Frobenius) mass, or
-// 40 sweeps.  Matrices and eigenvectors live in a global workspace (n^2 FP64 each, L2-resident at these sizes).
+// workgroup); a pair is rotated only while its off-diagonal entry exceeds both 1e-15 of the geometric mean of its
+// diagonal entries and 1e-15 of the matrix's Frobenius norm, and sweeps repeat until one rotates nothing (or 40).  The matrix being diagonalised lives in LDS when the batch's leading dimension is at most MG_LDS_LD
+// (every round reads and writes it twice), else in the global workspace; the eigenvectors always in the workspace.
 #include "xba_internal.h"
 
 namespace xba {
 
-#define MG_THREADS 256
+#define MG_THREADS 1024
 #define MG_EIG_FLOOR 1e-12  // marg.py:32 EIG_FLOOR, relative to the largest eigenvalue
 #define MG_MAX_SWEEPS 40
+#define MG_LDS_LD 128  // the largest leading dimension whose FP64 matrix (128 KiB) is kept in LDS
 
 __device__ inline double mg_block_sum(double v, double *red) {
     const int tid = threadIdx.x;
@@ -34,23 +36,27 @@ __device__ inline double mg_block_sum(double v, double *red) {
 
 // Eigen-decomposition of the symmetric n x n matrix A (leading dimension ld, overwritten: its diagonal ends as the
 // eigenvalues), eigenvectors into the columns of V.  cs: LDS for n/2 + 1 (c, s) pairs; red: LDS for the sums.
-__device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, double *red) {
+// Always inlined, so that every call site's pointers keep their address space (LDS or global) and the passes use
+// LDS or global instructions rather than generic (flat) ones.
+__device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, double *V, int n, int ld, double *cs,
+                                                                double *red) {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int e = tid; e < n * n; e += nt) V[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
     const int N = n + (n & 1);  // players of the round-robin; N - 1 is a dummy when n is odd
     const int np = N / 2;
-    __syncthreads();
+    __shared__ int rotated;
+    double fro = 0.0;
+    for (int e = tid; e < n * n; e += nt) {
+        const double v = A[(e / n) * ld + e % n];
+        fro += v * v;
+    }
+    const double tiny = 1e-15 * sqrt(mg_block_sum(fro, red));
     for (int sweep = 0; sweep < MG_MAX_SWEEPS; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int e = tid; e < n * n; e += nt) {
-            const int i = e / n, j = e % n;
-            const double v = A[i * ld + j];
-            tot += v * v;
-            if (i != j) off += v * v;
-        }
-        off = mg_block_sum(off, red);
-        tot = mg_block_sum(tot, red);
-        if (!(off > 1e-30 * tot)) break;
+        // a pair is left alone when its off-diagonal entry is below 1e-15 of the geometric mean of its two diagonal
+        // entries (the relative criterion) or below 1e-15 of the matrix's Frobenius norm (the rounding level of the
+        // rotations, so that sweeps terminate on semi-definite matrices); a sweep that rotates nothing ends it
+        if (tid == 0) rotated = 0;
+        __syncthreads();
         for (int rd = 0; rd < N - 1; rd++) {
             // pairs of this round: (N - 1, rd) and ((rd + k) mod (N - 1), (rd - k) mod (N - 1)), k = 1 .. np - 1
             for (int k = tid; k < np; k += nt) {
@@ -58,12 +64,13 @@ __device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, doubl
                 if (p > q) { const int x = p; p = q; q = x; }
                 double c = 1.0, s = 0.0;
                 if (q < n) {
-                    const double apq = A[p * ld + q];
-                    if (apq != 0.0) {
-                        const double th = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
+                    const double apq = A[p * ld + q], app = A[p * ld + p], aqq = A[q * ld + q];
+                    if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > tiny) {
+                        const double th = (aqq - app) / (2.0 * apq);
                         const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                         c = 1.0 / sqrt(t * t + 1.0);
                         s = t * c;
+                        rotated = 1;
                     }
                 }
                 cs[3 * k] = c;
@@ -72,6 +79,7 @@ __device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, doubl
             }
             __syncthreads();
             // rows: A <- J^T A
+#pragma unroll 4
             for (int e = tid; e < np * n; e += nt) {
                 const int k = e / n, j = e % n;
                 const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
@@ -82,9 +90,10 @@ __device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, doubl
                 A[q * ld + j] = s * ap + c * aq;
             }
             __syncthreads();
-            // columns: A <- A J, V <- V J
+            // columns: A <- A J, V <- V J (consecutive threads on one row: its entries share cache lines)
+#pragma unroll 4
             for (int e = tid; e < np * n; e += nt) {
-                const int k = e / n, i = e % n;
+                const int i = e / np, k = e % np;
                 const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
                 if (q >= n) continue;
                 const double c = cs[3 * k], s = cs[3 * k + 1];
@@ -98,6 +107,8 @@ __device__ void mg_jacobi(double *A, double *V, int n, int ld, double *cs, doubl
             }
             __syncthreads();
         }
+        if (!rotated) break;
+        __syncthreads();
     }
 }
 
@@ -190,11 +201,13 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
                                                                  double *bx_out, double *work) {
     __shared__ double cs[3 * (XBA_MAX_STATE / 2 + 1)];
     __shared__ double red[MG_THREADS / XBA_WAVE];
+    extern __shared__ double mg_dyn[];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int n = nn[w];
     const size_t L2 = (size_t)ld * ld;
     const double *Hw = H + L2 * w, *bw = b + (size_t)ld * w;
-    double *A = work + MG_WORK(ld) * w, *V = A + L2, *bx = V + 2 * L2, *d = bx + ld;  // bx: b at x0
+    double *Ag = work + MG_WORK(ld) * w, *V = Ag + L2, *bx = V + 2 * L2, *d = bx + ld;  // bx: b at x0
+    double *A = ld <= MG_LDS_LD ? mg_dyn : Ag;  // the matrix being diagonalised
     double *Sw = S + L2 * w, *rw = r + (size_t)ld * w;
     for (int e = tid; e < n * n; e += nt) {
         const int i = e / n, j = e % n;
@@ -222,7 +235,12 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
         if (bx_out) bx_out[(size_t)ld * w + i] = bx[i];
     }
     __syncthreads();
-    if (n > 0) mg_jacobi(A, V, n, ld, cs, red);
+    if (n > 0) {
+        if (ld <= MG_LDS_LD)
+            mg_jacobi(mg_dyn, V, n, ld, cs, red);
+        else
+            mg_jacobi(Ag, V, n, ld, cs, red);
+    }
     double lmax = -INFINITY;
     for (int i = 0; i < n; i++) lmax = fmax(lmax, A[i * ld + i]);
     const double floor_ = MG_EIG_FLOOR * fmax(lmax, 1e-300);

@@ -85,15 +85,47 @@ def schur_batch(Hs, bs, keeps, margs):
     mp = _vecs(margs, ld, torch, dev, np.int32)
     nk = torch.as_tensor(np.array([len(k) for k in keeps], np.int32), device=dev)
     nm = torch.as_tensor(np.array([len(m) for m in margs], np.int32), device=dev)
-    H2 = torch.zeros_like(H)
-    b2 = torch.zeros_like(b)
-    work = torch.empty(nb * (3 * ld * ld + 2 * ld), dtype=torch.float64, device=dev)
-    q = XbaSchurBatch(nb, ld, nk.data_ptr(), nm.data_ptr(), kp.data_ptr(), mp.data_ptr(), H.data_ptr(), b.data_ptr(),
-                      H2.data_ptr(), b2.data_ptr(), work.data_ptr())
-    st = torch.cuda.current_stream(dev).cuda_stream
-    _lib.check(_lib.load().xba_marg_schur(C.cast(C.byref(q), C.c_void_p), C.c_void_p(st)), "xba_marg_schur")
+    H2, b2 = schur_dev(H, b, kp, mp, nk, nm)
     H2h, b2h = H2.cpu().numpy(), b2.cpu().numpy()
     return [(H2h[w, :len(k), :len(k)].copy(), b2h[w, :len(k)].copy()) for w, k in enumerate(keeps)]
+
+
+def schur_dev(H, b, kp, mp, nk, nm, work=None):
+    """One `xba_marg_schur` launch on device tensors: H (nb, ld, ld), b (nb, ld) FP64, kp / mp (nb, ld) int32 index
+    lists with nk / nm (nb,) int32 entries.  Returns (H2, b2) device tensors; asynchronous on torch's current stream."""
+    import torch
+    nb, ld = int(H.shape[0]), int(H.shape[1])
+    H2 = torch.zeros_like(H)
+    b2 = torch.zeros_like(b)
+    if work is None:
+        work = torch.empty(nb * (3 * ld * ld + 2 * ld), dtype=torch.float64, device=H.device)
+    q = XbaSchurBatch(nb, ld, nk.data_ptr(), nm.data_ptr(), kp.data_ptr(), mp.data_ptr(), H.data_ptr(), b.data_ptr(),
+                      H2.data_ptr(), b2.data_ptr(), work.data_ptr())
+    st = torch.cuda.current_stream(H.device).cuda_stream
+    _lib.check(_lib.load().xba_marg_schur(C.cast(C.byref(q), C.c_void_p), C.c_void_p(st)), "xba_marg_schur")
+    return H2, b2
+
+
+def from_info_dev(H, b, n, blocks=None, want_bx=False, work=None):
+    """One `xba_prior_from_info` launch on device tensors: H (nb, ld, ld), b (nb, ld), n (nb,) int32; blocks None or
+    (blk0, boff, bdim, bcur, bx0) device tensors.  Returns (S, r, rank, lam, bx or None); asynchronous."""
+    import torch
+    nb, ld = int(H.shape[0]), int(H.shape[1])
+    S = torch.zeros_like(H)
+    r = torch.zeros_like(b)
+    rank = torch.zeros(nb, dtype=torch.int32, device=H.device)
+    lam = torch.zeros_like(b)
+    bx = torch.zeros_like(b) if want_bx else None
+    if work is None:
+        work = torch.empty(nb * (3 * ld * ld + 2 * ld), dtype=torch.float64, device=H.device)
+    ptr = lambda i: blocks[i].data_ptr() if blocks is not None else None
+    q = XbaPriorBatch(nb, ld, n.data_ptr(), H.data_ptr(), b.data_ptr(), ptr(0), ptr(1), ptr(2), ptr(3), ptr(4),
+                      S.data_ptr(), r.data_ptr(), rank.data_ptr(), lam.data_ptr(),
+                      bx.data_ptr() if want_bx else None, work.data_ptr())
+    st = torch.cuda.current_stream(H.device).cuda_stream
+    _lib.check(_lib.load().xba_prior_from_info(C.cast(C.byref(q), C.c_void_p), C.c_void_p(st)),
+               "xba_prior_from_info")
+    return S, r, rank, lam, bx
 
 
 def schur(H, b, keep, marg):
@@ -132,23 +164,11 @@ def from_info_batch(items, want_bx=False):
     with_blocks = len(boff) > 0
     T = lambda x, dt: torch.as_tensor(np.asarray(x, dt), device=dev).contiguous()
     n_t = T(ns, np.int32)
-    S = torch.zeros_like(H)
-    r = torch.zeros_like(b)
-    rank = torch.zeros(nb, dtype=torch.int32, device=dev)
-    lam = torch.zeros_like(b)
-    bx = torch.zeros_like(b) if want_bx else None
-    work = torch.empty(nb * (3 * ld * ld + 2 * ld), dtype=torch.float64, device=dev)
-    keep_alive = []
+    blocks = None
     if with_blocks:
-        keep_alive = [T(blk0, np.int32), T(boff, np.int32), T(bdim, np.int32), T(np.stack(bcur), np.float64),
-                      T(np.stack(bx0), np.float64)]
-    ptr = lambda i: keep_alive[i].data_ptr() if with_blocks else None
-    q = XbaPriorBatch(nb, ld, n_t.data_ptr(), H.data_ptr(), b.data_ptr(), ptr(0), ptr(1), ptr(2), ptr(3), ptr(4),
-                      S.data_ptr(), r.data_ptr(), rank.data_ptr(), lam.data_ptr(),
-                      bx.data_ptr() if want_bx else None, work.data_ptr())
-    st = torch.cuda.current_stream(dev).cuda_stream
-    _lib.check(_lib.load().xba_prior_from_info(C.cast(C.byref(q), C.c_void_p), C.c_void_p(st)),
-               "xba_prior_from_info")
+        blocks = (T(blk0, np.int32), T(boff, np.int32), T(bdim, np.int32), T(np.stack(bcur), np.float64),
+                  T(np.stack(bx0), np.float64))
+    S, r, rank, lam, bx = from_info_dev(H, b, n_t, blocks, want_bx)
     Sh, rh, kh, lh = S.cpu().numpy(), r.cpu().numpy(), rank.cpu().numpy(), lam.cpu().numpy()
     out = [(Sh[w, :int(kh[w]), :n].copy(), rh[w, :int(kh[w])].copy(), lh[w, :n].copy()) for w, n in enumerate(ns)]
     if want_bx:
