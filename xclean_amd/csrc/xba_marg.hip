@@ -21,6 +21,7 @@ namespace xba {
 #define MG_EIG_FLOOR 1e-12  // marg.py:32 EIG_FLOOR, relative to the largest eigenvalue
 #define MG_MAX_SWEEPS 40
 #define MG_LDS_LD 128  // the largest leading dimension whose FP64 matrix (128 KiB) is kept in LDS
+static_assert(MG_THREADS >= XBA_MAX_STATE, "the column pass gives every row at least one thread");
 
 __device__ inline double mg_block_sum(double v, double *red) {
     const int tid = threadIdx.x;
@@ -35,7 +36,8 @@ __device__ inline double mg_block_sum(double v, double *red) {
 }
 
 // Eigen-decomposition of the symmetric n x n matrix A (leading dimension ld, overwritten: its diagonal ends as the
-// eigenvalues), eigenvectors into the columns of V.  cs: LDS for n/2 + 1 (c, s) pairs; red: LDS for the sums.
+// eigenvalues), eigenvectors into the columns of V.  cs: LDS for n/2 + 1 (c, s) pairs; red: LDS for the sums.  The
+// workgroup has at least n threads (XBA_MAX_STATE <= MG_THREADS), so every row has a thread in the column pass.
 // Always inlined, so that every call site's pointers keep their address space (LDS or global) and the passes use
 // LDS or global instructions rather than generic (flat) ones.
 __device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, double *V, int n, int ld, double *cs,
@@ -45,6 +47,7 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, doubl
     const int N = n + (n & 1);  // players of the round-robin; N - 1 is a dummy when n is odd
     const int np = N / 2;
     __shared__ int rotated;
+    __shared__ int pqs[2 * (XBA_MAX_STATE / 2 + 1)];
     double fro = 0.0;
     for (int e = tid; e < n * n; e += nt) {
         const double v = A[(e / n) * ld + e % n];
@@ -72,38 +75,47 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, doubl
                         s = t * c;
                         rotated = 1;
                     }
+                } else {
+                    p = q = 0;  // the dummy player's pair: rotates nothing
                 }
-                cs[3 * k] = c;
-                cs[3 * k + 1] = s;
-                cs[3 * k + 2] = (double)(p * 65536 + q);
+                cs[2 * k] = c;
+                cs[2 * k + 1] = s;
+                pqs[2 * k] = p;
+                pqs[2 * k + 1] = q;
             }
             __syncthreads();
-            // rows: A <- J^T A
-#pragma unroll 4
-            for (int e = tid; e < np * n; e += nt) {
-                const int k = e / n, j = e % n;
-                const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
-                if (q >= n) continue;
-                const double c = cs[3 * k], s = cs[3 * k + 1];
-                const double ap = A[p * ld + j], aq = A[q * ld + j];
-                A[p * ld + j] = c * ap - s * aq;
-                A[q * ld + j] = s * ap + c * aq;
+            // rows: A <- J^T A; `per` threads per pair, each striding over the columns with its pair fixed
+            {
+                const int per = nt / np, k = tid / per, j0 = tid % per;
+                if (k < np) {
+                    const int p = pqs[2 * k], q = pqs[2 * k + 1];
+                    const double c = cs[2 * k], s = cs[2 * k + 1];
+                    if (s != 0.0)
+                        for (int j = j0; j < n; j += per) {
+                            const double ap = A[p * ld + j], aq = A[q * ld + j];
+                            A[p * ld + j] = c * ap - s * aq;
+                            A[q * ld + j] = s * ap + c * aq;
+                        }
+                }
             }
             __syncthreads();
-            // columns: A <- A J, V <- V J (consecutive threads on one row: its entries share cache lines)
-#pragma unroll 4
-            for (int e = tid; e < np * n; e += nt) {
-                const int i = e / np, k = e % np;
-                const int pq = (int)cs[3 * k + 2], p = pq / 65536, q = pq % 65536;
-                if (q >= n) continue;
-                const double c = cs[3 * k], s = cs[3 * k + 1];
-                double ap = A[i * ld + p], aq = A[i * ld + q];
-                A[i * ld + p] = c * ap - s * aq;
-                A[i * ld + q] = s * ap + c * aq;
-                ap = V[i * ld + p];
-                aq = V[i * ld + q];
-                V[i * ld + p] = c * ap - s * aq;
-                V[i * ld + q] = s * ap + c * aq;
+            // columns: A <- A J, V <- V J; `pr` threads per row, each striding over the pairs
+            {
+                const int pr = nt / n, i = tid / pr, k0 = tid % pr;
+                if (i < n)
+                    for (int k = k0; k < np; k += pr) {
+                        const double s = cs[2 * k + 1];
+                        if (s == 0.0) continue;
+                        const int p = pqs[2 * k], q = pqs[2 * k + 1];
+                        const double c = cs[2 * k];
+                        double ap = A[i * ld + p], aq = A[i * ld + q];
+                        A[i * ld + p] = c * ap - s * aq;
+                        A[i * ld + q] = s * ap + c * aq;
+                        ap = V[i * ld + p];
+                        aq = V[i * ld + q];
+                        V[i * ld + p] = c * ap - s * aq;
+                        V[i * ld + q] = s * ap + c * aq;
+                    }
             }
             __syncthreads();
         }
@@ -130,7 +142,7 @@ __device__ inline int mg_rank(const double *A, int ld, int n, int i) {
 __global__ __launch_bounds__(MG_THREADS) void k_marg_schur(int ld, const int32_t *nkeep, const int32_t *nmarg,
                                                             const int32_t *keep, const int32_t *marg, const double *H,
                                                             const double *b, double *H2, double *b2, double *work) {
-    __shared__ double cs[3 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ double cs[2 * (XBA_MAX_STATE / 2 + 1)];
     __shared__ double red[MG_THREADS / XBA_WAVE];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int nk = nkeep[w], m = nmarg[w];
@@ -199,7 +211,7 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
                                                                  const double *bcur, const double *bx0, double *S,
                                                                  double *r, int32_t *rank, double *lam,
                                                                  double *bx_out, double *work) {
-    __shared__ double cs[3 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ double cs[2 * (XBA_MAX_STATE / 2 + 1)];
     __shared__ double red[MG_THREADS / XBA_WAVE];
     extern __shared__ double mg_dyn[];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
