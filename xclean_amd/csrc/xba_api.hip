@@ -416,7 +416,14 @@ int xba_prior_from_info(const xba_prior_batch *q, void *stream) {
     if (q->nb == 0) return 0;
     if (!q->n || !q->H || !q->b || !q->S || !q->r || !q->rank || !q->work) return XBA_E_ARG;
     if (q->blk0 && (!q->boff || !q->bdim || !q->bcur || !q->bx0)) return XBA_E_ARG;
-    hipLaunchKernelGGL(k_prior_from_info, dim3(q->nb), dim3(MG_THREADS), 0, (hipStream_t)stream, q->ld, q->n, q->H,
+    const size_t shm = q->ld <= MG_LDS_LD ? (size_t)q->ld * q->ld * sizeof(double) : 0;
+    static bool attr = false;  // the LDS limit of the kernel, raised once per process
+    if (!attr) {
+        HIPCK(hipFuncSetAttribute((const void *)k_prior_from_info, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  MG_LDS_LD * MG_LDS_LD * (int)sizeof(double)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_prior_from_info, dim3(q->nb), dim3(MG_THREADS), shm, (hipStream_t)stream, q->ld, q->n, q->H,
                        q->b, q->blk0, q->boff, q->bdim, q->bcur, q->bx0, q->S, q->r, q->rank, q->lam, q->bx, q->work);
     HIPCK(hipGetLastError());
     return 0;
