@@ -98,6 +98,7 @@ struct xba_ctx {
     int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
     int lin_lds = 0, solve_lds_cap = 0, fast_nw = 8, fast_nw_trial = 8, fast_lds = 0, fast_lds_trial = 0;
     int nnarrow = 0, Rnarrow = 1, Rwide = 1, wide_nw = 8, wide_nw_trial = 8, wide_lds = 0, wide_lds_trial = 0, grid_wide = 1;
+    int grid_trial = 1, grid_wide_trial = 1;  // the trial kernel runs up to XBA_TRIAL_PER_CU workgroups per CU
     int npass = 0;
     int32_t *pass_l0 = nullptr, *sched = nullptr, *sched_ctr = nullptr;
     double *pass_cost = nullptr;
@@ -293,10 +294,10 @@ void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part 
     if (c->fast) {
         const KArgs<double> F = fast_args<T>(c, A);
         if (c->nnarrow > 0 && part != 1)
-            hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw_trial), c->fast_lds_trial,
-                               s, F, c->Rnarrow, 0, c->nnarrow, 2);
+            hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid_trial), dim3(XBA_WAVE * c->fast_nw_trial),
+                               c->fast_lds_trial, s, F, c->Rnarrow, 0, c->nnarrow, 2);
         if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw_trial),
+            hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide_trial), dim3(XBA_WAVE * c->wide_nw_trial),
                                c->wide_lds_trial, s, F, c->Rwide, c->nnarrow, c->nchunks, 3);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
@@ -844,11 +845,16 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         if (!pick(c->Rwide, c->wide_nw, c->wide_lds, c->wide_nw_trial, c->wide_lds_trial)) return fail(XBA_E_RANGE);
         per_cu = 1;
         c->grid_wide = std::max(1, std::min(c->nchunks - c->nnarrow, ncu));
+        // k_trial_fast is compiled for 4 waves per SIMD: two 8-wave workgroups per CU where the LDS holds both
+        auto tpc = [&](int lds) { return std::max(1, std::min(XBA_TRIAL_PER_CU, lds_max / std::max(lds, 1))); };
+        c->grid_wide_trial = std::max(1, std::min(c->nchunks - c->nnarrow, ncu * tpc(c->wide_lds_trial)));
+        c->grid_trial = std::max(1, std::min(c->nnarrow, ncu * tpc(c->fast_lds_trial)));
     } else {
         per_cu = std::max(1, std::min(8, lds_max / std::max(c->lin_lds, 1)));
     }
     c->grid = opt->grid > 0 ? opt->grid : std::max(1, std::min(c->fast ? c->nnarrow : c->nchunks, ncu * per_cu));
     if (c->nchunks == 0) c->grid = 1;
+    if (opt->grid > 0) c->grid_trial = opt->grid;
     // ---- device allocations
     auto &O = c->owned;
     c->chunks = dalloc<Chunk>(chunks.size(), O);

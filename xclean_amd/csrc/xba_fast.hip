@@ -36,7 +36,7 @@ enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
 struct FastLayout {
     int sc, sc64, wc, wd, tile, btile, ctl, wave, wave_stride, total;
     // offsets inside one wavefront's region
-    int o_oblm, o_obt, o_obact, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
+    int o_oblm, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
         o_lrow, o_lh, o_kown, o_ptab;
 };
 
@@ -46,7 +46,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     int o = 0;
     const int nc = trial ? 2 : 1, ts = 8;
     L.sc = o;    o = align16(o + nc * R * SC_STRIDE * ts);
-    L.sc64 = o;  o = align16(o + nc * R * SC64_STRIDE * 8);
+    L.sc64 = o;  o = align16(o + (trial ? 0 : R * SC64_STRIDE * 8));  // the trial reads FP64 poses from sc
     L.wc = o;    o = align16(o + 32 * ts);
     L.wd = o;    o = align16(o + 8 * 8);
     L.tile = o;  o = align16(o + (trial ? 6 * R * ts : (R * (R + 1) / 2) * TBLK * 8));  // trial: dp per slot
@@ -55,8 +55,6 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
-    L.o_obt = w;   w = align16(w + (trial ? FB_OBS * 4 : 0));
-    L.o_obact = w; w = align16(w + (trial ? FB_OBS * 4 : 0));
     L.o_rcq = w;   w = align16(w + FB_OBS * 8);
     L.o_u = w;     w = align16(w + (trial ? FB_OBS * 3 * ts : 0));
     L.o_lmask = w; w = align16(w + FB_LM * 4);
@@ -66,7 +64,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.o_poff = w;  w = align16(w + FB_LM * 4);
     L.o_slots = w; w = align16(w + (trial ? 0 : FB_HLP * 4));
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
-    L.o_lrec = w;  w = align16(w + FB_LM * FB_LREC * ts);
+    L.o_lrec = w;  w = align16(w + (trial ? 0 : FB_LM * FB_LREC * ts));
     L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
     L.o_lrow = w;  w = align16(w + (trial ? 0 : FB_LM * LA_N * 8));     // per landmark: Hll, bl sums, active count
     L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
@@ -77,18 +75,43 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     return L;
 }
 
+// Wavefront scans and sums with DPP lane moves (VALU only: no LDS round trip and no per-lane shuffle addresses held in
+// registers): lane i adds lane i - d for d = 1, 2, 4, 8 inside its row of 16 lanes, then the row-15 broadcast adds
+// row 0's (row 2's) total into row 1 (row 3) and the lane-31 broadcast adds the first half's total into rows 2 and 3.
+template <int CTRL, int ROWS>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
+}
+
+// Inclusive prefix sum over the wavefront.
 __device__ inline int wave_incl_scan(int v, int lane) {
-    for (int d = 1; d < XBA_WAVE; d <<= 1) {
-        int t = __shfl_up(v, d, XBA_WAVE);
-        if (lane >= d) v += t;
-    }
+    const int r = lane & 15, row = lane >> 4;
+    int t;
+    t = dpp_i<0x111, 0xf>(v); if (r >= 1) v += t;   // row_shr:1
+    t = dpp_i<0x112, 0xf>(v); if (r >= 2) v += t;   // row_shr:2
+    t = dpp_i<0x114, 0xf>(v); if (r >= 4) v += t;   // row_shr:4
+    t = dpp_i<0x118, 0xf>(v); if (r >= 8) v += t;   // row_shr:8
+    t = dpp_i<0x142, 0xa>(v); if (row & 1) v += t;  // row_bcast:15 into rows 1, 3
+    t = dpp_i<0x143, 0xc>(v); if (row >= 2) v += t; // row_bcast:31 into rows 2, 3
     return v;
 }
 
-// Sum over the wavefront in a fixed order (a butterfly), the result in every lane.
-__device__ inline double wave_sum(double v) {
-    for (int d = XBA_WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, XBA_WAVE);
-    return v;
+template <int CTRL, int ROWS>
+__device__ inline double dpp_d(double v);
+
+// Sum over the wavefront in a fixed order (the scan above, lane 63's total), the result in every lane.
+__device__ inline double wave_sum(double v, int lane) {
+    const int r = lane & 15, row = lane >> 4;
+    double t;
+    t = dpp_d<0x111, 0xf>(v); if (r >= 1) v += t;
+    t = dpp_d<0x112, 0xf>(v); if (r >= 2) v += t;
+    t = dpp_d<0x114, 0xf>(v); if (r >= 4) v += t;
+    t = dpp_d<0x118, 0xf>(v); if (r >= 8) v += t;
+    t = dpp_d<0x142, 0xa>(v); if (row & 1) v += t;
+    t = dpp_d<0x143, 0xc>(v); if (row >= 2) v += t;
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 template <typename T>
@@ -111,8 +134,8 @@ __device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane
         lnobs[lane] = n;
         for (int q = 0; q < n; q++) ob_lm[S - n + q] = lane;
     }
-    *base = __shfl(o0, 0, XBA_WAVE);  // the pass's observation rows are contiguous from here
-    return __shfl(S, B - 1, XBA_WAVE);
+    *base = __builtin_amdgcn_readlane(o0, 0);  // the pass's observation rows are contiguous from here
+    return __builtin_amdgcn_readlane(S, B - 1);
 }
 
 // Segmented sums over consecutive lanes (a segment: one landmark's observation rows), accumulated into the
@@ -192,7 +215,7 @@ __device__ inline int next_chunk(const KArgs<double> &A, int kind, int cbeg, int
 __device__ inline int next_pass(int *ctl, int lane) {
     int p = 0;
     if (lane == 0) p = __hip_atomic_fetch_add(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    return __shfl(p, 0, XBA_WAVE);
+    return __builtin_amdgcn_readlane(p, 0);
 }
 
 // ------------------------------------------------------------------------------------------ k_lin_fast --
@@ -238,16 +261,17 @@ __device__ inline double obs_row(const KArgs<double> &A, int k, int h, double a,
     double w = 0.0, c = 0.0, en2 = 0.0;
     if (act) {
         const double isg = A.isig[k];
-        for (int i = 0; i < nd; i++) {
+        // fixed trip counts (rows past nd hold zeros): the arrays stay in registers
+        for (int i = 0; i < 2; i++) {
             e[i] *= isg;
-            en2 += e[i] * e[i];
+            if (i < nd) en2 += e[i] * e[i];
         }
         huber(sqrt(en2), wd[5], &w, &c);
         if (A.ob_w) {
             w *= A.ob_w[k];
             c *= A.ob_w[k];
         }
-        for (int r = 0; r < nd; r++) {
+        for (int r = 0; r < 2; r++) {
             for (int j = 0; j < 3; j++) Jl[r][j] *= isg;
             for (int j = 0; j < 6; j++) {
                 Jt[r][j] *= isg;
@@ -515,10 +539,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 }
             }
             const int koff = wave_incl_scan(kb, lane) - kb;
-            const int ktot = __shfl(koff + kb, B - 1, XBA_WAVE);
+            const int ktot = __builtin_amdgcn_readlane(koff + kb, B - 1);
             const int npb = kb * (kb + 1) / 2;
             const int pof = wave_incl_scan(npb, lane) - npb;
-            const int ptot = __shfl(pof + npb, B - 1, XBA_WAVE);
+            const int ptot = __builtin_amdgcn_readlane(pof + npb, B - 1);
             if (lane < B) {
                 poff[lane] = pof;
                 loff[lane] = koff;
@@ -533,7 +557,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 for (int si = 0, o = pof; si < kb; si++)
                     for (int sj = 0; sj <= si; sj++, o++) ptab[o] = (uint16_t)((lane << 10) | (si << 5) | sj);
             }
-            const double pcost = wave_sum(lcost);
+            const double pcost = wave_sum(lcost, lane);
             if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
             // ---- U = L^-1 Hlp, one lane per observation: its two 3x6 terms G Jt and G Jh (G = w L^-1 Jl^T); the
@@ -632,14 +656,20 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
 }
 
 // ---------------------------------------------------------------------------------------- k_trial_fast --
+// Latency-bound (LDS reads, wave-local syncs, a few FP64 divides per observation): compiled for 4 waves per SIMD
+// (at most 128 VGPRs), so that two 8-wave workgroups share a CU.
+#define XBA_TRIAL_WAVES_PER_SIMD 4
+#define XBA_TRIAL_PER_CU 2
 template <int CLS>
-__global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, int cbeg, int cend, int kind) {
+__global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KArgs<double> A, int Rmax, int cbeg, int cend, int kind) {
     using T = double;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NW = blockDim.x / XBA_WAVE;
     const FastLayout FL = fast_layout(Rmax, 8, NW, true);
     T *sc = (T *)(smem + FL.sc), *wc = (T *)(smem + FL.wc), *dps = (T *)(smem + FL.tile);
-    double *sc64 = (double *)(smem + FL.sc64), *wd = (double *)(smem + FL.wd);
+    // In FP64 the slot records sc hold the camera poses (their first 12 entries) as the linearisation's FP64 copies
+    // do, and wc + 18 holds fx fy cx cy B huber_k: the observations read those, one copy of each value.
+    const double *wdc = wc + 18;
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
     int *ob_lm = (int *)(wb + FL.o_oblm);
@@ -661,19 +691,19 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;
         const int R = ch.R;
         T *sc2 = sc + R * SC_STRIDE;
-        double *sc642 = sc64 + R * SC64_STRIDE;
         if (wv == 0) {
-            load_slots<T>(sc, sc64, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane, smap);
+            load_slots<T>(sc, nullptr, A.Rwb, A.pwb, wp, W.slot0, ch.lo, R, lane, smap);
             load_wconst<T>(wc, wp, lane);
-            load_wd(wd, wp, lane);
         } else if (wv == 1 % NW) {
-            load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
+            load_slots<T>(sc2, nullptr, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
         }
-        if (NW == 1) load_slots<T>(sc2, sc642, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
+        if (NW == 1) load_slots<T>(sc2, nullptr, A.R2, A.p2, wp, W.slot0, ch.lo, R, lane, smap);
         for (int i = tid; i < 6 * R; i += blockDim.x) {
             int s = i / 6, j = i % 6;
             dps[i] = (T)A.dx[W.st0 + A.slot_off[W.slot0 + chunk_slot(ch, A.chunk_slots, s)] + j];
         }
+        __syncthreads();
+        for (int s = tid; s < R; s += blockDim.x) dir_slot<T>(dps + 6 * s, sc + s * SC_STRIDE, wc);
         __syncthreads();
         for (;;) {
             const int p = next_pass(ctl, lane);
@@ -702,22 +732,20 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                     const double a = A.lmp[3 * (size_t)q_l], b = A.lmp[3 * (size_t)q_l + 1],
                                  rho = A.lmp[3 * (size_t)q_l + 2];
                     double e[2];
-                    T Jl[2][3], Jt[2][6], Jh[2][6];
-                    if (obs_eval<T>(sc64 + q_h * SC64_STRIDE, sc64 + q_t * SC64_STRIDE, sc + q_h * SC_STRIDE,
-                                    sc + q_t * SC_STRIDE, wc, wd, a, b, rho, q_cam, A.z[2 * (size_t)q_k],
-                                    A.z[2 * (size_t)q_k + 1], false, true, e, Jl, Jt, Jh)) {
+                    T Jl[2][3], jdv[2];
+                    if (obs_dir(sc + q_h * SC_STRIDE, sc + q_t * SC_STRIDE, wdc, a, b, rho, q_cam, A.z[2 * (size_t)q_k],
+                                A.z[2 * (size_t)q_k + 1], dps + 6 * q_t, dps + 6 * q_h, e, Jl, jdv)) {
                         const int nd = q_cam == 0 ? 2 : 1;
                         const double isg = (double)A.isig[q_k];
                         double en2 = 0.0, w, cst;
-                        for (int i = 0; i < nd; i++) en2 += (e[i] * isg) * (e[i] * isg);
-                        huber(sqrt(en2), wd[5], &w, &cst);
+                        for (int i = 0; i < 2; i++)
+                            if (i < nd) en2 += (e[i] * isg) * (e[i] * isg);
+                        huber(sqrt(en2), wdc[5], &w, &cst);
                         if (A.ob_w) w *= A.ob_w[q_k];
                         const T isgT = A.isig[q_k];
                         T s[3] = {0, 0, 0};
-                        for (int r = 0; r < nd; r++) {
-                            T jd = 0;
-                            for (int j = 0; j < 6; j++) jd += Jt[r][j] * dps[6 * q_t + j] + Jh[r][j] * dps[6 * q_h + j];
-                            jd *= isgT * isgT;
+                        for (int r = 0; r < 2; r++) {
+                            const T jd = r < nd ? jdv[r] * (isgT * isgT) : (T)0;
                             for (int i = 0; i < 3; i++) s[i] += Jl[r][i] * jd;
                         }
                         u0 = (T)w * s[0];
@@ -768,7 +796,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                     const T ris = A.rho_isig[l];
                     if (ris > (T)0) {
                         double r0 = (nr - A.rho_meas[l]) * (double)ris, w, cc;
-                        huber(fabs(r0), wd[5], &w, &cc);
+                        huber(fabs(r0), wdc[5], &w, &cc);
                         lcost += cc;
                     }
                 }
@@ -781,8 +809,8 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                     const int qb = ob_lm[lane];
                     double e[2];
                     T Jl[2][3], Jt[2][6], Jh[2][6];
-                    bool act = obs_eval<T>(sc642 + q_h * SC64_STRIDE, sc642 + q_t * SC64_STRIDE, sc2 + q_h * SC_STRIDE,
-                                           sc2 + q_t * SC_STRIDE, wc, wd, ldbl[4 * qb], ldbl[4 * qb + 1],
+                    bool act = obs_eval<T>(sc2 + q_h * SC_STRIDE, sc2 + q_t * SC_STRIDE, sc2 + q_h * SC_STRIDE,
+                                           sc2 + q_t * SC_STRIDE, wc, wdc, ldbl[4 * qb], ldbl[4 * qb + 1],
                                            ldbl[4 * qb + 2], q_cam, A.z[2 * (size_t)q_k], A.z[2 * (size_t)q_k + 1],
                                            q_t == q_h, false, e, Jl, Jt, Jh);
                     if (!act) {
@@ -790,7 +818,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
                     } else {
                         double isg = (double)A.isig[q_k], en2 = e[0] * isg * (e[0] * isg), w;
                         if (q_cam == 0) en2 += e[1] * isg * (e[1] * isg);
-                        huber(sqrt(en2), wd[5], &w, &cq);
+                        huber(sqrt(en2), wdc[5], &w, &cq);
                         if (A.ob_w) cq *= A.ob_w[q_k];
                     }
                 }
@@ -799,7 +827,7 @@ __global__ __launch_bounds__(512) void k_trial_fast(KArgs<double> A, int Rmax, i
             WAVE_SYNC();
             if (lane < B)
                 for (int q = lobs0[lane]; q < lobs0[lane] + lnobs[lane]; q++) lcost += rcq[q];
-            const double pcost = wave_sum(lcost);
+            const double pcost = wave_sum(lcost, lane);
             if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
         }

@@ -62,7 +62,8 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int ts) {
 
 // --------------------------------------------------------------------------------------- per-slot caches --
 // Camera poses of the R tile slots of a work item of window w (slots lo + t, or map[t] when map is given):
-// Rwc = Rwb Rbc, pwc = pwb + Rwb pbc (vio/window.py:110-122), computed in FP64 and stored in the working precision.
+// Rwc = Rwb Rbc, pwc = pwb + Rwb pbc (vio/window.py:110-122), computed in FP64 and stored in the working precision
+// (sc) and, unless sc64 is null, in FP64.
 template <typename T>
 __device__ void load_slots(T *sc, double *sc64, const double *Rwb, const double *pwb, const double *wp, int slot0,
                            int lo, int R, int lane, const int32_t *map = nullptr) {
@@ -72,18 +73,18 @@ __device__ void load_slots(T *sc, double *sc64, const double *Rwb, const double 
         const double *Rb = Rwb + 9 * (size_t)ws;
         const double *pb = pwb + 3 * (size_t)ws;
         T *c = sc + s * SC_STRIDE;
-        double *c64 = sc64 + s * SC64_STRIDE;
+        double *c64 = sc64 ? sc64 + s * SC64_STRIDE : nullptr;
         for (int i = 0; i < 3; i++) {
             for (int j = 0; j < 3; j++) {
                 double acc = 0.0;
                 for (int k = 0; k < 3; k++) acc += Rb[3 * i + k] * Rbc[3 * k + j];
                 c[3 * i + j] = (T)acc;
-                c64[3 * i + j] = acc;
+                if (sc64) c64[3 * i + j] = acc;
             }
             double acc = pb[i];
             for (int k = 0; k < 3; k++) acc += Rb[3 * i + k] * pbc[k];
             c[9 + i] = (T)acc;
-            c64[9 + i] = acc;
+            if (sc64) c64[9 + i] = acc;
         }
         for (int i = 0; i < 9; i++) c[12 + i] = (T)Rb[i];
     }
@@ -204,6 +205,75 @@ __device__ inline bool obs_eval(const double *sh64, const double *st64, const T 
             Jh[r][3 + j] = rT * v;
         }
     return true;
+}
+
+// The same observation's residual, its landmark Jacobian Jl and, instead of the pose Jacobians, their product with
+// the pose steps [dphi, dp] of the target and host slots: jd = Jt [dphi_t, dp_t] + Jh [dphi_h, dp_h] (t != h), all in
+// FP64.  The camera-frame point moves by
+//     dq = [q]x Rcb dphi_t + R_ct (rho (R_wb,t [pbc]x dphi_t - dp_t - R_wb,h [pbc]x dphi_h + dp_h) - R_wc,h [m]x Rcb dphi_h),
+// obs_eval's T1 / T2 / translation columns applied to the steps, so jd = Dp dq.  Each slot's step enters through
+// v = [Rcb dphi, R_wb [pbc]x dphi - dp] (dir_slot), vt / vh of the target / host.  The world-frame vectors are
+// formed first and rotated by R_ct = R_wc,t^T last, so each pose is read once; Jl = Dp R_ct [R_wc,h e0, R_wc,h e1,
+// p_wc,h - p_wc,t] (minus the baseline in x for the right camera) is taken as (R_wc,t Dp^T)^T applied to those.
+__device__ inline bool obs_dir(const double *sh, const double *st, const double *wd, double a, double b, double rho,
+                               int cam, double z0, double z1, const double *vt, const double *vh, double e[2],
+                               double Jl[2][3], double jd[2]) {
+    const double *Rh = sh, *ph = sh + 9, *Rt = st, *pt = st + 9;
+    const double d0 = ph[0] - pt[0], d1 = ph[1] - pt[1], d2 = ph[2] - pt[2];
+    const double m0 = b * vh[2] - vh[1], m1 = vh[0] - a * vh[2], m2 = a * vh[1] - b * vh[0];
+    const double dd[3] = {d0, d1, d2};
+    double Y[3], u[3];
+    for (int i = 0; i < 3; i++) {
+        Y[i] = Rh[3 * i] * a + Rh[3 * i + 1] * b + Rh[3 * i + 2] + rho * dd[i];
+        u[i] = rho * (vt[3 + i] - vh[3 + i]) - (Rh[3 * i] * m0 + Rh[3 * i + 1] * m1 + Rh[3 * i + 2] * m2);
+    }
+    const double q0 = Rt[0] * Y[0] + Rt[3] * Y[1] + Rt[6] * Y[2];
+    const double q1 = Rt[1] * Y[0] + Rt[4] * Y[1] + Rt[7] * Y[2];
+    const double q2 = Rt[2] * Y[0] + Rt[5] * Y[1] + Rt[8] * Y[2];
+    if (!(q2 > 1e-9)) return false;
+    const double fx = wd[0], fy = wd[1], cx = wd[2], cy = wd[3], B = wd[4];
+    if (cam == 0) {
+        e[0] = fx * q0 / q2 + cx - z0;
+        e[1] = fy * q1 / q2 + cy - z1;
+    } else {
+        e[0] = fx * (q0 - rho * B) / q2 + cx - z0;
+        e[1] = 0.0;
+    }
+    const double iq = 1.0 / q2;
+    double Dp[2][3];
+    if (cam == 0) {
+        Dp[0][0] = fx * iq; Dp[0][1] = 0; Dp[0][2] = -fx * q0 * iq * iq;
+        Dp[1][0] = 0; Dp[1][1] = fy * iq; Dp[1][2] = -fy * q1 * iq * iq;
+    } else {
+        Dp[0][0] = fx * iq; Dp[0][1] = 0; Dp[0][2] = -fx * (q0 - rho * B) * iq * iq;
+        Dp[1][0] = 0; Dp[1][1] = 0; Dp[1][2] = 0;
+    }
+    // [q]x Rcb dphi_t, in the target camera
+    const double c0 = q1 * vt[2] - q2 * vt[1], c1 = q2 * vt[0] - q0 * vt[2], c2 = q0 * vt[1] - q1 * vt[0];
+    for (int r = 0; r < 2; r++) {
+        double D[3];  // R_wc,t Dp[r]^T
+        for (int k = 0; k < 3; k++) D[k] = Rt[3 * k] * Dp[r][0] + Rt[3 * k + 1] * Dp[r][1] + Rt[3 * k + 2] * Dp[r][2];
+        Jl[r][0] = D[0] * Rh[0] + D[1] * Rh[3] + D[2] * Rh[6];
+        Jl[r][1] = D[0] * Rh[1] + D[1] * Rh[4] + D[2] * Rh[7];
+        Jl[r][2] = D[0] * d0 + D[1] * d1 + D[2] * d2 - (cam == 1 ? Dp[r][0] * B : 0.0);
+        jd[r] = Dp[r][0] * c0 + Dp[r][1] * c1 + Dp[r][2] * c2 + D[0] * u[0] + D[1] * u[1] + D[2] * u[2];
+    }
+    return true;
+}
+
+// One slot's step [dphi, dp] (in d, overwritten) as obs_dir reads it: [Rcb dphi, R_wb [pbc]x dphi - dp], with the
+// slot record c (R_wb at c + 12) and the window constants wc (Rcb at wc, [pbc]x row-major at wc + 9).
+template <typename T>
+__device__ inline void dir_slot(T *d, const T *c, const T *wc) {
+    const T *Rcb = wc, *P = wc + 9, *Rwb = c + 12;
+    const T f0 = d[0], f1 = d[1], f2 = d[2], t0 = d[3], t1 = d[4], t2 = d[5];
+    const T x0 = P[1] * f1 + P[2] * f2, x1 = P[3] * f0 + P[5] * f2, x2 = P[6] * f0 + P[7] * f1;
+    T v[6];
+    for (int i = 0; i < 3; i++) v[i] = Rcb[3 * i] * f0 + Rcb[3 * i + 1] * f1 + Rcb[3 * i + 2] * f2;
+    v[3] = Rwb[0] * x0 + Rwb[1] * x1 + Rwb[2] * x2 - t0;
+    v[4] = Rwb[3] * x0 + Rwb[4] * x1 + Rwb[5] * x2 - t1;
+    v[5] = Rwb[6] * x0 + Rwb[7] * x1 + Rwb[8] * x2 - t2;
+    for (int i = 0; i < 6; i++) d[i] = v[i];
 }
 
 __device__ inline void huber(double en, double k, double *w, double *c) {
