@@ -35,7 +35,7 @@ def candidate_space(nlm, ncu, ns):
     auto_lpc = max(8, min(4096, int(round(nlm / (4.0 * ncu)))))
     ns = np.asarray(ns)
     big = [0]
-    for b in (96, 128, 256, 384, 512, -1):
+    for b in (64, 96, 112, 128, 144, 160, 256, 384, 512, -1):
         # keep thresholds that move at least one window across the tiled / one-workgroup boundary relative to 192
         if (b < 0 and (ns > 192).any()) or (b > 0 and ((ns > b) != (ns > 192)).any()):
             big.append(b)

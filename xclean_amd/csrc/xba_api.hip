@@ -135,6 +135,10 @@ struct xba_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t gstream = nullptr;
+    // the one-workgroup solves of the small windows run on `side` beside the tiled sequence of the large ones
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int maxn_small = 0;  // largest n among the windows k_solve factors
 };
 
 namespace {
@@ -329,6 +333,13 @@ void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 =
 // workgroup per window for the others.
 template <typename T>
 void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
+    const bool fork = c->side != nullptr;  // both kinds present: k_solve beside the tiled sequence
+    if (fork) {
+        (void)hipEventRecord(c->ev_fork, s);
+        (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, c->side, D, c->solve_lds_cap);
+        (void)hipEventRecord(c->ev_join, c->side);
+    }
     if (c->nbig > 0) {
         hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
         for (int k = 0; k < c->big_nbmax; k++) {
@@ -346,7 +357,9 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
         }
         hipLaunchKernelGGL(k_big_finish, dim3(c->nbig), dim3(256), 0, s, D);
     }
-    if (c->nbig < c->nwin)
+    if (fork)
+        (void)hipStreamWaitEvent(s, c->ev_join, 0);
+    else if (c->nbig < c->nwin)
         hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
 }
 
@@ -459,6 +472,9 @@ int xba_destroy(xba_ctx *c) {
     if (!c) return XBA_OK;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     for (void *p : c->owned) (void)hipFree(p);
     delete c;
     return XBA_OK;
@@ -585,6 +601,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             c->big_nmax = std::max(c->big_nmax, bw.n);
         }
         c->nbig = (int)bigs.size();
+        for (int w = 0; w < nwin; w++)
+            if (wins[w].big < 0) c->maxn_small = std::max(c->maxn_small, wins[w].n);
     }
     c->nstate = st;
     c->nh = h0;
@@ -977,11 +995,16 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     hipFuncAttributes fa;
     HIPCK(hipFuncGetAttributes(&fa, mx ? (const void *)k_solve<float> : (const void *)k_solve<double>));
     int cap = 160 * 1024 - (int)fa.sharedSizeBytes - 256;
-    int need = (int)std::min<int64_t>((int64_t)maxn * maxn * tsz, (int64_t)cap);
+    int need = (int)std::min<int64_t>((int64_t)c->maxn_small * c->maxn_small * tsz, (int64_t)cap);
     c->solve_lds_cap = std::max(0, need);
     if (c->solve_lds_cap > 64 * 1024)
         HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, c->solve_lds_cap));
+    if (c->nbig > 0 && c->nbig < c->nwin) {
+        HIPCK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPCK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
     {
         int rc = pack_measurements(c, nullptr);
         if (rc != XBA_OK) return fail(rc);
