@@ -16,6 +16,7 @@ namespace xba {
 // Per window: F = H + 1e-9 I factored in FP64 into fact64 (kept for the adjoint), dx = -F^-1 b.
 __global__ __launch_bounds__(256) void k_gn_solve(DArgs D) {
     __shared__ int sflag;
+    __shared__ double sinv[2 * CH_BS];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
     const int n = W.n;
@@ -23,11 +24,11 @@ __global__ __launch_bounds__(256) void k_gn_solve(DArgs D) {
     const double *H = D.H + W.h0, *bv = D.b + W.st0;
     for (int e = tid; e < n * n; e += nt) F[e] = H[e] + ((e / n == e % n) ? 1e-9 : 0.0);
     __syncthreads();
-    const bool ok = chol_wg<double>(F, n, &sflag);
+    const bool ok = chol_wg<double>(F, n, &sflag, sinv, n);
     if (ok) {
         for (int i = tid; i < n; i += nt) x[i] = bv[i];
         __syncthreads();
-        chol_solve_wg<double>(F, n, x);
+        chol_solve_wg<double>(F, n, x, n);
         for (int i = tid; i < n; i += nt) x[i] = -x[i];
     } else {
         for (int i = tid; i < n; i += nt) x[i] = NAN;
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(256) void k_adj_y(DArgs D, const double *g, double 
     const bool ok = D.wstate[(size_t)NWS * w + WS_CHOLOK] != 0.0;
     for (int i = tid; i < n; i += nt) yw[i] = ok ? g[W.st0 + i] : 0.0;
     __syncthreads();
-    if (ok) chol_solve_wg<double>(D.fact64 + W.h0, n, yw);
+    if (ok) chol_solve_wg<double>(D.fact64 + W.h0, n, yw, n);
 }
 
 // Camera pose of one slot in the layouts obs_eval reads: c64 = [Rwc 9, pwc 3], c = [Rwc 9, pwc 3, Rwb 9].

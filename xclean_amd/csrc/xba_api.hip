@@ -995,7 +995,8 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     hipFuncAttributes fa;
     HIPCK(hipFuncGetAttributes(&fa, mx ? (const void *)k_solve<float> : (const void *)k_solve<double>));
     int cap = 160 * 1024 - (int)fa.sharedSizeBytes - 256;
-    int need = (int)std::min<int64_t>((int64_t)c->maxn_small * c->maxn_small * tsz, (int64_t)cap);
+    // k_solve keeps a window's matrix in LDS, rows padded to a multiple of 4 entries, when it fits
+    int need = (int)std::min<int64_t>((int64_t)c->maxn_small * ((c->maxn_small + 3) & ~3) * tsz, (int64_t)cap);
     c->solve_lds_cap = std::max(0, need);
     if (c->solve_lds_cap > 64 * 1024)
         HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,

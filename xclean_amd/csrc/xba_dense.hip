@@ -532,77 +532,14 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
 }
 
 // ---------------------------------------------------------------------------------------- dense Cholesky --
-// In-place lower Cholesky of the n x n row-major A by one workgroup (a multiple of 64 threads), right-looking in
-// panels of 32 columns: the panel's diagonal block factored by the first wavefront (wavefront-synchronous), the
-// rows below it solved against that block one row per thread, then the trailing matrix updated over a 16 x 16
-// thread grid.  Every thread reads each pivot, so a failure exits uniformly.  Returns false on a non-positive or
-// non-finite pivot.
-// (the matrix may live in LDS or in global memory: wait for both before the lanes read each other's writes)
-#define CHOL_WAVE_SYNC()                                       \
+// Orders the LDS accesses of one wavefront's lanes (a lane reads what another lane of the wavefront wrote).
+#define CH_WAVE_SYNC()                                         \
     do {                                                       \
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
-        __builtin_amdgcn_s_waitcnt(0);                         \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+        __builtin_amdgcn_s_waitcnt(0xc07f);                    \
         __builtin_amdgcn_wave_barrier();                       \
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
     } while (0)
-
-template <typename T>
-__device__ bool chol_wg(T *A, int n, int *sflag) {
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    const int r = tid >> 4, c = tid & 15, nr = nt >> 4;
-    constexpr int BS = 32;
-    for (int j0 = 0; j0 < n; j0 += BS) {
-        const int nb = min(BS, n - j0), j1 = j0 + nb;
-        // diagonal block
-        if (wv == 0) {
-            int ok = 1;
-            for (int jj = j0; jj < j1; jj++) {
-                const T d = A[(size_t)jj * n + jj];
-                if (!(d > (T)0 && isfinite(d))) {
-                    ok = 0;
-                    break;
-                }
-                const T sd = sqrt(d), s = (T)1 / sd;
-                const int i = jj + 1 + lane;
-                if (i < j1) A[(size_t)i * n + jj] *= s;
-                CHOL_WAVE_SYNC();
-                if (lane == 0) A[(size_t)jj * n + jj] = sd;
-                for (int ii = jj + 1 + (lane >> 3); ii < j1; ii += 8) {
-                    T *Ai = A + (size_t)ii * n;
-                    const T aij = Ai[jj];
-                    for (int k = jj + 1 + (lane & 7); k <= ii; k += 8) Ai[k] -= aij * A[(size_t)k * n + jj];
-                }
-                CHOL_WAVE_SYNC();
-            }
-            if (lane == 0) *sflag = ok;
-        }
-        __syncthreads();
-        if (!*sflag) return false;
-        // rows below the block: L[i][j0:j1] = A[i][j0:j1] L_block^-T
-        for (int i = j1 + tid; i < n; i += nt) {
-            T *Ai = A + (size_t)i * n;
-            for (int jj = j0; jj < j1; jj++) {
-                const T *Lj = A + (size_t)jj * n;
-                T v = Ai[jj];
-                for (int m = j0; m < jj; m++) v -= Ai[m] * Lj[m];
-                Ai[jj] = v / Lj[jj];
-            }
-        }
-        __syncthreads();
-        // trailing update
-        for (int i = j1 + r; i < n; i += nr) {
-            T *Ai = A + (size_t)i * n;
-            for (int k = j1 + c; k <= i; k += 16) {
-                const T *Ak = A + (size_t)k * n;
-                T v = Ai[k];
-                for (int m = j0; m < j1; m++) v -= Ai[m] * Ak[m];
-                Ai[k] = v;
-            }
-        }
-        __syncthreads();
-    }
-    return true;
-}
 
 // Broadcast of lane jj's value (jj uniform) through the scalar unit.
 __device__ inline float lane_bcast(float v, int jj) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), jj)); }
@@ -613,11 +550,151 @@ __device__ inline double lane_bcast(double v, int jj) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// x <- (L L^T)^-1 x: forward then back substitution in blocks of 32 columns; the triangle of each block is solved
-// by the first wavefront with its row (forward) or column (backward) of L and the reciprocal pivots in registers,
-// the rest of the vector is updated by every thread.
+// In-place lower Cholesky of the n x n row-major A (leading dimension ld) by one workgroup (a multiple of 64
+// threads), right-looking in panels of CH_BS columns:
+//   1. the panel's diagonal block in the first wavefront's registers (lane i holds row i of the block): per
+//      column the pivot is broadcast through the scalar unit and the scaled column through 32 LDS entries
+//      (sinv[CH_BS..2 CH_BS)), which every lane reads back at uniform addresses; reciprocal pivots into sinv;
+//   2. the rows below solved against it, one row per thread in registers (the block read at wave-uniform
+//      addresses, so every read is a broadcast), written to the lower triangle and, transposed, to the unused
+//      upper triangle (row j0 + m holds column m of the panel, contiguous in the row index);
+//   3. the trailing lower triangle updated in 4 x 4 register tiles from the transposed copy: a tile's 4 rows
+//      and 4 columns are 4 consecutive entries of one row, one 4-wide load each when ld is a multiple of 4
+//      (consecutive threads then read consecutive 16-byte groups), and each loaded entry feeds 4 products.
+// The lower triangle holds the factor on return; the upper triangle is scratch.  Every thread reads the pivot
+// status, so a failure exits uniformly.  Returns false on a non-positive or non-finite pivot.  Inlined, so an
+// LDS-resident A gets LDS instructions.  sinv: 2 CH_BS entries of LDS.
+#define CH_BS 32
 template <typename T>
-__device__ void chol_solve_wg(const T *Lm, int n, T *x) {
+__device__ __attribute__((always_inline)) inline bool chol_wg(T *A, int n, int *sflag, T *sinv, int ld) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    T *scr = sinv + CH_BS;  // the current pivot column of the block, for the first wavefront
+    for (int j0 = 0; j0 < n; j0 += CH_BS) {
+        const int nb = min(CH_BS, n - j0), j1 = j0 + nb;
+        if (wv == 0) {
+            // lane i < nb: row j0 + i of the block, whole (its entries right of the diagonal are scratch: they only
+            // ever reach entries right of the diagonal); lanes past nb hold zeros and columns past nb take a unit
+            // pivot, so those steps change nothing.  No per-lane masks: every condition is on uniform values.
+            const bool in = lane < nb;
+            T row[CH_BS];
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++) row[jj] = (jj < nb && in) ? A[(size_t)(j0 + lane) * ld + j0 + jj] : (T)0;
+            int ok = 1;
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++) {
+                T d = jj < nb ? lane_bcast(row[jj], jj) : (T)1;
+                if (!(d > (T)0 && isfinite(d))) {  // (uniform) recorded; the sweep goes on with a unit pivot
+                    ok = 0;
+                    d = (T)1;
+                }
+                // l = L[lane][jj] for lane > jj; lane jj gets d / sqrt(d), the pivot (within an ulp of sqrt(d))
+                const T s = (T)1 / sqrt(d), l = row[jj] * s;
+                row[jj] = l;
+                if (lane < CH_BS) scr[lane] = l;
+                if (lane == 0) sinv[jj] = s;
+                CH_WAVE_SYNC();
+                // row[k] -= L[lane][jj] L[k][jj]; entries right of the diagonal are scratch, never stored
+#pragma unroll
+                for (int k = jj + 1; k < CH_BS; k++) {
+                    row[k] -= l * scr[k];
+                    // the update happens here: otherwise the compiler defers each row[k] to its use at step k and
+                    // keeps every earlier column's scratch values live until then
+                    asm volatile("" : "+v"(row[k]));
+                }
+                CH_WAVE_SYNC();
+            }
+            // whole rows back: the part right of the diagonal is never read as factor
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++)
+                if (jj < nb && in) A[(size_t)(j0 + lane) * ld + j0 + jj] = row[jj];
+            if (lane == 0) *sflag = ok;
+        }
+        __syncthreads();
+        if (!*sflag) return false;
+        // rows below the block (present only when the block is full, nb = CH_BS): L[i][j0:j1] = A[i][j0:j1] L^-T
+        for (int i = j1 + tid; i < n; i += nt) {
+            T *Ai = A + (size_t)i * ld + j0;
+            T x[CH_BS];
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++) x[jj] = Ai[jj];
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++) {
+                const T *Lj = A + (size_t)(j0 + jj) * ld + j0;
+                T v = x[jj];
+#pragma unroll
+                for (int m = 0; m < jj; m++) v -= x[m] * Lj[m];
+                x[jj] = v * sinv[jj];
+            }
+#pragma unroll
+            for (int jj = 0; jj < CH_BS; jj++) {
+                Ai[jj] = x[jj];
+                A[(size_t)(j0 + jj) * ld + i] = x[jj];
+            }
+        }
+        __syncthreads();
+        // trailing lower triangle: tile (bi, bk), bk <= bi, covers rows j1 + 4 bi.. and columns j1 + 4 bk..
+        const int M = n - j1, mt = (M + 3) >> 2, ntile = mt * (mt + 1) / 2;
+        for (int t = tid; t < ntile; t += nt) {
+            int bi = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+            while (bi * (bi + 1) / 2 > t) bi--;
+            const int bk = t - bi * (bi + 1) / 2;
+            const int i0 = j1 + 4 * bi, k0 = j1 + 4 * bk;
+            T acc[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = (T)0;
+            if ((ld & 3) == 0) {
+                // j1 is a multiple of CH_BS here (a panel with rows below is full), so i0, k0 and the row starts are
+                // multiples of 4: aligned 4-wide reads.  Entries past n lie in the row padding (ld >= n rounded up
+                // to 4); they only reach rows / columns that are not written.
+                typedef T T4 __attribute__((ext_vector_type(4)));
+                for (int m = 0; m < nb; m++) {
+                    const T *Pm = A + (size_t)(j0 + m) * ld;
+                    const T4 a = *(const T4 *)(Pm + i0), b = *(const T4 *)(Pm + k0);
+                    const T av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] += av[r] * bv[c];
+                }
+            } else {
+                for (int m = 0; m < nb; m++) {
+                    const T *Pm = A + (size_t)(j0 + m) * ld;
+                    T av[4], bv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        av[r] = Pm[min(i0 + r, n - 1)];  // rows past n: not written
+                        bv[r] = Pm[min(k0 + r, n - 1)];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] += av[r] * bv[c];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int i = i0 + r, k = k0 + c;
+                    if (i < n && k <= i) A[(size_t)i * ld + k] -= acc[r][c];
+                }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// x <- (L L^T)^-1 x with the lower triangle of Lm (leading dimension ld; the upper triangle is not read as
+// factor): forward then back substitution in blocks of 32 columns.  The triangle of each block is solved by the
+// first wavefront with its row (forward) or column (backward) of L in registers: each step's solution entry is
+// broadcast through the scalar unit and stored by lane 0, and every lane subtracts its multiple of it (a lane whose
+// entry is already solved then only updates a value that is not read again, so no per-lane masks are needed).
+// The rest of the vector is updated by every thread.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void chol_solve_wg(const T *Lm, int n, T *x, int ld) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     constexpr int BS = 32;
     for (int j0 = 0; j0 < n; j0 += BS) {
@@ -625,23 +702,22 @@ __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
         if (wv == 0) {
             const bool in = lane < nb;
             T xi = in ? x[j0 + lane] : (T)0;
-            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * n + j0 + lane] : (T)0;
+            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * ld + j0 + lane] : (T)0;
             T row[BS];
 #pragma unroll
-            for (int jj = 0; jj < BS; jj++) row[jj] = (in && jj < lane) ? Lm[(size_t)(j0 + lane) * n + j0 + jj] : (T)0;
+            for (int jj = 0; jj < BS; jj++) row[jj] = (jj < nb && in) ? Lm[(size_t)(j0 + lane) * ld + j0 + jj] : (T)0;
 #pragma unroll
             for (int jj = 0; jj < BS; jj++) {
                 if (jj < nb) {
                     const T xj = lane_bcast(xi, jj) * lane_bcast(dinv, jj);
-                    if (lane == jj) xi = xj;
-                    else xi -= row[jj] * xj;
+                    if (lane == 0) x[j0 + jj] = xj;
+                    xi -= row[jj] * xj;
                 }
             }
-            if (in) x[j0 + lane] = xi;
         }
         __syncthreads();
         for (int i = j0 + nb + tid; i < n; i += nt) {
-            const T *Li = Lm + (size_t)i * n + j0;
+            const T *Li = Lm + (size_t)i * ld + j0;
             T acc = x[i];
             for (int jj = 0; jj < nb; jj++) acc -= Li[jj] * x[j0 + jj];
             x[i] = acc;
@@ -653,25 +729,23 @@ __device__ void chol_solve_wg(const T *Lm, int n, T *x) {
         if (wv == 0) {
             const bool in = lane < nb;
             T xi = in ? x[j0 + lane] : (T)0;
-            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * n + j0 + lane] : (T)0;
+            const T dinv = in ? (T)1 / Lm[(size_t)(j0 + lane) * ld + j0 + lane] : (T)0;
             T col[BS];
 #pragma unroll
-            for (int jj = 0; jj < BS; jj++)
-                col[jj] = (in && jj > lane && jj < nb) ? Lm[(size_t)(j0 + jj) * n + j0 + lane] : (T)0;
+            for (int jj = 0; jj < BS; jj++) col[jj] = (jj < nb && in) ? Lm[(size_t)(j0 + jj) * ld + j0 + lane] : (T)0;
 #pragma unroll
             for (int jj = BS - 1; jj >= 0; jj--) {
                 if (jj < nb) {
                     const T xj = lane_bcast(xi, jj) * lane_bcast(dinv, jj);
-                    if (lane == jj) xi = xj;
-                    else xi -= col[jj] * xj;
+                    if (lane == 0) x[j0 + jj] = xj;
+                    xi -= col[jj] * xj;
                 }
             }
-            if (in) x[j0 + lane] = xi;
         }
         __syncthreads();
         for (int i = tid; i < j0; i += nt) {
             T acc = x[i];
-            for (int jj = 0; jj < nb; jj++) acc -= Lm[(size_t)(j0 + jj) * n + i] * x[j0 + jj];
+            for (int jj = 0; jj < nb; jj++) acc -= Lm[(size_t)(j0 + jj) * ld + i] * x[j0 + jj];
             x[i] = acc;
         }
         __syncthreads();
@@ -718,6 +792,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
     __shared__ T sy[XBA_MAX_STATE];
+    __shared__ T sinv[2 * CH_BS];
     __shared__ double sred[32];
     __shared__ int sflag;
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -727,8 +802,12 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     double *ws = D.wstate + (size_t)NWS * w;
     const double lam = ws[WS_LAM];
     const double *H = D.H + W.h0, *bv = D.b + W.st0;
-    const bool in_lds = (size_t)n * n * sizeof(T) <= (size_t)lds_cap;
-    T *F = in_lds ? (T *)smem : (sizeof(T) == 8 ? (T *)(D.fact64 + W.h0) : (T *)(D.fact32 + W.h0));
+    // in LDS the rows are padded to a multiple of 4 entries
+    const int ldp = (n + 3) & ~3;
+    const bool in_lds = (size_t)n * ldp * sizeof(T) <= (size_t)lds_cap;
+    const int ld = in_lds ? ldp : n;
+    T *Fl = (T *)smem, *Fg = sizeof(T) == 8 ? (T *)(D.fact64 + W.h0) : (T *)(D.fact32 + W.h0);
+    T *F = in_lds ? Fl : Fg;
     // scaling (mixed) and the damped matrix in the working precision
     for (int i = tid; i < n; i += nt) {
         double hd = H[(size_t)i * n + i] + (lam * H[(size_t)i * n + i] + 1e-9);
@@ -739,10 +818,11 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
         for (int j = tid & 63; j < n; j += 64) {
             double h = H[(size_t)i * n + j];
             if (i == j) h += lam * h + 1e-9;
-            F[(size_t)i * n + j] = (T)(h * ss[i] * ss[j]);
+            F[(size_t)i * ld + j] = (T)(h * ss[i] * ss[j]);
         }
     __syncthreads();
-    bool ok = chol_wg<T>(F, n, &sflag);
+    // the factor and solves inlined once per memory space
+    bool ok = in_lds ? chol_wg<T>(Fl, n, &sflag, sinv, ld) : chol_wg<T>(Fg, n, &sflag, sinv, ld);
     int nref = 0;
     if (ok) {
         for (int i = tid; i < n; i += nt) {
@@ -754,24 +834,37 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
         for (int it = 0; it < maxit; it++) {
             for (int i = tid; i < n; i += nt) sy[i] = (T)(sr[i] * ss[i]);
             __syncthreads();
-            chol_solve_wg<T>(F, n, sy);
+            if (in_lds)
+                chol_solve_wg<T>(Fl, n, sy, ld);
+            else
+                chol_solve_wg<T>(Fg, n, sy, ld);
             for (int i = tid; i < n; i += nt) sx[i] += (double)sy[i] * ss[i];
             __syncthreads();
             nref = it + 1;
             if (it + 1 == maxit) break;
-            // residual r = b - Hd x in FP64: one wavefront per row, lanes along the row
+            // residual r = b - Hd x in FP64, entry i by thread i from column i of the symmetric H (row j's entry i:
+            // the threads of a wavefront read consecutive addresses)
             const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
             double rmax = 0.0, bmax = 0.0;
-            for (int i = wv; i < n; i += nw) {
-                const double *Hi = H + (size_t)i * n;
+            for (int i = tid; i < n; i += nt) {
+                // 16 column entries in flight per thread, summed in column order
                 double acc = 0.0;
-                for (int j = lane; j < n; j += 64) acc += Hi[j] * sx[j];
-                acc = dwave_sum(acc);
-                const double ri = bv[i] - acc - (lam * Hi[i] + 1e-9) * sx[i];
-                if (lane == 0) sr[i] = ri;
+                int j = 0;
+                for (; j + 16 <= n; j += 16) {
+                    double h[16];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) h[q] = H[(size_t)(j + q) * n + i];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) acc += h[q] * sx[j + q];
+                }
+                for (; j < n; j++) acc += H[(size_t)j * n + i] * sx[j];
+                const double ri = bv[i] - acc - (lam * H[(size_t)i * n + i] + 1e-9) * sx[i];
+                sr[i] = ri;
                 rmax = fmax(rmax, fabs(ri));
                 bmax = fmax(bmax, fabs(bv[i]));
             }
+            rmax = dwave_max(rmax);
+            bmax = dwave_max(bmax);
             if (lane == 0) {
                 sred[2 * wv] = rmax;
                 sred[2 * wv + 1] = bmax;

@@ -163,6 +163,7 @@ __global__ void k_np_schur(NpArgs A, int n) {
 __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
     __shared__ double sred[256];
     __shared__ int sflag;
+    __shared__ double sinv[2 * CH_BS];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (np_skip(A)) return;  // (uniform: no kernel of this launch writes the flag)
     // chi2 of the current residuals, fixed-order reduction
@@ -236,11 +237,11 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
     for (int i = tid; i < n; i += nt) A.H[(size_t)i * n + i] += lam * fmax(A.H[(size_t)i * n + i], 1e-9);
     __syncthreads();
     // Cholesky (the damped reduced system is symmetric positive definite) in place, then solve
-    bool ok = chol_wg<double>(A.H, n, &sflag);
+    bool ok = chol_wg<double>(A.H, n, &sflag, sinv, n);
     if (ok) {
         for (int i = tid; i < n; i += nt) A.dx[i] = A.bp[i];
         __syncthreads();
-        chol_solve_wg<double>(A.H, n, A.dx);
+        chol_solve_wg<double>(A.H, n, A.dx, n);
         if (A.dx_hist)
             for (int i = tid; i < n; i += nt) A.dx_hist[(size_t)it * n + i] = A.dx[i];
         // trial poses: T_new = exp(dx) T (free slots), rotation projected onto SO(3) by polar iteration
