@@ -235,6 +235,33 @@ def test_tiled_and_one_workgroup_windows_in_one_batch_with_graph_replay():
         assert _rel(pb.lmp, ref.lmp) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_side_stream_solves_graph_replays_match_eager_every_iteration(precision):
+    """Windows on both sides of the tiling threshold: the small ones are solved on the second stream beside the tiled
+    sequence and joined back.  Four replays of the captured LM iteration, each compared with the eager iteration from
+    the same state, with host synchronisation and eager copies between the replays."""
+    from xclean_amd.window import BatchSolver
+    pbs = _large_windows() + [synth.make_window(P=P, L=L, views=(2, min(10, P)), seed=[8, i])[0]
+                              for i, (P, L) in enumerate([(5, 80), (12, 600), (25, 900)])]
+    a = [pb.copy() for pb in pbs]
+    b = [pb.copy() for pb in pbs]
+    A = BatchSolver(a, precision=precision, use_graph=False)
+    B = BatchSolver(b, precision=precision, use_graph=True)
+    A.upload()
+    B.upload()
+    bar = 1e-12 if precision == "fp64" else 1e-10
+    for it in range(4):
+        A.iterate(1)
+        B.iterate(1)
+        A.download()
+        B.download()
+        for pa, pg in zip(a, b):
+            assert _rel(pg.pwb, pa.pwb) < bar, it
+            assert _rel(pg.Rwb, pa.Rwb) < bar, it
+            assert _rel(pg.lmp, pa.lmp) < bar, it
+    assert [s["accepted"] for s in A.stats()] == [s["accepted"] for s in B.stats()]
+
+
 def test_config1_visual_inertial_window_mixed_and_fp64_match_oracle():
     """Config 1: 20 visual-inertial keyframes (n = 300, tiled Cholesky), 20,000 landmarks, ~110,000 observation
     rows, 19 IMU factors.  FP64 within 1e-9 and mixed precision within 1e-6 of the oracle's first update."""

@@ -430,11 +430,14 @@ int xba_prior_from_info(const xba_prior_batch *q, void *stream) {
     if (q->nb == 0) return 0;
     if (!q->n || !q->H || !q->b || !q->S || !q->r || !q->rank || !q->work) return XBA_E_ARG;
     if (q->blk0 && (!q->boff || !q->bdim || !q->bcur || !q->bx0)) return XBA_E_ARG;
-    const size_t shm = q->ld <= MG_LDS_LD ? (size_t)q->ld * q->ld * sizeof(double) : 0;
+    // LDS: room for the packed matrix and the eigenvectors of any matrix of order up to MG_PACK_N (the order rounded
+    // up to even), and, when the leading dimension allows it, for an unpacked matrix of a larger order
+    const int ld2 = std::min(q->ld + (q->ld & 1), MG_PACK_N);
+    const size_t shm = (size_t)std::max(MG_PACK_DOUBLES(ld2), q->ld <= MG_LDS_LD ? q->ld * q->ld : 0) * sizeof(double);
     static bool attr = false;  // the LDS limit of the kernel, raised once per process
     if (!attr) {
-        HIPCK(hipFuncSetAttribute((const void *)k_prior_from_info, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  MG_LDS_LD * MG_LDS_LD * (int)sizeof(double)));
+        const int mx = std::max(MG_PACK_DOUBLES(MG_PACK_N), MG_LDS_LD * MG_LDS_LD) * (int)sizeof(double);
+        HIPCK(hipFuncSetAttribute((const void *)k_prior_from_info, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
         attr = true;
     }
     hipLaunchKernelGGL(k_prior_from_info, dim3(q->nb), dim3(MG_THREADS), shm, (hipStream_t)stream, q->ld, q->n, q->H,

@@ -11,8 +11,12 @@
 // round-robin parallel ordering: each round rotates n/2 disjoint index pairs at once (rotation angles from the
 // current 2x2 blocks, then the row pass, then the column pass and the eigenvector update, each spread over the whole
 // workgroup); a pair is rotated only while its off-diagonal entry exceeds both 1e-15 of the geometric mean of its
-// diagonal entries and 1e-15 of the matrix's Frobenius norm, and sweeps repeat until one rotates nothing (or 40).  The matrix being diagonalised lives in LDS when the batch's leading dimension is at most MG_LDS_LD
-// (every round reads and writes it twice), else in the global workspace; the eigenvectors always in the workspace.
+// diagonal entries and 1e-15 of the matrix's Frobenius norm, and sweeps repeat until one rotates nothing (or 40).
+// Square roots (xba_prior_from_info) of matrices of order up to MG_PACK_N keep both the matrix (its upper triangle,
+// packed) and the eigenvectors in LDS and apply a round as one pass over the 2x2 blocks of the round's pairs (each
+// block B <- G_k B G_l^T read and written once, the eigenvector columns beside them); larger ones keep the matrix in
+// LDS when the batch's leading dimension is at most MG_LDS_LD (row pass, then column pass), else in the global
+// workspace, and the eigenvectors in the workspace.
 #include "xba_internal.h"
 
 namespace xba {
@@ -21,6 +25,9 @@ namespace xba {
 #define MG_EIG_FLOOR 1e-12  // marg.py:32 EIG_FLOOR, relative to the largest eigenvalue
 #define MG_MAX_SWEEPS 40
 #define MG_LDS_LD 128  // the largest leading dimension whose FP64 matrix (128 KiB) is kept in LDS
+// the largest (even) order whose packed upper triangle and eigenvectors share the LDS (1.5 n^2 + 0.5 n doubles)
+#define MG_PACK_N 112
+#define MG_PACK_DOUBLES(n2) ((n2) * ((n2) + 1) / 2 + (n2) * (n2))
 static_assert(MG_THREADS >= XBA_MAX_STATE, "the column pass gives every row at least one thread");
 
 __device__ inline double mg_block_sum(double v, double *red) {
@@ -36,18 +43,17 @@ __device__ inline double mg_block_sum(double v, double *red) {
 }
 
 // Eigen-decomposition of the symmetric n x n matrix A (leading dimension ld, overwritten: its diagonal ends as the
-// eigenvalues), eigenvectors into the columns of V.  cs: LDS for n/2 + 1 (c, s) pairs; red: LDS for the sums.  The
+// eigenvalues), eigenvectors into the columns of V.  cs, pqs: LDS for n/2 + 1 (c, s) and (p, q) pairs; rotated: an
+// LDS flag; red: LDS for the sums.  The
 // workgroup has at least n threads (XBA_MAX_STATE <= MG_THREADS), so every row has a thread in the column pass.
 // Always inlined, so that every call site's pointers keep their address space (LDS or global) and the passes use
 // LDS or global instructions rather than generic (flat) ones.
 __device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, double *V, int n, int ld, double *cs,
-                                                                double *red) {
+                                                                int *pqs, int &rotated, double *red) {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int e = tid; e < n * n; e += nt) V[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
     const int N = n + (n & 1);  // players of the round-robin; N - 1 is a dummy when n is odd
     const int np = N / 2;
-    __shared__ int rotated;
-    __shared__ int pqs[2 * (XBA_MAX_STATE / 2 + 1)];
     double fro = 0.0;
     for (int e = tid; e < n * n; e += nt) {
         const double v = A[(e / n) * ld + e % n];
@@ -124,6 +130,105 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi(double *A, doubl
     }
 }
 
+// Packed upper triangle: entry (i, j) of an order-n2 symmetric matrix.
+__device__ inline int mg_pk(int i, int j, int n2) {
+    if (i > j) {
+        const int t = i;
+        i = j;
+        j = t;
+    }
+    return i * (2 * n2 - i + 1) / 2 + (j - i);
+}
+
+// The same cyclic Jacobi on an even order n2 with everything in LDS: P the packed upper triangle (overwritten; its
+// diagonal ends as the eigenvalues), V (n2 x n2, leading dimension n2) the eigenvectors.  A round computes the
+// rotations G_k = [[c, -s], [s, c]] of its pairs, then, in one pass, every 2x2 block B of pairs (k, l), k <= l, as
+// G_k B G_l^T and every row of V's pair columns as V G_k^T.  An odd order is padded by the caller with a decoupled
+// zero row and column, whose off-diagonal entries stay exactly zero, so it is never rotated.
+__device__ __attribute__((always_inline)) inline void mg_jacobi_lds(double *P, double *V, int n2, double *cs,
+                                                                    int *pqs, int &rotated, double *red) {
+    const int tid = threadIdx.x, nt = blockDim.x, np = n2 / 2;
+    for (int e = tid; e < n2 * n2; e += nt) V[e] = (e / n2 == e % n2) ? 1.0 : 0.0;
+    double fro = 0.0;
+    for (int i = tid; i < n2; i += nt)
+        for (int j = i; j < n2; j++) {
+            const double v = P[mg_pk(i, j, n2)];
+            fro += (i == j ? 1.0 : 2.0) * v * v;
+        }
+    const double tiny = 1e-15 * sqrt(mg_block_sum(fro, red));
+    const int nblk = np * (np + 1) / 2, nwork = nblk + n2 * np;
+    for (int sweep = 0; sweep < MG_MAX_SWEEPS; sweep++) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int rd = 0; rd < n2 - 1; rd++) {
+            // pairs of this round: (n2 - 1, rd) and ((rd + k) mod (n2 - 1), (rd - k) mod (n2 - 1)), k = 1 .. np - 1
+            for (int k = tid; k < np; k += nt) {
+                int p = k == 0 ? n2 - 1 : (rd + k) % (n2 - 1), q = k == 0 ? rd : (rd - k + n2 - 1) % (n2 - 1);
+                if (p > q) {
+                    const int x = p;
+                    p = q;
+                    q = x;
+                }
+                double c = 1.0, sn = 0.0;
+                const double apq = P[mg_pk(p, q, n2)], app = P[mg_pk(p, p, n2)], aqq = P[mg_pk(q, q, n2)];
+                if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > tiny) {
+                    const double th = (aqq - app) / (2.0 * apq);
+                    const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    sn = t * c;
+                    rotated = 1;
+                }
+                cs[2 * k] = c;
+                cs[2 * k + 1] = sn;
+                pqs[2 * k] = p;
+                pqs[2 * k + 1] = q;
+            }
+            __syncthreads();
+            for (int b = tid; b < nwork; b += nt) {
+                if (b < nblk) {
+                    // block (k, l), k <= l, from its triangular index
+                    int l = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+                    while ((l + 1) * (l + 2) / 2 <= b) l++;
+                    while (l * (l + 1) / 2 > b) l--;
+                    const int k = b - l * (l + 1) / 2;
+                    const double ck = cs[2 * k], sk = cs[2 * k + 1], cl = cs[2 * l], sl = cs[2 * l + 1];
+                    if (sk == 0.0 && sl == 0.0) continue;
+                    const int pk = pqs[2 * k], qk = pqs[2 * k + 1], pl = pqs[2 * l], ql = pqs[2 * l + 1];
+                    const int i00 = mg_pk(pk, pl, n2), i01 = mg_pk(pk, ql, n2), i10 = mg_pk(qk, pl, n2),
+                              i11 = mg_pk(qk, ql, n2);
+                    const double b00 = P[i00], b01 = P[i01], b10 = P[i10], b11 = P[i11];
+                    const double l00 = ck * b00 - sk * b10, l01 = ck * b01 - sk * b11;  // G_k B
+                    const double l10 = sk * b00 + ck * b10, l11 = sk * b01 + ck * b11;
+                    const double n00 = cl * l00 - sl * l01, n01 = sl * l00 + cl * l01;  // (G_k B) G_l^T
+                    const double n10 = cl * l10 - sl * l11, n11 = sl * l10 + cl * l11;
+                    if (k == l) {  // the diagonal block: (p, q) and (q, p) share one entry
+                        P[i00] = n00;
+                        P[i01] = 0.5 * (n01 + n10);
+                        P[i11] = n11;
+                    } else {
+                        P[i00] = n00;
+                        P[i01] = n01;
+                        P[i10] = n10;
+                        P[i11] = n11;
+                    }
+                } else {
+                    const int e = b - nblk, i = e / np, k = e % np;
+                    const double sk = cs[2 * k + 1];
+                    if (sk == 0.0) continue;
+                    const double ck = cs[2 * k];
+                    const int p = pqs[2 * k], q = pqs[2 * k + 1];
+                    const double vp = V[i * n2 + p], vq = V[i * n2 + q];
+                    V[i * n2 + p] = ck * vp - sk * vq;
+                    V[i * n2 + q] = sk * vp + ck * vq;
+                }
+            }
+            __syncthreads();
+        }
+        if (!rotated) break;
+        __syncthreads();
+    }
+}
+
 // The rank of eigenvalue i in ascending order (ties by index).
 __device__ inline int mg_rank(const double *A, int ld, int n, int i) {
     const double li = A[i * ld + i];
@@ -143,6 +248,8 @@ __global__ __launch_bounds__(MG_THREADS) void k_marg_schur(int ld, const int32_t
                                                             const int32_t *keep, const int32_t *marg, const double *H,
                                                             const double *b, double *H2, double *b2, double *work) {
     __shared__ double cs[2 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ int pqs[2 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ int rotated;
     __shared__ double red[MG_THREADS / XBA_WAVE];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int nk = nkeep[w], m = nmarg[w];
@@ -156,7 +263,7 @@ __global__ __launch_bounds__(MG_THREADS) void k_marg_schur(int ld, const int32_t
         A[i * ld + j] = 0.5 * (Hw[mp[i] * ld + mp[j]] + Hw[mp[j] * ld + mp[i]]);
     }
     __syncthreads();
-    if (m > 0) mg_jacobi(A, V, m, ld, cs, red);
+    if (m > 0) mg_jacobi(A, V, m, ld, cs, pqs, rotated, red);
     // Hinv = V_ok diag(1 / l) V_ok^T over the eigenvalues above the floor relative to the largest one
     double lmax = -INFINITY;
     for (int i = 0; i < m; i++) lmax = fmax(lmax, A[i * ld + i]);
@@ -212,6 +319,8 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
                                                                  double *r, int32_t *rank, double *lam,
                                                                  double *bx_out, double *work) {
     __shared__ double cs[2 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ int pqs[2 * (XBA_MAX_STATE / 2 + 1)];
+    __shared__ int rotated;
     __shared__ double red[MG_THREADS / XBA_WAVE];
     extern __shared__ double mg_dyn[];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -219,11 +328,23 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
     const size_t L2 = (size_t)ld * ld;
     const double *Hw = H + L2 * w, *bw = b + (size_t)ld * w;
     double *Ag = work + MG_WORK(ld) * w, *V = Ag + L2, *bx = V + 2 * L2, *d = bx + ld;  // bx: b at x0
-    double *A = ld <= MG_LDS_LD ? mg_dyn : Ag;  // the matrix being diagonalised
+    const int n2 = n + (n & 1);
+    // per matrix: the launch holds at least MG_PACK_DOUBLES(min(ld rounded up to even, MG_PACK_N)) doubles of LDS
+    const bool packed = n2 <= MG_PACK_N;
+    double *A = ld <= MG_LDS_LD ? mg_dyn : Ag;  // the matrix being diagonalised (unpacked paths)
+    double *Pk = mg_dyn, *Vl = mg_dyn + n2 * (n2 + 1) / 2;  // the packed path
     double *Sw = S + L2 * w, *rw = r + (size_t)ld * w;
-    for (int e = tid; e < n * n; e += nt) {
-        const int i = e / n, j = e % n;
-        A[i * ld + j] = 0.5 * (Hw[i * ld + j] + Hw[j * ld + i]);
+    if (packed) {
+        for (int e = tid; e < n2 * n2; e += nt) {
+            const int i = e / n2, j = e % n2;
+            if (i > j) continue;
+            Pk[mg_pk(i, j, n2)] = (i < n && j < n) ? 0.5 * (Hw[i * ld + j] + Hw[j * ld + i]) : 0.0;
+        }
+    } else {
+        for (int e = tid; e < n * n; e += nt) {
+            const int i = e / n, j = e % n;
+            A[i * ld + j] = 0.5 * (Hw[i * ld + j] + Hw[j * ld + i]);
+        }
     }
     // b at the first linearisation points: b - H d, d = x_cur [-] x0 per block (Log(R0^T R), p - p0, ...)
     for (int i = tid; i < n; i += nt) d[i] = 0.0;
@@ -247,32 +368,40 @@ __global__ __launch_bounds__(MG_THREADS) void k_prior_from_info(int ld, const in
         if (bx_out) bx_out[(size_t)ld * w + i] = bx[i];
     }
     __syncthreads();
+    // eigenvalues into d (free again), eigenvectors in the columns of Ve (leading dimension ldv)
+    const double *Ve = packed ? Vl : V;
+    const int ldv = packed ? n2 : ld;
     if (n > 0) {
-        if (ld <= MG_LDS_LD)
-            mg_jacobi(mg_dyn, V, n, ld, cs, red);
+        if (packed)
+            mg_jacobi_lds(Pk, Vl, n2, cs, pqs, rotated, red);
+        else if (ld <= MG_LDS_LD)
+            mg_jacobi(mg_dyn, V, n, ld, cs, pqs, rotated, red);
         else
-            mg_jacobi(Ag, V, n, ld, cs, red);
+            mg_jacobi(Ag, V, n, ld, cs, pqs, rotated, red);
     }
+    for (int i = tid; i < n; i += nt) d[i] = packed ? Pk[mg_pk(i, i, n2)] : A[i * ld + i];
+    __syncthreads();
     double lmax = -INFINITY;
-    for (int i = 0; i < n; i++) lmax = fmax(lmax, A[i * ld + i]);
+    for (int i = 0; i < n; i++) lmax = fmax(lmax, d[i]);
     const double floor_ = MG_EIG_FLOOR * fmax(lmax, 1e-300);
     int nkeep = 0;
-    for (int i = 0; i < n; i++) nkeep += A[i * ld + i] > floor_ ? 1 : 0;
+    for (int i = 0; i < n; i++) nkeep += d[i] > floor_ ? 1 : 0;
     // rows of S in ascending eigenvalue order among the kept ones (eigh's order); the rest of S and r zero
     for (int e = tid; e < n * n; e += nt) Sw[(e / n) * ld + e % n] = 0.0;
     for (int i = tid; i < n; i += nt) rw[i] = 0.0;
     __syncthreads();
     for (int i = tid; i < n; i += nt) {
-        const double l = A[i * ld + i];
-        const int rk = mg_rank(A, ld, n, i);
+        const double l = d[i];
+        int rk = 0;  // ascending rank, ties by index
+        for (int j = 0; j < n; j++) rk += (d[j] < l || (d[j] == l && j < i)) ? 1 : 0;
         if (lam) lam[(size_t)ld * w + rk] = l;
         if (!(l > floor_)) continue;
         const int row = rk - (n - nkeep);  // the kept eigenvalues are the nkeep largest
         const double sl = sqrt(l);
         double vb = 0.0;
         for (int j = 0; j < n; j++) {
-            Sw[row * ld + j] = sl * V[j * ld + i];
-            vb += V[j * ld + i] * bx[j];
+            Sw[row * ld + j] = sl * Ve[j * ldv + i];
+            vb += Ve[j * ldv + i] * bx[j];
         }
         rw[row] = vb / sl;
     }
