@@ -157,6 +157,27 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi_lds(double *P, d
         }
     const double tiny = 1e-15 * sqrt(mg_block_sum(fro, red));
     const int nblk = np * (np + 1) / 2, nwork = nblk + n2 * np;
+    // this thread's work items, the same every round: block (k, l) of pairs, or (row i, pair k) of V
+    constexpr int MAXI = (MG_PACK_N / 2 * (MG_PACK_N / 2 + 1) / 2 + MG_PACK_N * MG_PACK_N / 2 + MG_THREADS - 1) /
+                         MG_THREADS;
+    int ia[MAXI], ib[MAXI];
+#pragma unroll
+    for (int it = 0; it < MAXI; it++) {
+        const int b = tid + it * nt;
+        ia[it] = -1;
+        ib[it] = 0;
+        if (b < nblk) {
+            int l = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+            while ((l + 1) * (l + 2) / 2 <= b) l++;
+            while (l * (l + 1) / 2 > b) l--;
+            ia[it] = b - l * (l + 1) / 2;  // k
+            ib[it] = l;
+        } else if (b < nwork) {
+            const int e = b - nblk;
+            ia[it] = np + e / np;  // np + row
+            ib[it] = e % np;       // pair
+        }
+    }
     for (int sweep = 0; sweep < MG_MAX_SWEEPS; sweep++) {
         if (tid == 0) rotated = 0;
         __syncthreads();
@@ -184,13 +205,12 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi_lds(double *P, d
                 pqs[2 * k + 1] = q;
             }
             __syncthreads();
-            for (int b = tid; b < nwork; b += nt) {
-                if (b < nblk) {
-                    // block (k, l), k <= l, from its triangular index
-                    int l = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
-                    while ((l + 1) * (l + 2) / 2 <= b) l++;
-                    while (l * (l + 1) / 2 > b) l--;
-                    const int k = b - l * (l + 1) / 2;
+#pragma unroll
+            for (int it = 0; it < MAXI; it++) {
+                const int a = ia[it];
+                if (a < 0) continue;
+                if (a < np) {  // block (k, l) = (a, ib)
+                    const int k = a, l = ib[it];
                     const double ck = cs[2 * k], sk = cs[2 * k + 1], cl = cs[2 * l], sl = cs[2 * l + 1];
                     if (sk == 0.0 && sl == 0.0) continue;
                     const int pk = pqs[2 * k], qk = pqs[2 * k + 1], pl = pqs[2 * l], ql = pqs[2 * l + 1];
@@ -211,8 +231,8 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi_lds(double *P, d
                         P[i10] = n10;
                         P[i11] = n11;
                     }
-                } else {
-                    const int e = b - nblk, i = e / np, k = e % np;
+                } else {  // row a - np of V, pair ib
+                    const int i = a - np, k = ib[it];
                     const double sk = cs[2 * k + 1];
                     if (sk == 0.0) continue;
                     const double ck = cs[2 * k];
