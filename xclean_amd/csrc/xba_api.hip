@@ -135,9 +135,11 @@ struct xba_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t gstream = nullptr;
-    // the one-workgroup solves of the small windows run on `side` beside the tiled sequence of the large ones
+    // a second stream for independent launches of one iteration: the one-workgroup solves of the small windows
+    // beside the tiled sequence of the large ones, the wide work items' assembly and trial beside the narrow ones
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool solve_fork = false, wide_fork = false;
     int maxn_small = 0;  // largest n among the windows k_solve factors
 };
 
@@ -278,16 +280,31 @@ void launch_terms(xba_ctx *c, const DArgs &D, hipStream_t s, int trial) {
 }
 
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.
+// The wide work items' launch beside the narrow one (both kinds present, whole iteration): fork before the narrow
+// launch, join after it.  Their outputs (per-chunk partials, per-pass costs, per-landmark records) are disjoint.
+inline bool fork_wide(xba_ctx *c, hipStream_t s, int part) {
+    if (part >= 0 || !c->wide_fork) return false;
+    (void)hipEventRecord(c->ev_fork, s);
+    (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    return true;
+}
+inline void join_wide(xba_ctx *c, hipStream_t s) {
+    (void)hipEventRecord(c->ev_join, c->side);
+    (void)hipStreamWaitEvent(s, c->ev_join, 0);
+}
+
 template <typename T>
 void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
         const KArgs<double> F = fast_args<T>(c, A);
+        const bool fork = fork_wide(c, s, part);
         if (c->nnarrow > 0 && part != 1)
             hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, F,
                                c->Rnarrow, 0, c->nnarrow, 0);
         if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds, s, F,
-                               c->Rwide, c->nnarrow, c->nchunks, 1);
+            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds,
+                               fork ? c->side : s, F, c->Rwide, c->nnarrow, c->nchunks, 1);
+        if (fork) join_wide(c, s);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_lin<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -297,12 +314,14 @@ template <typename T>
 void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
         const KArgs<double> F = fast_args<T>(c, A);
+        const bool fork = fork_wide(c, s, part);
         if (c->nnarrow > 0 && part != 1)
             hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid_trial), dim3(XBA_WAVE * c->fast_nw_trial),
                                c->fast_lds_trial, s, F, c->Rnarrow, 0, c->nnarrow, 2);
         if (c->nchunks > c->nnarrow && part != 0)
             hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide_trial), dim3(XBA_WAVE * c->wide_nw_trial),
-                               c->wide_lds_trial, s, F, c->Rwide, c->nnarrow, c->nchunks, 3);
+                               c->wide_lds_trial, fork ? c->side : s, F, c->Rwide, c->nnarrow, c->nchunks, 3);
+        if (fork) join_wide(c, s);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -333,7 +352,7 @@ void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 =
 // workgroup per window for the others.
 template <typename T>
 void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
-    const bool fork = c->side != nullptr;  // both kinds present: k_solve beside the tiled sequence
+    const bool fork = c->solve_fork;  // both kinds present: k_solve beside the tiled sequence
     if (fork) {
         (void)hipEventRecord(c->ev_fork, s);
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
@@ -1004,7 +1023,9 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     if (c->solve_lds_cap > 64 * 1024)
         HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, c->solve_lds_cap));
-    if (c->nbig > 0 && c->nbig < c->nwin) {
+    c->solve_fork = c->nbig > 0 && c->nbig < c->nwin;
+    c->wide_fork = c->fast && c->nnarrow > 0 && c->nchunks > c->nnarrow;
+    if (c->solve_fork || c->wide_fork) {
         HIPCK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIPCK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPCK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
