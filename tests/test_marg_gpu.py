@@ -65,6 +65,24 @@ def test_from_info_batch_matches_oracle():
         assert np.allclose(np.sum(S * S, 1), ev[len(ev) - len(S):], rtol=1e-10, atol=1e-12 * ev.max())
 
 
+def test_from_info_every_matrix_layout_matches_oracle():
+    """One batch (leading dimension 120) through each in-kernel layout: orders up to 112 keep the packed upper
+    triangle and the eigenvectors in LDS (112 the largest, 101 and 7 odd, padded by a decoupled zero row), 120 keeps
+    the whole matrix in LDS with the eigenvectors in the global workspace."""
+    from xclean_amd import marg
+    rng = np.random.default_rng(3)
+    items = [(_pd(n, rng, 1e6), rng.standard_normal(n)) for n in (120, 112, 101, 7)]
+    H = _psd(101, 95, rng)
+    items.append((H, H @ rng.standard_normal(101)))
+    got = marg.from_info_batch(items)
+    for (H, b), (S, r, lam) in zip(items, got):
+        Sr, rr = O.prior_from_info(H, b)
+        assert S.shape == Sr.shape
+        assert _rel(S.T @ S, Sr.T @ Sr) < 1e-10 and _rel(S.T @ r, Sr.T @ rr) < 1e-10
+        ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+        assert np.abs(lam - ev).max() < 1e-12 * ev.max()
+
+
 def _state(rng, R=None):
     return (expm(_hat(rng.standard_normal(3))) if R is None else R, rng.standard_normal(3), rng.standard_normal(3),
             rng.standard_normal(3), rng.standard_normal(3))
