@@ -252,12 +252,13 @@ __device__ inline double obs_row(const KArgs<double> &A, int k, int h, double a,
     const int t = obs_tile(info), cam = (info >> 12) & 1, valid = (info >> 13) & 1;
     const int nd = cam == 0 ? 2 : 1;
     o.t = t;
-    double e[2] = {0.0, 0.0}, Jl[2][3], Jt[2][6], Jh[2][6];
+    double ev[2] = {0.0, 0.0}, Jl[2][3], Jt[2][6], Jh[2][6];
     bool act = false;
     if (valid)
         act = obs_eval<double>(sc64 + h * SC64_STRIDE, sc64 + t * SC64_STRIDE, sc + h * SC_STRIDE,
                                sc + t * SC_STRIDE, wc, wd, a, b, rho, cam, A.z[2 * (size_t)k], A.z[2 * (size_t)k + 1],
-                               t == h, true, e, Jl, Jt, Jh);
+                               t == h, true, ev, Jl, Jt, Jh);
+    double e[2] = {ev[0], ev[1]};  // a fresh copy, only ever indexed by constants
     double w = 0.0, c = 0.0, en2 = 0.0;
     if (act) {
         const double isg = A.isig[k];

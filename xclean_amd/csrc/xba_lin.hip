@@ -137,13 +137,18 @@ __device__ inline bool obs_eval(const double *sh64, const double *st64, const T 
     double q2 = Rt[2] * Y0 + Rt[5] * Y1 + Rt[8] * Y2;
     if (!(q2 > 1e-9)) return false;
     const double fxd = wd[0], fyd = wd[1], cxd = wd[2], cyd = wd[3], Bd = wd[4];
+    // both entries written once, after the branch (stores in the branches get merged into one store through a
+    // selected address, which keeps the caller's array in scratch memory)
+    double e0, e1;
     if (cam == 0) {
-        e[0] = fxd * q0 / q2 + cxd - z0;
-        e[1] = fyd * q1 / q2 + cyd - z1;
+        e0 = fxd * q0 / q2 + cxd - z0;
+        e1 = fyd * q1 / q2 + cyd - z1;
     } else {
-        e[0] = fxd * (q0 - rho * Bd) / q2 + cxd - z0;
-        e[1] = 0.0;
+        e0 = fxd * (q0 - rho * Bd) / q2 + cxd - z0;
+        e1 = 0.0;
     }
+    e[0] = e0;
+    e[1] = e1;
     if (!jac) return true;
     const T fx = wc[18], fy = wc[19], B = wc[22];
     const T *RhT = sh, *RtT = st;
@@ -171,8 +176,13 @@ __device__ inline bool obs_eval(const double *sh64, const double *st64, const T 
         Jl[r][2] = Dp[r][0] * r0 + Dp[r][1] * r1 + Dp[r][2] * r2;
     }
     if (same) {
+        // (stores in the order of the general branch's last ones, Jt then Jh: the compiler merges the two branches'
+        // final stores, and stores to different entries would leave the caller's arrays in scratch memory)
         for (int r = 0; r < 2; r++)
-            for (int j = 0; j < 6; j++) Jt[r][j] = Jh[r][j] = (T)0;
+            for (int j = 0; j < 6; j++) {
+                Jt[r][j] = (T)0;
+                Jh[r][j] = (T)0;
+            }
         return true;
     }
     const T *Rcb = wc, *Px = wc + 9, *Rwbt = st + 12, *Rwbh = sh + 12;
@@ -277,13 +287,16 @@ __device__ inline void dir_slot(T *d, const T *c, const T *wc) {
 }
 
 __device__ inline void huber(double en, double k, double *w, double *c) {
+    double ww, cc;  // each output stored once, after the branch (see obs_eval)
     if (en <= k) {
-        *w = 1.0;
-        *c = 0.5 * en * en;
+        ww = 1.0;
+        cc = 0.5 * en * en;
     } else {
-        *w = k / en;
-        *c = k * (en - 0.5 * k);
+        ww = k / en;
+        cc = k * (en - 0.5 * k);
     }
+    *w = ww;
+    *c = cc;
 }
 
 // Tile accumulation: FP64 adds directly; FP32 adds with Kahan compensation (c holds the negated lost low part).
