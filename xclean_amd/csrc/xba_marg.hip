@@ -157,7 +157,8 @@ __device__ __attribute__((always_inline)) inline void mg_jacobi_lds(double *P, d
         }
     const double tiny = 1e-15 * sqrt(mg_block_sum(fro, red));
     const int nblk = np * (np + 1) / 2, nwork = nblk + n2 * np;
-    // this thread's work items, the same every round: block (k, l) of pairs, or (row i, pair k) of V
+    // this thread's work items, the same every round: block (k, l) of pairs, or (row i, pair k) of V (the launch has
+    // MG_THREADS threads, so MAXI items per thread cover every order up to MG_PACK_N)
     constexpr int MAXI = (MG_PACK_N / 2 * (MG_PACK_N / 2 + 1) / 2 + MG_PACK_N * MG_PACK_N / 2 + MG_THREADS - 1) /
                          MG_THREADS;
     int ia[MAXI], ib[MAXI];
