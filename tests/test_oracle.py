@@ -11,6 +11,7 @@ Each check is chosen so that a plausible mistake in the restatement fails it:
 import os
 
 import numpy as np
+from scipy.spatial.transform import Rotation
 import pytest
 
 from oracle import oracle as O
@@ -233,31 +234,21 @@ def test_one_lm_step_equals_dense_damped_solve():
     lam = 1e-4
     ref = pb.copy()
     st, dxh, _acc = O.window_solve(ref, iters=1, lam0=lam, history=True)
-    _c, H6, b6, *_ = O.window_build(pb.copy(), lam)
-    H, b, _cr = O.window_full_system(pb.copy(), H6, b6)
     Hf, bf, n6 = _dense_problem(pb, lam)
-    Hf[:n6, :n6] += H - H6          # the prior's information on the pose block
-    bf[:n6] += b - b6
-    Hf[np.arange(n6), np.arange(n6)] += lam * np.diag(H) + 1e-9
+    # the prior's information on the pose block, formed here (vio/marg.py:120-127: H = S^T S, b = S^T (S d + r))
+    pr = pb.prior
+    cols = np.concatenate([np.arange(6 * s, 6 * s + d) for s, d in zip(pr.slots(), pr.dims)])
+    d = np.concatenate([np.concatenate([Rotation.from_matrix(x0[0].T @ pb.Rwb[s]).as_rotvec(), pb.pwb[s] - x0[1]])
+                        for s, x0 in zip(pr.slots(), pr.x0)])
+    Hf[np.ix_(cols, cols)] += pr.S.T @ pr.S
+    bf[cols] += pr.S.T @ (pr.S @ d + pr.r)
+    # the reduced diagonal damping lam diag(H) + 1e-9 (vio/window.py:933-935), H the reduced system: the Schur
+    # complement of the (Marquardt-damped) landmark blocks of the dense system, prior included
+    App, Apl, All = Hf[:n6, :n6], Hf[:n6, n6:], Hf[n6:, n6:]
+    Hred = App - Apl @ np.linalg.solve(All, Apl.T)
+    Hf[np.arange(n6), np.arange(n6)] += lam * np.diag(Hred) + 1e-9
     x = -np.linalg.solve(Hf, bf)
     assert _rel(dxh[0], x[:n6]) < 1e-8
-
-
-def test_gauss_newton_step_at_noise_free_truth_is_zero():
-    pb, tr = synth.make_window(P=5, L=80, views=(2, 5), seed=[9, 4], outlier_frac=0.0, px_sigma=1.0)
-    # replace the measurements by exact projections of the truth and start at the truth
-    pb.Rwb[:] = tr["Rwb"]
-    pb.pwb[:] = tr["pwb"]
-    pb.lmp[:] = tr["lmp"]
-    pb.rho_meas[:] = tr["lmp"][:, 2]
-    pb.prior = None
-    for l in range(pb.L):
-        for k in range(pb.ob_start[l], pb.ob_start[l + 1]):
-            e, *_ = O.obs_residual_jac(pb, l, k)
-            pb.ob_z[k, :len(e)] += e
-    st, dxh, _ = O.window_solve(pb.copy(), iters=1, history=True)
-    assert st["cost0"] < 1e-12
-    assert np.abs(dxh[0]).max() < 1e-9
 
 
 # ----------------------------------------------------------------------------------------------- IMU factor --

@@ -342,9 +342,11 @@ def _project(Rwc, pwc, X):
 
 def make_window(P=10, L=2000, views=(2, 10), seed=0, vi=False, stereo="prior", outlier_frac=0.05,
                 px_sigma=FEATURE_SIGMA_PX, pose_sigma_t=0.02, pose_sigma_r_deg=0.5, depth=(1.0, 20.0),
-                spacing=0.5, posefree=False, gauge=True):
+                spacing=0.5, posefree=False, gauge=True, noise_scale=1.0):
     """A seeded inverse-depth stereo window (module docstring).  ``views`` = (min, max) views per landmark including
-    the host's.  ``posefree`` moves landmarks whose only observation is the host-left one into ``Problem.pf``."""
+    the host's.  ``posefree`` moves landmarks whose only observation is the host-left one into ``Problem.pf``.
+    ``noise_scale`` multiplies the pixel and stereo-depth noise drawn (0: every measurement is the exact projection
+    of the true world point, whitened as if its sigma were ``px_sigma``); the random stream is the same."""
     rng = np.random.default_rng(seed)
     fx, fy, cx, cy, Wd, Hd, B = (CAM[k] for k in ("fx", "fy", "cx", "cy", "W", "H", "B"))
     if vi:
@@ -396,13 +398,13 @@ def make_window(P=10, L=2000, views=(2, 10), seed=0, vi=False, stereo="prior", o
     cnt = np.bincount(lm_idx, minlength=L)
     obs_l, obs_t = lm_idx, sl_idx
     n = len(obs_l)
-    zl = np.stack([uu[obs_t, obs_l], vv[obs_t, obs_l]], 1) + rng.normal(0, px_sigma, (n, 2))
+    zl = np.stack([uu[obs_t, obs_l], vv[obs_t, obs_l]], 1) + noise_scale * rng.normal(0, px_sigma, (n, 2))
     is_host = obs_t == host[obs_l]
     out = (rng.uniform(0, 1, n) < outlier_frac) & ~is_host
     zl[out] = np.stack([rng.uniform(0, Wd, out.sum()), rng.uniform(0, Hd, out.sum())], 1)
     rho_t = 1.0 / d
     if stereo == "ur":
-        zr = (fx * (q[obs_t, obs_l, 0] - B) / z[obs_t, obs_l] + cx) + rng.normal(0, px_sigma, n)
+        zr = (fx * (q[obs_t, obs_l, 0] - B) / z[obs_t, obs_l] + cx) + noise_scale * rng.normal(0, px_sigma, n)
         zr[out] = rng.uniform(0, Wd, out.sum())
         # interleave: left then right per view
         ob_pose = np.repeat(obs_t, 2)
@@ -419,7 +421,7 @@ def make_window(P=10, L=2000, views=(2, 10), seed=0, vi=False, stereo="prior", o
         ob_cam = np.zeros(n, np.uint8)
         ob_z = zl
         sig_rho = DISPARITY_NOISE_PX / (fx * B)
-        rho_meas = np.clip(rho_t + rng.normal(0, sig_rho, L), 1e-3, None)
+        rho_meas = np.clip(rho_t + noise_scale * rng.normal(0, sig_rho, L), 1e-3, None)
         rho_isig = np.full(L, 1.0 / sig_rho)
         rho0 = rho_meas.copy()
     ob_start = np.zeros(L + 1, np.int64)
