@@ -132,8 +132,12 @@ int xba_iterate(xba_ctx *ctx, int32_t iters, void *stream);
 /* Write the state back into the batch's device state arrays. */
 int xba_download(xba_ctx *ctx, void *stream);
 
-/* Per-window statistics after xba_iterate, (device) out (nwin, 8): cost0, cost, accepted, iters, lam,
- * last Cholesky status (1 ok, 0 failed), last accept (1/0), refinement steps used. */
+/* Per-window statistics after xba_iterate, (device) out (nwin, XBA_NSTATS): cost0, cost, accepted, iters, lam,
+ * last Cholesky status (1 ok, 0 failed), last accept (1/0), refinement solves of the last iteration, FP64
+ * fallbacks since the upload (mixed precision: iterations whose FP32 factor failed or whose refinement did not
+ * converge, re-linearised and solved again in FP64), last FP32 refinement converged (1/0: 0 when this iteration
+ * fell back; always 1 in FP64). */
+#define XBA_NSTATS 10
 int xba_stats(xba_ctx *ctx, double *out_dev, void *stream);
 
 /* The last iteration's full reduced update dx (sum of window state dims, FP64, device out). */
@@ -261,6 +265,14 @@ typedef struct {
 } xba_preint_batch;
 
 int xba_preintegrate(const xba_preint_batch *b, void *stream);
+
+/* Replaces vio/imu_factor.py:1051-1063 `sqrt_information(pre, kappa, bias_sigma_g, bias_sigma_a)` (numba
+ * sqrt_info_core, :707-746) for nf intervals already preintegrated: W (nf, 225, lower triangular, row-major) =
+ * L^-1 with L L^T = kappa C + J_bg sbg2 J_bg^T + J_ba sba2 J_ba^T, from C (nf, 225) and J (nf, 54) in the layout
+ * xba_preintegrate writes and par (nf, 3) = kappa, sbg2, sba2 per interval.  w_ok (nf) is 0 (and W zero) when the
+ * matrix is not positive definite.  Every pointer is a device pointer.  Asynchronous on `stream`. */
+int xba_sqrt_information(int32_t nf, const double *C, const double *J, const double *par, double *W, int32_t *w_ok,
+                         void *stream);
 
 /* ---- Marginalisation algebra of the square-root prior, batched ----------------------------------------------
  * Replaces vio/marg.py:165-177 `schur` and :73-84 `Prior.from_info` (with :212-242 `combine`'s re-expression of b

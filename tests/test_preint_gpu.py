@@ -140,3 +140,62 @@ def test_single_interval_api_matches_reference_signature():
     assert pre.ok and not pre.extrapolated and pre.n_inner == r["info"][2]
     assert _rel(pre.cov, r["C"]) < 1e-12 and np.abs(pre.dR - r["dR"]).max() < 1e-13
     assert pre.JRg.shape == (3, 3) and pre.cov9.shape == (9, 9)
+
+
+def test_sqrt_information_honours_kappa_and_bias_sigmas():
+    """vio/imu_factor.py:1051-1063: W for the caller's kappa and bias sigmas, against the oracle's sqrt_info_core
+    restatement and against the defining identity W^T W = (kappa C + BIASCOV)^-1."""
+    from xclean_amd import imu
+    t, g, a = _stream(12)
+    noise = imu.ImuNoise(1.7e-4, 2.0e-3, 1.9e-5, 3.0e-3, rate_hz=200.0)
+    pres = [imu.preintegrate(t, g, a, t0, t1, None, np.full(3, 0.01), np.full(3, -0.02), noise)
+            for t0, t1 in ((0.01, 0.31), (0.2, 0.7), (0.05, 0.06))]
+    for kappa, sg, sa in ((1.0, 0.0, 0.0), (4.0, 0.0, 0.0), (2.5, 1e-3, 2e-2), (0.3, 5e-4, 0.0)):
+        Ws = imu.sqrt_information_many(pres, kappa, sg, sa)
+        for p, W in zip(pres, Ws):
+            want = O.sqrt_info(p.cov, kappa, sg ** 2, sa ** 2, p.J)
+            assert W is not None and want is not None
+            assert _rel(W, want) < 1e-12
+            A = kappa * p.cov.copy()
+            A[:9, :9] += sg ** 2 * p.J[:, :3] @ p.J[:, :3].T + sa ** 2 * p.J[:, 3:] @ p.J[:, 3:].T
+            assert _rel(W.T @ W, np.linalg.inv(A)) < 1e-8
+            assert np.abs(np.triu(W, 1)).max() == 0.0
+        W1 = imu.sqrt_information(pres[1], kappa, sg, sa)
+        assert _rel(W1, Ws[1]) == 0.0
+    # kappa scales W by 1 / sqrt(kappa) when there is no bias term
+    W1 = imu.sqrt_information(pres[0], 1.0)
+    W4 = imu.sqrt_information(pres[0], 4.0)
+    assert _rel(W4, 0.5 * W1) < 1e-13
+
+
+def test_sqrt_information_refusals():
+    from xclean_amd import imu
+    t, g, a = _stream(13)
+    noise = imu.ImuNoise(1.7e-4, 2.0e-3, 1.9e-5, 3.0e-3, rate_hz=200.0)
+    good = imu.preintegrate(t, g, a, 0.02, 0.3, None, np.zeros(3), np.zeros(3), noise)
+    refused = imu.preintegrate(t, g, a, 0.3, 0.2, None, np.zeros(3), np.zeros(3), noise)
+    assert not refused.ok
+    with pytest.raises(ValueError):
+        imu.sqrt_information(refused, 1.0)
+    for bad in (0.0, -1.0, float("nan"), float("inf")):
+        with pytest.raises(ValueError):
+            imu.sqrt_information(good, bad)
+    # a mean-only interval has a zero covariance: not positive definite
+    mean_only = imu.preintegrate(t, g, a, 0.02, 0.3, None, np.zeros(3), np.zeros(3), None)
+    assert imu.sqrt_information_many([mean_only])[0] is None
+    with pytest.raises(ValueError):
+        imu.sqrt_information(mean_only)
+
+
+def test_engine_call_pair_preintegrate_then_sqrt_information():
+    """The engine's call pair (vio/engine_live.py:898, 1134): preintegrate(...) then sqrt_information(pre, kappa)
+    equals the whitened batch launch at that kappa."""
+    from xclean_amd import imu
+    t, g, a = _stream(14)
+    noise = imu.ImuNoise(1.7e-4, 2.0e-3, 1.9e-5, 3.0e-3)
+    pre = imu.preintegrate(t, g, a, 0.0123, 0.4567, None, np.zeros(3), np.zeros(3), noise)
+    W = imu.sqrt_information(pre, 3.0)
+    batch = imu.preintegrate_many([(t, g, a)], [(0, 0.0123, 0.4567, None, np.zeros(3), np.zeros(3))], noise,
+                                  kappa=3.0, whiten=True)[0]
+    assert _rel(W, batch.W) < 1e-13
+    assert _rel(W, O.sqrt_info(pre.cov, 3.0, 0.0, 0.0, pre.J)) < 1e-12

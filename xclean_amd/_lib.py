@@ -15,6 +15,7 @@ _LIB = None
 XBA_FP64, XBA_MIXED = 0, 1
 XBA_WPAR = 24
 XBA_IMU_REC = 301
+XBA_NSTATS = 10  # columns of xba_stats (include/xba.h)
 
 vp = C.c_void_p
 i32p = C.POINTER(C.c_int32)
@@ -62,7 +63,7 @@ class XbaError(RuntimeError):
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
-           "xba_np_solve", "xba_np_id_solve", "xba_preintegrate",
+           "xba_np_solve", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
            "xba_marg_schur", "xba_prior_from_info",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
@@ -93,6 +94,7 @@ def load():
     L.xba_np_id_solve.argtypes = [C.POINTER(XbaNpidProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_copy_kernel.argtypes = [vp, vp, C.c_int64, vp]
     L.xba_preintegrate.argtypes = [vp, vp]
+    L.xba_sqrt_information.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
     L.xba_marg_schur.argtypes = [vp, vp]
     L.xba_prior_from_info.argtypes = [vp, vp]
     L.xba_strerror.restype = C.c_char_p

@@ -78,9 +78,9 @@ __global__ void k_stats(int nwin, const double *ws, double *out) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwin) return;
     const double *s = ws + (size_t)NWS * w;
-    double *o = out + 8 * (size_t)w;
+    double *o = out + XBA_NSTATS * (size_t)w;
     o[0] = s[WS_COST0]; o[1] = s[WS_CUR]; o[2] = s[WS_ACC]; o[3] = s[WS_ITERS]; o[4] = s[WS_LAM];
-    o[5] = s[WS_CHOLOK]; o[6] = s[WS_ACCEPT]; o[7] = s[WS_NREF];
+    o[5] = s[WS_CHOLOK]; o[6] = s[WS_ACCEPT]; o[7] = s[WS_NREF]; o[8] = s[WS_NFALLBACK]; o[9] = s[WS_CONV];
 }
 
 __global__ void k_cost_out(int nwin, const double *ws, double *out) {
@@ -140,6 +140,7 @@ struct xba_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool solve_fork = false, wide_fork = false;
+    int fb_lds_cap = 0;  // LDS of the FP64 fallback of the mixed solve
     int maxn_small = 0;  // largest n among the windows k_solve factors
 };
 
@@ -225,6 +226,7 @@ DArgs dargs(xba_ctx *c) {
     D.partc = c->partc;
     D.partc2 = c->partc2;
     D.nref_max = 6;
+    D.fallback_on = c->opt.precision == XBA_MIXED ? 1 : 0;
     D.nbig = c->nbig;
     D.bigs = c->bigs;
     D.bigF32 = c->bigF32;
@@ -270,6 +272,7 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
     F.pass_l0 = A.pass_l0; F.sched = A.sched; F.sched_ctr = A.sched_ctr; F.pass_cost = A.pass_cost;
     F.ob_w = A.ob_w;
     F.chunk_slots = A.chunk_slots;
+    F.only_fallback = A.only_fallback;
     return F;
 }
 
@@ -293,16 +296,18 @@ inline void join_wide(xba_ctx *c, hipStream_t s) {
     (void)hipStreamWaitEvent(s, c->ev_join, 0);
 }
 
-template <typename T>
+// RT: the observation record type of the multi-wavefront kernels (T, or double for the mixed solve's FP64
+// re-assembly of the windows it flagged, with A.only_fallback set).
+template <typename T, typename RT = T>
 void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
         const KArgs<double> F = fast_args<T>(c, A);
         const bool fork = fork_wide(c, s, part);
         if (c->nnarrow > 0 && part != 1)
-            hipLaunchKernelGGL((k_lin_fast<T, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, F,
+            hipLaunchKernelGGL((k_lin_fast<RT, 0>), dim3(c->grid), dim3(XBA_WAVE * c->fast_nw), c->fast_lds, s, F,
                                c->Rnarrow, 0, c->nnarrow, 0);
         if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_lin_fast<T, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds,
+            hipLaunchKernelGGL((k_lin_fast<RT, 1>), dim3(c->grid_wide), dim3(XBA_WAVE * c->wide_nw), c->wide_lds,
                                fork ? c->side : s, F, c->Rwide, c->nnarrow, c->nchunks, 1);
         if (fork) join_wide(c, s);
     } else if (part != 1) {
@@ -356,7 +361,7 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     if (fork) {
         (void)hipEventRecord(c->ev_fork, s);
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, c->side, D, c->solve_lds_cap);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, c->side, D, c->solve_lds_cap, 0);
         (void)hipEventRecord(c->ev_join, c->side);
     }
     if (c->nbig > 0) {
@@ -379,7 +384,22 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     if (fork)
         (void)hipStreamWaitEvent(s, c->ev_join, 0);
     else if (c->nbig < c->nwin)
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap);
+        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap, 0);
+    // mixed precision: the windows flagged above are linearised again with FP64 observation records (the
+    // multi-wavefront kernels; the FP32 records perturb H by ~1e-7 relative, which an ill-conditioned window
+    // amplifies) and solved in FP64.  Workgroups of the other windows exit at once.
+    if (sizeof(T) == 4 && D.fallback_on) {
+        if (c->fast) {
+            KArgs<T> A = kargs<T>(c);
+            A.only_fallback = 1;
+            launch_lin_kernel<T, double>(c, A, s);
+            DArgs Df = D;
+            Df.only_fallback = 1;
+            launch_reduce<T>(c, Df, s);
+            hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, Df, 0);  // the IMU records are current
+        }
+        hipLaunchKernelGGL(k_solve<double>, dim3(c->nwin), dim3(512), c->fb_lds_cap, s, D, c->fb_lds_cap, 1);
+    }
 }
 
 template <typename T>
@@ -428,6 +448,17 @@ int xba_preintegrate(const xba_preint_batch *b, void *stream) {
     const int grid = (b->nf + PRE_NG - 1) / PRE_NG;
     hipLaunchKernelGGL(k_preint, dim3(grid), dim3(PRE_NG * PRE_G), 0, (hipStream_t)stream, b->t, b->gyr, b->acc,
                        b->s0, b->ns, b->par, b->nf, b->dR, b->dv, b->dp, b->J, b->C, b->info, b->W, b->w_ok, b->rec);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+int xba_sqrt_information(int32_t nf, const double *C, const double *J, const double *par, double *W, int32_t *w_ok,
+                         void *stream) {
+    if (nf < 0) return XBA_E_ARG;
+    if (nf == 0) return 0;
+    if (!C || !J || !par || !W || !w_ok) return XBA_E_ARG;
+    hipLaunchKernelGGL(k_sqrt_info, dim3((nf + 63) / 64), dim3(64), 0, (hipStream_t)stream, (int)nf, C, J, par, W,
+                       w_ok);
     HIPCK(hipGetLastError());
     return 0;
 }
@@ -997,10 +1028,10 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         if (mx) {
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        } else {
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         }
+        // FP64 records: the FP64 path, and the mixed solve's FP64 re-assembly of flagged windows (same layout)
+        HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
@@ -1023,6 +1054,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     if (c->solve_lds_cap > 64 * 1024)
         HIPCK(hipFuncSetAttribute(mx ? (const void *)k_solve<float> : (const void *)k_solve<double>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, c->solve_lds_cap));
+    if (mx) {  // the FP64 fallback of the mixed solve: the matrix in LDS when it fits, else in global memory
+        HIPCK(hipFuncGetAttributes(&fa, (const void *)k_solve<double>));
+        const int cap64 = 160 * 1024 - (int)fa.sharedSizeBytes - 256;
+        const int64_t n64 = std::max(c->maxn, 1);
+        c->fb_lds_cap = (int)std::max<int64_t>(0, std::min<int64_t>(n64 * ((n64 + 3) & ~3) * 8, cap64));
+        if (c->fb_lds_cap > 64 * 1024)
+            HIPCK(hipFuncSetAttribute((const void *)k_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      c->fb_lds_cap));
+    }
     c->solve_fork = c->nbig > 0 && c->nbig < c->nwin;
     c->wide_fork = c->fast && c->nnarrow > 0 && c->nchunks > c->nnarrow;
     if (c->solve_fork || c->wide_fork) {

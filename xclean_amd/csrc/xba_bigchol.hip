@@ -380,7 +380,31 @@ __global__ __launch_bounds__(1024) void k_big_trsv(DArgs D, int it) {
             __syncthreads();
         }
     }
-    for (int i = tid; i < n; i += nt) x[i] = (it == 0 ? 0.0 : x[i]) + (double)sv[i] * ss[i];
+    double cmax = 0.0, xmax = 0.0;
+    for (int i = tid; i < n; i += nt) {
+        const double c = (double)sv[i] * ss[i], xi = (it == 0 ? 0.0 : x[i]) + c;
+        x[i] = xi;
+        cmax = fmax(cmax, fabs(c));
+        xmax = fmax(xmax, fabs(xi));
+    }
+    if (sizeof(T) == 4) {  // refinement: converged once the correction is small against the solution
+        cmax = dwave_max(cmax);
+        xmax = dwave_max(xmax);
+        __syncthreads();  // sred is free again (its last readers were before the solves)
+        if ((tid & 63) == 0) {
+            sred[2 * (tid >> 6)] = cmax;
+            sred[2 * (tid >> 6) + 1] = xmax;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, c = 0.0;
+            for (int q = 0; q < nt / 64; q++) {
+                a = fmax(a, sred[2 * q]);
+                c = fmax(c, sred[2 * q + 1]);
+            }
+            if (a <= XBA_REFINE_TOL * c) ws[WS_BIGCONV] = 1.0;
+        }
+    }
     if (tid == 0) ws[WS_NREF] = it + 1;
 }
 
@@ -434,13 +458,21 @@ __global__ __launch_bounds__(256) void k_big_finish(DArgs D) {
     const int n = B.n, tid = threadIdx.x, nt = blockDim.x;
     double *ws = D.wstate + (size_t)NWS * B.win;
     const bool ok = ws[WS_BIGOK] != 0.0;
+    // mixed: converged by the residual or the correction test; FP64 (one solve, no refinement): always
+    const bool conv = !D.fallback_on || ws[WS_BIGCONV] != 0.0;
+    const bool fb = D.fallback_on && !(ok && conv);  // left to the FP64 fallback (k_solve<double>, fallback mode)
     const double *x = D.bigV + B.v0;
-    for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = ok ? -x[i] : NAN;
-    __syncthreads();
-    if (ok) trial_state(D, W, tid, nt);
+    if (!fb) {
+        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = ok ? -x[i] : NAN;
+        __syncthreads();
+        if (ok) trial_state(D, W, tid, nt);
+    }
     if (tid == 0) {
         ws[WS_CHOLOK] = ok ? 1.0 : 0.0;
         if (!ok) ws[WS_NREF] = 0.0;
+        ws[WS_CONV] = (ok && conv) ? 1.0 : 0.0;
+        ws[WS_FALLBACK] = fb ? 1.0 : 0.0;
+        if (fb) ws[WS_NFALLBACK] += 1.0;
     }
 }
 

@@ -257,6 +257,8 @@ struct DArgs {
     const double *partc, *partc2;
     double *imu_chi2;                     // (nimu) or null: whitened chi2 of every IMU factor
     int32_t nref_max;
+    int32_t fallback_on;                  // mixed precision: windows whose FP32 solve fails go to FP64
+    int32_t only_fallback;                // k_reduce / k_terms: only the windows flagged WS_FALLBACK
     int32_t nbig;                         // windows solved by the tiled Cholesky
     const BigWin *bigs;                   // (nbig)
     float *bigF32;                        // tiled factors (sum np^2), mixed precision
@@ -293,6 +295,7 @@ struct Acc {  // FP64: plain sum; FP32: Kahan-compensated sum
 template <typename T>
 __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH_lo, const T *partb_lo) {
     const int w = blockIdx.y;
+    if (D.only_fallback && D.wstate[(size_t)NWS * w + WS_FALLBACK] == 0.0) return;
     const WinInfo W = D.wins[w];
     const int n = W.n;
     const bool kah = sizeof(T) == 4;
@@ -428,6 +431,7 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
     __shared__ int scol[XBA_MAX_STATE];
     __shared__ double sW[225], sr[15], sJi[225], sJj[225], sred[256];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (D.only_fallback && D.wstate[(size_t)NWS * w + WS_FALLBACK] == 0.0) return;
     const WinInfo W = D.wins[w];
     const int n = W.n;
     const double *R = trial ? D.R2 : D.Rwb, *p = trial ? D.p2 : D.pwb;
@@ -785,10 +789,14 @@ __device__ inline void trial_state(const DArgs &D, const WinInfo &W, int tid, in
 
 // ------------------------------------------------------------------------------------------- k_solve --
 // Damped Cholesky solve of every window, dx = -Hd^-1 b, then the trial state.  FP64 (T = double) factors Hd
-// itself; MIXED (T = float) factors the Jacobi-scaled Hd in FP32 and refines x in FP64 against Hd.
-// `lam_mode`: 0 the window's lambda; the factor is kept in fact64/fact32 for the adjoint.
+// itself; MIXED (T = float) factors the Jacobi-scaled Hd in FP32 and refines x in FP64 against Hd until the last
+// correction is at most XBA_REFINE_TOL of x or the residual at most 1e-15 of b (at most nref_max solves).  A mixed
+// window whose FP32 factor fails or whose refinement does not meet that test within the budget is flagged
+// (WS_FALLBACK) and left to the FP64 fallback: k_solve<double> launched with `fallback` = 1 after every mixed solve
+// of the iteration (one-workgroup and tiled), which solves exactly the flagged windows, small or tiled, in FP64
+// (in LDS when the matrix fits `lds_cap`, else in the window's slot of the FP64 factor buffer).
 template <typename T>
-__global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
+__global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap, int fallback) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
     __shared__ T sy[XBA_MAX_STATE];
@@ -796,10 +804,15 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     __shared__ double sred[32];
     __shared__ int sflag;
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
     const WinInfo W = D.wins[w];
-    if (W.big >= 0) return;  // factored by the tiled kernels (k_big_*)
-    const int n = W.n;
     double *ws = D.wstate + (size_t)NWS * w;
+    if (fallback) {
+        if (ws[WS_FALLBACK] == 0.0) return;  // written by this iteration's mixed solve of the window
+    } else if (W.big >= 0) {
+        return;  // factored by the tiled kernels (k_big_*)
+    }
+    const int n = W.n;
     const double lam = ws[WS_LAM];
     const double *H = D.H + W.h0, *bv = D.b + W.st0;
     // in LDS the rows are padded to a multiple of 4 entries
@@ -824,6 +837,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
     // the factor and solves inlined once per memory space
     bool ok = in_lds ? chol_wg<T>(Fl, n, &sflag, sinv, ld) : chol_wg<T>(Fg, n, &sflag, sinv, ld);
     int nref = 0;
+    bool conv = sizeof(T) == 8;
     if (ok) {
         for (int i = tid; i < n; i += nt) {
             sx[i] = 0.0;
@@ -838,13 +852,37 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
                 chol_solve_wg<T>(Fl, n, sy, ld);
             else
                 chol_solve_wg<T>(Fg, n, sy, ld);
-            for (int i = tid; i < n; i += nt) sx[i] += (double)sy[i] * ss[i];
+            double cmax = 0.0, xmax = 0.0;
+            for (int i = tid; i < n; i += nt) {
+                const double c = (double)sy[i] * ss[i], xi = sx[i] + c;
+                sx[i] = xi;
+                cmax = fmax(cmax, fabs(c));
+                xmax = fmax(xmax, fabs(xi));
+            }
             __syncthreads();
             nref = it + 1;
+            if (sizeof(T) == 8) break;
+            // the correction against the solution (block maxima; NaN compares false and never converges)
+            cmax = dwave_max(cmax);
+            xmax = dwave_max(xmax);
+            if (lane == 0) {
+                sred[2 * wv] = cmax;
+                sred[2 * wv + 1] = xmax;
+            }
+            __syncthreads();
+            double cn = 0.0, xn = 0.0;
+            for (int q = 0; q < nw; q++) {
+                cn = fmax(cn, sred[2 * q]);
+                xn = fmax(xn, sred[2 * q + 1]);
+            }
+            __syncthreads();
+            if (cn <= XBA_REFINE_TOL * xn) {
+                conv = true;
+                break;
+            }
             if (it + 1 == maxit) break;
             // residual r = b - Hd x in FP64, entry i by thread i from column i of the symmetric H (row j's entry i:
             // the threads of a wavefront read consecutive addresses)
-            const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
             double rmax = 0.0, bmax = 0.0;
             for (int i = tid; i < n; i += nt) {
                 // 16 column entries in flight per thread, summed in column order
@@ -876,17 +914,31 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap) {
                 bn = fmax(bn, sred[2 * q + 1]);
             }
             __syncthreads();
-            if (rn <= 1e-15 * bn) break;
+            if (rn <= 1e-15 * bn) {
+                conv = true;
+                break;
+            }
         }
-        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
-        __syncthreads();
-        trial_state(D, W, tid, nt);
-    } else {
-        for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
+    }
+    // mixed: a failed factor or an unconverged refinement leaves the window to the FP64 fallback
+    const bool fb = sizeof(T) == 4 && D.fallback_on && !(ok && conv);
+    if (!fb) {
+        if (ok) {
+            for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = -sx[i];
+            __syncthreads();
+            trial_state(D, W, tid, nt);
+        } else {
+            for (int i = tid; i < n; i += nt) D.dx[W.st0 + i] = NAN;
+        }
     }
     if (tid == 0) {
         ws[WS_CHOLOK] = ok ? 1.0 : 0.0;
-        ws[WS_NREF] = nref;
+        if (!fallback) {  // the fallback leaves the mixed solve's refinement record as it was
+            ws[WS_NREF] = nref;
+            ws[WS_CONV] = conv ? 1.0 : 0.0;
+            ws[WS_FALLBACK] = fb ? 1.0 : 0.0;
+            if (fb) ws[WS_NFALLBACK] += 1.0;
+        }
     }
 }
 
@@ -935,7 +987,7 @@ __global__ __launch_bounds__(256) void k_accept(DArgs D) {
 
 template __global__ void k_reduce<float>(DArgs, const float *, const float *, const float *, const float *);
 template __global__ void k_reduce<double>(DArgs, const double *, const double *, const double *, const double *);
-template __global__ void k_solve<float>(DArgs, int);
-template __global__ void k_solve<double>(DArgs, int);
+template __global__ void k_solve<float>(DArgs, int, int);
+template __global__ void k_solve<double>(DArgs, int, int);
 
 }  // namespace xba

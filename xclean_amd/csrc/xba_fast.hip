@@ -325,6 +325,8 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
         const int c = next_chunk(A, kind, cbeg, cend, ctl);
         if (c < 0) break;
         const Chunk ch = A.chunks[c];
+        // the mixed solve's FP64 re-assembly visits only the windows it flagged (uniform per workgroup)
+        if (A.only_fallback && A.wstate[(size_t)NWS * ch.win + WS_FALLBACK] == 0.0) continue;
         const int32_t *smap = ch.smap >= 0 ? A.chunk_slots + ch.smap : nullptr;
         const WinInfo W = A.wins[ch.win];
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;

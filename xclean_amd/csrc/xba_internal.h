@@ -64,7 +64,11 @@ enum {
     WS_COSTTRIAL_REST,
     WS_BIGOK,       // tiled Cholesky of the window: 1 while every pivot so far was positive and finite
     WS_BIGCONV,     // tiled solve: 1 once the refinement has converged in this LM iteration
-    NWS = 16
+    WS_FALLBACK,    // mixed precision: 1 when this iteration's FP32 factor failed or its refinement did not converge,
+                    // so the FP64 solve (k_solve<double> in fallback mode) produced the update
+    WS_NFALLBACK,   // FP64 fallbacks since the last upload
+    WS_CONV,        // 1 when this iteration's solve met the refinement criterion (always 1 in FP64)
+    NWS = 20
 };
 
 // A window whose reduced system is factored by the tiled multi-workgroup Cholesky: its matrix is padded to np, a
@@ -73,6 +77,9 @@ enum {
 // (or ones) at s0 of the scale buffer; its refinement vectors (solution x, residual r, per-workgroup residual and
 // right-hand-side maxima) at v0 of the refinement buffer (3 np entries).
 #define XBA_TILE 64
+// Mixed-precision refinement stops once the last FP64-accumulated correction is at most this fraction of the
+// solution (max norms), or once the FP64 residual is at most 1e-15 of the right-hand side.
+#define XBA_REFINE_TOL 1e-10
 #define XBA_RES_ROWS 4  // rows per workgroup of the refinement residual
 struct BigWin {
     int32_t win, n, np, nb;  // window, state dimension, padded dimension, tiles per side (np / XBA_TILE)
@@ -125,4 +132,5 @@ struct KArgs {
     double *pass_cost;         // multi-wave kernels: robust cost of every pass (summed per work item in pass order)
     const double *ob_w;        // (nobs) per-observation weight multiplier, or null (1)
     const int32_t *chunk_slots; // tile slot lists of the work items with smap >= 0
+    int32_t only_fallback;     // 1: only the work items of windows flagged WS_FALLBACK (the FP64 re-assembly)
 };

@@ -107,6 +107,50 @@ __device__ inline void pre_qcol(int c, const double *q, const PreLds &S, double 
     }
 }
 
+// A <- kappa A + J_bg sbg2 J_bg^T + J_ba sba2 J_ba^T on the nine preintegration rows (vio/imu_factor.py:707-722),
+// A row-major 15 x 15, J row-major 9 x 6 (columns bg, ba).
+__device__ inline void pre_info_matrix(double *A, const double *J, double kappa, double sbg2, double sba2) {
+    for (int i = 0; i < 225; i++) A[i] = kappa * A[i];
+    if (sbg2 > 0.0 || sba2 > 0.0)
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) {
+                double s = 0.0;
+                for (int q = 0; q < 3; q++)
+                    s += sbg2 * J[6 * i + q] * J[6 * j + q] + sba2 * J[6 * i + 3 + q] * J[6 * j + 3 + q];
+                A[15 * i + j] += s;
+            }
+}
+
+// The whitening of vio/imu_factor.py:723-746 in place: the lower Cholesky factor L of A (lower triangle read), then
+// W = L^-1 column by column over L's own storage (column j of W needs L(i, q) for q >= j only, and L(i, j) is read
+// before W(i, j) replaces it); the upper triangle is zeroed.  False (A then undefined) when A is not positive
+// definite.
+__device__ inline bool pre_whiten_inplace(double *A) {
+    for (int j = 0; j < 15; j++) {
+        double s = A[15 * j + j];
+        for (int q = 0; q < j; q++) s -= A[15 * j + q] * A[15 * j + q];
+        if (!(s > 0.0)) return false;
+        const double dj = sqrt(s);
+        A[15 * j + j] = dj;
+        for (int i = j + 1; i < 15; i++) {
+            s = A[15 * i + j];
+            for (int q = 0; q < j; q++) s -= A[15 * i + q] * A[15 * j + q];
+            A[15 * i + j] = s / dj;
+        }
+    }
+    for (int j = 0; j < 15; j++) {
+        const double ljj = A[15 * j + j];
+        A[15 * j + j] = 1.0 / ljj;
+        for (int i = j + 1; i < 15; i++) {
+            double s = 0.0;
+            for (int q = j; q < i; q++) s += A[15 * i + q] * A[15 * q + j];
+            A[15 * i + j] = -s / A[15 * i + i];
+        }
+        for (int i = 0; i < j; i++) A[15 * i + j] = 0.0;
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(64) void k_preint(const double *__restrict__ ts, const double *__restrict__ gyr,
                                                 const double *__restrict__ acc, const int64_t *__restrict__ s0,
                                                 const int32_t *__restrict__ ns, const double *__restrict__ par, int nf,
@@ -456,57 +500,17 @@ __global__ __launch_bounds__(64) void k_preint(const double *__restrict__ ts, co
         for (int i = 0; i < 9; i++) Js[6 * i + c] = J[i];
     WAVE_SYNC();
     if (c == 0) {
-        const double kappa = pr[PP_KAPPA], sbg2 = pr[PP_SBG2], sba2 = pr[PP_SBA2];
         double *A = S.Y;
-        for (int i = 0; i < 225; i++) A[i] = kappa * A[i];
-        if (sbg2 > 0.0 || sba2 > 0.0)
-            for (int i = 0; i < 9; i++)
-                for (int j = 0; j < 9; j++) {
-                    double s = 0.0;
-                    for (int q = 0; q < 3; q++)
-                        s += sbg2 * Js[6 * i + q] * Js[6 * j + q] + sba2 * Js[6 * i + 3 + q] * Js[6 * j + 3 + q];
-                    A[15 * i + j] += s;
-                }
-        // lower Cholesky in place (below and on the diagonal of A)
-        bool ok = true;
-        for (int j = 0; j < 15 && ok; j++) {
-            double s = A[15 * j + j];
-            for (int q = 0; q < j; q++) s -= A[15 * j + q] * A[15 * j + q];
-            if (!(s > 0.0)) {
-                ok = false;
-                break;
-            }
-            const double dj = sqrt(s);
-            A[15 * j + j] = dj;
-            for (int i = j + 1; i < 15; i++) {
-                s = A[15 * i + j];
-                for (int q = 0; q < j; q++) s -= A[15 * i + q] * A[15 * j + q];
-                A[15 * i + j] = s / dj;
-            }
-        }
-        // W = L^-1, column by column (W's column j depends on L only), into the upper triangle's mirror: W is
-        // written to the output directly
+        pre_info_matrix(A, Js, pr[PP_KAPPA], pr[PP_SBG2], pr[PP_SBA2]);
+        const bool ok = pre_whiten_inplace(A);
+        // W (or zero when kappa C + BIASCOV is not positive definite) to the output and into the solver's record
         double *Wd = oW ? oW + 225 * (size_t)f : nullptr;
         double *rec = orec ? orec + (size_t)XBA_IMU_REC * f : nullptr;
         double *Wr = rec ? rec + 76 : nullptr;
         for (int i = 0; i < 225; i++) {
-            if (Wd) Wd[i] = 0.0;
-            if (Wr) Wr[i] = 0.0;
-        }
-        if (ok) {
-            for (int j = 0; j < 15; j++) {
-                double col[15];
-                col[j] = 1.0 / A[15 * j + j];
-                for (int i = j + 1; i < 15; i++) {
-                    double s = 0.0;
-                    for (int q = j; q < i; q++) s += A[15 * i + q] * col[q];
-                    col[i] = -s / A[15 * i + i];
-                }
-                for (int i = j; i < 15; i++) {
-                    if (Wd) Wd[15 * i + j] = col[i];
-                    if (Wr) Wr[15 * i + j] = col[i];
-                }
-            }
+            const double v = ok ? A[i] : 0.0;
+            if (Wd) Wd[i] = v;
+            if (Wr) Wr[i] = v;
         }
         if (wok) wok[f] = ok ? 1 : 0;
         if (rec) {  // the window solver's IMU record: dR 9, dv 3, dp 3, dt 1, J 54, bg_lin 3, ba_lin 3, W 225
@@ -521,6 +525,24 @@ __global__ __launch_bounds__(64) void k_preint(const double *__restrict__ ts, co
             for (int i = 0; i < 54; i++) rec[16 + i] = Js[i];
         }
     }
+}
+
+// vio/imu_factor.py:1051-1063 `sqrt_information` for intervals already preintegrated: one thread per interval,
+// W = L^-1 with L L^T = kappa C + BIASCOV formed and factored in W's own storage (C, J in the Preint layout:
+// row-major 15 x 15 residual-order covariance and 9 x 6 bias Jacobians; par (nf, 3): kappa, sbg2, sba2).  w_ok 0 and
+// W zero when the matrix is not positive definite.
+__global__ __launch_bounds__(64) void k_sqrt_info(int nf, const double *__restrict__ C, const double *__restrict__ J,
+                                                  const double *__restrict__ par, double *W, int32_t *w_ok) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    double *A = W + 225 * (size_t)f;
+    const double *Cf = C + 225 * (size_t)f;
+    for (int i = 0; i < 225; i++) A[i] = Cf[i];
+    pre_info_matrix(A, J + 54 * (size_t)f, par[3 * (size_t)f], par[3 * (size_t)f + 1], par[3 * (size_t)f + 2]);
+    const bool ok = pre_whiten_inplace(A);
+    if (!ok)
+        for (int i = 0; i < 225; i++) A[i] = 0.0;
+    w_ok[f] = ok ? 1 : 0;
 }
 
 }  // namespace xba

@@ -19,53 +19,78 @@ import numpy as np
 
 from . import _lib
 
+# Status codes of a preintegration request (the reference's values, vio/imu_factor.py:178-184).
 OK, REFUSED_BAD_INTERVAL, REFUSED_NO_DATA, REFUSED_BEFORE_FIRST_SAMPLE, REFUSED_NONFINITE, REFUSED_NO_RATE = range(6)
-STATUS_TEXT = {OK: "ok", REFUSED_BAD_INTERVAL: "t1 <= t0 or non-finite interval",
-               REFUSED_NO_DATA: "no IMU sample released", REFUSED_BEFORE_FIRST_SAMPLE: "t0 before the first sample",
-               REFUSED_NONFINITE: "non-finite IMU sample inside the interval",
-               REFUSED_NO_RATE: "no IMU sample period (rate not declared, fewer than two released stamps)"}
+STATUS_TEXT = {
+    OK: "ok",
+    REFUSED_BAD_INTERVAL: "refused: the interval is empty, reversed or not finite",
+    REFUSED_NO_DATA: "refused: the horizon releases no sample",
+    REFUSED_BEFORE_FIRST_SAMPLE: "refused: the interval starts before the stream's first sample",
+    REFUSED_NONFINITE: "refused: a sample the interval uses is not finite",
+    REFUSED_NO_RATE: "refused: no sample period for the covariance (no declared rate, under two released stamps)",
+}
 I_STATUS, I_NSUB, I_NINNER, I_EXTRAP, I_TEND, I_GAPMAX, I_NREL, I_TNOM, I_SMAX = range(9)
 NPAR = 17
 
+# calib["imu_noise"] keys in ImuNoise's argument order (Kalibr/EuRoC names)
+CALIB_KEYS = ("gyroscope_noise_density", "accelerometer_noise_density", "gyroscope_random_walk",
+              "accelerometer_random_walk")
+
+
+def _finite_positive(label, value):
+    x = float(value)
+    if math.isfinite(x) and x > 0.0:
+        return x
+    raise ValueError("%s must be a finite positive number (got %r)" % (label, value))
+
+
+def _median_spacing(stamps):
+    """The median gap of a stamp sequence, or 0.0 when there are fewer than two stamps or it is not a positive
+    finite number."""
+    t = np.asarray(stamps, dtype=np.float64).reshape(-1)
+    if t.size < 2:
+        return 0.0
+    gap = float(np.median(np.diff(t)))
+    return gap if (math.isfinite(gap) and gap > 0.0) else 0.0
+
 
 class ImuNoise:
-    """vio/imu_factor.py:897-940: the declared continuous densities (sigma_g, sigma_a white; rw_g, rw_a bias walks)
-    and the optional sample rate of the discrete noise model.  Non-positive or non-finite densities are refused."""
+    """Continuous-time IMU noise of vio/imu_factor.py:897-940: white gyro and accelerometer densities sigma_g,
+    sigma_a and bias random walks rw_g, rw_a (Kalibr units), plus the sample rate of the discrete noise model
+    (None: measured from the stamps of each request).  Every density is required and must be positive and finite;
+    nothing is substituted for a missing one."""
     __slots__ = ("sigma_g", "sigma_a", "rw_g", "rw_a", "rate_hz")
 
     def __init__(self, sigma_g, sigma_a, rw_g, rw_a, rate_hz=None):
-        vals = [float(sigma_g), float(sigma_a), float(rw_g), float(rw_a)]
-        for name, v in zip(self.__slots__, vals):
-            if not (math.isfinite(v) and v > 0.0):
-                raise ValueError("ImuNoise.%s = %r: a declared density must be finite and > 0" % (name, v))
-        self.sigma_g, self.sigma_a, self.rw_g, self.rw_a = vals
-        if rate_hz is not None:
-            rate_hz = float(rate_hz)
-            if not (math.isfinite(rate_hz) and rate_hz > 0.0):
-                raise ValueError("ImuNoise.rate_hz = %r: must be finite and > 0, or None (measured)" % rate_hz)
-        self.rate_hz = rate_hz
+        self.sigma_g = _finite_positive("gyro noise density sigma_g", sigma_g)
+        self.sigma_a = _finite_positive("accelerometer noise density sigma_a", sigma_a)
+        self.rw_g = _finite_positive("gyro bias random walk rw_g", rw_g)
+        self.rw_a = _finite_positive("accelerometer bias random walk rw_a", rw_a)
+        self.rate_hz = None if rate_hz is None else _finite_positive("IMU sample rate rate_hz", rate_hz)
 
     @classmethod
     def from_calib(cls, calib, t=None):
-        """Densities from calib `imu_noise`, the rate from `imu_rate_hz` or 1 / the median spacing of `t`."""
-        n = (calib or {}).get("imu_noise") or {}
-        rate = float((calib or {}).get("imu_rate_hz") or 0.0)
-        if not rate > 0.0 and t is not None:
-            tt = np.asarray(t, dtype=np.float64)
-            if tt.shape[0] >= 2:
-                med = float(np.median(np.diff(tt)))
-                rate = 1.0 / med if med > 0.0 else 0.0
-        try:
-            return cls(n["gyroscope_noise_density"], n["accelerometer_noise_density"], n["gyroscope_random_walk"],
-                       n["accelerometer_random_walk"], rate if rate > 0.0 else None)
-        except KeyError as e:
-            raise ValueError("calib imu_noise lacks %s: the IMU factor needs all four declared densities" % e)
+        """The four densities of calib["imu_noise"] (all required); the rate is calib["imu_rate_hz"] when that is
+        positive, else 1 / the median spacing of the stamps `t` when given, else left to be measured per request."""
+        calib = calib or {}
+        table = calib.get("imu_noise") or {}
+        absent = [k for k in CALIB_KEYS if k not in table]
+        if absent:
+            raise ValueError("IMU noise calibration is incomplete: %s not given" % ", ".join(absent))
+        declared = float(calib.get("imu_rate_hz") or 0.0)
+        rate = declared if declared > 0.0 else 0.0
+        if rate == 0.0 and t is not None:
+            gap = _median_spacing(t)
+            rate = 1.0 / gap if gap > 0.0 else 0.0
+        return cls(*(table[k] for k in CALIB_KEYS), rate_hz=rate if rate > 0.0 else None)
 
     def squares(self):
         return self.sigma_g ** 2, self.sigma_a ** 2, self.rw_g ** 2, self.rw_a ** 2
 
 
 def _as_noise(noise):
+    """An ImuNoise from an ImuNoise, a calib imu_noise table, a (sigma_g, sigma_a, rw_g, rw_a[, rate]) sequence or
+    None (mean only)."""
     if noise is None or isinstance(noise, ImuNoise):
         return noise
     if isinstance(noise, dict):
@@ -74,49 +99,34 @@ def _as_noise(noise):
 
 
 def t_nom(noise, t, horizon):
-    """vio/imu_factor.py:958-970: 1 / the declared rate, else the median spacing of the released stamps (t <=
-    horizon), else 0 (a covariance is then refused)."""
+    """The nominal sample period of the sample-support rule (vio/imu_factor.py:958-970): 1 / the declared rate,
+    else the median spacing of the stamps released by `horizon` (t <= horizon), else 0 (no covariance then)."""
     if noise is None:
         return 0.0
     if noise.rate_hz is not None:
         return 1.0 / noise.rate_hz
-    t = np.asarray(t, np.float64)
-    nrel = int(np.searchsorted(t, horizon, side="right"))
-    if nrel < 2:
-        return 0.0
-    med = float(np.median(np.diff(t[:nrel])))
-    return med if (math.isfinite(med) and med > 0.0) else 0.0
+    stamps = np.asarray(t, np.float64)
+    return _median_spacing(stamps[:int(np.searchsorted(stamps, horizon, side="right"))])
+
+
+def _block(r0, c0):
+    return property(lambda self: self.J[r0:r0 + 3, c0:c0 + 3])
 
 
 class Preint:
-    """One preintegrated interval [t0, t1] in the body frame at t0 (vio/imu_factor.py:973-1005).  `ok` is False
-    for a refused request (see `reason`); then no other field is meaningful.  `W` is the whitening when it was
-    requested (None when not positive definite)."""
+    """One preintegrated interval [t0, t1] in the body frame at t0, with the fields of vio/imu_factor.py:973-1005.
+    When `ok` is False the request was refused (`reason` says why) and only the status fields mean anything.  `W`
+    is set by a whitened launch (None when the information is not positive definite)."""
     __slots__ = ("t0", "t1", "dt", "dR", "dv", "dp", "J", "bg_lin", "ba_lin", "cov", "status", "n_sub", "n_inner",
                  "extrap_span", "t_data_end", "gap_max", "n_released", "t_nom", "white_scale_max", "W")
 
-    @property
-    def ok(self):
-        return self.status == OK
-
-    @property
-    def reason(self):
-        return STATUS_TEXT.get(self.status, "status %d" % self.status)
-
-    @property
-    def extrapolated(self):
-        return self.extrap_span > 0.0
-
-    @property
-    def single_sample(self):
-        return self.n_inner == 0
-
-    JRg = property(lambda s: s.J[0:3, 0:3])
-    Jvg = property(lambda s: s.J[3:6, 0:3])
-    Jva = property(lambda s: s.J[3:6, 3:6])
-    Jpg = property(lambda s: s.J[6:9, 0:3])
-    Jpa = property(lambda s: s.J[6:9, 3:6])
-    cov9 = property(lambda s: s.cov[0:9, 0:9])
+    ok = property(lambda self: self.status == OK)
+    reason = property(lambda self: STATUS_TEXT.get(self.status, "unknown status %d" % self.status))
+    extrapolated = property(lambda self: self.extrap_span > 0.0)    # ran past the last released sample
+    single_sample = property(lambda self: self.n_inner == 0)        # no released sample strictly inside
+    # bias Jacobian blocks: rows (phi, v, p) x columns (bg, ba)
+    JRg, Jvg, Jva, Jpg, Jpa = _block(0, 0), _block(3, 0), _block(3, 3), _block(6, 0), _block(6, 3)
+    cov9 = property(lambda self: self.cov[:9, :9])
 
 
 class XbaPreintBatch(C.Structure):
@@ -231,12 +241,39 @@ def preintegrate(t, gyro, acc, t0, t1, horizon, bg_lin, ba_lin, noise):
     return preintegrate_many([(t, gyro, acc)], [(0, t0, t1, horizon, bg_lin, ba_lin)], noise)[0]
 
 
+def sqrt_information_many(pres, kappa=1.0, bias_sigma_g=0.0, bias_sigma_a=0.0, device=None):
+    """`sqrt_information` of many preintegrated intervals in one `xba_sqrt_information` launch: a list of W (15 x 15
+    lower triangular) with W^T W = (kappa C + BIASCOV)^-1, None where that matrix is not positive definite.
+    Raises ValueError for a refused interval or a kappa that is not finite and positive."""
+    import torch
+    pres = list(pres)
+    kappa = _finite_positive("kappa", kappa)
+    for p in pres:
+        if not p.ok:
+            raise ValueError("no information for a refused preintegration (%s)" % p.reason)
+    if not pres:
+        return []
+    L = _lib.load()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    nf = len(pres)
+    f64 = lambda x: torch.as_tensor(np.ascontiguousarray(x, np.float64), device=dev).contiguous()
+    Cm = f64(np.stack([p.cov for p in pres]))
+    Jm = f64(np.stack([p.J for p in pres]))
+    par = f64(np.tile([kappa, float(bias_sigma_g) ** 2, float(bias_sigma_a) ** 2], (nf, 1)))
+    W = torch.empty((nf, 15, 15), dtype=torch.float64, device=dev)
+    ok = torch.empty(nf, dtype=torch.int32, device=dev)
+    st = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(L.xba_sqrt_information(nf, Cm.data_ptr(), Jm.data_ptr(), par.data_ptr(), W.data_ptr(), ok.data_ptr(),
+                                      C.c_void_p(st)), "xba_sqrt_information")
+    Wh, okh = W.cpu().numpy(), ok.cpu().numpy()
+    return [Wh[f] if okh[f] else None for f in range(nf)]
+
+
 def sqrt_information(pre, kappa=1.0, bias_sigma_g=0.0, bias_sigma_a=0.0):
-    """vio/imu_factor.py:1051-1063 through the batch kernel's whitening: W with W^T W = (kappa C + BIASCOV)^-1.
-    Needs the interval's samples, so it is offered as `preintegrate_many(..., whiten=True)`; this helper only
-    validates like the reference and returns the W computed there."""
-    if not pre.ok:
-        raise ValueError("refused preintegration (%s) has no information" % pre.reason)
-    if pre.W is None:
-        raise ValueError("no whitening: preintegrate with whiten=True (or the covariance is not positive definite)")
-    return pre.W
+    """vio/imu_factor.py:1051-1063: W (15 x 15, lower triangular) with W^T W = (kappa C + BIASCOV)^-1 for this
+    kappa and these bias sigmas, computed on the GPU from the interval's covariance and bias Jacobians.  Raises
+    ValueError for a refused interval, a bad kappa or an information matrix that is not positive definite."""
+    W = sqrt_information_many([pre], kappa, bias_sigma_g, bias_sigma_a)[0]
+    if W is None:
+        raise ValueError("kappa C + BIASCOV of this interval is not positive definite (%s)" % pre.reason)
+    return W

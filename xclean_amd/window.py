@@ -306,11 +306,12 @@ class BatchSolver:
                 pb.pf.lmp[:] = lmw[w.L0:]
 
     def stats(self):
-        out = self.torch.zeros((len(self.packed), 8), dtype=self.torch.float64, device=self.dev)
+        out = self.torch.zeros((len(self.packed), _lib.XBA_NSTATS), dtype=self.torch.float64, device=self.dev)
         _lib.check(self.L.xba_stats(self.ctx, out.data_ptr(), self.stream), "xba_stats")
         a = out.cpu().numpy()
         return [{"cost0": r[0], "cost": r[1], "accepted": int(r[2]), "iters": int(r[3]), "lam": r[4],
-                 "chol_ok": bool(r[5]), "accept_last": bool(r[6]), "refine_steps": int(r[7])} for r in a]
+                 "chol_ok": bool(r[5]), "accept_last": bool(r[6]), "refine_steps": int(r[7]),
+                 "fallback": int(r[8]), "refine_converged": bool(r[9])} for r in a]
 
     def last_dx(self):
         out = self.torch.zeros(max(self.n_state, 1), dtype=self.torch.float64, device=self.dev)
