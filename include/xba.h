@@ -203,6 +203,20 @@ typedef struct xba_np_problem {
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                  double *dx_hist_out, void *stream);
 
+/* ---- the Euclidean local BA without allocation ---------------------------------------------------------------
+ * No counterpart in the reference: xba_np_solve for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments, results and return codes
+ * as xba_np_solve; its workspace is one static device array per GPU (about 13 MB, held until the process ends)
+ * sized for nk <= XBA_NOALLOC_MAX_NK, nl <= XBA_NOALLOC_MAX_NL, N <= XBA_NOALLOC_MAX_N (larger problems return
+ * XBA_E_SIZE), and the grouping of observations by landmark runs on the device.  No hipMalloc, no hipFree, no host
+ * heap.  `stream` must belong to the current device.  Every call on one GPU shares that array: calls that overlap
+ * in time on one GPU (two host threads, or two streams) race and must be serialised by the caller.  Synchronous. */
+#define XBA_NOALLOC_MAX_NK 32
+#define XBA_NOALLOC_MAX_NL 8192
+#define XBA_NOALLOC_MAX_N 65536
+int xba_np_solve_noalloc(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                         double *dx_hist_out, void *stream);
+
 /* ---- the inverse-depth local-BA drop-in (backend/ba.py:1212-1342 `_solve_np_id`) --------------------------------
  * poses (device, nk x 4 x 4, T_wc, in/out); rho (device, nl, in/out) inverse depth and mbear (device, nl x 3, in/out)
  * bearing (x_n, y_n, 1) of every landmark in its host camera frame; hidx (device int32, nl) the host keyframe or -1,

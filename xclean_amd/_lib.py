@@ -63,7 +63,7 @@ class XbaError(RuntimeError):
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
-           "xba_np_solve", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
+           "xba_np_solve", "xba_np_solve_noalloc", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
            "xba_marg_schur", "xba_prior_from_info",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
@@ -91,6 +91,7 @@ def load():
     L.xba_gn_step.argtypes = [vp, vp, vp]
     L.xba_backward.argtypes = [vp, vp, vp, vp, vp]
     L.xba_np_solve.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
+    L.xba_np_solve_noalloc.argtypes = [C.POINTER(XbaNpProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_np_id_solve.argtypes = [C.POINTER(XbaNpidProblem), C.c_int32, C.c_double, C.c_int32, vp, vp, vp]
     L.xba_copy_kernel.argtypes = [vp, vp, C.c_int64, vp]
     L.xba_preintegrate.argtypes = [vp, vp]

@@ -47,12 +47,13 @@ def _rig_intrinsics(rig):
 
 
 def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None, rotf=None, device=None,
-              history=False, _policy=0):
+              history=False, _policy=0, noalloc=False):
     """backend/ba.py:713 ``_solve_np``.  Returns (poses T_wc (nk,4,4), pts (nl,3), stats dict) as numpy arrays,
     stats = {chi2_0, chi2_n, rms_0, rms_n}.  ``history=True`` also returns the (iters, 6 nk) pose updates.
 
     ``prior`` = (c (nl,3), rho0 (nl), invsig (nl) or scalar, valid (nl)); ``rotf`` = [(a, b, R (3,3), sigma), ...].
-    Inputs may be numpy arrays or torch tensors; they are copied to the GPU once per call."""
+    Inputs may be numpy arrays or torch tensors; they are copied to the GPU once per call.  ``noalloc``: solve
+    through ``xba_np_solve_noalloc`` (the library's static workspace, for problems within XBA_NOALLOC_MAX_*)."""
     import torch
     _lib.require_gpu()
     L = _lib.load()
@@ -103,9 +104,10 @@ def _solve_np(poses, pts, okf, olm, ouv, rig, free, iters, lam0=1e-4, prior=None
     dxh = torch.zeros((max(iters, 0), D_POSE * nk), dtype=torch.float64, device=dev) if history else None
     stream = torch.cuda.current_stream(dev).cuda_stream
     with torch.cuda.device(dev):
-        rc = L.xba_np_solve(C.byref(q), int(iters), float(lam0), _lib.XBA_FP64, C.c_void_p(stats.data_ptr()),
-                            C.c_void_p(dxh.data_ptr()) if dxh is not None else None, C.c_void_p(stream))
-    _lib.check(rc, "xba_np_solve")
+        entry = L.xba_np_solve_noalloc if noalloc else L.xba_np_solve
+        rc = entry(C.byref(q), int(iters), float(lam0), _lib.XBA_FP64, C.c_void_p(stats.data_ptr()),
+                   C.c_void_p(dxh.data_ptr()) if dxh is not None else None, C.c_void_p(stream))
+    _lib.check(rc, "xba_np_solve_noalloc" if noalloc else "xba_np_solve")
     s = stats.cpu().numpy()
     st = {"chi2_0": float(s[0]), "chi2_n": float(s[1]), "rms_0": float(s[2]), "rms_n": float(s[3])}
     out = (d_poses.cpu().numpy(), d_pts.cpu().numpy(), st)

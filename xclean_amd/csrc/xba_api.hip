@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
+#include <mutex>
 #include <vector>
 
 #include "xba_internal.h"
@@ -1296,15 +1297,155 @@ int xba_backward(xba_ctx *c, const double *g_dx, double *grad_z, double *grad_w,
     return XBA_OK;
 }
 
-int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
-                 double *dx_hist_out, void *stream) {
+}  // extern "C"
+
+namespace {
+// Argument checks shared by the Euclidean local-BA entry points (backend/ba.py:713-869).
+int np_check(const xba_np_problem *q, int32_t iters, int32_t precision, const double *stats_out) {
     if (!q || iters < 0 || q->nk < 1 || q->nl < 0 || q->N < 0 || !q->poses || !stats_out) return XBA_E_ARG;
     if (precision != XBA_FP64) return XBA_E_ARG;  // the reference solve is FP64 throughout
     if ((q->nl > 0 && !q->pts) || (q->N > 0 && (!q->okf || !q->olm || !q->ouv)) || !q->free_mask) return XBA_E_ARG;
     if (q->pr_c && (!q->pr_rho0 || !q->pr_invsig || !q->pr_valid)) return XBA_E_ARG;
     if (q->n_rot < 0 || (q->n_rot > 0 && (!q->rot_ab || !q->rot_M || !q->rot_sig))) return XBA_E_ARG;
+    if (6 * q->nk > XBA_MAX_STATE) return XBA_E_SIZE;
+    return XBA_OK;
+}
+
+// The problem's fields into the kernel arguments (the workspace pointers are set by the caller).
+NpArgs np_args(const xba_np_problem *q, double *dx_hist_out) {
+    NpArgs A{};
+    A.nk = q->nk; A.nl = q->nl; A.N = q->N; A.n_rot = q->n_rot; A.has_prior = q->pr_c ? 1 : 0;
+    A.fx = q->fx; A.cx = q->cx; A.cy = q->cy; A.K = q->huber_k;
+    A.pts = q->pts; A.okf = q->okf; A.olm = q->olm; A.ouv = q->ouv; A.free_mask = q->free_mask;
+    A.pr_c = q->pr_c; A.pr_rho0 = q->pr_rho0; A.pr_invsig = q->pr_invsig; A.pr_valid = q->pr_valid;
+    A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
+    A.dx_hist = dx_hist_out;
+    A.policy = q->lm_policy ? 1 : 0;
+    A.rot_h = A.policy;
+    A.jstride = 20;
+    A.rwoff = 18;
+    return A;
+}
+
+// Workspace of the Euclidean solve, carved from `base` in 256-byte-aligned regions (base NULL: sizes only).
+// Returns the doubles used.  The int32 regions (order N, lstart nl + 1, fill nl, err 1) follow the doubles.
+struct NpWs {
+    int32_t *order, *lstart, *fill, *err;
+};
+constexpr int64_t np_round(int64_t n) { return (n + 31) & ~(int64_t)31; }
+constexpr int64_t np_ws_doubles(int64_t nk, int64_t nl, int64_t N) {
+    return 3 * np_round(16 * nk) + 2 * np_round(3 * nl) + np_round(20 * N) + np_round(N) + np_round(9 * nl) +
+           np_round(3 * nl) + np_round(36 * nk * nk) + 2 * np_round(6 * nk) + np_round(NST) +
+           np_round((N + 2 * nl + 2 + 1) / 2 + 1);
+}
+int64_t np_carve(double *base, int nk, int nl, int N, NpArgs &A, NpWs &W) {
+    int64_t o = 0;
+    auto take = [&](int64_t n) {
+        double *p = base ? base + o : nullptr;
+        o += np_round(n);
+        return p;
+    };
+    const int64_t n = 6 * (int64_t)nk;
+    A.Tcw = take(16 * (int64_t)nk);
+    A.Tnew = take(16 * (int64_t)nk);
+    A.Tbest = take(16 * (int64_t)nk);
+    A.pnew = take(3 * (int64_t)nl);
+    A.pbest = take(3 * (int64_t)nl);
+    A.J = take(20 * (int64_t)N);
+    A.r2 = take(N);
+    A.Hlli = take(9 * (int64_t)nl);
+    A.bl = take(3 * (int64_t)nl);
+    A.H = take(n * n);
+    A.bp = take(n);
+    A.dx = take(n);
+    A.st = take(NST);
+    int32_t *ip = (int32_t *)take(((int64_t)N + 2 * (int64_t)nl + 2 + 1) / 2 + 1);
+    W.order = ip;
+    W.lstart = ip ? ip + N : nullptr;
+    W.fill = ip ? ip + N + nl + 1 : nullptr;
+    W.err = ip ? ip + N + 2 * (int64_t)nl + 1 : nullptr;
+    return o;
+}
+
+// The LM iterations of the Euclidean solve on a planned problem (A.order / A.lstart ready on the device):
+// shared by xba_np_solve and xba_np_solve_noalloc.  Synchronous.
+int np_solve_launch(const xba_np_problem *q, NpArgs &A, int32_t iters, double lam0, double *stats_out,
+                    double *dx_hist_out, hipStream_t s) {
     const int nk = q->nk, nl = q->nl, N = q->N, n = 6 * nk;
-    if (n > XBA_MAX_STATE) return XBA_E_SIZE;
+    const double st0[NST] = {lam0, 0, 0, 0, 1, 0, 0, 0, 0, 0, -1, -1, -1, 0, 0, 0};
+    HIPCK(hipMemcpyAsync(A.st, st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    if (dx_hist_out && iters) {
+        const int64_t m = (int64_t)iters * n;
+        k_np_fill_f64<<<(int)std::min<int64_t>((m + 255) / 256, 1024), 256, 0, s>>>(dx_hist_out, m, 0.0);
+    }
+    const int gk = (nk + 255) / 256, gN = (N + 255) / 256, gl = (nl + 255) / 256;
+    const int gz = std::min((n * n + 255) / 256, 1024), gp = (3 * nl + 255) / 256;
+    k_np_inv_poses<<<gk, 256, 0, s>>>(nk, q->poses, A.Tcw);
+    if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
+    k_np_chi<<<1, 256, 0, s>>>(A, 5);
+    for (int it = 0; it < iters; it++) {
+        if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
+        if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 0);
+        if (gl) k_np_lm<<<gl, 256, 0, s>>>(A);
+        k_np_zero<<<gz, 256, 0, s>>>(A, n);
+        if (gl) k_np_schur<<<gl, 256, 0, s>>>(A, n);
+        k_np_solve<<<1, 256, 0, s>>>(A, n, it);
+        if (gl) k_np_backsub<<<gl, 256, 0, s>>>(A);
+        if (A.policy) {
+            k_np_apply<<<1, 256, 0, s>>>(A, n, 0);
+        } else {
+            if (gN) k_np_trial_r2<<<gN, 256, 0, s>>>(A);
+            k_np_accept<<<1, 256, 0, s>>>(A);
+            if (gp) k_np_commit_pts<<<gp, 256, 0, s>>>(A);
+        }
+    }
+    if (A.policy) {
+        k_np_project<<<gk, 256, 0, s>>>(nk, A.Tcw);
+        if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
+        k_np_wstats<<<1, 256, 0, s>>>(A, stats_out);
+    } else {
+        if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
+        k_np_chi<<<1, 256, 0, s>>>(A, 6);
+        k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
+    }
+    k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tcw, q->poses);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(s));
+    return XBA_OK;
+}
+
+// The allocation-free entry point's workspace: one array per GPU, for the problem sizes XBA_NOALLOC_MAX_*.  Host
+// code never passes the array's name to a kernel: its per-device address comes from hipGetSymbolAddress, taken once
+// per device (with that device current) on the entry point's first call there.
+constexpr int64_t NP_SCRATCH_DOUBLES = np_ws_doubles(XBA_NOALLOC_MAX_NK, XBA_NOALLOC_MAX_NL, XBA_NOALLOC_MAX_N);
+static __device__ __align__(256) double xba_np_scratch[NP_SCRATCH_DOUBLES];
+#define XBA_MAX_DEVICES 64
+std::mutex g_np_scratch_mu;
+double *g_np_scratch[XBA_MAX_DEVICES];
+
+int np_scratch_ptr(double **out) {
+    int dev = -1;
+    HIPCK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= XBA_MAX_DEVICES) return XBA_E_ARG;
+    std::lock_guard<std::mutex> lock(g_np_scratch_mu);
+    if (!g_np_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(xba_np_scratch));
+        if (e != hipSuccess || !p) return XBA_E_HIP;
+        g_np_scratch[dev] = (double *)p;
+    }
+    *out = g_np_scratch[dev];
+    return XBA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                 double *dx_hist_out, void *stream) {
+    const int rc0 = np_check(q, iters, precision, stats_out);
+    if (rc0 != XBA_OK) return rc0;
+    const int nk = q->nk, nl = q->nl, N = q->N;
     hipStream_t s = (hipStream_t)stream;
 
     // host planning: observations grouped by landmark (stable), index validation
@@ -1328,88 +1469,58 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
         for (int k = 0; k < N; k++) order[fill[olm[k]]++] = k;
     }
 
-    std::vector<void *> owned;
-    auto release = [&]() {
-        for (void *p : owned) (void)hipFree(p);
-    };
-    NpArgs A{};
-    A.nk = nk; A.nl = nl; A.N = N; A.n_rot = q->n_rot; A.has_prior = q->pr_c ? 1 : 0;
-    A.fx = q->fx; A.cx = q->cx; A.cy = q->cy; A.K = q->huber_k;
-    A.pts = q->pts; A.okf = q->okf; A.olm = q->olm; A.ouv = q->ouv; A.free_mask = q->free_mask;
-    A.pr_c = q->pr_c; A.pr_rho0 = q->pr_rho0; A.pr_invsig = q->pr_invsig; A.pr_valid = q->pr_valid;
-    A.rot_ab = q->rot_ab; A.rot_M = q->rot_M; A.rot_sig = q->rot_sig;
-    A.dx_hist = dx_hist_out;
-    A.policy = q->lm_policy ? 1 : 0;
-    A.rot_h = A.policy;
-    A.jstride = 20;
-    A.rwoff = 18;
-    A.Tcw = dalloc<double>(16 * (size_t)nk, owned);
-    A.Tnew = dalloc<double>(16 * (size_t)nk, owned);
-    A.pnew = dalloc<double>(3 * (size_t)nl, owned);
-    A.J = dalloc<double>(20 * (size_t)N, owned);
-    A.r2 = dalloc<double>(N, owned);
-    A.Hlli = dalloc<double>(9 * (size_t)nl, owned);
-    A.bl = dalloc<double>(3 * (size_t)nl, owned);
-    A.H = dalloc<double>((size_t)n * n, owned);
-    A.bp = dalloc<double>(n, owned);
-    A.dx = dalloc<double>(n, owned);
-    A.st = dalloc<double>(NST, owned);
-    A.Tbest = dalloc<double>(16 * (size_t)nk, owned);
-    A.pbest = dalloc<double>(3 * (size_t)nl, owned);
-    int32_t *d_order = dalloc<int32_t>(N, owned), *d_lstart = dalloc<int32_t>(nl + 1, owned);
-    if (owned.size() != 15) {
-        release();
-        return XBA_E_HIP;
-    }
-    A.order = d_order;
-    A.lstart = d_lstart;
-    const double st0[NST] = {lam0, 0, 0, 0, 1, 0, 0, 0, 0, 0, -1, -1, -1, 0, 0, 0};
-    auto run = [&]() -> int {
-        if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
-        HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
-        HIPCK(hipMemcpyAsync(A.st, st0, sizeof(st0), hipMemcpyHostToDevice, s));
-        if (dx_hist_out && iters)
-            HIPCK(hipMemsetAsync(dx_hist_out, 0, sizeof(double) * (size_t)iters * n, s));
-        const int gk = (nk + 255) / 256, gN = (N + 255) / 256, gl = (nl + 255) / 256;
-        const int gz = std::min((n * n + 255) / 256, 1024), gp = (3 * nl + 255) / 256;
-        k_np_inv_poses<<<gk, 256, 0, s>>>(nk, q->poses, A.Tcw);
-        if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
-        k_np_chi<<<1, 256, 0, s>>>(A, 5);
-        for (int it = 0; it < iters; it++) {
-            if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
-            if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 0);
-            if (gl) k_np_lm<<<gl, 256, 0, s>>>(A);
-            k_np_zero<<<gz, 256, 0, s>>>(A, n);
-            if (gl) k_np_schur<<<gl, 256, 0, s>>>(A, n);
-            k_np_solve<<<1, 256, 0, s>>>(A, n, it);
-            if (gl) k_np_backsub<<<gl, 256, 0, s>>>(A);
-            if (A.policy) {
-                k_np_apply<<<1, 256, 0, s>>>(A, n, 0);
-            } else {
-                if (gN) k_np_trial_r2<<<gN, 256, 0, s>>>(A);
-                k_np_accept<<<1, 256, 0, s>>>(A);
-                if (gp) k_np_commit_pts<<<gp, 256, 0, s>>>(A);
-            }
-        }
-        if (A.policy) {
-            k_np_project<<<gk, 256, 0, s>>>(nk, A.Tcw);
-            if (gN) k_np_obs<<<gN, 256, 0, s>>>(A);
-            k_np_wstats<<<1, 256, 0, s>>>(A, stats_out);
-        } else {
-            if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
-            k_np_chi<<<1, 256, 0, s>>>(A, 6);
-            k_np_stats<<<1, 1, 0, s>>>(A.st, N, stats_out);
-        }
-        k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tcw, q->poses);
-        HIPCK(hipGetLastError());
-        HIPCK(hipStreamSynchronize(s));
-        return XBA_OK;
-    };
-    const int rc = run();
-    release();
+    NpArgs A = np_args(q, dx_hist_out);
+    NpWs W{};
+    const int64_t need = np_carve(nullptr, nk, nl, N, A, W);
+    double *ws = nullptr;
+    if (hipMalloc(&ws, sizeof(double) * need) != hipSuccess) return XBA_E_HIP;
+    np_carve(ws, nk, nl, N, A, W);
+    A.order = W.order;
+    A.lstart = W.lstart;
+    int rc = XBA_OK;
+    if (N && hipMemcpyAsync(W.order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = XBA_E_HIP;
+    if (rc == XBA_OK &&
+        hipMemcpyAsync(W.lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = XBA_E_HIP;
+    if (rc == XBA_OK) rc = np_solve_launch(q, A, iters, lam0, stats_out, dx_hist_out, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(ws);
     return rc;
 }
 
+int xba_np_solve_noalloc(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
+                         double *dx_hist_out, void *stream) {
+    const int rc0 = np_check(q, iters, precision, stats_out);
+    if (rc0 != XBA_OK) return rc0;
+    const int nk = q->nk, nl = q->nl, N = q->N;
+    if (nk > XBA_NOALLOC_MAX_NK || nl > XBA_NOALLOC_MAX_NL || N > XBA_NOALLOC_MAX_N) return XBA_E_SIZE;
+    hipStream_t s = (hipStream_t)stream;
+    double *base = nullptr;
+    const int rcs = np_scratch_ptr(&base);
+    if (rcs != XBA_OK) return rcs;
+    NpArgs A = np_args(q, dx_hist_out);
+    NpWs W{};
+    if (np_carve(nullptr, nk, nl, N, A, W) > NP_SCRATCH_DOUBLES) return XBA_E_SIZE;
+    np_carve(base, nk, nl, N, A, W);
+    A.order = W.order;
+    A.lstart = W.lstart;
+    // device planning: counts (the array keeps the previous call's contents, so it is cleared first), offsets,
+    // scatter, ordered segments; the index checks read back once
+    const int nrot2 = 2 * q->n_rot, gc = (std::max(N, nrot2) + 255) / 256;
+    k_np_fill_i32<<<(nl + 1 + 255) / 256, 256, 0, s>>>(W.lstart, nl + 1, 0);
+    k_np_fill_i32<<<1, 64, 0, s>>>(W.err, 1, 0);
+    if (gc) k_np_plan_count<<<gc, 256, 0, s>>>(N, nk, nl, q->okf, q->olm, nrot2, q->rot_ab, W.lstart, W.err);
+    k_np_plan_scan<<<1, 1024, 0, s>>>(nl, W.lstart, W.fill, W.err);
+    if (N) k_np_plan_scatter<<<(N + 255) / 256, 256, 0, s>>>(N, nl, q->olm, W.fill, W.order, W.err);
+    if (nl) k_np_plan_sort<<<(nl + 255) / 256, 256, 0, s>>>(nl, W.lstart, W.order, W.err);
+    HIPCK(hipGetLastError());
+    int32_t herr = 0;
+    HIPCK(hipMemcpyAsync(&herr, W.err, sizeof(herr), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (herr) return XBA_E_RANGE;
+    return np_solve_launch(q, A, iters, lam0, stats_out, dx_hist_out, s);
+}
 
 int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                     double *dx_hist_out, void *stream) {

@@ -410,6 +410,85 @@ __global__ void k_np_stats(const double *st, int N, double *stats) {
 }
 
 
+// ---- device-side planning of the Euclidean solve (the allocation-free entry point): observations grouped by
+// landmark in index order, as the host plan of xba_np_solve groups them, and the index checks.  err[0] != 0 after
+// k_np_plan_count when an index is out of range; the later kernels then write nothing.
+__global__ void k_np_fill_i32(int32_t *a, int n, int32_t v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+__global__ void k_np_fill_f64(double *a, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+// counts per landmark into lstart[l + 1] (zeroed), the range checks of okf / olm and of the rotation factors' slots
+__global__ void k_np_plan_count(int N, int nk, int nl, const int32_t *okf, const int32_t *olm, int nrot2,
+                                const int32_t *rot_ab, int32_t *lstart, int32_t *err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) {
+        const int f = okf[k], l = olm[k];
+        if (f < 0 || f >= nk || l < 0 || l >= nl)
+            atomicOr(err, 1);
+        else
+            atomicAdd(lstart + l + 1, 1);
+    }
+    if (k < nrot2 && (rot_ab[k] < 0 || rot_ab[k] >= nk)) atomicOr(err, 1);
+}
+
+// one workgroup: lstart <- exclusive offsets (lstart[0] = 0), fill <- a copy of the first nl offsets
+__global__ __launch_bounds__(1024) void k_np_plan_scan(int nl, int32_t *lstart, int32_t *fill, const int32_t *err) {
+    __shared__ int32_t part[1024];
+    __shared__ int32_t carry;
+    if (*err) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 1; base <= nl; base += nt) {
+        const int i = base + tid;
+        int32_t v = i <= nl ? lstart[i] : 0;
+        part[tid] = v;
+        __syncthreads();
+        for (int d = 1; d < nt; d <<= 1) {  // inclusive Hillis-Steele scan of the block
+            const int32_t add = tid >= d ? part[tid - d] : 0;
+            __syncthreads();
+            part[tid] += add;
+            __syncthreads();
+        }
+        const int32_t c0 = carry;
+        if (i <= nl) lstart[i] = c0 + part[tid];
+        __syncthreads();
+        if (tid == nt - 1) carry = c0 + part[tid];
+        __syncthreads();
+    }
+    for (int l = tid; l < nl; l += nt) fill[l] = lstart[l];
+}
+
+// observation k to its landmark's segment (arbitrary order inside the segment; k_np_plan_sort orders it)
+__global__ void k_np_plan_scatter(int N, int nl, const int32_t *olm, int32_t *fill, int32_t *order,
+                                  const int32_t *err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N || *err) return;
+    const int l = olm[k];
+    if (l < 0 || l >= nl) return;
+    order[atomicAdd(fill + l, 1)] = k;
+}
+
+// one thread per landmark: its segment in ascending observation index (insertion sort; segments are short)
+__global__ void k_np_plan_sort(int nl, const int32_t *lstart, int32_t *order, const int32_t *err) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl || *err) return;
+    const int a = lstart[l], b = lstart[l + 1];
+    for (int i = a + 1; i < b; i++) {
+        const int32_t v = order[i];
+        int j = i - 1;
+        while (j >= a && order[j] > v) {
+            order[j + 1] = order[j];
+            j--;
+        }
+        order[j + 1] = v;
+    }
+}
+
 // ============================================================= the inverse-depth local BA (ba.py:1212-1342) ==
 // Host frame (R_wc, t_wc) of landmark l: the window pose Twc[hidx] or the constant hext row (ba.py:1116-1135).
 __device__ inline void id_host(const NpArgs &A, const double *Twc, int l, double *R, double *t) {
