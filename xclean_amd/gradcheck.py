@@ -86,7 +86,8 @@ def differentiation_check(problems, iters=10, per_window=200, seed=0, h=1e-6, pr
     of L = g . dx for a random g reusing the step's Cholesky factor, then the adjoint checked against FP64 central
     differences at ``per_window`` coordinates per window and against the FP64 path's own adjoint.  Mutates
     ``problems`` (their state becomes the LM solution; ob_w is set when absent).  Returns timings (ms; the solver's
-    planning and graph capture excluded) and per-window relative errors."""
+    planning and graph capture excluded), per-window relative errors, and the sampled coordinates with the adjoint's,
+    the FP64 adjoint's and the central differences' values there and the loss weights g."""
     import time
 
     import torch
@@ -138,7 +139,8 @@ def differentiation_check(problems, iters=10, per_window=200, seed=0, h=1e-6, pr
     ad64 = adjoint_at(coords, gz64, gw64, problems)
     rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
     return dict(forward_ms=(t1 - t0) * 1e3, backward_ms=(t2 - t1) * 1e3, lm_iters=iters,
-                coords_per_window=[len(c) for c in coords],
+                coords_per_window=[len(c) for c in coords], coords=coords, adjoint=ad, adjoint_fp64=ad64,
+                fd=fd, g=g.cpu().numpy(),
                 rel_err_vs_fd=[rel(a, f) for a, f in zip(ad, fd)],
                 rel_err_fp64_adjoint_vs_fd=[rel(a, f) for a, f in zip(ad64, fd)],
                 rel_err_vs_fp64_adjoint=[rel(a, b) for a, b in zip(ad, ad64)],
