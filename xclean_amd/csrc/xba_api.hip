@@ -142,6 +142,7 @@ struct xba_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool solve_fork = false, wide_fork = false;
     int fb_lds_cap = 0;  // LDS of the FP64 fallback of the mixed solve
+    int fb_R = 1, fb_nw = 1, fb_lds = 0, fb_grid = 1;  // its re-assembly launch: tile, waves, LDS, grid
     int maxn_small = 0;  // largest n among the windows k_solve factors
 };
 
@@ -228,6 +229,9 @@ DArgs dargs(xba_ctx *c) {
     D.partc2 = c->partc2;
     D.nref_max = 6;
     D.fallback_on = c->opt.precision == XBA_MIXED ? 1 : 0;
+    D.fallback_relin = (D.fallback_on && c->fast) ? 1 : 0;
+    D.partH64 = c->partH64;
+    D.partb64 = c->partb64;
     D.nbig = c->nbig;
     D.bigs = c->bigs;
     D.bigF32 = c->bigF32;
@@ -336,12 +340,13 @@ void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part 
 template <typename T>
 void launch_reduce(xba_ctx *c, const DArgs &D, hipStream_t s) {
     int gx = (int)(((int64_t)c->maxn * c->maxn + 255) / 256);
+    const int gy = D.only_fallback ? 1 : c->nwin;  // the fallback's blocks visit the flagged windows themselves
     if (c->fast)
-        hipLaunchKernelGGL(k_reduce<double>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D,
+        hipLaunchKernelGGL(k_reduce<double>, dim3(std::max(gx, 1), gy), dim3(256), 0, s, D,
                            (const double *)c->partH64, (const double *)c->partb64, (const double *)nullptr,
                            (const double *)nullptr);
     else
-        hipLaunchKernelGGL(k_reduce<T>, dim3(std::max(gx, 1), c->nwin), dim3(256), 0, s, D, (const T *)c->partH,
+        hipLaunchKernelGGL(k_reduce<T>, dim3(std::max(gx, 1), gy), dim3(256), 0, s, D, (const T *)c->partH,
                            (const T *)c->partb, (const T *)c->partH_lo, (const T *)c->partb_lo);
 }
 
@@ -390,14 +395,14 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     // multi-wavefront kernels; the FP32 records perturb H by ~1e-7 relative, which an ill-conditioned window
     // amplifies) and solved in FP64.  Workgroups of the other windows exit at once.
     if (sizeof(T) == 4 && D.fallback_on) {
-        if (c->fast) {
+        if (D.fallback_relin && c->nchunks > 0) {
+            // every work item in one launch of the wide kernel's shape (its tile holds any work item's slots); the
+            // reduction and terms of a flagged window run inside its FP64 solve below (the IMU records are current)
             KArgs<T> A = kargs<T>(c);
             A.only_fallback = 1;
-            launch_lin_kernel<T, double>(c, A, s);
-            DArgs Df = D;
-            Df.only_fallback = 1;
-            launch_reduce<T>(c, Df, s);
-            hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, Df, 0);  // the IMU records are current
+            const KArgs<double> F = fast_args<T>(c, A);
+            hipLaunchKernelGGL((k_lin_fast<double, 1>), dim3(c->fb_grid), dim3(XBA_WAVE * c->fb_nw), c->fb_lds, s, F,
+                               c->fb_R, 0, c->nchunks, 1);
         }
         hipLaunchKernelGGL(k_solve<double>, dim3(c->nwin), dim3(512), c->fb_lds_cap, s, D, c->fb_lds_cap, 1);
     }
@@ -915,6 +920,12 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         };
         if (!pick(c->Rnarrow, c->fast_nw, c->fast_lds, c->fast_nw_trial, c->fast_lds_trial)) return fail(XBA_E_RANGE);
         if (!pick(c->Rwide, c->wide_nw, c->wide_lds, c->wide_nw_trial, c->wide_lds_trial)) return fail(XBA_E_RANGE);
+        {  // the mixed fallback's re-assembly: one launch over every work item, tile of the widest
+            int nwt, ldst;
+            c->fb_R = std::max(c->Rnarrow, c->Rwide);
+            if (!pick(c->fb_R, c->fb_nw, c->fb_lds, nwt, ldst)) return fail(XBA_E_RANGE);
+            c->fb_grid = std::max(1, std::min(c->nchunks, ncu));
+        }
         per_cu = 1;
         c->grid_wide = std::max(1, std::min(c->nchunks - c->nnarrow, ncu));
         // k_trial_fast is compiled for 4 waves per SIMD: two 8-wave workgroups per CU where the LDS holds both

@@ -270,6 +270,10 @@ struct DArgs {
     // buffer in units of 36 entries) that hold it, in work-item order; for every slot, its partial gradient
     // segments (units of 6 entries)
     const int32_t *pair_start, *pair_ent, *bslot_start, *bslot_ent;
+    // the FP64 partials of the multi-wavefront kernels: the mixed fallback (fallback_relin) reduces a flagged
+    // window's re-assembled system inside its FP64 solve
+    const double *partH64, *partb64;
+    int32_t fallback_relin;
 };
 
 // ------------------------------------------------------------------------------------------- k_reduce --
@@ -293,13 +297,11 @@ struct Acc {  // FP64: plain sum; FP32: Kahan-compensated sum
 };
 
 template <typename T>
-__global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH_lo, const T *partb_lo) {
-    const int w = blockIdx.y;
-    if (D.only_fallback && D.wstate[(size_t)NWS * w + WS_FALLBACK] == 0.0) return;
+__device__ inline void reduce_window(const DArgs &D, int w, int64_t idx, const T *partH, const T *partb,
+                                     const T *partH_lo, const T *partb_lo) {
     const WinInfo W = D.wins[w];
     const int n = W.n;
     const bool kah = sizeof(T) == 4;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < (int64_t)n * n) {
         int a = (int)(idx / n), bcol = (int)(idx % n);
         int ca = D.colmap[W.st0 + a], cb = D.colmap[W.st0 + bcol];
@@ -355,6 +357,19 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
         for (int c = W.c2; c < W.c3; c++) cs += D.partc[c];
         D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
     }
+}
+
+// grid (ceil(max n^2 / 256), nwin): entry blockIdx.x * 256 + threadIdx.x of window blockIdx.y.  With
+// D.only_fallback (grid (.., 1)): that entry of every window flagged WS_FALLBACK, the flags read by every block.
+template <typename T>
+__global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH_lo, const T *partb_lo) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!D.only_fallback) {
+        reduce_window<T>(D, blockIdx.y, idx, partH, partb, partH_lo, partb_lo);
+        return;
+    }
+    for (int w = 0; w < D.nwin; w++)
+        if (D.wstate[(size_t)NWS * w + WS_FALLBACK] != 0.0) reduce_window<T>(D, w, idx, partH, partb, partH_lo, partb_lo);
 }
 
 // --------------------------------------------------------------------------------------------- k_imu --
@@ -426,12 +441,10 @@ __global__ __launch_bounds__(256) void k_imu(DArgs D, int trial) {
 // ------------------------------------------------------------------------------------------- k_terms --
 // The prior and IMU terms of one window (one workgroup per window).  trial == 0: add H, b and the cost at the
 // current state, then c_tot = visual + rest (cost0 on the first call); trial == 1: cost only, at the trial state.
-__global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
-    __shared__ double sd[XBA_MAX_STATE], sres[XBA_MAX_STATE];
-    __shared__ int scol[XBA_MAX_STATE];
-    __shared__ double sW[225], sr[15], sJi[225], sJj[225], sred[256];
-    const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    if (D.only_fallback && D.wstate[(size_t)NWS * w + WS_FALLBACK] == 0.0) return;
+// The terms of window w by the calling workgroup; sd, sres (doubles) and scol (ints) are XBA_MAX_STATE entries of
+// LDS scratch each.
+__device__ inline void terms_window(const DArgs &D, int w, int trial, double *sd, double *sres, int *scol) {
+    const int tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
     const int n = W.n;
     const double *R = trial ? D.R2 : D.Rwb, *p = trial ? D.p2 : D.pwb;
@@ -533,6 +546,12 @@ __global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
+    __shared__ double sd[XBA_MAX_STATE], sres[XBA_MAX_STATE];
+    __shared__ int scol[XBA_MAX_STATE];
+    terms_window(D, blockIdx.x, trial, sd, sres, scol);
 }
 
 // ---------------------------------------------------------------------------------------- dense Cholesky --
@@ -809,6 +828,15 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap, int fallbac
     double *ws = D.wstate + (size_t)NWS * w;
     if (fallback) {
         if (ws[WS_FALLBACK] == 0.0) return;  // written by this iteration's mixed solve of the window
+        if (D.fallback_relin) {
+            // the window's system re-assembled with FP64 records: its reduction and prior / IMU terms here
+            for (int64_t idx = tid; idx < (int64_t)W.n * W.n; idx += nt)
+                reduce_window<double>(D, w, idx, D.partH64, D.partb64, (const double *)nullptr,
+                                      (const double *)nullptr);
+            __syncthreads();
+            terms_window(D, w, 0, sx, sr, (int *)ss);
+            __syncthreads();
+        }
     } else if (W.big >= 0) {
         return;  // factored by the tiled kernels (k_big_*)
     }

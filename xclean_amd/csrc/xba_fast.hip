@@ -55,7 +55,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
-    L.o_rcq = w;   w = align16(w + FB_OBS * 8);
+    L.o_rcq = w;   w = align16(w + (trial ? FB_OBS * 8 : 0));
     L.o_u = w;     w = align16(w + (trial ? FB_OBS * 3 * ts : 0));
     L.o_lmask = w; w = align16(w + FB_LM * 4);
     L.o_lobs0 = w; w = align16(w + FB_LM * 4);
@@ -66,10 +66,18 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + (trial ? 0 : FB_LM * FB_LREC * ts));
     L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
-    L.o_lrow = w;  w = align16(w + (trial ? 0 : FB_LM * LA_N * 8));     // per landmark: Hll, bl sums, active count
+    L.o_lrow = w;  // (trial: unused)
+    if (!trial) {
+        // the observation costs and the per-landmark sums (Hll, bl, active count) live from the observation phase
+        // to the landmark phase, U from after it: they share U's storage (one wavefront's phases are ordered by its
+        // wave syncs), so that eight wavefronts of a 16-slot tile fit the LDS
+        static_assert(FB_OBS * 8 + FB_LM * LA_N * 8 <= FB_HLP * 18 * 8, "U storage holds the phase-1 arrays");
+        L.o_rcq = L.o_hlp;
+        L.o_lrow = L.o_hlp + FB_OBS * 8;
+    }
     L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
     L.o_kown = w;  w = align16(w + (trial ? 0 : FB_HLP));                  // per touched-slot entry: its landmark
-    L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: landmark, si, sj
+    L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: first entry, si, sj
     L.wave_stride = w;
     L.total = o + nw * w;
     return L;
@@ -211,6 +219,38 @@ __device__ inline int next_chunk(const KArgs<double> &A, int kind, int cbeg, int
     return ctl[0];
 }
 
+// The mixed solve's FP64 re-assembly (A.only_fallback): this workgroup's share of the launch's work items,
+// sched[i] with i = cbeg + blockIdx.x + k gridDim.x, k = 0, 1, ..., keeping those whose window was flagged
+// WS_FALLBACK; one per call, -1 when none is left.  The threads read a round of blockDim.x items' flags together,
+// so a launch with nothing flagged costs one round per workgroup (no hand-out counter).  ctl[2] holds the next k
+// (0 before the first call), ctl[3] the round's first flagged thread.
+__device__ inline int next_fallback_chunk(const KArgs<double> &A, int cbeg, int cend, int *ctl) {
+    for (;;) {
+        __syncthreads();
+        const int k0 = ctl[2];
+        if (cbeg + (int)blockIdx.x + k0 * (int)gridDim.x >= cend) return -1;  // uniform
+        if (threadIdx.x == 0) ctl[3] = INT_MAX;
+        __syncthreads();
+        const int i = cbeg + (int)blockIdx.x + (k0 + (int)threadIdx.x) * (int)gridDim.x;
+        if (i < cend) {
+            const int c = A.sched[i];
+            if (A.wstate[(size_t)NWS * A.chunks[c].win + WS_FALLBACK] != 0.0) atomicMin(ctl + 3, (int)threadIdx.x);
+        }
+        __syncthreads();
+        const int t = ctl[3];
+        __syncthreads();
+        if (threadIdx.x == 0) ctl[2] = t == INT_MAX ? k0 + (int)blockDim.x : k0 + t + 1;
+        if (t == INT_MAX) continue;
+        const int c = A.sched[cbeg + (int)blockIdx.x + (k0 + t) * (int)gridDim.x];
+        if (threadIdx.x == 0) {
+            ctl[0] = c;
+            ctl[1] = A.chunks[c].p0;
+        }
+        __syncthreads();
+        return c;
+    }
+}
+
 // The next pass of the current work item for this wavefront (uniform across the wavefront).
 __device__ inline int next_pass(int *ctl, int lane) {
     int p = 0;
@@ -321,12 +361,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     uint16_t *ptab = (uint16_t *)(wb + FL.o_ptab);
     int *ctl = (int *)(smem + FL.ctl);
 
+    if (threadIdx.x == 0) ctl[2] = 0;
     for (;;) {
-        const int c = next_chunk(A, kind, cbeg, cend, ctl);
+        // the mixed solve's FP64 re-assembly visits only the windows it flagged
+        const int c = A.only_fallback ? next_fallback_chunk(A, cbeg, cend, ctl) : next_chunk(A, kind, cbeg, cend, ctl);
         if (c < 0) break;
         const Chunk ch = A.chunks[c];
-        // the mixed solve's FP64 re-assembly visits only the windows it flagged (uniform per workgroup)
-        if (A.only_fallback && A.wstate[(size_t)NWS * ch.win + WS_FALLBACK] == 0.0) continue;
         const int32_t *smap = ch.smap >= 0 ? A.chunk_slots + ch.smap : nullptr;
         const WinInfo W = A.wins[ch.win];
         const double *wp = A.win_par + (size_t)XBA_WPAR * ch.win;
@@ -465,6 +505,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             //      Hll = L L^T
             double lcost = 0.0;
             int kb = 0;
+            bool lok = false;
             unsigned msk = 0u;
             if (lane < B) {
                 const int l = lb + lane, li = pl_li;
@@ -526,6 +567,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                         lr[LR_G + 1] = i10 * bl[0] + i11 * bl[1];
                         lr[LR_G + 2] = i20 * bl[0] + i21 * bl[1] + i22 * bl[2];
                     }
+                    lok = ok;
                     lr[LR_OK] = ok ? 1.0 : 0.0;
                     lr[LR_MASKED] = 1.0;  // masked in: its pose-pose blocks are added even when the det guard fails
                     A.lmok[l] = ok ? 1 : 0;
@@ -543,7 +585,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             const int koff = wave_incl_scan(kb, lane) - kb;
             const int ktot = __builtin_amdgcn_readlane(koff + kb, B - 1);
-            const int npb = kb * (kb + 1) / 2;
+            const int npb = lok ? kb * (kb + 1) / 2 : 0;  // a landmark that failed the det guard has no Schur part
             const int pof = wave_incl_scan(npb, lane) - npb;
             const int ptot = __builtin_amdgcn_readlane(pof + npb, B - 1);
             if (lane < B) {
@@ -557,8 +599,8 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     m &= m - 1u;
                     i++;
                 }
-                for (int si = 0, o = pof; si < kb; si++)
-                    for (int sj = 0; sj <= si; sj++, o++) ptab[o] = (uint16_t)((lane << 10) | (si << 5) | sj);
+                for (int si = 0, o = pof; si < kb && lok; si++)
+                    for (int sj = 0; sj <= si; sj++, o++) ptab[o] = (uint16_t)((koff << 10) | (si << 5) | sj);
             }
             const double pcost = wave_sum(lcost, lane);
             if (lane == 0) A.pass_cost[p] = pcost;
@@ -603,16 +645,14 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 }
             }
             WAVE_SYNC();
-            // ---- reduced system: one lane per (landmark, touched block pair), -U2^T U1 in two halves of three
-            //      rows held in registers, then the half's atomics (lower triangle only on the diagonal blocks)
+            // ---- reduced system: one lane per (landmark, touched block pair) of the landmarks that passed the det
+            //      guard (the table entry holds the landmark's first touched-slot entry and the pair), -U2^T U1 in
+            //      two halves of three rows held in registers, then the half's atomics (lower triangle only on the
+            //      diagonal blocks)
             for (int o = lane; o < ptot; o += XBA_WAVE) {
                 const int pk = ptab[o];
-                const int bb = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
-                const T *lr = lrec + bb * FB_LREC;
-                const int base = loff[bb];
+                const int base = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
                 const int s2 = slots[base + si], s1 = slots[base + sj];
-                const bool schur = lr[LR_OK] != 0.0;
-                if (!schur) continue;
                 const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
                 T a1[18];
                 for (int e = 0; e < 18; e++) a1[e] = U1[e];
