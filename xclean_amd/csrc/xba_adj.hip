@@ -16,7 +16,7 @@ namespace xba {
 // Per window: F = H + 1e-9 I factored in FP64 into fact64 (kept for the adjoint), dx = -F^-1 b.
 __global__ __launch_bounds__(256) void k_gn_solve(DArgs D) {
     __shared__ int sflag;
-    __shared__ double sinv[2 * CH_BS];
+    __shared__ double sinv[CH_SCRATCH];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
     const int n = W.n;

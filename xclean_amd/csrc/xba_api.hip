@@ -1378,6 +1378,40 @@ int64_t np_carve(double *base, int nk, int nl, int N, NpArgs &A, NpWs &W) {
     return o;
 }
 
+#ifndef XBA_NP_SCHUR_LDS
+#define XBA_NP_SCHUR_LDS 1
+#endif
+// The reduced system of a local-BA solve (k_np_schur_pairs): in each workgroup's LDS when the upper triangle and
+// the gradient fit (a few workgroups, each flushed once into H), else with global atomics.
+template <bool ID>
+int np_launch_schur(const NpArgs &A, int n, hipStream_t s) {
+    const int nblk = (A.nl + NP_LB - 1) / NP_LB;
+    if (nblk == 0) return XBA_OK;
+    const size_t lds = ((size_t)n * (n + 1) / 2 + n) * sizeof(double);
+    constexpr int cap = 150 * 1024;
+    if (XBA_NP_SCHUR_LDS && lds <= (size_t)cap) {
+        HIPCK(hipFuncSetAttribute((const void *)k_np_schur_pairs<ID, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  cap));
+        hipLaunchKernelGGL((k_np_schur_pairs<ID, true>), dim3(std::min(nblk, 2 * 256)), dim3(256), lds, s, A, n);
+    } else {
+        hipLaunchKernelGGL((k_np_schur_pairs<ID, false>), dim3(std::min(nblk, 1024)), dim3(256), 0, s, A, n);
+    }
+    return XBA_OK;
+}
+
+// Dynamic LDS of k_np_solve: the damped n x n matrix with rows padded to a multiple of 4 entries when it fits.
+int np_solve_lds(int n) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void *)k_np_solve) != hipSuccess) return -1;
+    const int cap = 160 * 1024 - (int)fa.sharedSizeBytes - 256;  // beside its static LDS
+    const size_t need = (size_t)n * ((n + 3) & ~3) * sizeof(double);
+    if (cap <= 0 || need > (size_t)cap) return 0;
+    if (hipFuncSetAttribute((const void *)k_np_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) !=
+        hipSuccess)
+        return -1;
+    return (int)need;
+}
+
 // The LM iterations of the Euclidean solve on a planned problem (A.order / A.lstart ready on the device):
 // shared by xba_np_solve and xba_np_solve_noalloc.  Synchronous.
 int np_solve_launch(const xba_np_problem *q, NpArgs &A, int32_t iters, double lam0, double *stats_out,
@@ -1391,6 +1425,8 @@ int np_solve_launch(const xba_np_problem *q, NpArgs &A, int32_t iters, double la
     }
     const int gk = (nk + 255) / 256, gN = (N + 255) / 256, gl = (nl + 255) / 256;
     const int gz = std::min((n * n + 255) / 256, 1024), gp = (3 * nl + 255) / 256;
+    const int slds = np_solve_lds(n);
+    if (slds < 0) return XBA_E_HIP;
     k_np_inv_poses<<<gk, 256, 0, s>>>(nk, q->poses, A.Tcw);
     if (gN) k_np_r2_only<<<gN, 256, 0, s>>>(A);
     k_np_chi<<<1, 256, 0, s>>>(A, 5);
@@ -1399,8 +1435,8 @@ int np_solve_launch(const xba_np_problem *q, NpArgs &A, int32_t iters, double la
         if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 0);
         if (gl) k_np_lm<<<gl, 256, 0, s>>>(A);
         k_np_zero<<<gz, 256, 0, s>>>(A, n);
-        if (gl) k_np_schur<<<gl, 256, 0, s>>>(A, n);
-        k_np_solve<<<1, 256, 0, s>>>(A, n, it);
+        if (gl && np_launch_schur<false>(A, n, s) != XBA_OK) return XBA_E_HIP;
+        k_np_solve<<<1, 512, slds, s>>>(A, n, it, slds);
         if (gl) k_np_backsub<<<gl, 256, 0, s>>>(A);
         if (A.policy) {
             k_np_apply<<<1, 256, 0, s>>>(A, n, 0);
@@ -1616,6 +1652,11 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
     A.order = d_order;
     A.lstart = d_lstart;
     const double st0[NST] = {lam0, 0, 0, 0, 1, 0, 0, 0, 0, 0, -1, -1, -1, 0, 0, 0};
+    const int slds = np_solve_lds(n);
+    if (slds < 0) {
+        release();
+        return XBA_E_HIP;
+    }
     auto run = [&]() -> int {
         if (N) HIPCK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(d_lstart, lstart.data(), sizeof(int32_t) * (nl + 1), hipMemcpyHostToDevice, s));
@@ -1634,8 +1675,8 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
             if (A.policy) k_np_gate<<<1, 256, 0, s>>>(A, 1);
             if (gl) k_id_lm<<<gl, 256, 0, s>>>(A);
             k_np_zero<<<gz, 256, 0, s>>>(A, n);
-            if (gl) k_id_schur<<<gl, 256, 0, s>>>(A, n);
-            k_np_solve<<<1, 256, 0, s>>>(A, n, it);
+            if (gl && np_launch_schur<true>(A, n, s) != XBA_OK) return XBA_E_HIP;
+            k_np_solve<<<1, 512, slds, s>>>(A, n, it, slds);
             k_np_inv_poses<<<gk, 256, 0, s>>>(nk, A.Tnew, A.Twn);
             if (gl) k_id_backsub<<<gl, 256, 0, s>>>(A);
             if (A.policy) {

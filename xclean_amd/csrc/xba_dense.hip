@@ -586,8 +586,10 @@ __device__ inline double lane_bcast(double v, int jj) {
 //      (consecutive threads then read consecutive 16-byte groups), and each loaded entry feeds 4 products.
 // The lower triangle holds the factor on return; the upper triangle is scratch.  Every thread reads the pivot
 // status, so a failure exits uniformly.  Returns false on a non-positive or non-finite pivot.  Inlined, so an
-// LDS-resident A gets LDS instructions.  sinv: 2 CH_BS entries of LDS.
+// LDS-resident A gets LDS instructions.  sinv: CH_SCRATCH entries of LDS (the pivots' inverse square roots and
+// the current pivot column).
 #define CH_BS 32
+#define CH_SCRATCH (2 * CH_BS)
 template <typename T>
 __device__ __attribute__((always_inline)) inline bool chol_wg(T *A, int n, int *sflag, T *sinv, int ld) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
@@ -819,7 +821,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap, int fallbac
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sx[XBA_MAX_STATE], sr[XBA_MAX_STATE], ss[XBA_MAX_STATE];
     __shared__ T sy[XBA_MAX_STATE];
-    __shared__ T sinv[2 * CH_BS];
+    __shared__ T sinv[CH_SCRATCH];
     __shared__ double sred[32];
     __shared__ int sflag;
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;

@@ -127,43 +127,16 @@ __global__ void k_np_zero(NpArgs A, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) A.bp[i] = 0.0;
 }
 
-// Reduced system: H = Hpp - sum_l sum_{a,b} (Hpl_a Hlli) Hpl_b^T, bp = -Jpose^T rw - (Hpl_a Hlli) bl (ba.py:766-823).
-// One thread per landmark; FP64 atomics into the small n x n system.
-__global__ void k_np_schur(NpArgs A, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl || np_skip(A)) return;
-    const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
-    for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
-        const int ka = A.order[qa], ki = 6 * A.okf[ka];
-        const double *Ja = A.J + 20 * (size_t)ka;
-        double Hpa[18], T3[18];
-        hpl_of(Ja, Hpa);
-        for (int i = 0; i < 6; i++)
-            for (int c = 0; c < 3; c++)
-                T3[3 * i + c] = Hpa[3 * i] * Hi[c] + Hpa[3 * i + 1] * Hi[3 + c] + Hpa[3 * i + 2] * Hi[6 + c];
-        for (int i = 0; i < 6; i++) {
-            double g = -(Ja[i] * Ja[18] + Ja[6 + i] * Ja[19]) - (T3[3 * i] * bl[0] + T3[3 * i + 1] * bl[1] + T3[3 * i + 2] * bl[2]);
-            atomicAdd(A.bp + ki + i, g);
-            for (int c = 0; c < 6; c++) atomicAdd(A.H + (size_t)(ki + i) * n + ki + c, Ja[i] * Ja[c] + Ja[6 + i] * Ja[6 + c]);
-        }
-        for (int qb = A.lstart[j]; qb < A.lstart[j + 1]; qb++) {
-            const int kb = A.order[qb], kk = 6 * A.okf[kb];
-            double Hpb[18];
-            hpl_of(A.J + 20 * (size_t)kb, Hpb);
-            for (int i = 0; i < 6; i++)
-                for (int c = 0; c < 6; c++)
-                    atomicAdd(A.H + (size_t)(ki + i) * n + kk + c,
-                              -(T3[3 * i] * Hpb[3 * c] + T3[3 * i + 1] * Hpb[3 * c + 1] + T3[3 * i + 2] * Hpb[3 * c + 2]));
-        }
-    }
-}
-
 // so3 log via the quaternion and Jr^-1 (for the rotation factors) come from xba_dense.hip.
 // One workgroup: chi2, rotation-factor gradient, held slots, damping, Cholesky, solve, trial poses (ba.py:825-856).
-__global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
-    __shared__ double sred[256];
+// 512 threads.  The damped matrix is factored in LDS (rows padded to a multiple of 4 entries) when it fits
+// `lds_cap` bytes of dynamic LDS, else in place in H.
+__global__ __launch_bounds__(512) void k_np_solve(NpArgs A, int n, int it, int lds_cap) {
+    extern __shared__ __align__(16) double sF[];
+    __shared__ double sred[512];
+    __shared__ double sxv[XBA_MAX_STATE];
     __shared__ int sflag;
-    __shared__ double sinv[2 * CH_BS];
+    __shared__ double sinv[CH_SCRATCH];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (np_skip(A)) return;  // (uniform: no kernel of this launch writes the flag)
     // chi2 of the current residuals, fixed-order reduction
@@ -224,24 +197,35 @@ __global__ __launch_bounds__(256) void k_np_solve(NpArgs A, int n, int it) {
             }
         }
     __syncthreads();
-    // held slots: rows and columns leave the solve (ba.py:829-836)
+    // held slots: rows and columns leave the solve (ba.py:829-836); damping Hred_ii += lam max(Hred_ii, 1e-9)
+    // (ba.py:837-838); the result in the factor's storage
+    const int ldp = (n + 3) & ~3;
+    const bool in_lds = (size_t)n * ldp * sizeof(double) <= (size_t)lds_cap;
+    const int ld = in_lds ? ldp : n;
+    double *F = in_lds ? sF : A.H;
+    const double lam = A.st[0];
     for (int e = tid; e < n * n; e += nt) {
         const int i = e / n, c = e % n;
         const bool hi_ = !A.free_mask[i / 6], hc = !A.free_mask[c / 6];
-        if (hi_ || hc) A.H[e] = (hi_ && hc && i / 6 == c / 6 && i == c) ? 1.0 : 0.0;
+        double h = A.H[e];
+        if (hi_ || hc) h = (hi_ && hc && i / 6 == c / 6 && i == c) ? 1.0 : 0.0;
+        if (i == c) h += lam * fmax(h, 1e-9);
+        F[(size_t)i * ld + c] = h;
     }
     for (int i = tid; i < n; i += nt)
         if (!A.free_mask[i / 6]) A.bp[i] = 0.0;
     __syncthreads();
-    const double lam = A.st[0];
-    for (int i = tid; i < n; i += nt) A.H[(size_t)i * n + i] += lam * fmax(A.H[(size_t)i * n + i], 1e-9);
-    __syncthreads();
-    // Cholesky (the damped reduced system is symmetric positive definite) in place, then solve
-    bool ok = chol_wg<double>(A.H, n, &sflag, sinv, n);
+    // Cholesky (the damped reduced system is symmetric positive definite), then solve
+    bool ok = in_lds ? chol_wg<double>(sF, n, &sflag, sinv, ld) : chol_wg<double>(A.H, n, &sflag, sinv, ld);
     if (ok) {
-        for (int i = tid; i < n; i += nt) A.dx[i] = A.bp[i];
+        for (int i = tid; i < n; i += nt) sxv[i] = A.bp[i];
         __syncthreads();
-        chol_solve_wg<double>(A.H, n, A.dx, n);
+        if (in_lds)
+            chol_solve_wg<double>(sF, n, sxv, ld);
+        else
+            chol_solve_wg<double>(A.H, n, sxv, ld);
+        for (int i = tid; i < n; i += nt) A.dx[i] = sxv[i];
+        __syncthreads();
         if (A.dx_hist)
             for (int i = tid; i < n; i += nt) A.dx_hist[(size_t)it * n + i] = A.dx[i];
         // trial poses: T_new = exp(dx) T (free slots), rotation projected onto SO(3) by polar iteration
@@ -597,53 +581,105 @@ __device__ inline void id_hpl(const double *J, int sa, double *Hpl) {  // (6x3) 
         for (int c = 0; c < 3; c++) Hpl[3 * i + c] = Jp[i] * Jl[c] + Jp[6 + i] * Jl[3 + c];
 }
 
-// Reduced system, one thread per landmark, FP64 atomics (ba.py:1266-1305): every observation contributes its
-// observer and (in-window) host pose blocks to Hpp and the gradient; H -= (Hpl_a Hlli) Hpl_b^T over the landmark's
-// (observation, slot) entries; bp -= (Hpl_a Hlli) bl.
-__global__ void k_id_schur(NpArgs A, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nl || np_skip(A)) return;
-    const double *Hi = A.Hlli + 9 * (size_t)j, *bl = A.bl + 3 * (size_t)j;
-    const int hs = A.hidx[j];
-    for (int qa = A.lstart[j]; qa < A.lstart[j + 1]; qa++) {
-        const int ka = A.order[qa];
-        const double *Ja = A.J + 32 * (size_t)ka, *rw = Ja + 30;
-        const int sla[2] = {A.okf[ka], hs};
-        for (int sa = 0; sa < 2; sa++) {
-            if (sla[sa] < 0) continue;
-            const double *Jp = Ja + 12 * sa;
-            const int ki = 6 * sla[sa];
-            for (int sb = 0; sb < 2; sb++) {
-                if (sla[sb] < 0) continue;
-                const double *Jq = Ja + 12 * sb;
-                const int kk = 6 * sla[sb];
-                for (int i = 0; i < 6; i++)
-                    for (int c = 0; c < 6; c++)
-                        atomicAdd(A.H + (size_t)(ki + i) * n + kk + c, Jp[i] * Jq[c] + Jp[6 + i] * Jq[6 + c]);
+// ---- the reduced system of both local-BA solves, one thread per (observation block, observation block) pair of a
+// landmark (ba.py:759-823 Euclidean: one pose block per observation; :1266-1305 inverse depth: the observer's and
+// the in-window host's).  A thread's pair (a, sa) x (b, sb) adds -(Hpl_a Hlli) Hpl_b^T (plus Jp_a^T Jp_b when a == b,
+// and the gradient of (a, sa) when also sa == sb) to the upper triangle of H only: pairs whose row block lies
+// below their column block are the transposes of pairs that are kept, and the flush mirrors the triangle.  When the
+// triangle and the gradient fit (INLDS), a workgroup accumulates into its LDS copy (LDS atomics) and adds it to H
+// and bp once; otherwise straight into H and bp (global atomics).  Grid: workgroups of 256 threads over blocks of
+// NP_LB landmarks; the pairs of a block are numbered by a prefix sum of (blocks per landmark)^2.
+#define NP_LB 64
+template <bool ID>
+__device__ inline int np_block_slot(const NpArgs &A, int k, int sb, int j) {
+    return ID ? (sb == 0 ? A.okf[k] : A.hidx[j]) : A.okf[k];
+}
+
+__device__ inline int np_tri(int r, int c, int n) { return r * n - r * (r - 1) / 2 + (c - r); }  // r <= c
+
+template <bool ID, bool INLDS>
+__global__ __launch_bounds__(256) void k_np_schur_pairs(NpArgs A, int n) {
+    extern __shared__ double tri[];  // INLDS: n (n + 1) / 2 upper-triangle entries, then the n gradient entries
+    __shared__ int spfx[NP_LB + 1];
+    if (np_skip(A)) return;  // uniform
+    constexpr int NB = ID ? 2 : 1, JS = ID ? 32 : 20;
+    const int tid = threadIdx.x, ntri = n * (n + 1) / 2;
+    double *Hacc = INLDS ? tri : A.H, *bacc = INLDS ? tri + ntri : A.bp;
+    if (INLDS) {
+        for (int e = tid; e < ntri + n; e += 256) tri[e] = 0.0;
+        __syncthreads();
+    }
+    auto add = [&](double *p, double v) {
+        if (INLDS)
+            __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicAdd(p, v);
+    };
+    for (int l0 = blockIdx.x * NP_LB; l0 < A.nl; l0 += gridDim.x * NP_LB) {
+        const int l = l0 + tid;
+        const int m = (tid < NP_LB && l < A.nl) ? NB * (A.lstart[l + 1] - A.lstart[l]) : 0;  // its blocks
+        if (tid < NP_LB) spfx[tid + 1] = m * m;
+        if (tid == 0) spfx[0] = 0;
+        __syncthreads();
+        for (int d = 1; d < NP_LB; d <<= 1) {  // inclusive scan of spfx[1..NP_LB]
+            const int v = (tid < NP_LB && tid + 1 > d) ? spfx[tid + 1 - d] : 0;
+            __syncthreads();
+            if (tid < NP_LB) spfx[tid + 1] += v;
+            __syncthreads();
+        }
+        const int tot = spfx[NP_LB];
+        for (int p = tid; p < tot; p += 256) {
+            int lo = 0, hi = NP_LB - 1;  // the last landmark whose first pair is at or before p
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (spfx[mid] <= p) lo = mid; else hi = mid - 1;
             }
-            double Hpa[18], T3[18];
-            id_hpl(Ja, sa, Hpa);
+            const int j = l0 + lo, q0 = A.lstart[j], mj = NB * (A.lstart[j + 1] - q0), q = p - spfx[lo];
+            const int ea = q / mj, eb = q % mj;  // observation blocks a and b of landmark j
+            const int ka = A.order[q0 + ea / NB], kb = A.order[q0 + eb / NB], sa = ea % NB, sb = eb % NB;
+            const int s1 = np_block_slot<ID>(A, ka, sa, j), s2 = np_block_slot<ID>(A, kb, sb, j);
+            if (s1 < 0 || s2 < 0 || s1 > s2) continue;  // an out-of-window host, or a transposed pair
+            const double *Ja = A.J + JS * (size_t)ka, *Jb = A.J + JS * (size_t)kb;
+            const double *Pa = Ja + 12 * sa, *Pb = Jb + 12 * sb;  // the 2 x 6 pose blocks
+            const double *Hi = A.Hlli + 9 * (size_t)j;
+            double Hpa[18], Hpb[18], T3[18];
+            if (ID) {
+                id_hpl(Ja, sa, Hpa);
+                id_hpl(Jb, sb, Hpb);
+            } else {
+                hpl_of(Ja, Hpa);
+                hpl_of(Jb, Hpb);
+            }
             for (int i = 0; i < 6; i++)
                 for (int c = 0; c < 3; c++)
                     T3[3 * i + c] = Hpa[3 * i] * Hi[c] + Hpa[3 * i + 1] * Hi[3 + c] + Hpa[3 * i + 2] * Hi[6 + c];
+            const bool same_obs = ka == kb, diag = s1 == s2;
+            const int r0 = 6 * s1, c0 = 6 * s2;
             for (int i = 0; i < 6; i++)
-                atomicAdd(A.bp + ki + i, -(Jp[i] * rw[0] + Jp[6 + i] * rw[1]) -
-                                             (T3[3 * i] * bl[0] + T3[3 * i + 1] * bl[1] + T3[3 * i + 2] * bl[2]));
-            for (int qb = A.lstart[j]; qb < A.lstart[j + 1]; qb++) {
-                const int kb = A.order[qb];
-                const int slb[2] = {A.okf[kb], hs};
-                for (int sb = 0; sb < 2; sb++) {
-                    if (slb[sb] < 0) continue;
-                    double Hpb[18];
-                    id_hpl(A.J + 32 * (size_t)kb, sb, Hpb);
-                    const int kk = 6 * slb[sb];
-                    for (int i = 0; i < 6; i++)
-                        for (int c = 0; c < 6; c++)
-                            atomicAdd(A.H + (size_t)(ki + i) * n + kk + c,
-                                      -(T3[3 * i] * Hpb[3 * c] + T3[3 * i + 1] * Hpb[3 * c + 1] + T3[3 * i + 2] * Hpb[3 * c + 2]));
+                for (int c = diag ? i : 0; c < 6; c++) {
+                    double v = -(T3[3 * i] * Hpb[3 * c] + T3[3 * i + 1] * Hpb[3 * c + 1] + T3[3 * i + 2] * Hpb[3 * c + 2]);
+                    if (same_obs) v += Pa[i] * Pb[c] + Pa[6 + i] * Pb[6 + c];
+                    add(INLDS ? Hacc + np_tri(r0 + i, c0 + c, n) : Hacc + (size_t)(r0 + i) * n + c0 + c, v);
+                    if (!INLDS && (r0 + i) != (c0 + c)) add(Hacc + (size_t)(c0 + c) * n + r0 + i, v);
                 }
+            if (same_obs && sa == sb) {
+                const double *bl = A.bl + 3 * (size_t)j, *rw = Ja + (ID ? 30 : 18);
+                for (int i = 0; i < 6; i++)
+                    add(bacc + r0 + i, -(Pa[i] * rw[0] + Pa[6 + i] * rw[1]) -
+                                           (T3[3 * i] * bl[0] + T3[3 * i + 1] * bl[1] + T3[3 * i + 2] * bl[2]));
             }
         }
+        __syncthreads();
+    }
+    if (INLDS) {  // the workgroup's triangle and gradient into H (both triangles) and bp
+        __syncthreads();
+        for (int e = tid; e < n * n; e += 256) {
+            const int r = e / n, c = e % n;
+            const double v = r <= c ? tri[np_tri(r, c, n)] : tri[np_tri(c, r, n)];
+            if (v != 0.0) atomicAdd(A.H + e, v);
+        }
+        for (int i = tid; i < n; i += 256)
+            if (tri[ntri + i] != 0.0) atomicAdd(A.bp + i, tri[ntri + i]);
     }
 }
 
