@@ -45,6 +45,21 @@ def _measured_traffic():
     return float(m.group(1)) if m else None
 
 
+CFG4_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "assembly_config4_traffic.md")
+
+
+def _measured_config4_assembly_bytes():
+    """HBM bytes of configuration 4's assembly launches (k_lin_fast narrow + wide, one LM iteration) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None when that summary is absent."""
+    import re
+    try:
+        txt = open(CFG4_TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"assembly_bytes_per_iteration\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
 def single_window_configs(torch, synth, BatchSolver, log):
     """Configurations 1 (20 visual-inertial keyframes, n = 300) and 4 (60 visual-inertial keyframes, n = 900,
     200,000 landmarks): one LM iteration from the initial state replayed from its graph, mixed precision and FP64,
@@ -80,6 +95,14 @@ def single_window_configs(torch, synth, BatchSolver, log):
             del b
         rec["dx_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(dxs["mixed"] - dxs["fp64"]) /
                                                 np.linalg.norm(dxs["fp64"]))
+        if cfg == 4:
+            hb = _measured_config4_assembly_bytes()
+            if hb is not None:
+                ms_asm = rec["mixed"]["kernel_ms"]["lin_schur"] + rec["mixed"]["kernel_ms"]["lin_schur_wide"]
+                rec["assembly_hbm"] = {"bytes_per_iteration": hb, "kernel_ms": ms_asm,
+                                       "achieved_gbs": hb / (ms_asm * 1e-3) / 1e9,
+                                       "frac": hb / (ms_asm * 1e-3) / 1e9 / 8000.0,
+                                       "source": os.path.relpath(CFG4_TRAFFIC_SUMMARY, ROOT)}
         log("bench: config %d: mixed %.3e / fp64 %.3e obs/s per LM iteration, mixed-vs-fp64 dx err %.2e" % (
             cfg, rec["mixed"]["value"], rec["fp64"]["value"], rec["dx_rel_err_mixed_vs_fp64"]))
         out["config%d" % cfg] = rec
@@ -181,6 +204,64 @@ def preintegration_config(torch, synth, log, cpu_budget=3.0, nwin=256, nkf=20, s
     return rec
 
 
+def _marg_roofline(ns, ms):
+    """The marginalisation launches against the FP64 peak: algorithmic FLOPs (roofline.marg_flops, 6 columns
+    marginalised from each window's n + 6) over their HIP-event time.  Bound by the serial Jacobi rounds of the
+    largest matrix, not by arithmetic."""
+    from xclean_amd import roofline
+    fl = float(sum(roofline.marg_flops(n, 6) for n in ns))
+    tf = fl / (ms * 1e-3) / 1e12
+    return {"bound": "latency (serial Jacobi rounds)", "achieved": tf, "peak": roofline.FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": tf / roofline.FP64_PEAK_TFLOPS, "algorithmic_flops": fl,
+            "kernels": "k_marg_schur + k_prior_from_info"}
+
+
+def local_ba_config(torch, synth, log, cpu_budget=3.0, reps=20):
+    """The local-BA drop-in (backend/ba.py:713 _solve_np, 5 LM iterations, FP64) at tests/test_ba_solver.py's
+    problem (make_problem: 15 keyframes, 600 points, 4 views) and at 30 keyframes / 5,000 points / 6 views: median
+    and p95 per call (inputs on the host, as the reference's callers hand them over, so each call includes the
+    upload and the download), the allocation-free entry point beside it, and the oracle's C restatement of
+    _solve_np on one host thread.  Reference bars: local BA p95 per keyframe 10 ms (tier 1) / 5 ms (tier 2),
+    src/xclean/obs/budget.py:84-85."""
+    from oracle import oracle as O
+    from xclean_amd.ba import _solve_np
+    out = {"bars_ms": {"tier1_p95": 10.0, "tier2_p95": 5.0}}
+    for name, (nk, nl, op) in (("test_ba_solver_nk15_nl600", (15, 600, 4)), ("nk30_nl5000", (30, 5000, 6))):
+        p = synth.make_problem(nk=nk, nl=nl, obs_per=op, seed=5)
+        args = (p["poses0"], p["pts0"], p["okf"], p["olm"], p["ouv"], p["rig"], 0, 5)
+        rec = {"keyframes": nk, "points": len(p["pts0"]), "obs": int(len(p["okf"])), "lm_iters": 5}
+        for mode in (False, True):
+            for _ in range(3):
+                _solve_np(*args, noalloc=mode)
+            ts = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                a = time.perf_counter()
+                r = _solve_np(*args, noalloc=mode)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - a)
+            key = "noalloc" if mode else "gpu"
+            rec[key] = {"ms_median": 1e3 * float(np.median(ts)), "ms_p95": 1e3 * float(np.percentile(ts, 95)),
+                        "obs_per_s_per_lm_iteration": len(p["okf"]) * 5 / float(np.median(ts))}
+        e0 = float(np.sqrt(np.mean(np.sum((p["poses0"][:, :3, 3] - p["poses_true"][:, :3, 3]) ** 2, 1))))
+        e1 = float(np.sqrt(np.mean(np.sum((r[0][:, :3, 3] - p["poses_true"][:, :3, 3]) ** 2, 1))))
+        rec["pose_error_ratio"] = e1 / e0
+        if cpu_budget > 0:
+            tc, n = 0.0, 0
+            while n < 3 or tc < cpu_budget / 2:
+                a = time.perf_counter()
+                O.solve_np(*args)
+                tc += time.perf_counter() - a
+                n += 1
+            rec["cpu_baseline"] = {"ms": 1e3 * tc / n, "cores": 1, "kind": "port",
+                                   "sample": "%d calls, oracle's C restatement of _solve_np, one host thread" % n}
+        log("bench: local BA %s: %.3f ms per call (p95 %.3f; no-alloc %.3f)%s" % (
+            name, rec["gpu"]["ms_median"], rec["gpu"]["ms_p95"], rec["noalloc"]["ms_median"],
+            "; cpu oracle %.3f ms" % rec["cpu_baseline"]["ms"] if "cpu_baseline" in rec else ""))
+        out[name] = rec
+    return out
+
+
 def marginalisation_config(torch, pbs, log, cpu_budget=3.0, reps=10):
     """Marginalisation of every window's oldest keyframe at once (vio/engine_live.py:690-739 per window): the
     visual rows of its hosted landmarks and the prior linearised at lambda = 0 (one batched linearisation), its 6
@@ -263,6 +344,7 @@ def marginalisation_config(torch, pbs, log, cpu_budget=3.0, reps=10):
                        % (len(Hs), min(ns), max(ns)),
            "windows": len(Hs), "n_max": int(max(ns)), "linearise_and_copy_ms": lin_ms, "schur_ms": ts, "sqrt_ms": tf,
            "windows_per_s": len(Hs) / ((ts + tf) * 1e-3), "max_rel_err_vs_oracle_first8": err,
+           "roofline": _marg_roofline(ns, ts + tf),
            "cpu_baseline": {"value": done / tc, "unit": "windows/s", "cores": 1, "kind": "port",
                             "sample": "first %d windows, numpy oracle of vio/marg.py (schur + from_info), one BLAS "
                                       "thread" % done}}
@@ -362,6 +444,8 @@ def main():
                     help="diagnostic: run only the batched marginalisation and print its JSON")
     ap.add_argument("--preint-only", action="store_true",
                     help="diagnostic: run only the batched IMU preintegration and print its JSON")
+    ap.add_argument("--local-ba-only", action="store_true",
+                    help="diagnostic: run only the local-BA drop-in timings and print their JSON")
     ap.add_argument("--configs-only", action="store_true",
                     help="diagnostic: run only configurations 1 and 4 (for profiling the tiled factorisation) and "
                          "print their JSON")
@@ -378,6 +462,9 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
+    if args.local_ba_only:
+        print(json.dumps(local_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)))
+        return
     if args.configs_only:
         print(json.dumps(single_window_configs(torch, synth, BatchSolver, log)))
         return
@@ -512,6 +599,7 @@ def main():
                                                               cpu_budget=0.0 if args.no_cpu else 3.0)
     margin = None if args.no_extra else marginalisation_config(torch, pbs, log,
                                                                cpu_budget=0.0 if args.no_cpu else 3.0)
+    local_ba = None if args.no_extra else local_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)
 
     cpu = None
     if not args.no_cpu:
@@ -581,6 +669,7 @@ def main():
         "differentiation": diff,
         "preintegration": preint,
         "marginalisation": margin,
+        "local_ba": local_ba,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

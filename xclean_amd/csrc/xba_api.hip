@@ -284,7 +284,7 @@ KArgs<double> fast_args(const xba_ctx *c, const KArgs<T> &A) {
 // Prior and IMU terms of every window: the IMU factors in parallel (k_imu), then one workgroup per window.
 void launch_terms(xba_ctx *c, const DArgs &D, hipStream_t s, int trial) {
     if (c->nimu > 0) hipLaunchKernelGGL(k_imu, dim3(c->nimu), dim3(256), 0, s, D, trial);
-    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(256), 0, s, D, trial);
+    hipLaunchKernelGGL(k_terms, dim3(c->nwin), dim3(1024), 0, s, D, trial);
 }
 
 // The linearisation half of one LM iteration: k_lin, k_reduce, k_terms.

@@ -296,6 +296,24 @@ struct Acc {  // FP64: plain sum; FP32: Kahan-compensated sum
     __device__ inline double value() const { return (double)s - (double)c; }
 };
 
+// Deterministic sum over the block: thread t adds v[i] for i = a0 + t, a0 + t + nt, ... (and likewise over [b0, b1)) in
+// order, then a butterfly inside each wavefront and the wavefronts' partials in order.  Called by every thread of the
+// block; every thread gets the sum.  sred: blockDim.x / 64 doubles of LDS.
+__device__ inline double block_sum_ranges(const double *v, int a0, int a1, int b0, int b1, double *sred) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double s = 0.0;
+    for (int c = a0 + tid; c < a1; c += nt) s += v[c];
+    for (int c = b0 + tid; c < b1; c += nt) s += v[c];
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) sred[tid >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < nt / 64; q++) t += sred[q];
+    __syncthreads();
+    return t;
+}
+
 template <typename T>
 __device__ inline void reduce_window(const DArgs &D, int w, int64_t idx, const T *partH, const T *partb,
                                      const T *partH_lo, const T *partb_lo) {
@@ -351,25 +369,32 @@ __device__ inline void reduce_window(const DArgs &D, int w, int64_t idx, const T
         }
         D.b[W.st0 + idx] = v.value();
     }
-    if (idx == 0) {
-        double cs = 0.0;
-        for (int c = W.c0; c < W.c1; c++) cs += D.partc[c];
-        for (int c = W.c2; c < W.c3; c++) cs += D.partc[c];
-        D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
-    }
+}
+
+// The window's robust visual cost, the work items' pass-cost sums added over the block (fixed order).  Called by
+// every thread of a block; thread 0 stores it.
+__device__ inline void window_visual_cost(const DArgs &D, int w, double *sred) {
+    const WinInfo W = D.wins[w];
+    const double cs = block_sum_ranges(D.partc, W.c0, W.c1, W.c2, W.c3, sred);
+    if (threadIdx.x == 0) D.wstate[(size_t)NWS * w + WS_COSTVIS] = cs;
 }
 
 // grid (ceil(max n^2 / 256), nwin): entry blockIdx.x * 256 + threadIdx.x of window blockIdx.y.  With
 // D.only_fallback (grid (.., 1)): that entry of every window flagged WS_FALLBACK, the flags read by every block.
 template <typename T>
 __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH_lo, const T *partb_lo) {
+    __shared__ double sred[16];
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!D.only_fallback) {
         reduce_window<T>(D, blockIdx.y, idx, partH, partb, partH_lo, partb_lo);
+        if (blockIdx.x == 0) window_visual_cost(D, blockIdx.y, sred);
         return;
     }
     for (int w = 0; w < D.nwin; w++)
-        if (D.wstate[(size_t)NWS * w + WS_FALLBACK] != 0.0) reduce_window<T>(D, w, idx, partH, partb, partH_lo, partb_lo);
+        if (D.wstate[(size_t)NWS * w + WS_FALLBACK] != 0.0) {
+            reduce_window<T>(D, w, idx, partH, partb, partH_lo, partb_lo);
+            if (blockIdx.x == 0) window_visual_cost(D, w, sred);
+        }
 }
 
 // --------------------------------------------------------------------------------------------- k_imu --
@@ -548,7 +573,7 @@ __device__ inline void terms_window(const DArgs &D, int w, int trial, double *sd
     }
 }
 
-__global__ __launch_bounds__(256) void k_terms(DArgs D, int trial) {
+__global__ __launch_bounds__(1024) void k_terms(DArgs D, int trial) {
     __shared__ double sd[XBA_MAX_STATE], sres[XBA_MAX_STATE];
     __shared__ int scol[XBA_MAX_STATE];
     terms_window(D, blockIdx.x, trial, sd, sres, scol);
@@ -835,6 +860,7 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap, int fallbac
             for (int64_t idx = tid; idx < (int64_t)W.n * W.n; idx += nt)
                 reduce_window<double>(D, w, idx, D.partH64, D.partb64, (const double *)nullptr,
                                       (const double *)nullptr);
+            window_visual_cost(D, w, sred);
             __syncthreads();
             terms_window(D, w, 0, sx, sr, (int *)ss);
             __syncthreads();
@@ -976,15 +1002,16 @@ __global__ __launch_bounds__(512) void k_solve(DArgs D, int lds_cap, int fallbac
 // c_new = trial visual cost (chunk sums in order) + prior/IMU trial cost; accept iff c_new < c_tot.
 __global__ __launch_bounds__(256) void k_accept(DArgs D) {
     __shared__ int sacc;
+    __shared__ double sred[16];
     const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const WinInfo W = D.wins[w];
     double *ws = D.wstate + (size_t)NWS * w;
+    const bool chol_ok = ws[WS_CHOLOK] != 0.0;  // (uniform)
+    const double cvis = chol_ok ? block_sum_ranges(D.partc2, W.c0, W.c1, W.c2, W.c3, sred) : 0.0;
     if (tid == 0) {
         int acc = 0;
-        if (ws[WS_CHOLOK] != 0.0) {
-            double cn = 0.0;
-            for (int c = W.c0; c < W.c1; c++) cn += D.partc2[c];
-            for (int c = W.c2; c < W.c3; c++) cn += D.partc2[c];
+        if (chol_ok) {
+            double cn = cvis;
             cn += ws[WS_COSTTRIAL_REST];
             ws[WS_CNEW] = cn;
             acc = cn < ws[WS_CTOT];

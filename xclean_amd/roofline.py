@@ -99,3 +99,17 @@ def preint_flops(n_sub):
 def preint_bytes(n_samples_read, n_intervals, whiten_and_records=True):
     return PREINT_BYTES_SAMPLE * float(n_samples_read) + n_intervals * (
         PREINT_BYTES_OUT_W_REC if whiten_and_records else PREINT_BYTES_OUT)
+
+
+def eig_flops(n):
+    """Algorithmic FLOPs of one symmetric eigen-decomposition with eigenvectors of order n: the 9 n^3 of the
+    standard count (tridiagonal reduction 4/3 n^3, eigenvectors and back-transformation the rest)."""
+    return 9.0 * float(n) ** 3
+
+
+def marg_flops(n_keep, n_marg):
+    """Algorithmic FLOPs of one marginalisation of the square-root prior: the Schur complement (the eigen-
+    decomposition of the marginalised block and the rank update of the kept one) and the square root of the result
+    (one eigen-decomposition of the kept block)."""
+    k, m = float(n_keep), float(n_marg)
+    return eig_flops(m) + 2.0 * k * k * m + eig_flops(k)
