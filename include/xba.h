@@ -203,6 +203,17 @@ typedef struct xba_np_problem {
 int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t precision, double *stats_out,
                  double *dx_hist_out, void *stream);
 
+/* ---- ordered observation pairs (backend/ba.py:902-927 build_pairs) -------------------------------------------
+ * Replaces build_pairs(olm_sorted, nl) -> (offs, pa, pb).  Device pointers: olm_sorted (n_obs, int32) is the landmark
+ * of every observation, ascending; offs (nl + 1, int32) receives the first observation of every landmark and
+ * offs[nl] = n_obs; pa, pb (cap, int32) receive every ordered pair (a, b) of observations of one landmark, landmarks
+ * ascending, a-major then b.  *npair (host) receives the pair count sum_l c_l^2.  pa == NULL: offs and *npair only
+ * (the size query).  cap < *npair: XBA_E_SIZE and nothing written to pa, pb.  An index outside [0, nl) or a list
+ * that is not ascending: XBA_E_RANGE (the reference's bincount raises on the first and returns segments that are not
+ * the landmarks' on the second).  Synchronous (the count is read back). */
+int xba_build_pairs(const int32_t *olm_sorted, int64_t n_obs, int32_t nl, int32_t *offs, int32_t *pa, int32_t *pb,
+                    int64_t cap, int64_t *npair, void *stream);
+
 /* ---- the Euclidean local BA without allocation ---------------------------------------------------------------
  * No counterpart in the reference: xba_np_solve for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments, results and return codes

@@ -16,6 +16,7 @@ XBA_FP64, XBA_MIXED = 0, 1
 XBA_WPAR = 24
 XBA_IMU_REC = 301
 XBA_NSTATS = 10  # columns of xba_stats (include/xba.h)
+XBA_E_ARG, XBA_E_HIP, XBA_E_TOO_MANY_OBS, XBA_E_RANGE, XBA_E_SIZE, XBA_E_STATE = -1, -2, -3, -4, -5, -6
 
 vp = C.c_void_p
 i32p = C.POINTER(C.c_int32)
@@ -64,7 +65,7 @@ class XbaError(RuntimeError):
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
            "xba_np_solve", "xba_np_solve_noalloc", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
-           "xba_marg_schur", "xba_prior_from_info",
+           "xba_marg_schur", "xba_prior_from_info", "xba_build_pairs",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
 
@@ -98,6 +99,7 @@ def load():
     L.xba_sqrt_information.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
     L.xba_marg_schur.argtypes = [vp, vp]
     L.xba_prior_from_info.argtypes = [vp, vp]
+    L.xba_build_pairs.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_int64, C.POINTER(C.c_int64), vp]
     L.xba_strerror.restype = C.c_char_p
     L.xba_strerror.argtypes = [C.c_int]
     L.xba_version.restype = C.c_char_p

@@ -193,6 +193,49 @@ def _solve_np_id(poses, rho, mbear, hidx, hpose_ext, okf, olm, ouv, rig, free, i
     return out
 
 
+def build_pairs_device(olm_sorted, nl):
+    """backend/ba.py:902-927 ``build_pairs`` on device tensors: ``olm_sorted`` (int32 CUDA tensor, ascending) ->
+    (offs (nl + 1,), pa (npair,), pb (npair,)) int32 CUDA tensors, built by libxba ``xba_build_pairs`` on the tensor's
+    stream.  Raises ValueError for a landmark index outside [0, nl) or a list that is not ascending."""
+    import torch
+    _lib.require_gpu()
+    L = _lib.load()
+    if olm_sorted.dtype != torch.int32 or not olm_sorted.is_cuda:
+        raise ValueError("olm_sorted must be an int32 CUDA tensor")
+    olm_sorted = olm_sorted.contiguous().reshape(-1)
+    dev = olm_sorted.device
+    nl = int(nl)
+    offs = torch.empty(nl + 1, dtype=torch.int32, device=dev)
+    n = C.c_int64(0)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        optr = C.c_void_p(olm_sorted.data_ptr()) if olm_sorted.numel() else None
+        rc = L.xba_build_pairs(optr, int(olm_sorted.numel()), nl, C.c_void_p(offs.data_ptr()), None, None, 0,
+                               C.byref(n), C.c_void_p(stream))
+        if rc == _lib.XBA_E_RANGE:
+            raise ValueError("olm_sorted must be ascending with every index in [0, %d)" % nl)
+        _lib.check(rc, "xba_build_pairs")
+        pa = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+        pb = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+        rc = L.xba_build_pairs(optr, int(olm_sorted.numel()), nl, C.c_void_p(offs.data_ptr()),
+                               C.c_void_p(pa.data_ptr()), C.c_void_p(pb.data_ptr()), int(pa.numel()), C.byref(n),
+                               C.c_void_p(stream))
+        _lib.check(rc, "xba_build_pairs")
+    return offs, pa[:n.value], pb[:n.value]
+
+
+def build_pairs(olm_sorted, nl, device=None):
+    """backend/ba.py:902-927 ``build_pairs(olm_sorted, nl)`` -> (offs, pa, pb) as int32 numpy arrays: offs[l] the
+    first observation of landmark l (nl + 1 entries), and every ordered observation pair (a, b) inside a landmark,
+    landmark-major, a-major then b.  The lists are built on the GPU (``build_pairs_device``)."""
+    import torch
+    _lib.require_gpu()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    o = torch.as_tensor(np.ascontiguousarray(np.asarray(olm_sorted).ravel(), dtype=np.int32)).to(dev)
+    offs, pa, pb = build_pairs_device(o, nl)
+    return offs.cpu().numpy(), pa.cpu().numpy(), pb.cpu().numpy()
+
+
 def _npair(olm, nl):
     """backend/ba.py:902-927 build_pairs' count: every ordered observation pair inside a landmark, sum c_l^2."""
     c = np.bincount(np.asarray(olm, np.int64).ravel(), minlength=nl) if len(olm) else np.zeros(0, np.int64)

@@ -21,6 +21,7 @@
 #include "xba_adj.hip"
 #include "xba_preint.hip"
 #include "xba_marg.hip"
+#include "xba_pairs.hip"
 
 using namespace xba;
 
@@ -520,8 +521,8 @@ const char *xba_strerror(int code) {
         case XBA_E_ARG: return "invalid argument";
         case XBA_E_HIP: return "HIP runtime error";
         case XBA_E_TOO_MANY_OBS: return "a landmark has more than XBA_MAX_OBS_PER_LM observations";
-        case XBA_E_RANGE: return "a landmark couples slots further apart than the slot tile allows";
-        case XBA_E_SIZE: return "a reduced system exceeds XBA_MAX_STATE";
+        case XBA_E_RANGE: return "an index is out of range or out of order, or a landmark couples slots further apart than the slot tile allows";
+        case XBA_E_SIZE: return "a size exceeds a limit (XBA_MAX_STATE, a static workspace, or the output capacity given)";
         case XBA_E_STATE: return "call order (upload before iterate)";
         default: return "unknown error";
     }
@@ -1704,6 +1705,51 @@ int xba_np_id_solve(const xba_npid_problem *q, int32_t iters, double lam0, int32
     };
     const int rc = run();
     release();
+    return rc;
+}
+
+int xba_build_pairs(const int32_t *olm_sorted, int64_t n_obs, int32_t nl, int32_t *offs, int32_t *pa, int32_t *pb,
+                    int64_t cap, int64_t *npair, void *stream) {
+    if (n_obs < 0 || nl < 0 || !offs || !npair || (n_obs && !olm_sorted) || (pa && (!pb || cap < 0)))
+        return XBA_E_ARG;
+    if (n_obs > INT32_MAX) return XBA_E_ARG;   // observation indices are int32
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t nb = (int32_t)(((int64_t)nl + 1 + PAIRS_BS - 1) / PAIRS_BS);   // blocks over l in [0, nl]
+    // workspace: err (padded to 8 bytes), bsum (nb + 1), base (nl + 1)
+    char *ws = nullptr;
+    const size_t bytes = 8 + sizeof(int64_t) * ((size_t)nb + 1 + (size_t)nl + 1);
+    if (hipMalloc(&ws, bytes) != hipSuccess) return XBA_E_HIP;
+    int32_t *err = (int32_t *)ws;
+    int64_t *bsum = (int64_t *)(ws + 8), *base = bsum + nb + 1;
+    int rc = XBA_OK;
+    auto run = [&]() -> int {
+        HIPCK(hipMemsetAsync(err, 0, sizeof(int32_t), s));
+        const int64_t span = std::max<int64_t>(n_obs, (int64_t)nl + 1);
+        const int g = (int)std::min<int64_t>((span + 255) / 256, 4096);
+        k_pairs_offs<<<g, 256, 0, s>>>(olm_sorted, n_obs, nl, offs, err);
+        k_pairs_bsum<<<nb, PAIRS_BS, 0, s>>>(offs, nl, bsum, err);
+        k_pairs_bscan<<<1, PAIRS_BS, 0, s>>>(bsum, nb, err);
+        k_pairs_base<<<nb, PAIRS_BS, 0, s>>>(offs, nl, bsum, base, err);
+        HIPCK(hipGetLastError());
+        int32_t herr = 0;
+        int64_t tot = 0;
+        HIPCK(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(&tot, base + nl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (herr) return XBA_E_RANGE;
+        *npair = tot;
+        if (!pa) return XBA_OK;
+        if (cap < tot) return XBA_E_SIZE;
+        if (tot) {
+            const int ge = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+            k_pairs_expand<<<ge, 256, 0, s>>>(offs, base, nl, tot, pa, pb);
+            HIPCK(hipGetLastError());
+        }
+        HIPCK(hipStreamSynchronize(s));
+        return XBA_OK;
+    };
+    rc = run();
+    (void)hipFree(ws);
     return rc;
 }
 

@@ -105,7 +105,14 @@ class Prior:
         return int(sum(self.dims))
 
     def bind(self, index_of_sid, off=None, slot_dims=None):
-        self._idx = [int(index_of_sid[s]) for s in self.sids]
+        """Map the prior's slot ids to problem slots.  A prior block may cover the first 6 dims of a 15-dim slot,
+        never more dims than the slot has (RuntimeError)."""
+        idx = [int(index_of_sid[s]) for s in self.sids]
+        if slot_dims is not None:
+            for s, d, i in zip(self.sids, self.dims, idx):
+                if d > int(slot_dims[i]):
+                    raise RuntimeError("prior block of slot %s spans %d dims; the slot has %d" % (s, d, slot_dims[i]))
+        self._idx = idx
 
     def slots(self):
         return np.asarray(self._idx, np.int32)
