@@ -522,3 +522,83 @@ def make_batch(nwin=256, seed=0, P_range=(5, 30), L_range=(500, 30000), views_ma
         pb, _t = make_window(P=P, L=max(L, 4), views=(2, min(views_max, P)), seed=[20261203, 2, seed, w])
         out.append(pb)
     return out
+
+
+def make_gba(nk=60, nl=3000, track=(2, 10), seed=0, reach=15, px_sigma=0.5, disp_sigma=0.5, outlier_frac=0.02,
+             stereo_frac=0.8, prior_valid_frac=0.9, pose_sigma_t=0.05, pose_sigma_r_deg=0.5, depth=(1.0, 20.0),
+             spacing=0.5, noise_scale=1.0):
+    """A seeded whole-trajectory problem in the argument form of backend/global_ba.py:694 ``global_ba_id``.
+
+    ``nk`` keyframes on the VO trajectory model (camera = body), ``nl`` landmarks, each hosted at a random keyframe
+    at a uniform pixel and depth U(depth), observed by its host and by up to track[1] - 1 keyframes within
+    ``reach`` of it that see it (track length U{track}).  Pixels carry N(0, px_sigma); ``outlier_frac`` of the
+    non-host rows are shifted by U(10, 40) px.  ``stereo_frac`` of the rows carry a disparity fB / z + N(0,
+    disp_sigma) (the others 0: none measured).  The landmark prior holds the host's measured bearing and inverse
+    depth (whitening fx / px_sigma and fB / disp_sigma; ``prior_valid_frac`` of them valid).  The state starts at
+    the measured bearings and inverse depths and at poses perturbed by a random walk (pose_sigma_t m, pose_sigma_r
+    deg per step); keyframe 0 is at the truth.  ``noise_scale`` scales every measurement noise (0: exact).
+
+    -> dict(kf_ids, poses (nk,4,4) T_wc, rho, mbear, host_idx, okf, olm, ouv, odisp, rig, prior, truth)."""
+    import types
+    rng = np.random.default_rng(seed)
+    fx, cx, cy, Wd, Hd, B = (CAM[k] for k in ("fx", "cx", "cy", "W", "H", "B"))
+    fB = fx * B
+    Rwc, pwc = _vo_trajectory(rng, nk, spacing)
+    host = rng.integers(0, nk, nl)
+    u0, v0 = rng.uniform(0, Wd, nl), rng.uniform(0, Hd, nl)
+    dep = rng.uniform(depth[0], depth[1], nl)
+    m_true = np.stack([(u0 - cx) / fx, (v0 - cy) / fx, np.ones(nl)], 1)
+    Xw = np.einsum("lij,lj->li", Rwc[host], m_true * dep[:, None]) + pwc[host]
+    offs = np.arange(-reach, reach + 1)
+    cand = host[:, None] + offs[None, :]                                   # (nl, 2 reach + 1)
+    inside = (cand >= 0) & (cand < nk)
+    c = np.clip(cand, 0, nk - 1)
+    q = np.einsum("lkji,lkj->lki", Rwc[c], Xw[:, None, :] - pwc[c])          # camera-frame points
+    z = q[..., 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        uu, vv = fx * q[..., 0] / z + cx, fx * q[..., 1] / z + cy
+    vis = inside & (z > 0.3) & (uu >= 0) & (uu < Wd) & (vv >= 0) & (vv < Hd)
+    vis[:, reach] = True                                                    # the host row
+    ntr = rng.integers(track[0], track[1] + 1, nl)
+    score = np.where(vis, np.abs(offs)[None, :] + rng.uniform(0, 0.5, vis.shape), np.inf)
+    score[:, reach] = -1.0
+    rank = np.argsort(np.argsort(score, axis=1), axis=1)
+    take = (rank < ntr[:, None]) & np.isfinite(score)
+    lm_i, k_i = np.nonzero(take)
+    okf = cand[lm_i, k_i]
+    n = len(okf)
+    ouv = np.stack([uu[lm_i, k_i], vv[lm_i, k_i]], 1) + noise_scale * rng.normal(0, px_sigma, (n, 2))
+    is_host = okf == host[lm_i]
+    bad = (rng.uniform(0, 1, n) < outlier_frac) & ~is_host
+    ang = rng.uniform(0, 2 * math.pi, n)
+    ouv[bad] += rng.uniform(10, 40, n)[bad, None] * np.stack([np.cos(ang), np.sin(ang)], 1)[bad]
+    zt = z[lm_i, k_i]
+    odisp = fB / zt + noise_scale * rng.normal(0, disp_sigma, n)
+    odisp[rng.uniform(0, 1, n) >= stereo_frac] = 0.0
+    odisp = np.maximum(odisp, 0.0)
+    # the host row's measurement gives the prior and the starting landmark state
+    hrow = np.flatnonzero(is_host)
+    m0 = np.zeros((nl, 2))
+    m0[lm_i[hrow]] = np.stack([(ouv[hrow, 0] - cx) / fx, (ouv[hrow, 1] - cy) / fx], 1)
+    rho0 = 1.0 / dep + noise_scale * rng.normal(0, disp_sigma / fB, nl)
+    rho0 = np.maximum(rho0, 1e-3)
+    valid = (rng.uniform(0, 1, nl) < prior_valid_frac).astype(np.float64)
+    prior = (m0, rho0, np.full(nl, fx / px_sigma), np.full(nl, fB / disp_sigma), valid)
+    mbear = np.concatenate([m0, np.ones((nl, 1))], 1)
+    # poses: keyframe 0 exact, then a random walk of errors
+    dth = np.cumsum(rng.normal(0, math.radians(pose_sigma_r_deg), (nk, 3)), 0)
+    dpt = np.cumsum(rng.normal(0, pose_sigma_t, (nk, 3)), 0)
+    dth[0] = 0.0
+    dpt[0] = 0.0
+    poses = np.zeros((nk, 4, 4))
+    for k in range(nk):
+        poses[k, :3, :3] = so3_exp(dth[k]) @ Rwc[k]
+        poses[k, :3, 3] = pwc[k] + dpt[k]
+        poses[k, 3, 3] = 1.0
+    truth = np.zeros((nk, 4, 4))
+    truth[:, :3, :3], truth[:, :3, 3], truth[:, 3, 3] = Rwc, pwc, 1.0
+    rig = types.SimpleNamespace(fx_rect=fx, cx=cx, cy=cy, fB=fB, rho_sigma=0.5 / fB)
+    return {"kf_ids": np.arange(nk), "poses": poses, "rho": rho0.copy(), "mbear": mbear, "host_idx": host,
+            "okf": okf.astype(np.int64), "olm": lm_i.astype(np.int64), "ouv": ouv, "odisp": odisp, "rig": rig,
+            "prior": prior, "truth": {"poses": truth, "rho": 1.0 / dep, "mbear": m_true, "Xw": Xw,
+                                      "outlier": bad}}
