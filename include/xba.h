@@ -214,6 +214,49 @@ int xba_np_solve(const xba_np_problem *q, int32_t iters, double lam0, int32_t pr
 int xba_build_pairs(const int32_t *olm_sorted, int64_t n_obs, int32_t nl, int32_t *offs, int32_t *pa, int32_t *pb,
                     int64_t cap, int64_t *npair, void *stream);
 
+/* ---- run-end global BA in inverse depth (backend/global_ba.py:694-1785 global_ba_id; its damped reduced camera
+ * system by the deflated PCG of backend/gba_linsolve.py:447-478) -------------------------------------------------
+ * Replaces global_ba_id on the path with every XTEN_GBA_* mode unset and no inertial term, rotation factors, welds
+ * or nuisance states: the reprojection rows (Huber on the pixel norm, optional per-keyframe weights), the per-row
+ * disparity rows, the landmark prior, the gauge keyframes, LM with the reference's accept rule and best-state
+ * bookkeeping, the 3-sigma MAD rejection and the second pass (re-weighted prior, no disparity rows), the final SO(3)
+ * projection.  Every damped reduced system is solved by the deflated PCG to rtol (rtol_direct at or below
+ * XBA_GBA_DIRECT_MAX unknowns, where the reference solves directly); a solve that does not reach it is reported and
+ * its step not used (lambda x10), as the reference does for an unconverged PCG step.
+ * Device pointers; poses (nk, 16) row-major 4x4 T_wc, rho (nl), mbear (nl, 3) are overwritten with the result.
+ * stats (host, XBA_GBA_NSTATS doubles, indices XBA_GBA_ST_*); hist (host, hist_cap x XBA_GBA_HIST doubles per LM
+ * iteration: pass, iteration, lambda, chi, chi2 (NaN without a step), accepted, PCG iterations, relative residual,
+ * PCG reason, restarts) or NULL.  Synchronous; allocates its workspace. */
+#define XBA_GBA_DIRECT_MAX 30000
+#define XBA_GBA_MAX_NK 65536
+#define XBA_GBA_HIST 10
+typedef struct {
+    int32_t nk, nl, N;
+    double *poses, *rho, *mbear;
+    const int32_t *host, *okf, *olm;
+    const double *ouv;
+    const double *odisp;          /* (N) disparity [px], 0 = none; NULL: no disparity rows */
+    const double *vt_w;           /* (nk) vision weight per keyframe or NULL */
+    const uint8_t *free_mask;     /* (nk) 0 = gauge keyframe */
+    const double *pr_m0, *pr_rho0, *pr_wbear, *pr_invsig, *pr_valid;  /* (nl,2), (nl) x 4; all NULL: no prior */
+    double fx, cx, cy, fB, rho_sigma, rho_floor;
+} xba_gba_problem;
+typedef struct {
+    int32_t iters, passes, maxiter, seg_len;  /* seg_len 0: the derived segment length (capped at 23 keyframes) */
+    double lam0, reject_sigma, time_budget_s; /* time_budget_s <= 0: none */
+    double rtol, rtol_direct;
+} xba_gba_options;
+enum {
+    XBA_GBA_ST_RMS0 = 0, XBA_GBA_ST_RMS1, XBA_GBA_ST_OBS, XBA_GBA_ST_REJECTED, XBA_GBA_ST_THR, XBA_GBA_ST_STEREO,
+    XBA_GBA_ST_CHI0V, XBA_GBA_ST_CHI0S, XBA_GBA_ST_CHIBEST, XBA_GBA_ST_ACC, XBA_GBA_ST_BEST,      /* last pass */
+    XBA_GBA_ST_P1_CHI0V, XBA_GBA_ST_P1_CHI0S, XBA_GBA_ST_P1_CHIBEST, XBA_GBA_ST_P1_ACC, XBA_GBA_ST_P1_BEST,
+    XBA_GBA_ST_PASSES, XBA_GBA_ST_SIGMA, XBA_GBA_ST_BIAS, XBA_GBA_ST_SIGMA1, XBA_GBA_ST_SOLVES, XBA_GBA_ST_PCG_IT,
+    XBA_GBA_ST_UNCONV, XBA_GBA_ST_SECONDS, XBA_GBA_ST_NSEG, XBA_GBA_ST_SEGLEN, XBA_GBA_ST_NNZB, XBA_GBA_ST_ITERS_RUN,
+    XBA_GBA_NSTATS
+};
+int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o, double *stats, double *hist, int32_t hist_cap,
+                  void *stream);
+
 /* ---- the Euclidean local BA without allocation ---------------------------------------------------------------
  * No counterpart in the reference: xba_np_solve for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments, results and return codes

@@ -1,0 +1,103 @@
+"""Run-end global BA (backend/global_ba.py:694 global_ba_id, default path) on the GPU against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import gba_oracle as G
+from xclean_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ARGS = ("kf_ids", "poses", "rho", "mbear", "host_idx", "okf", "olm", "ouv", "rig", "odisp", "prior")
+
+
+def _args(g, **over):
+    a = {k: g[k] for k in ARGS}
+    a.update(over)
+    return a
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _compare(out, ref, tol):
+    for i, name in enumerate(("poses", "rho", "mbear")):
+        e = _rel(out[i], ref[i])
+        print(name, "rel err", e, flush=True)
+        assert e < tol, name
+
+
+@pytest.mark.parametrize("stereo,prior", [(True, True), (False, True), (False, False)])
+def test_gba_one_iteration_matches_the_oracle(stereo, prior):
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=30, nl=1500, seed=21)
+    kw = dict(iters=1, passes=1)
+    if not stereo:
+        kw["odisp"] = None
+    if not prior:
+        kw["prior"] = None
+    print("nk 30 nl 1500 rows", len(g["okf"]), "stereo", stereo, "prior", prior, flush=True)
+    h = []
+    out = global_ba_id(**_args(g, **kw), history=h)
+    hr = []
+    ref = G.global_ba_id(**_args(g, **kw), history=hr)
+    print(h, [(e["lam"], e["chi"], e["chi2"], e["accepted"]) for e in hr])
+    assert h[0]["pcg_reason"] == "tol" and h[0]["relres"] <= 1e-12
+    assert abs(h[0]["chi"] - hr[0]["chi"]) <= 1e-12 * hr[0]["chi"]
+    assert abs(h[0]["chi2"] - hr[0]["chi2"]) <= 1e-9 * hr[0]["chi2"]
+    assert h[0]["accepted"] == hr[0]["accepted"]
+    _compare(out, ref, 1e-9)
+    assert out[3]["observations"] == ref[3]["observations"]
+    assert abs(out[3]["rms_px_before"] - ref[3]["rms_px_before"]) <= 1e-12 * ref[3]["rms_px_before"]
+
+
+def test_gba_default_two_pass_solve_matches_the_oracle():
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=50, nl=3000, seed=8, outlier_frac=0.05)
+    h, hr = [], []
+    out = global_ba_id(**_args(g), history=h)
+    ref = G.global_ba_id(**_args(g), history=hr)
+    so, sr = out[3], ref[3]
+    print({k: so[k] for k in ("rms_px_before", "rms_px_after", "rejected", "observations")}, so["gba_solver"])
+    print({k: sr[k] for k in ("rms_px_before", "rms_px_after", "rejected", "observations")})
+    assert [e["accepted"] for e in h] == [e["accepted"] for e in hr]
+    assert so["rejected"] == sr["rejected"] and so["observations"] == sr["observations"]
+    assert abs(so["reject_threshold_px"] - sr["reject_threshold_px"]) <= 1e-9 * sr["reject_threshold_px"]
+    assert abs(so["rms_px_after"] - sr["rms_px_after"]) <= 1e-9 * sr["rms_px_after"]
+    for k in ("accepted", "best_updates"):
+        assert so["lm"][k] == sr["lm"][k] and so["lm_pass1"][k] == sr["lm_pass1"][k]
+    assert abs(so["measured_disparity_sigma_px"] - sr["measured_disparity_sigma_px"]) <= 1e-9
+    assert abs(so["measured_disparity_bias_px"] - sr["measured_disparity_bias_px"]) <= 1e-9
+    _compare(out, ref, 1e-8)
+
+
+def test_gba_pcg_matches_the_oracle_pcg_on_a_larger_trajectory():
+    """300 keyframes at the reference's PCG tolerance 1e-8: GPU and oracle PCG take about the same number of
+    iterations and reach the same step."""
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=300, nl=12000, seed=4)
+    h, hr = [], []
+    out = global_ba_id(**_args(g, iters=2, passes=1), history=h, rtol_direct=1e-8)
+    ref = G.global_ba_id(**_args(g, iters=2, passes=1), history=hr, solver="dpcg")
+    print([(e["pcg_iters"], e["relres"], e["pcg_reason"]) for e in h])
+    assert all(e["pcg_reason"] == "tol" for e in h)
+    assert [e["accepted"] for e in h] == [e["accepted"] for e in hr]
+    _compare(out, ref, 1e-5)
+
+
+def test_gba_edge_cases():
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=12, nl=400, seed=5)
+    # per-keyframe vision weights, a constant beta, two gauge keyframes, a landmark without rows
+    vt = np.linspace(0.5, 1.5, 12)
+    keep = g["olm"] != 7
+    a = _args(g, okf=g["okf"][keep], olm=g["olm"][keep], ouv=g["ouv"][keep], odisp=g["odisp"][keep])
+    out = global_ba_id(**a, vt_weights=vt, beta0=0.01, fixed=(0, 5), iters=3)
+    ref = G.global_ba_id(**dict(a, prior=(g["prior"][0], g["prior"][1] * 1.01) + tuple(g["prior"][2:])),
+                         vt_weights=vt, fixed=(0, 5), iters=3)
+    _compare(out, ref, 1e-8)
+    assert np.allclose(out[0][[0, 5]], g["poses"][[0, 5]], atol=1e-12)
+    with pytest.raises(ValueError):
+        global_ba_id(**_args(g, okf=g["okf"] + 100))
+    with pytest.raises(NotImplementedError):
+        global_ba_id(**_args(g), rotf=[1])

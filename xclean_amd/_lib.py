@@ -57,6 +57,26 @@ class XbaNpidProblem(C.Structure):
                 ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp), ("lm_policy", C.c_int32)]
 
 
+class XbaGbaProblem(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("nl", C.c_int32), ("N", C.c_int32), ("poses", vp), ("rho", vp), ("mbear", vp),
+                ("host", vp), ("okf", vp), ("olm", vp), ("ouv", vp), ("odisp", vp), ("vt_w", vp), ("free_mask", vp),
+                ("pr_m0", vp), ("pr_rho0", vp), ("pr_wbear", vp), ("pr_invsig", vp), ("pr_valid", vp),
+                ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("fB", C.c_double),
+                ("rho_sigma", C.c_double), ("rho_floor", C.c_double)]
+
+
+class XbaGbaOptions(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("passes", C.c_int32), ("maxiter", C.c_int32), ("seg_len", C.c_int32),
+                ("lam0", C.c_double), ("reject_sigma", C.c_double), ("time_budget_s", C.c_double),
+                ("rtol", C.c_double), ("rtol_direct", C.c_double)]
+
+
+XBA_GBA_NSTATS = 28
+XBA_GBA_HIST = 10
+XBA_GBA_DIRECT_MAX = 30000
+XBA_GBA_MAX_NK = 65536
+
+
 class XbaError(RuntimeError):
     pass
 
@@ -65,7 +85,7 @@ class XbaError(RuntimeError):
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
            "xba_np_solve", "xba_np_solve_noalloc", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
-           "xba_marg_schur", "xba_prior_from_info", "xba_build_pairs",
+           "xba_marg_schur", "xba_prior_from_info", "xba_build_pairs", "xba_gba_solve",
            "xba_copy_kernel", "xba_strerror", "xba_version")
 
 
@@ -99,6 +119,7 @@ def load():
     L.xba_sqrt_information.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
     L.xba_marg_schur.argtypes = [vp, vp]
     L.xba_prior_from_info.argtypes = [vp, vp]
+    L.xba_gba_solve.argtypes = [C.POINTER(XbaGbaProblem), C.POINTER(XbaGbaOptions), vp, vp, C.c_int32, vp]
     L.xba_build_pairs.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_int64, C.POINTER(C.c_int64), vp]
     L.xba_strerror.restype = C.c_char_p
     L.xba_strerror.argtypes = [C.c_int]

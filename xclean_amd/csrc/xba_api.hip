@@ -22,6 +22,7 @@
 #include "xba_preint.hip"
 #include "xba_marg.hip"
 #include "xba_pairs.hip"
+#include "xba_gba.hip"
 
 using namespace xba;
 
