@@ -262,6 +262,79 @@ def local_ba_config(torch, synth, log, cpu_budget=3.0, reps=20):
     return out
 
 
+GBA_ROW_RECORD_B = 108 * 8     # per-row products of one assembly (xba_gba.hip RR_*), written once and read once
+GBA_ROW_INPUT_B = 4 + 16 + 8   # observing keyframe, pixel, disparity
+GBA_LM_RECORD_B = 27 * 8       # per landmark: damped inverse, bl, W_h
+
+
+def global_ba_config(torch, synth, log, cpu_budget=10.0, nk=6000, nl=450000, iters=8):
+    """Run-end global BA (backend/global_ba.py:694 global_ba_id, gba_linsolve.py's deflated PCG) on one trajectory of
+    ``nk`` keyframes (6 nk unknowns: the PCG regime above 30,000) and ``nl`` landmarks: LM iterations of pass 1
+    timed end to end (assembly, PCG set-up and recursion, trial cost, accept), the default two-pass solve beside
+    it, the phase times from HIP events, and the oracle's numpy/scipy restatement on a 2,000-keyframe sample."""
+    from xclean_amd.gba import global_ba_id
+    from xclean_amd import roofline
+    args = ("kf_ids", "poses", "rho", "mbear", "host_idx", "okf", "olm", "ouv", "rig", "odisp", "prior")
+    g = synth.make_gba(nk=nk, nl=nl, seed=20261204)
+    a = {k: g[k] for k in args}
+    global_ba_id(**a, iters=1, passes=1)
+    torch.cuda.synchronize()
+    h = []
+    t0 = time.perf_counter()
+    res = global_ba_id(**a, iters=iters, passes=1, history=h, profile=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = res[3]
+    gs = st["gba_solver"]
+    rows = gs["rows_last_pass"]
+    n_it = len(h)
+    t1 = time.perf_counter()
+    res2 = global_ba_id(**a)
+    torch.cuda.synchronize()
+    dt2 = time.perf_counter() - t1
+    asm_bytes = rows * (2 * GBA_ROW_RECORD_B + GBA_ROW_INPUT_B) + nl * 2 * GBA_LM_RECORD_B + gs["blocks"] * 288
+    asm_ms = gs["assembly_ms"]
+    gbs = asm_bytes / (asm_ms * 1e-3) / 1e9
+    rec = {"workload": "one trajectory: %d keyframes (%d unknowns), %d landmarks, %d observation rows (host rows "
+                       "dropped), disparity rows and the landmark prior, keyframe 0 held" % (nk, 6 * nk, nl, rows),
+           "unit": "obs rows/s per LM iteration",
+           "value": rows * n_it / dt, "ms_per_lm_iteration": 1e3 * dt / n_it, "lm_iterations": n_it,
+           "accepted": int(sum(e["accepted"] for e in h)),
+           "pcg_iterations_per_solve": gs["pcg_iters"] / max(gs["solves"], 1), "pcg_unconverged": gs["unconverged"],
+           "segments": gs["segments"], "seg_len": gs["seg_len"], "blocks": gs["blocks"],
+           "phase_ms": {"assembly": asm_ms, "pcg_setup": gs["setup_ms"], "pcg_recursion": gs["pcg_ms"]},
+           "default_two_pass": {"seconds": dt2, "rms_px": [res2[3]["rms_px_before"], res2[3]["rms_px_after"]],
+                                "rejected": res2[3]["rejected"], "lm_iterations": res2[3]["gba_solver"]["iters_run"]},
+           "roofline": {"bound": "hbm", "kernel": "k_gba_rows + k_gba_assemble + k_gba_mirror (the reduced system)",
+                        "achieved": gbs, "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / roofline.HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes": asm_bytes,
+                        "bytes_note": "per row its inputs (%d B) and its product record written and read once "
+                                      "(2 x %d B), per landmark its record written and read once (2 x %d B), every "
+                                      "6x6 block written once" % (GBA_ROW_INPUT_B, GBA_ROW_RECORD_B, GBA_LM_RECORD_B)}}
+    log("bench: global BA %d kf / %d rows: %.2f ms per LM iteration (assembly %.2f, PCG set-up %.2f, recursion "
+        "%.2f ms, %.1f PCG iterations per solve); two-pass default %.2f s" % (
+            nk, rows, rec["ms_per_lm_iteration"], asm_ms, gs["setup_ms"], gs["pcg_ms"],
+            rec["pcg_iterations_per_solve"], dt2))
+    if cpu_budget > 0:
+        from oracle import gba_oracle as G
+        gs_ = synth.make_gba(nk=2000, nl=150000, seed=20261204)
+        ac = {k: gs_[k] for k in args}
+        tc, n = 0.0, 0
+        while n < 1 or tc < cpu_budget:
+            t = time.perf_counter()
+            G.global_ba_id(**ac, iters=1, passes=1)
+            tc += time.perf_counter() - t
+            n += 1
+        crow = int(np.sum(gs_["host_idx"][gs_["olm"]] != gs_["okf"]))
+        rec["cpu_baseline"] = {"value": crow * n / tc, "unit": "obs rows/s per LM iteration", "cores": 1,
+                               "kind": "port", "sample": "%d x one LM iteration (system, direct solve, trial) of the "
+                                                         "oracle's numpy/scipy restatement on a 2,000-keyframe, "
+                                                         "150,000-landmark trajectory (%d rows)" % (n, crow)}
+        log("bench: global BA cpu oracle %.3e rows/s per LM iteration" % rec["cpu_baseline"]["value"])
+    return rec
+
+
 def marginalisation_config(torch, pbs, log, cpu_budget=3.0, reps=10):
     """Marginalisation of every window's oldest keyframe at once (vio/engine_live.py:690-739 per window): the
     visual rows of its hosted landmarks and the prior linearised at lambda = 0 (one batched linearisation), its 6
@@ -446,6 +519,8 @@ def main():
                     help="diagnostic: run only the batched IMU preintegration and print its JSON")
     ap.add_argument("--local-ba-only", action="store_true",
                     help="diagnostic: run only the local-BA drop-in timings and print their JSON")
+    ap.add_argument("--global-ba-only", action="store_true",
+                    help="diagnostic: run only the global-BA timings and print their JSON")
     ap.add_argument("--configs-only", action="store_true",
                     help="diagnostic: run only configurations 1 and 4 (for profiling the tiled factorisation) and "
                          "print their JSON")
@@ -464,6 +539,9 @@ def main():
 
     if args.local_ba_only:
         print(json.dumps(local_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)))
+        return
+    if args.global_ba_only:
+        print(json.dumps(global_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 10.0)))
         return
     if args.configs_only:
         print(json.dumps(single_window_configs(torch, synth, BatchSolver, log)))
@@ -600,6 +678,7 @@ def main():
     margin = None if args.no_extra else marginalisation_config(torch, pbs, log,
                                                                cpu_budget=0.0 if args.no_cpu else 3.0)
     local_ba = None if args.no_extra else local_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 3.0)
+    gba = None if args.no_extra else global_ba_config(torch, synth, log, cpu_budget=0.0 if args.no_cpu else 10.0)
 
     cpu = None
     if not args.no_cpu:
@@ -670,6 +749,7 @@ def main():
         "preintegration": preint,
         "marginalisation": margin,
         "local_ba": local_ba,
+        "global_ba": gba,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -243,8 +243,11 @@ typedef struct {
 } xba_gba_problem;
 typedef struct {
     int32_t iters, passes, maxiter, seg_len;  /* seg_len 0: the derived segment length (capped at 23 keyframes) */
+    int32_t profile;                          /* 1: time the phases with HIP events (XBA_GBA_ST_*_MS) */
     double lam0, reject_sigma, time_budget_s; /* time_budget_s <= 0: none */
     double rtol, rtol_direct;
+    double rtol_direct_floor;  /* at or below XBA_GBA_DIRECT_MAX unknowns a PCG that stops short of rtol_direct at
+                                  its attainable accuracy is still used when its best relative residual is <= this */
 } xba_gba_options;
 enum {
     XBA_GBA_ST_RMS0 = 0, XBA_GBA_ST_RMS1, XBA_GBA_ST_OBS, XBA_GBA_ST_REJECTED, XBA_GBA_ST_THR, XBA_GBA_ST_STEREO,
@@ -252,6 +255,10 @@ enum {
     XBA_GBA_ST_P1_CHI0V, XBA_GBA_ST_P1_CHI0S, XBA_GBA_ST_P1_CHIBEST, XBA_GBA_ST_P1_ACC, XBA_GBA_ST_P1_BEST,
     XBA_GBA_ST_PASSES, XBA_GBA_ST_SIGMA, XBA_GBA_ST_BIAS, XBA_GBA_ST_SIGMA1, XBA_GBA_ST_SOLVES, XBA_GBA_ST_PCG_IT,
     XBA_GBA_ST_UNCONV, XBA_GBA_ST_SECONDS, XBA_GBA_ST_NSEG, XBA_GBA_ST_SEGLEN, XBA_GBA_ST_NNZB, XBA_GBA_ST_ITERS_RUN,
+    XBA_GBA_ST_ASM_MS,   /* mean device time of one reduced-system assembly (HIP events; profile on) */
+    XBA_GBA_ST_SETUP_MS, /* mean device time of one PCG set-up: smoother, deflation space, coarse inverse */
+    XBA_GBA_ST_PCG_MS,   /* mean device time of one PCG recursion */
+    XBA_GBA_ST_ROWS,     /* observation rows of the last pass */
     XBA_GBA_NSTATS
 };
 int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o, double *stats, double *hist, int32_t hist_cap,

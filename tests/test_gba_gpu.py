@@ -101,3 +101,26 @@ def test_gba_edge_cases():
         global_ba_id(**_args(g, okf=g["okf"] + 100))
     with pytest.raises(NotImplementedError):
         global_ba_id(**_args(g), rotf=[1])
+
+
+def test_gba_full_size_trajectory_drives_the_noise_free_residuals_down():
+    """6,000 keyframes (36,000 unknowns: the PCG regime of the reference above 30,000), 450,000 landmarks, exact
+    measurements: every PCG solve meets 1e-8 on its true residual until the step is within rounding of the
+    solution, every accepted step lowers the cost, and the pixel rms falls by more than 1,000x (the trajectory's
+    slow modes stay Marquardt-damped: they are not resolved within ten iterations, so the poses themselves are
+    not compared with the truth)."""
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=6000, nl=450000, seed=9, noise_scale=0.0, outlier_frac=0.0, pose_sigma_t=0.002,
+                       pose_sigma_r_deg=0.02)
+    print("rows", len(g["okf"]), flush=True)
+    h = []
+    poses, rho, mb, st = global_ba_id(**_args(g), iters=10, passes=1, history=h)
+    print([(e["pcg_iters"], e["pcg_reason"], e["accepted"], e["chi"]) for e in h], st["rms_px_before"],
+          st["rms_px_after"])
+    for e in h:
+        assert e["pcg_reason"] == "tol" or e["chi"] < 1e-6 * h[0]["chi"], e
+        if e["pcg_reason"] == "tol":
+            assert e["relres"] <= 1e-8
+        if e["accepted"]:
+            assert e["chi2"] <= e["chi"]
+    assert st["rms_px_after"] < 1e-3 * st["rms_px_before"]

@@ -23,7 +23,8 @@ from . import _lib
 
 DISPARITY_NOISE_PX = 0.5   # core/derive.py:60
 RTOL = 1e-8                # gba_linsolve.py:57, above XBA_GBA_DIRECT_MAX unknowns
-RTOL_DIRECT = 1e-12        # at or below it the reference solves directly: the PCG is driven to this instead
+RTOL_DIRECT = 1e-12        # at or below it the reference solves directly: the PCG is driven to this instead,
+RTOL_DIRECT_FLOOR = 1e-9   # and its best iterate used when the recursion stops short of it within this
 MAXITER = 2000             # gba_linsolve.py:62
 
 PCG_REASONS = ("running", "tol", "maxiter", "breakdown", "divergence", "stagnation", "zero_rhs", "nonfinite_rhs",
@@ -46,7 +47,8 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
                  welds=None, iters=8, fixed=(0,), lam0=1e-4, verbose=False, time_budget_s=None, passes=2,
                  reject_sigma=3.0, want_cov=0, rho_floor=1e-4, inertial=None, scale_bias=False, beta0=0.0,
                  rotf_huber=None, vt_weights=None, dof=None, _screen=True, device=None, rtol=RTOL,
-                 rtol_direct=RTOL_DIRECT, maxiter=MAXITER, seg_len=None, history=None):
+                 rtol_direct=RTOL_DIRECT, rtol_direct_floor=RTOL_DIRECT_FLOOR, maxiter=MAXITER, seg_len=None,
+                 history=None, profile=False):
     """backend/global_ba.py:694: (poses T_wc (nk,4,4), rho (nl,), mbear (nl,3), stats).
 
     poses: (nk,4,4) T_wc; rho, mbear: the landmarks' inverse depth and bearing in their host camera; host_idx (nl,);
@@ -123,9 +125,10 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     o = _lib.XbaGbaOptions()
     o.iters, o.passes, o.maxiter = int(iters), int(passes), int(maxiter)
     o.seg_len = int(seg_len or 0)
+    o.profile = 1 if profile else 0
     o.lam0, o.reject_sigma = float(lam0), float(reject_sigma)
     o.time_budget_s = float(time_budget_s) if time_budget_s else 0.0
-    o.rtol, o.rtol_direct = float(rtol), float(rtol_direct)
+    o.rtol, o.rtol_direct, o.rtol_direct_floor = float(rtol), float(rtol_direct), float(rtol_direct_floor)
     stats = np.zeros(_lib.XBA_GBA_NSTATS)
     hcap = 2 * max(int(iters), 0) + 2
     hist = np.zeros((hcap, _lib.XBA_GBA_HIST))
@@ -170,7 +173,9 @@ def _stats(s, nk, nl, beta0):
           "measured_disparity_sigma_px": _f(s[17]), "measured_disparity_bias_px": _f(s[18]),
           "declared_disparity_sigma_px": DISPARITY_NOISE_PX, "lm": lm, "seconds": float(s[23]),
           "gba_solver": {"mode": "dpcg", "solves": int(s[20]), "pcg_iters": int(s[21]), "unconverged": int(s[22]),
-                         "segments": int(s[24]), "seg_len": int(s[25]), "blocks": int(s[26])}}
+                         "segments": int(s[24]), "seg_len": int(s[25]), "blocks": int(s[26]),
+                         "rows_last_pass": int(s[31]), "iters_run": int(s[27]),
+                         "assembly_ms": _f(s[28]), "setup_ms": _f(s[29]), "pcg_ms": _f(s[30])}}
     if int(s[16]) == 2:
         st["reject_threshold_px"] = float(s[4])
         st["lm_pass1"] = {"chi0_vision": float(s[11]), "chi0_stereo": float(s[12]), "chi_best": float(s[13]),
