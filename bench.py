@@ -72,6 +72,8 @@ def single_window_configs(torch, synth, BatchSolver, log):
         dxs = {}
         for prec in ("mixed", "fp64"):
             b = BatchSolver([pb.copy()], precision=prec, use_graph=True)
+            from xclean_amd import autotune
+            tune = autotune.tune(b, repeats=5)
             b.upload(1e-4)
             b.iterate(1)
             dxs[prec] = b.last_dx()[0]
@@ -89,6 +91,7 @@ def single_window_configs(torch, synth, BatchSolver, log):
             st = b.stats()[0]
             n = pb.n_state()
             rec[prec] = {"value": pb.M / t, "unit": "obs/s per LM iteration", "ms_per_iter": t * 1e3,
+                         "plan": tune["options"],
                          "solve_ms": k["chol_solve"], "refinement_solves": st["refine_steps"],
                          "solve_effective_tflops": (n ** 3 / 3.0 + 2.0 * n * n) / (k["chol_solve"] * 1e-3) / 1e12,
                          "kernel_ms": k}

@@ -41,7 +41,8 @@ def candidate_space(nlm, ncu, ns):
             big.append(b)
     return dict(
         slot_tile=[0, 8, 12, 20, 24],
-        lm_per_chunk=[0] + sorted({max(8, auto_lpc // 2), min(4096, auto_lpc * 2)} - {auto_lpc}),
+        lm_per_chunk=[0] + sorted({max(8, auto_lpc // 2), min(4096, auto_lpc * 2), min(4096, auto_lpc * 4)}
+                                  - {auto_lpc}),
         big_n=big,
         grid=[0, 2 * ncu],
     )
