@@ -163,3 +163,43 @@ def test_gpu_chi2_vanishes_at_the_noise_free_truth(vi):
     assert np.isfinite(chi).all()
     assert chi.max() < 1e-18            # whitened squared residual: |e| < 1e-9 px at sigma 1 px
     assert np.abs(r["b"]).max() < 1e-6 * max(1.0, np.abs(r["H"]).max()) ** 0.5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_failed_cholesky_raises_lambda_and_keeps_the_state_like_the_oracle(precision):
+    """A non-finite prior makes the damped reduced system non-finite: the Cholesky fails on every iteration
+    (vio/window.py:936-941), lambda grows x10 per iteration, nothing is accepted and the state is not touched."""
+    from xclean_amd.window import BatchSolver
+    pb, _ = synth.make_window(P=6, L=200, views=(2, 6), seed=[3, 3])
+    pb.prior.S[0, 0] = np.nan
+    ref = pb.copy()
+    sr, dxr, ar = O.window_solve(ref, iters=3, lam0=1e-4, history=True)
+    assert not ar.any() and abs(sr["lam"] - 0.1) <= 1e-12
+    before = (pb.Rwb.copy(), pb.pwb.copy(), pb.lmp.copy())
+    bs = BatchSolver([pb], precision=precision, use_graph=True)
+    bs.upload(1e-4)
+    bs.iterate(3)
+    s = bs.stats()[0]
+    assert not s["chol_ok"] and s["accepted"] == 0
+    assert abs(s["lam"] - sr["lam"]) <= 1e-12 * sr["lam"]
+    bs.download()
+    for a, b in zip((pb.Rwb, pb.pwb, pb.lmp), before):
+        assert np.array_equal(a, b)
+
+
+def test_landmark_with_more_rows_than_a_wavefront_is_refused():
+    """A landmark with more than XBA_MAX_OBS_PER_LM (64) observation rows: the planner refuses the window with
+    XBA_E_TOO_MANY_OBS (one wavefront lane per row of a landmark) instead of solving something else."""
+    from xclean_amd import _lib
+    from xclean_amd.window import BatchSolver
+    pb, _ = synth.make_window(P=8, L=50, views=(2, 8), seed=[3, 4])
+    k0, k1 = pb.ob_start[0], pb.ob_start[1]
+    rep = np.concatenate([np.arange(k0, k1)] * (70 // (k1 - k0) + 1))[:70]
+    rest = np.arange(k1, pb.M)
+    idx = np.concatenate([rep, rest])
+    for k in ("ob_pose", "ob_cam", "ob_z", "ob_isig", "ob_valid"):
+        setattr(pb, k, np.ascontiguousarray(getattr(pb, k)[idx]))
+    pb.ob_start = np.concatenate([[0], pb.ob_start[1:] - (k1 - k0) + 70])
+    assert pb.ob_start[1] == 70 and pb.ob_start[-1] == pb.M
+    with pytest.raises(_lib.XbaError, match="XBA_MAX_OBS_PER_LM"):
+        BatchSolver([pb], precision="fp64", use_graph=False)
