@@ -240,6 +240,18 @@ typedef struct {
     const uint8_t *free_mask;     /* (nk) 0 = gauge keyframe */
     const double *pr_m0, *pr_rho0, *pr_wbear, *pr_invsig, *pr_valid;  /* (nl,2), (nl) x 4; all NULL: no prior */
     double fx, cx, cy, fB, rho_sigma, rho_floor;
+    /* gyro rotation factors (global_ba.py:1904-1967 _rot_blocks): keyframe pairs (n_rot, 2), the measured relative
+       rotation M (n_rot, 9, row-major), sigma (n_rot); rot_huber > 0: the per-factor Huber threshold (rotf_huber) */
+    int32_t n_rot;
+    const int32_t *rot_ab;
+    const double *rot_M, *rot_sig;
+    double rot_huber;
+    /* loop welds (global_ba.py:1969-2078 _weld_blocks, beta held): keyframe pairs (n_weld, 2), the measured relative
+       pose Z (n_weld, 16, row-major 4x4, translation already divided by 1 + beta), the information diagonal
+       (n_weld, 6) */
+    int32_t n_weld;
+    const int32_t *weld_ab;
+    const double *weld_Z, *weld_w;
 } xba_gba_problem;
 typedef struct {
     int32_t iters, passes, maxiter, seg_len;  /* seg_len 0: the derived segment length (capped at 23 keyframes) */

@@ -5,7 +5,8 @@ check or time the product path; the product package (xclean_amd/) never imports 
 
 Functions and the reference code they restate (paths relative to the reference's src/xclean/):
   build_pairs        backend/ba.py:902-927
-  (the global-BA functions follow below with their own citations)
+  (the global-BA functions follow below with their own citations; pose_factor_terms restates _rot_blocks and
+  _weld_blocks of backend/global_ba.py:1904-2078)
 
 Pins (tests/test_gba_oracle.py): build_pairs against a brute-force loop over every landmark's observation pairs.
 """
@@ -112,7 +113,76 @@ def prior_rows(prior, rho, mbear):
     return rp, J0, swp, valid > 0.5
 
 
-def gba_system(T_cw, rho, mbear, host, okf, olm, ouv, hk, cam, d_meas, prior, lam, free, vt_weights=None):
+def so3_log(R):
+    """Rotation vector of R (scipy's quaternion-based conversion)."""
+    from scipy.spatial.transform import Rotation
+    return Rotation.from_matrix(np.asarray(R, np.float64)).as_rotvec()
+
+
+def jr_inv(phi):
+    """Inverse right Jacobian of SO(3): I + [phi]x / 2 + (1/t^2 - (1 + cos t) / (2 t sin t)) [phi]x^2."""
+    phi = np.asarray(phi, np.float64)
+    t = float(np.linalg.norm(phi))
+    K = _hat(phi)
+    c = 1.0 / 12.0 + t * t / 720.0 if t < 1e-4 else 1.0 / (t * t) - (1.0 + np.cos(t)) / (2.0 * t * np.sin(t))
+    return np.eye(3) + 0.5 * K + c * (K @ K)
+
+
+def se3_log(T):
+    """[rho, phi] of T: phi = Log(R), rho = V(phi)^-1 t (V^-1 = Jr^-1(-phi))."""
+    phi = so3_log(T[:3, :3])
+    return np.concatenate([jr_inv(-phi) @ T[:3, 3], phi])
+
+
+def adjoint(T):
+    """Ad_T in the [rho, phi] order: [[R, [t]x R], [0, R]]."""
+    R, t = T[:3, :3], T[:3, 3]
+    A = np.zeros((6, 6))
+    A[:3, :3] = R
+    A[:3, 3:] = _hat(t) @ R
+    A[3:, 3:] = R
+    return A
+
+
+def pose_factor_terms(T_cw, n, rotf=None, rotf_huber=None, welds=None, beta=0.0):
+    """global_ba.py:1904-1967 (_rot_blocks) and 1969-2078 (_weld_blocks, beta held): the dense (n, n) Hessian blocks
+    and gradient of the gyro rotation factors (a, b, M, sig) and the loop welds (a, b, Z, w)."""
+    H = np.zeros((n, n))
+    g = np.zeros(n)
+    for (a, b, M, sig) in (rotf or []):
+        A = T_cw[a][:3, :3] @ T_cw[b][:3, :3].T
+        e = so3_log(np.asarray(M, np.float64).T @ A)
+        w = 1.0 / max(sig, 1e-9)
+        if rotf_huber is not None:
+            en = w * float(np.linalg.norm(e))
+            if en > rotf_huber:
+                w = w * float(np.sqrt(rotf_huber / en))
+        r = w * e
+        J = jr_inv(e)
+        Ja, Jb = w * (J @ A.T), -w * J
+        ia, ib = slice(6 * a + 3, 6 * a + 6), slice(6 * b + 3, 6 * b + 6)
+        H[ia, ia] += Ja.T @ Ja
+        H[ia, ib] += Ja.T @ Jb
+        H[ib, ia] += Jb.T @ Ja
+        H[ib, ib] += Jb.T @ Jb
+        g[ia] -= Ja.T @ r
+        g[ib] -= Jb.T @ r
+    for (a, b, Z, w) in (welds or []):
+        Zb = np.array(Z, np.float64)
+        Zb[:3, 3] /= (1.0 + beta)
+        Ti, Tj = se3_inv(T_cw[a]), se3_inv(T_cw[b])
+        e = se3_log(se3_inv(Zb) @ (se3_inv(Ti) @ Tj))
+        W = np.diag(np.asarray(w, np.float64)) if np.ndim(w) else np.eye(6) * float(w)
+        Ja, Jb = adjoint(se3_inv(Tj) @ Ti), -np.eye(6)
+        for bi, J1 in ((a, Ja), (b, Jb)):
+            g[6 * bi:6 * bi + 6] -= J1.T @ (W @ e)
+            for bj, J2 in ((a, Ja), (b, Jb)):
+                H[6 * bi:6 * bi + 6, 6 * bj:6 * bj + 6] += J1.T @ W @ J2
+    return H, g
+
+
+def gba_system(T_cw, rho, mbear, host, okf, olm, ouv, hk, cam, d_meas, prior, lam, free, vt_weights=None, rotf=None, rotf_huber=None,
+               welds=None, beta=0.0):
     """global_ba.py:1003-1197 and 1326-1336: the damped reduced camera system of one LM iteration.
 
     cam = (fx, cx, cy, fB, inv_sig_s).  -> dict(H (n,n) scipy CSR with the gauge rows identity and the damping on
@@ -197,6 +267,10 @@ def gba_system(T_cw, rho, mbear, host, okf, olm, ouv, hk, cam, d_meas, prior, la
     U = (Jp.T @ Jr).tocsr()
     H = (Hpp - U @ Hl_sp @ U.T).tolil()
     bp = -np.asarray(Jp.T @ rw).ravel() - U @ np.einsum("lab,lb->la", Hlli, bl).reshape(-1)
+    if rotf or welds:
+        Hf, gf = pose_factor_terms(T_cw, n, rotf, rotf_huber, welds, beta)
+        H = (H.tocsr() + sp.csr_matrix(Hf)).tolil()
+        bp = bp + gf
     for i in np.flatnonzero(~np.asarray(free, bool)):
         s = slice(D * i, D * i + D)
         H[s, :] = 0.0
@@ -390,9 +464,9 @@ def dpcg(H, bp, T_cw, free, rtol=RTOL, maxiter=MAXITER, seg_len=None):
 # ---------------------------------------------------------------------------------------------------------------
 # the LM driver: global_ba.py:694-1785
 # ---------------------------------------------------------------------------------------------------------------
-def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=None, prior=None, iters=8,
-                 fixed=(0,), lam0=1e-4, passes=2, reject_sigma=3.0, rho_floor=1e-4, vt_weights=None,
-                 solver="direct", history=None):
+def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=None, prior=None, rotf=None,
+                 welds=None, iters=8, fixed=(0,), lam0=1e-4, passes=2, reject_sigma=3.0, rho_floor=1e-4,
+                 rotf_huber=None, vt_weights=None, solver="direct", history=None):
     """global_ba.py:694-1785, the default path.  poses: (nk,4,4) T_wc; -> (poses T_wc, rho, mbear, stats).
 
     solver "direct" solves every damped reduced system exactly (scipy spsolve, the reference's path up to 30,000
@@ -431,7 +505,8 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     n_acc = n_best = 0
     n_stereo = int((d_meas > 1e-6).sum()) if d_meas is not None else 0
     for it in range(iters):
-        sysm = gba_system(T_cw, rho, mbear, host_idx, okf, olm, ouv, hk, cam, d_meas, prior, lam, free, vt_weights)
+        sysm = gba_system(T_cw, rho, mbear, host_idx, okf, olm, ouv, hk, cam, d_meas, prior, lam, free, vt_weights,
+                          rotf=rotf, rotf_huber=rotf_huber, welds=welds)
         chi = sysm["chi"]
         H, bp = sysm["H"], sysm["bp"]
         if solver == "dpcg" or H.shape[0] > DIRECT_MAX:
@@ -493,8 +568,9 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
         if n_rej and keep.sum() > 0.5 * N:
             # pass 2 re-solves the kept rows from pass 1's state: the vision rows only (no disparity rows)
             p2, rho2, m2, st = global_ba_id(kf_ids, se3_inv(T_cw), rho, mbear, host_idx, okf[keep], olm[keep],
-                                            ouv[keep], rig, prior=prior_rw, iters=iters, fixed=fixed, lam0=lam0,
-                                            passes=1, rho_floor=rho_floor, vt_weights=vt_weights, solver=solver,
+                                            ouv[keep], rig, prior=prior_rw, rotf=rotf, welds=welds, iters=iters,
+                                            fixed=fixed, lam0=lam0, passes=1, rho_floor=rho_floor,
+                                            rotf_huber=rotf_huber, vt_weights=vt_weights, solver=solver,
                                             history=history)
             st.update(stereo_rows=n_stereo, rms_px_before=rms0, observations=N, rejected=n_rej,
                       rejected_pct=100.0 * n_rej / max(N, 1), reject_threshold_px=thr, lm_pass1=lm)

@@ -100,7 +100,9 @@ def test_gba_edge_cases():
     with pytest.raises(ValueError):
         global_ba_id(**_args(g, okf=g["okf"] + 100))
     with pytest.raises(NotImplementedError):
-        global_ba_id(**_args(g), rotf=[1])
+        global_ba_id(**_args(g), inertial={"vel": None})
+    with pytest.raises(ValueError):
+        global_ba_id(**_args(g), rotf=[(0, 99, np.eye(3), 0.01)])
 
 
 def test_gba_full_size_trajectory_drives_the_noise_free_residuals_down():
@@ -124,3 +126,39 @@ def test_gba_full_size_trajectory_drives_the_noise_free_residuals_down():
         if e["accepted"]:
             assert e["chi2"] <= e["chi"]
     assert st["rms_px_after"] < 1e-3 * st["rms_px_before"]
+
+
+def _factors(g, huber):
+    """Gyro rotation factors between consecutive keyframes (measured from the truth with 0.2 deg noise) and three
+    loop welds from the true relative poses."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(17)
+    Tt = G.se3_inv(g["truth"]["poses"])
+    nk = len(Tt)
+    rotf = []
+    for a in range(nk - 1):
+        A = Tt[a][:3, :3] @ Tt[a + 1][:3, :3].T
+        rotf.append((a, a + 1, A @ Rotation.from_rotvec(rng.normal(0, 0.0035, 3)).as_matrix(), 0.004))
+    welds = []
+    for a, b in ((0, nk // 2), (3, nk - 2), (nk // 3, nk - 1)):
+        welds.append((a, b, Tt[a] @ G.se3_inv(Tt[b]), np.array([2e3, 2e3, 2e3, 5e4, 5e4, 5e4])))
+    welds.append((1, nk - 3, Tt[1] @ G.se3_inv(Tt[nk - 3]), 1e3))   # a scalar weight
+    return dict(rotf=rotf, welds=welds, rotf_huber=1.345 if huber else None)
+
+
+@pytest.mark.parametrize("huber", [False, True])
+def test_gba_rotation_factors_and_welds_match_the_oracle(huber):
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=40, nl=2500, seed=23, pose_sigma_t=0.02, pose_sigma_r_deg=0.3)
+    f = _factors(g, huber)
+    h, hr = [], []
+    out = global_ba_id(**_args(g), **f, iters=4, history=h)
+    ref = G.global_ba_id(**_args(g), **f, iters=4, history=hr)
+    print([(e["accepted"], e["pcg_reason"]) for e in h], [e["accepted"] for e in hr])
+    assert [e["accepted"] for e in h] == [e["accepted"] for e in hr]
+    for a, b in zip(h, hr):
+        assert abs(a["chi"] - b["chi"]) <= 1e-9 * b["chi"]
+    _compare(out, ref, 1e-8)
+    # the factors pull the trajectory: the solve with them differs from the one without
+    out0 = global_ba_id(**_args(g), iters=4)
+    assert _rel(out0[0], out[0]) > 1e-6

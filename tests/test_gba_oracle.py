@@ -69,6 +69,19 @@ def test_gba_residuals_vanish_at_the_noise_free_truth():
     assert np.allclose(Pc, q, rtol=0, atol=1e-9)
 
 
+def _pose_factor_rows(T_cw, rotf, welds):
+    """Whitened rotation-factor rows w Log(M^T R_a R_b^T) and weld rows W^(1/2) Log(Z^-1 T_cw_a T_wc_b)."""
+    from scipy.spatial.transform import Rotation
+    out = []
+    for a, b, M, sig in rotf:
+        A = T_cw[a][:3, :3] @ T_cw[b][:3, :3].T
+        out.append(Rotation.from_matrix(M.T @ A).as_rotvec() / sig)
+    for a, b, Z, w in welds:
+        E = G.se3_inv(Z) @ T_cw[a] @ G.se3_inv(T_cw[b])
+        out.append(np.sqrt(np.asarray(w, np.float64)) * G.se3_log(E))
+    return np.concatenate(out) if out else np.zeros(0)
+
+
 def _whitened_rows(T_cw, rho, mb, g, keep, sw, sws, okd, prior, swp):
     """Every whitened residual of the system with frozen IRLS weights: vision (2 per row), disparity (1 per row
     with one), prior (3 per valid landmark)."""
@@ -83,13 +96,23 @@ def _whitened_rows(T_cw, rho, mb, g, keep, sw, sws, okd, prior, swp):
     return np.concatenate([(r * sw[:, None]).ravel(), rs * sws, (rp * swp)[ok].ravel()])
 
 
-@pytest.mark.parametrize("lam", [0.0, 1e-3])
-def test_gba_reduced_system_matches_a_dense_fd_schur_complement(lam):
+@pytest.mark.parametrize("lam,factors", [(0.0, False), (1e-3, False), (1e-3, True)])
+def test_gba_reduced_system_matches_a_dense_fd_schur_complement(lam, factors):
     """The reduced system of gba_system against a finite-difference Jacobian of every whitened residual (poses by a
     left-multiplied twist, landmarks by (a, b, rho)), the dense normal equations, the landmark damping
-    lam max(tr/3, 1e-9), a dense Schur complement, the gauge rows and the pose damping."""
+    lam max(tr/3, 1e-9), a dense Schur complement, the gauge rows and the pose damping; with gyro rotation factors
+    (a non-zero residual: their Jacobian is exact) and loop welds (at zero residual, where the reference's weld
+    Jacobian, which leaves out Jr^-1(e), is exact)."""
     import scipy.linalg as sla
+    from scipy.spatial.transform import Rotation
     g = synth.make_gba(nk=6, nl=40, seed=11, reach=6, pose_sigma_t=0.01)
+    T0 = G.se3_inv(g["poses"])
+    rotf, welds = [], []
+    if factors:
+        for a, b, sig in ((1, 2, 0.01), (3, 5, 0.02)):
+            A = T0[a][:3, :3] @ T0[b][:3, :3].T
+            rotf.append((a, b, A @ Rotation.from_rotvec([0.01, -0.02, 0.015]).as_matrix(), sig))
+        welds.append((2, 4, T0[2] @ G.se3_inv(T0[4]), np.array([4e2, 5e2, 6e2, 1e3, 2e3, 3e3])))
     rig = g["rig"]
     nk, nl = len(g["poses"]), len(g["rho"])
     T_cw = G.se3_inv(g["poses"])
@@ -100,7 +123,8 @@ def test_gba_reduced_system_matches_a_dense_fd_schur_complement(lam):
     free = np.ones(nk, bool)
     free[0] = False
     cam = (rig.fx_rect, rig.cx, rig.cy, rig.fB, 1.0 / rig.rho_sigma)
-    out = G.gba_system(T_cw, g["rho"], g["mbear"], g["host_idx"], okf, olm, ouv, hk, cam, d, g["prior"], lam, free)
+    out = G.gba_system(T_cw, g["rho"], g["mbear"], g["host_idx"], okf, olm, ouv, hk, cam, d, g["prior"], lam, free,
+                       rotf=rotf, welds=welds)
     # frozen weights at the linearisation point
     r, Pc, _, _ = G.gba_geom(T_cw, g["rho"], g["mbear"], g["host_idx"], okf, olm, ouv, *cam[:3])
     sw = np.sqrt(G._huber_w(np.linalg.norm(r, axis=1)))
@@ -117,7 +141,7 @@ def test_gba_reduced_system_matches_a_dense_fd_schur_complement(lam):
         lm = x[6 * nk:].reshape(nl, 3)
         mb = g["mbear"].copy()
         mb[:, :2] += lm[:, :2]
-        return _whitened_rows(Tn, g["rho"] + lm[:, 2], mb, *args)
+        return np.concatenate([_whitened_rows(Tn, g["rho"] + lm[:, 2], mb, *args), _pose_factor_rows(Tn, rotf, welds)])
 
     nx = 6 * nk + 3 * nl
     h = 1e-6

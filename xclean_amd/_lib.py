@@ -62,7 +62,9 @@ class XbaGbaProblem(C.Structure):
                 ("host", vp), ("okf", vp), ("olm", vp), ("ouv", vp), ("odisp", vp), ("vt_w", vp), ("free_mask", vp),
                 ("pr_m0", vp), ("pr_rho0", vp), ("pr_wbear", vp), ("pr_invsig", vp), ("pr_valid", vp),
                 ("fx", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("fB", C.c_double),
-                ("rho_sigma", C.c_double), ("rho_floor", C.c_double)]
+                ("rho_sigma", C.c_double), ("rho_floor", C.c_double),
+                ("n_rot", C.c_int32), ("rot_ab", vp), ("rot_M", vp), ("rot_sig", vp), ("rot_huber", C.c_double),
+                ("n_weld", C.c_int32), ("weld_ab", vp), ("weld_Z", vp), ("weld_w", vp)]
 
 
 class XbaGbaOptions(C.Structure):

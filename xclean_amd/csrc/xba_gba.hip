@@ -216,6 +216,14 @@ __global__ void k_gba_check_host(int nl, int nk, const int32_t *host, int32_t *e
     if (l < nl && (host[l] < 0 || host[l] >= nk)) atomicOr(err, 1);
 }
 
+// every pose factor's keyframes in range (err |= 1 otherwise)
+__global__ void k_gba_check_pairs(int nf, int nk, const int32_t *ab, int32_t *err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int a = ab[2 * f], b = ab[2 * f + 1];
+    if (a < 0 || a >= nk || b < 0 || b >= nk) atomicOr(err, 1);
+}
+
 // every kept row to its landmark's segment (arbitrary order; k_gba_sort_rows orders each segment by input index)
 __global__ void k_gba_scatter(int N, const int32_t *okf, const int32_t *olm, const int32_t *host,
                               const uint8_t *keep_in, int32_t *fill, int32_t *src) {
@@ -493,6 +501,141 @@ __global__ void k_gba_mirror(int64_t nnzb, const int32_t *brow, const int32_t *r
     double *Lb = Hb + 36 * bi;
     for (int i = 0; i < 6; i++)
         for (int j = 0; j < 6; j++) Lb[6 * i + j] = U[6 * j + i];
+}
+
+// keyframe pairs of the pose factors into the block pattern: (a,a), (a,b), (b,a), (b,b)
+__global__ void k_gba_pattern_pairs(int nf, const int32_t *ab, uint32_t *bits, int64_t wpr) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int a = ab[2 * f], b = ab[2 * f + 1];
+    const int p[2] = {a, b};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) atomicOr(bits + (size_t)p[i] * wpr + (p[j] >> 5), 1u << (p[j] & 31));
+}
+
+__device__ inline double *block_of(const Dev &G, int p, int q) {
+    int lo = G.rowptr[p], hi = G.rowptr[p + 1];
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (G.col[m] < q) lo = m + 1;
+        else hi = m;
+    }
+    return G.Hb + 36 * (size_t)lo;
+}
+
+// the pose factors' Hessian blocks and gradient, added to the assembled system (global atomics; one thread per
+// factor).  Rotation factor (a, b, M, sig) (global_ba.py:1921-1960): e = Log(M^T R_a R_b^T), w = 1 / max(sig, 1e-9)
+// (Huber-reduced by sqrt(k / (w |e|)) above k), Ja = w Jr^-1(e) A^T, Jb = -w Jr^-1(e) on the rotation part of each
+// twist.  Weld (a, b, Z, W) (global_ba.py:2034-2067): e = Log_SE3(Z^-1 T_cw_a T_wc_b), Ja = Ad(T_cw_b T_wc_a),
+// Jb = -I on the whole twist.
+__global__ void k_gba_pose_factors(Dev G, int n_rot, const int32_t *rot_ab, const double *rot_M, const double *rot_sig,
+                                   double rot_huber, int n_weld, const int32_t *weld_ab, const double *weld_Z,
+                                   const double *weld_w) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_rot + n_weld) return;
+    if (f < n_rot) {
+        const int a = rot_ab[2 * f], b = rot_ab[2 * f + 1];
+        const double *Ta = G.Tcw + 16 * (size_t)a, *Tb = G.Tcw + 16 * (size_t)b;
+        double Ra[9], Rb[9], Am[9], E[9], e[3], Jr[9], Ja[9], Jb[9];
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 3; c++) {
+                Ra[3 * i + c] = Ta[4 * i + c];
+                Rb[3 * i + c] = Tb[4 * i + c];
+            }
+        xba::mmt3(Ra, Rb, Am);
+        xba::mtm3(rot_M + 9 * (size_t)f, Am, E);
+        xba::log_quat(E, e);
+        double w = 1.0 / fmax(rot_sig[f], 1e-9);
+        if (rot_huber > 0.0) {
+            const double en = w * sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (en > rot_huber) w *= sqrt(rot_huber / en);
+        }
+        xba::jr_inv(e, Jr);
+        xba::mmt3(Jr, Am, Ja);
+        for (int i = 0; i < 9; i++) {
+            Ja[i] *= w;
+            Jb[i] = -w * Jr[i];
+        }
+        const double r[3] = {w * e[0], w * e[1], w * e[2]};
+        const int p[2] = {a, b};
+        const double *J[2] = {Ja, Jb};
+        for (int u = 0; u < 2; u++) {
+            for (int i = 0; i < 3; i++) {
+                double gi = 0.0;
+                for (int m = 0; m < 3; m++) gi += J[u][3 * m + i] * r[m];
+                atomicAdd(G.bp + 6 * (size_t)p[u] + 3 + i, -gi);
+            }
+            for (int v = 0; v < 2; v++) {
+                double *B = block_of(G, p[u], p[v]);
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) {
+                        double hv = 0.0;
+                        for (int m = 0; m < 3; m++) hv += J[u][3 * m + i] * J[v][3 * m + c];
+                        atomicAdd(B + 6 * (3 + i) + 3 + c, hv);
+                    }
+            }
+        }
+        return;
+    }
+    const int fw = f - n_rot;
+    const int a = weld_ab[2 * fw], b = weld_ab[2 * fw + 1];
+    const double *Ta = G.Tcw + 16 * (size_t)a, *Tb = G.Tcw + 16 * (size_t)b, *Z = weld_Z + 16 * (size_t)fw;
+    // rel = T_cw_a T_wc_b; X = Z^-1 rel
+    double Rb_t[9], tb_w[3], Rrel[9], trel[3];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) Rb_t[3 * i + c] = Tb[4 * c + i];
+    for (int i = 0; i < 3; i++) tb_w[i] = -(Rb_t[3 * i] * Tb[3] + Rb_t[3 * i + 1] * Tb[7] + Rb_t[3 * i + 2] * Tb[11]);
+    for (int i = 0; i < 3; i++) {
+        for (int c = 0; c < 3; c++)
+            Rrel[3 * i + c] = Ta[4 * i] * Rb_t[c] + Ta[4 * i + 1] * Rb_t[3 + c] + Ta[4 * i + 2] * Rb_t[6 + c];
+        trel[i] = Ta[4 * i] * tb_w[0] + Ta[4 * i + 1] * tb_w[1] + Ta[4 * i + 2] * tb_w[2] + Ta[4 * i + 3];
+    }
+    double RX[9], tX[3];
+    for (int i = 0; i < 3; i++) {
+        for (int c = 0; c < 3; c++)
+            RX[3 * i + c] = Z[i] * Rrel[c] + Z[4 + i] * Rrel[3 + c] + Z[8 + i] * Rrel[6 + c];  // Z_R^T Rrel
+        tX[i] = Z[i] * (trel[0] - Z[3]) + Z[4 + i] * (trel[1] - Z[7]) + Z[8 + i] * (trel[2] - Z[11]);
+    }
+    double e[6], phi[3], mphi[3], Vi[9];
+    xba::log_quat(RX, phi);
+    for (int i = 0; i < 3; i++) mphi[i] = -phi[i];
+    xba::jr_inv(mphi, Vi);  // V(phi)^-1 = Jr^-1(-phi)
+    for (int i = 0; i < 3; i++) e[i] = Vi[3 * i] * tX[0] + Vi[3 * i + 1] * tX[1] + Vi[3 * i + 2] * tX[2];
+    for (int i = 0; i < 3; i++) e[3 + i] = phi[i];
+    // Ja = Ad(T_cw_b T_wc_a): R = R_b R_a^T, t = t_b - R t_a
+    double R[9], tt[3], tx[9], Ja[36];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) R[3 * i + c] = Tb[4 * i] * Ta[4 * c] + Tb[4 * i + 1] * Ta[4 * c + 1] + Tb[4 * i + 2] * Ta[4 * c + 2];
+    for (int i = 0; i < 3; i++) tt[i] = Tb[4 * i + 3] - (R[3 * i] * Ta[3] + R[3 * i + 1] * Ta[7] + R[3 * i + 2] * Ta[11]);
+    const double hx[9] = {0, -tt[2], tt[1], tt[2], 0, -tt[0], -tt[1], tt[0], 0};
+    xba::mm3(hx, R, tx);
+    for (int i = 0; i < 36; i++) Ja[i] = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) {
+            Ja[6 * i + c] = R[3 * i + c];
+            Ja[6 * i + 3 + c] = tx[3 * i + c];
+            Ja[6 * (3 + i) + 3 + c] = R[3 * i + c];
+        }
+    const double *Wd = weld_w + 6 * (size_t)fw;
+    double We[6];
+    for (int i = 0; i < 6; i++) We[i] = Wd[i] * e[i];
+    // gradient: -Ja^T W e into a, +W e into b (Jb = -I)
+    for (int i = 0; i < 6; i++) {
+        double gi = 0.0;
+        for (int m = 0; m < 6; m++) gi += Ja[6 * m + i] * We[m];
+        atomicAdd(G.bp + 6 * (size_t)a + i, -gi);
+        atomicAdd(G.bp + 6 * (size_t)b + i, We[i]);
+    }
+    double *Baa = block_of(G, a, a), *Bab = block_of(G, a, b), *Bba = block_of(G, b, a), *Bbb = block_of(G, b, b);
+    for (int i = 0; i < 6; i++)
+        for (int c = 0; c < 6; c++) {
+            double haa = 0.0;
+            for (int m = 0; m < 6; m++) haa += Ja[6 * m + i] * Wd[m] * Ja[6 * m + c];
+            atomicAdd(Baa + 6 * i + c, haa);
+            atomicAdd(Bab + 6 * i + c, -Ja[6 * c + i] * Wd[c]);  // Ja^T W (-I)
+            atomicAdd(Bba + 6 * i + c, -Wd[i] * Ja[6 * i + c]);  // (-I) W Ja
+            if (i == c) atomicAdd(Bbb + 6 * i + c, Wd[i]);
+        }
 }
 
 // gauge rows and columns of the fixed keyframes to identity, their b to zero, then H += diag(lam max(diag, 1e-9))
@@ -1590,6 +1733,8 @@ class Solver {
         P.lstart = A.get<int32_t>(nl + 1);
         GCK(hipMemsetAsync(P.lstart, 0, sizeof(int32_t) * (nl + 1), s));
         k_gba_check_host<<<nblk(nl, 256), 256, 0, s>>>(nl, nk, q.host, err);
+        if (q.n_rot) k_gba_check_pairs<<<nblk(q.n_rot, 128), 128, 0, s>>>(q.n_rot, nk, q.rot_ab, err);
+        if (q.n_weld) k_gba_check_pairs<<<nblk(q.n_weld, 128), 128, 0, s>>>(q.n_weld, nk, q.weld_ab, err);
         GCK(hipGetLastError());
         if (read1(err)) throw RangeFail{};
         if (Nin) k_gba_count<<<nblk(Nin, 256), 256, 0, s>>>(Nin, nk, nl, okf, olm, q.host, keep_in, P.lstart, err);
@@ -1621,6 +1766,8 @@ class Solver {
         uint32_t *bits = A.get<uint32_t>((size_t)nk * wpr);
         GCK(hipMemsetAsync(bits, 0, sizeof(uint32_t) * nk * wpr, s));
         k_gba_pattern<<<nblk(std::max(nk, nl), 256), 256, 0, s>>>(nk, nl, P.lstart, P.rkf, q.host, bits, wpr);
+        if (q.n_rot) k_gba_pattern_pairs<<<nblk(q.n_rot, 128), 128, 0, s>>>(q.n_rot, q.rot_ab, bits, wpr);
+        if (q.n_weld) k_gba_pattern_pairs<<<nblk(q.n_weld, 128), 128, 0, s>>>(q.n_weld, q.weld_ab, bits, wpr);
         P.rowptr = A.get<int32_t>(nk + 1);
         k_gba_rowcount<<<nblk(nk, 4), 256, 0, s>>>(nk, bits, wpr, P.rowptr);
         k_gba_scan<<<1, 1024, 0, s>>>(nk, P.rowptr);
@@ -1861,6 +2008,10 @@ class Solver {
             k_gba_rows<<<nblk(nl, 128), 128, 0, s>>>(G, rrec, whs);
             k_gba_assemble<<<P.nchunks, GBA_WG, P.lds, s>>>(G, P.chunks, rrec, whs);
             k_gba_mirror<<<nblk(P.nnzb, 256), 256, 0, s>>>(P.nnzb, P.brow, P.rowptr, P.col, P.Hb);
+            if (q.n_rot + q.n_weld)
+                k_gba_pose_factors<<<nblk(q.n_rot + q.n_weld, 64), 64, 0, s>>>(G, q.n_rot, q.rot_ab, q.rot_M,
+                                                                              q.rot_sig, q.rot_huber, q.n_weld,
+                                                                              q.weld_ab, q.weld_Z, q.weld_w);
             const int e1 = mark();
             if (e0 >= 0) asm_ev.push_back({e0, e1});
             k_gba_gauge<<<nblk(std::max<int64_t>(P.nnzb, nk), 256), 256, 0, s>>>(G, P.brow, P.nnzb);
@@ -1932,6 +2083,9 @@ extern "C" int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o,
     const bool pr = q->pr_m0 != nullptr;
     if (pr && (!q->pr_rho0 || !q->pr_wbear || !q->pr_invsig || !q->pr_valid)) return XBA_E_ARG;
     if (o->iters < 0 || o->passes < 1 || o->maxiter < 1 || o->seg_len < 0 || o->seg_len > GBA_SEG_MAX) return XBA_E_ARG;
+    if (q->n_rot < 0 || q->n_weld < 0 || (q->n_rot && (!q->rot_ab || !q->rot_M || !q->rot_sig)) ||
+        (q->n_weld && (!q->weld_ab || !q->weld_Z || !q->weld_w)))
+        return XBA_E_ARG;
     hipStream_t s = (hipStream_t)stream;
     for (int i = 0; i < XBA_GBA_NSTATS; i++) stats[i] = NAN;
     try {

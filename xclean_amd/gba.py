@@ -6,11 +6,11 @@ the two-level deflated PCG of backend/gba_linsolve.py (segment smoother, rigid-m
 true-residual stopping rule); the host only marshals arrays and reads the statistics.
 
 Covered: the reprojection rows with the Huber IRLS weight and optional per-keyframe weights (``vt_weights``), the
-per-observation disparity rows (``odisp``), the landmark prior (with a constant ``beta0``), the gauge keyframes
-(``fixed``), the LM accept / damping / best-state rules, the wall-clock budget, the MAD rejection and the second
-pass with the re-weighted prior, the final SO(3) projection.  Not covered (NotImplementedError): the inertial term,
-rotation factors, welds, the disparity offset states, the marginal covariances and the XTEN_GBA_SCREEN /
-ACCEPT / ITERBUDGET / BESTFIX modes.
+per-observation disparity rows (``odisp``), the landmark prior (with a constant ``beta0``), the gyro rotation factors
+(``rotf``, ``rotf_huber``), the loop welds (``welds``), the gauge keyframes (``fixed``), the LM accept / damping /
+best-state rules, the wall-clock budget, the MAD rejection and the second pass with the re-weighted prior, the final
+SO(3) projection.  Not covered (NotImplementedError): the inertial term, the disparity offset states, the marginal
+covariances and the XTEN_GBA_SCREEN / ACCEPT / ITERBUDGET / BESTFIX modes.
 """
 from __future__ import annotations
 
@@ -53,10 +53,11 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
 
     poses: (nk,4,4) T_wc; rho, mbear: the landmarks' inverse depth and bearing in their host camera; host_idx (nl,);
     okf / olm / ouv: the observing keyframe, landmark and pixel of every row; rig: fx_rect, cx, cy and, with
-    ``odisp``, fB and rho_sigma; prior: (m0 (nl,2), rho0, wbear, invsig, valid).  ``history``, a list, receives one
+    ``odisp``, fB and rho_sigma; prior: (m0 (nl,2), rho0, wbear, invsig, valid); rotf: [(a, b, M (3,3), sigma)];
+    welds: [(a, b, Z (4,4), w scalar or (6,) information diagonal)].  ``history``, a list, receives one
     dict per LM iteration (pass, it, lam, chi, chi2, accepted, pcg_iters, relres, pcg_reason, restarts)."""
     import torch
-    for name, v in (("rotf", rotf), ("welds", welds), ("inertial", inertial), ("dof", dof)):
+    for name, v in (("inertial", inertial), ("dof", dof)):
         if v:
             raise NotImplementedError("global_ba_id(%s=...) is not implemented on the GPU" % name)
     if want_cov:
@@ -118,6 +119,27 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
             d = t(np.asarray(arr, np.float64), np.float64)
             keep.append(d)
             setattr(q, name, d.data_ptr())
+    if rotf:
+        ab = np.array([[int(f[0]), int(f[1])] for f in rotf], np.int32)
+        Ms = np.stack([np.asarray(f[2], np.float64).reshape(3, 3) for f in rotf])
+        sg = np.array([float(f[3]) for f in rotf], np.float64)
+        for name, arr, dt in (("rot_ab", ab, np.int32), ("rot_M", Ms, np.float64), ("rot_sig", sg, np.float64)):
+            d = t(arr, dt)
+            keep.append(d)
+            setattr(q, name, d.data_ptr())
+        q.n_rot = len(rotf)
+        q.rot_huber = float(rotf_huber) if rotf_huber is not None else 0.0
+    if welds:
+        ab = np.array([[int(f[0]), int(f[1])] for f in welds], np.int32)
+        Zs = np.stack([np.asarray(f[2], np.float64).reshape(4, 4) for f in welds]).copy()
+        Zs[:, :3, 3] /= (1.0 + float(beta0))     # a weld's translation carries the map's scale (global_ba.py:2018)
+        ws = np.stack([np.asarray(f[3], np.float64) * np.ones(6) if np.ndim(f[3]) == 0
+                       else np.asarray(f[3], np.float64).reshape(6) for f in welds])
+        for name, arr, dt in (("weld_ab", ab, np.int32), ("weld_Z", Zs, np.float64), ("weld_w", ws, np.float64)):
+            d = t(arr, dt)
+            keep.append(d)
+            setattr(q, name, d.data_ptr())
+        q.n_weld = len(welds)
     q.fx, q.cx, q.cy = float(rig.fx_rect), float(rig.cx), float(rig.cy)
     q.fB = fB
     q.rho_sigma = float(getattr(rig, "rho_sigma", 0.0) or 0.0) if odisp is not None else 0.0
