@@ -260,6 +260,13 @@ typedef struct {
     double rtol, rtol_direct;
     double rtol_direct_floor;  /* at or below XBA_GBA_DIRECT_MAX unknowns a PCG that stops short of rtol_direct at
                                   its attainable accuracy is still used when its best relative residual is <= this */
+    /* marginal covariances (global_ba.py:2080-2126 _marginal_cov): want_cov > 0 samples that many free keyframes
+       evenly and returns the 6x6 diagonal blocks of the inverse of the last LM iteration's damped reduced system
+       (6 PCG solves each, to rtol_direct), only at or below XBA_GBA_DIRECT_MAX unknowns; cov_ids (host, want_cov
+       int32) and cov (host, want_cov x 36) receive XBA_GBA_ST_NCOV of them */
+    int32_t want_cov;
+    int32_t *cov_ids;
+    double *cov;
 } xba_gba_options;
 enum {
     XBA_GBA_ST_RMS0 = 0, XBA_GBA_ST_RMS1, XBA_GBA_ST_OBS, XBA_GBA_ST_REJECTED, XBA_GBA_ST_THR, XBA_GBA_ST_STEREO,
@@ -271,6 +278,7 @@ enum {
     XBA_GBA_ST_SETUP_MS, /* mean device time of one PCG set-up: smoother, deflation space, coarse inverse */
     XBA_GBA_ST_PCG_MS,   /* mean device time of one PCG recursion */
     XBA_GBA_ST_ROWS,     /* observation rows of the last pass */
+    XBA_GBA_ST_NCOV,     /* marginal covariance blocks returned */
     XBA_GBA_NSTATS
 };
 int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o, double *stats, double *hist, int32_t hist_cap,

@@ -464,8 +464,30 @@ def dpcg(H, bp, T_cw, free, rtol=RTOL, maxiter=MAXITER, seg_len=None):
 # ---------------------------------------------------------------------------------------------------------------
 # the LM driver: global_ba.py:694-1785
 # ---------------------------------------------------------------------------------------------------------------
+def marginal_cov(H, nk, want, free):
+    """global_ba.py:2080-2126: the 6x6 diagonal blocks of H^-1 (H the last damped reduced system) for want free
+    keyframes sampled evenly, symmetrised; {} above DIRECT_MAX unknowns."""
+    import scipy.sparse.linalg as spl
+    ids = [i for i in range(nk) if free[i]]
+    if not ids or H.shape[0] > DIRECT_MAX:
+        return {}
+    step = max(1, len(ids) // max(want, 1))
+    ids = ids[::step][:want]
+    lu = spl.splu(H.tocsc())
+    out = {}
+    for i in ids:
+        C = np.zeros((D, D))
+        for c in range(D):
+            e = np.zeros(H.shape[0])
+            e[D * i + c] = 1.0
+            C[:, c] = lu.solve(e)[D * i:D * i + D]
+        if np.isfinite(C).all():
+            out[i] = 0.5 * (C + C.T)
+    return out
+
+
 def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=None, prior=None, rotf=None,
-                 welds=None, iters=8, fixed=(0,), lam0=1e-4, passes=2, reject_sigma=3.0, rho_floor=1e-4,
+                 welds=None, iters=8, fixed=(0,), lam0=1e-4, passes=2, reject_sigma=3.0, want_cov=0, rho_floor=1e-4,
                  rotf_huber=None, vt_weights=None, solver="direct", history=None):
     """global_ba.py:694-1785, the default path.  poses: (nk,4,4) T_wc; -> (poses T_wc, rho, mbear, stats).
 
@@ -501,6 +523,7 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     chi0_vision = float(np.sum(r0 ** 2))
     chi0_stereo = stereo_chi(Pc0[:, 2], d_meas, fB, inv_sig_s)
     lam = lam0
+    last_H = None
     best = (T_cw.copy(), rho.copy(), mbear.copy(), chi0_vision)
     n_acc = n_best = 0
     n_stereo = int((d_meas > 1e-6).sum()) if d_meas is not None else 0
@@ -509,6 +532,7 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
                           rotf=rotf, rotf_huber=rotf_huber, welds=welds)
         chi = sysm["chi"]
         H, bp = sysm["H"], sysm["bp"]
+        last_H = H
         if solver == "dpcg" or H.shape[0] > DIRECT_MAX:
             dx, info = dpcg(H, bp, T_cw, free)
             if not info["converged"]:
@@ -569,7 +593,7 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
             # pass 2 re-solves the kept rows from pass 1's state: the vision rows only (no disparity rows)
             p2, rho2, m2, st = global_ba_id(kf_ids, se3_inv(T_cw), rho, mbear, host_idx, okf[keep], olm[keep],
                                             ouv[keep], rig, prior=prior_rw, rotf=rotf, welds=welds, iters=iters,
-                                            fixed=fixed, lam0=lam0, passes=1, rho_floor=rho_floor,
+                                            fixed=fixed, lam0=lam0, passes=1, want_cov=want_cov, rho_floor=rho_floor,
                                             rotf_huber=rotf_huber, vt_weights=vt_weights, solver=solver,
                                             history=history)
             st.update(stereo_rows=n_stereo, rms_px_before=rms0, observations=N, rejected=n_rej,
@@ -589,7 +613,9 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
             rp = ((rho - rho0P) * invsigP)[okp]
             sig = float(np.median(np.abs(rp)) * 1.4826 * DISPARITY_NOISE_PX)
             bias = float(np.median(rp) * DISPARITY_NOISE_PX)
+    cov = marginal_cov(last_H, nk, want_cov, free) if (want_cov and last_H is not None) else {}
     st = {"rms_px_before": rms0, "rms_px_after": float(np.sqrt(np.mean(np.sum(rn ** 2, 1)))) if N else float("nan"),
+          "cov": cov,
           "keyframes": nk, "landmarks": nl, "observations": N, "rejected": n_rej,
           "rejected_pct": 100.0 * n_rej / max(N, 1), "stereo_rows": n_stereo, "lm": lm,
           "measured_disparity_sigma_px": sig, "measured_disparity_bias_px": bias}

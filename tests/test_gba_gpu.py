@@ -162,3 +162,19 @@ def test_gba_rotation_factors_and_welds_match_the_oracle(huber):
     # the factors pull the trajectory: the solve with them differs from the one without
     out0 = global_ba_id(**_args(g), iters=4)
     assert _rel(out0[0], out[0]) > 1e-6
+
+
+def test_gba_marginal_covariances_match_the_oracle():
+    """want_cov (the tracker asks for 200): the 6x6 diagonal blocks of the last damped reduced system's inverse for
+    evenly sampled free keyframes (global_ba.py:2080-2126), GPU PCG columns against the oracle's sparse LU."""
+    from xclean_amd.gba import global_ba_id
+    g = synth.make_gba(nk=40, nl=2500, seed=29)
+    out = global_ba_id(**_args(g), iters=3, want_cov=7)
+    ref = G.global_ba_id(**_args(g), iters=3, want_cov=7)
+    co, cr = out[3]["cov"], ref[3]["cov"]
+    print(sorted(co), sorted(cr))
+    assert sorted(co) == sorted(cr) and len(co) == 7 and 0 not in co
+    for k in cr:
+        assert np.abs(co[k] - cr[k]).max() <= 1e-8 * np.abs(cr[k]).max()
+        assert np.allclose(co[k], co[k].T)
+    assert global_ba_id(**_args(g), iters=1, passes=1)[3]["cov"] == {}

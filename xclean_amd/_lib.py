@@ -70,10 +70,11 @@ class XbaGbaProblem(C.Structure):
 class XbaGbaOptions(C.Structure):
     _fields_ = [("iters", C.c_int32), ("passes", C.c_int32), ("maxiter", C.c_int32), ("seg_len", C.c_int32),
                 ("profile", C.c_int32), ("lam0", C.c_double), ("reject_sigma", C.c_double), ("time_budget_s", C.c_double),
-                ("rtol", C.c_double), ("rtol_direct", C.c_double), ("rtol_direct_floor", C.c_double)]
+                ("rtol", C.c_double), ("rtol_direct", C.c_double), ("rtol_direct_floor", C.c_double),
+                ("want_cov", C.c_int32), ("cov_ids", vp), ("cov", vp)]
 
 
-XBA_GBA_NSTATS = 32
+XBA_GBA_NSTATS = 33
 XBA_GBA_HIST = 10
 XBA_GBA_DIRECT_MAX = 30000
 XBA_GBA_MAX_NK = 65536

@@ -1822,11 +1822,10 @@ class Solver {
 
     void sum_parts(int np, int stage) { k_pcg_ctl<<<1, 1024, 0, s>>>(st, part, np, stage); }
 
-    // the DPCG solve of the assembled, gauged, damped system P.Hb / P.bp into x (gba_linsolve.py:447-478)
-    PcgInfo dpcg(const Pass &P, double rtol) {
-        PcgInfo inf;
-        const int nred = std::min(GBA_RED, nblk(n, 256));
-        const int nsp = std::min(GBA_RED, nblk(nk, 4));
+    // the PCG set-up of the assembled system (segment inverses, deflation basis, coarse inverse); false on a set-up
+    // failure (a block not numerically positive definite)
+    int set_nc = 0, set_ldE = 0;
+    bool setup(const Pass &P) {
         const int es0 = mark();
         GCK(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
         k_seg_inv<<<nseg, 1024, sizeof(double) * 36 * (size_t)seg_max * seg_max, s>>>(segk, boff, P.rowptr, P.col, P.brow,
@@ -1851,12 +1850,28 @@ class Solver {
         GCK(hipGetLastError());
         const int es1 = mark();
         if (es0 >= 0) setup_ev.push_back({es0, es1});
-        if (read1(flag)) {
+        set_nc = nc;
+        set_ldE = ldE;
+        return read1(flag) == 0;
+    }
+
+    // the DPCG solve of the assembled, gauged, damped system P.Hb / P.bp into x (gba_linsolve.py:447-478)
+    PcgInfo dpcg(const Pass &P, double rtol) {
+        if (!setup(P)) {
+            PcgInfo inf;
             inf.reason = PR_SETUP;
             return inf;
         }
-        // |b|
-        const double *b = P.bp;
+        return recur(P, P.bp, rtol);
+    }
+
+    // the recursion (gba_linsolve.py:353-444) for right-hand side b with the current set-up, into x
+    PcgInfo recur(const Pass &P, const double *b, double rtol) {
+        PcgInfo inf;
+        const int nred = std::min(GBA_RED, nblk(n, 256));
+        const int nsp = std::min(GBA_RED, nblk(nk, 4));
+        const int nc = set_nc, ldE = set_ldE;
+        const int es1 = mark();
         k_norm2<<<nred, 256, 0, s>>>(n, b, part);
         Pcg h{};
         h.maxiter = o.maxiter;
@@ -1901,11 +1916,11 @@ class Solver {
                 restart_tail(P, nc, ldE, nsp, 1);
                 continue;
             }
-            for (int i = 0; i < K; i++) iteration(P, nc, ldE, nsp, nred);
+            for (int i = 0; i < K; i++) iteration(P, b, nc, ldE, nsp, nred);
             GCK(hipGetLastError());
         }
         const int es2 = mark();
-        if (es0 >= 0) pcg_ev.push_back({es1, es2});
+        if (es1 >= 0) pcg_ev.push_back({es1, es2});
         inf.reason = h.reason;
         inf.iters = h.it;
         inf.matvecs = h.matvecs;
@@ -1927,13 +1942,13 @@ class Solver {
     }
 
     // one CG iteration of gba_linsolve.py:395-444 as a fixed launch sequence
-    void iteration(const Pass &P, int nc, int ldE, int nsp, int nred) {
+    void iteration(const Pass &P, const double *b, int nc, int ldE, int nsp, int nred) {
         sum_parts(1, CT_START);  // part[0] unused by START
         k_spmv<<<nsp, 256, 0, s>>>(nk, P.rowptr, P.col, P.Hb, pv, w, nullptr, 0, 0, st, part);
         sum_parts(nsp, CT_ALPHA);
         k_axpy<<<nred, 256, 0, s>>>(n, st, x, r, pv, w, part);
         sum_parts(nred, CT_CHECK);
-        k_spmv<<<nsp, 256, 0, s>>>(nk, P.rowptr, P.col, P.Hb, x, rt, P.bp, 1, 0, st, part);
+        k_spmv<<<nsp, 256, 0, s>>>(nk, P.rowptr, P.col, P.Hb, x, rt, b, 1, 0, st, part);
         sum_parts(nsp, CT_TRUE);
         k_copybest<<<nred, 256, 0, s>>>(n, st, x, xb);
         k_minv<<<nseg, 256, 0, s>>>(segk, boff, Binv, r, z, st, 0, part);
@@ -1942,6 +1957,52 @@ class Solver {
         k_zt<<<nseg, XBA_WAVE, 0, s>>>(segk, rcount, Zg, hz, yc, st, 0);
         k_einv<<<nblk(nc, 4), 256, 0, s>>>(nc, ldE, E, yc, cc, st, 0);
         k_zapply<<<nseg, 256, 0, s>>>(segk, rcount, Zg, cc, z, pv, st, 0, 0, 0);
+    }
+
+    // global_ba.py:2080-2126: the 6x6 diagonal blocks of the inverse of the last damped reduced system for want
+    // free keyframes sampled evenly (every step-th, step = max(1, free / want)), column by column with the PCG
+    int marginal_cov(const Pass &P, int want, int32_t *ids_out, double *cov_out) {
+        if (want <= 0 || n > XBA_GBA_DIRECT_MAX) return 0;
+        std::vector<uint8_t> fm(nk);
+        GCK(hipMemcpyAsync(fm.data(), q.free_mask, nk, hipMemcpyDeviceToHost, s));
+        GCK(hipStreamSynchronize(s));
+        std::vector<int> ids;
+        for (int i = 0; i < nk; i++)
+            if (fm[i]) ids.push_back(i);
+        if (ids.empty()) return 0;
+        const int step = std::max<int>(1, (int)ids.size() / std::max(want, 1));
+        std::vector<int> pick;
+        for (size_t i = 0; i < ids.size() && (int)pick.size() < want; i += step) pick.push_back(ids[i]);
+        if (!setup(P)) return 0;
+        double *e = A.get<double>(n);
+        const double one = 1.0;
+        int cnt = 0;
+        for (int i : pick) {
+            double Cm[36];
+            bool ok = true;
+            for (int c = 0; c < 6 && ok; c++) {
+                GCK(hipMemsetAsync(e, 0, sizeof(double) * n, s));
+                GCK(hipMemcpyAsync(e + 6 * (size_t)i + c, &one, sizeof(double), hipMemcpyHostToDevice, s));
+                const PcgInfo inf = recur(P, e, o.rtol_direct);
+                const bool usable = inf.converged || (inf.reason != PR_SETUP && std::isfinite(inf.relres) &&
+                                                      inf.relres <= o.rtol_direct_floor);
+                if (!usable) {
+                    ok = false;
+                    break;
+                }
+                double col[6];
+                GCK(hipMemcpyAsync(col, x + 6 * (size_t)i, sizeof(col), hipMemcpyDeviceToHost, s));
+                GCK(hipStreamSynchronize(s));
+                for (int r = 0; r < 6; r++) Cm[6 * r + c] = col[r];
+            }
+            for (int v = 0; v < 36 && ok; v++) ok = std::isfinite(Cm[v]);
+            if (!ok) continue;
+            ids_out[cnt] = i;
+            for (int r = 0; r < 6; r++)
+                for (int c = 0; c < 6; c++) cov_out[36 * (size_t)cnt + 6 * r + c] = 0.5 * (Cm[6 * r + c] + Cm[6 * c + r]);
+            cnt++;
+        }
+        return cnt;
     }
 
     double kth(const double *v, int cnt, int64_t k) {
@@ -2172,6 +2233,11 @@ extern "C" int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o,
                 iters_run += ps2.iters_run;
             }
         }
+        // the marginal covariances from the last LM iteration's system of the last pass (before the projection,
+        // which does not touch it)
+        stats[XBA_GBA_ST_NCOV] = 0;
+        if (o->want_cov > 0 && o->cov_ids && o->cov && last.iters_run > 0)
+            stats[XBA_GBA_ST_NCOV] = S.marginal_cov(*Pf, o->want_cov, o->cov_ids, o->cov);
         // the final SO(3) projection and the statistics of the returned state
         xba::k_np_project<<<nblk(S.nk, 128), 128, 0, s>>>(S.nk, S.T[S.cur]);
         GCK(hipGetLastError());

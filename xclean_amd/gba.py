@@ -9,8 +9,9 @@ Covered: the reprojection rows with the Huber IRLS weight and optional per-keyfr
 per-observation disparity rows (``odisp``), the landmark prior (with a constant ``beta0``), the gyro rotation factors
 (``rotf``, ``rotf_huber``), the loop welds (``welds``), the gauge keyframes (``fixed``), the LM accept / damping /
 best-state rules, the wall-clock budget, the MAD rejection and the second pass with the re-weighted prior, the final
-SO(3) projection.  Not covered (NotImplementedError): the inertial term, the disparity offset states, the marginal
-covariances and the XTEN_GBA_SCREEN / ACCEPT / ITERBUDGET / BESTFIX modes.
+SO(3) projection, the sampled marginal covariances (``want_cov``, ``stats["cov"]``).  Not covered
+(NotImplementedError): the inertial term, the disparity offset states and the XTEN_GBA_SCREEN / ACCEPT / ITERBUDGET /
+BESTFIX modes.
 """
 from __future__ import annotations
 
@@ -60,8 +61,6 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     for name, v in (("inertial", inertial), ("dof", dof)):
         if v:
             raise NotImplementedError("global_ba_id(%s=...) is not implemented on the GPU" % name)
-    if want_cov:
-        raise NotImplementedError("global_ba_id(want_cov>0): marginal covariances are not implemented on the GPU")
     if _screen:
         _check_modes()
     _lib.require_gpu()
@@ -151,6 +150,12 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     o.lam0, o.reject_sigma = float(lam0), float(reject_sigma)
     o.time_budget_s = float(time_budget_s) if time_budget_s else 0.0
     o.rtol, o.rtol_direct, o.rtol_direct_floor = float(rtol), float(rtol_direct), float(rtol_direct_floor)
+    want = max(int(want_cov or 0), 0)
+    cov_ids = np.zeros(max(want, 1), np.int32)
+    cov = np.zeros((max(want, 1), 6, 6))
+    o.want_cov = want
+    o.cov_ids = cov_ids.ctypes.data_as(C.c_void_p)
+    o.cov = cov.ctypes.data_as(C.c_void_p)
     stats = np.zeros(_lib.XBA_GBA_NSTATS)
     hcap = 2 * max(int(iters), 0) + 2
     hist = np.zeros((hcap, _lib.XBA_GBA_HIST))
@@ -166,6 +171,7 @@ def global_ba_id(kf_ids, poses, rho, mbear, host_idx, okf, olm, ouv, rig, odisp=
     out_mb = d_mb.cpu().numpy()
     del keep
     st = _stats(stats, nk, nl, beta0)
+    st["cov"] = {int(cov_ids[i]): cov[i].copy() for i in range(int(stats[32]))}
     n_it = int(stats[27])
     if history is not None:
         for row in hist[:min(n_it, hcap)]:
