@@ -1307,6 +1307,7 @@ __global__ void k_coarse_scale(int nc, int ldE, double *E, const double *dsc) {
 
 // Blocked sweep of the scaled coarse matrix over 64 x 64 tiles (A <- -A^-1 after every tile step k):
 //   P = A_kk^-1;  S_i = A_ik P (i != k);  A_ij -= S_i A_kj (i, j != k);  A_ik = S_i, A_ki = S_i^T, A_kk = -P
+// on the lower triangle of tiles (below)
 #define SW_T 64
 __device__ inline void tile_load(double *dst, const double *A, int ld, int ti, int tj) {
     for (int e = threadIdx.x; e < SW_T * SW_T; e += blockDim.x) {
@@ -1315,9 +1316,9 @@ __device__ inline void tile_load(double *dst, const double *A, int ld, int ti, i
     }
 }
 
-__global__ __launch_bounds__(256) void k_sw_pivot(double *A, int ld, int k, double *P, int32_t *fail) {
+__global__ __launch_bounds__(1024) void k_sw_pivot(double *A, int ld, int k, double *P, int32_t *fail) {
     __shared__ double sT[SW_T * SW_T];
-    __shared__ double ck[SW_T];  // R * MAXR = 4 * 16 = 64 rows
+    __shared__ double ck[SW_T];  // R * MAXR = 16 * 4 = 64 rows
     tile_load(sT, A, ld, k, k);
     __syncthreads();
     for (int e = threadIdx.x; e < SW_T * SW_T; e += blockDim.x) {  // symmetrise the pivot tile
@@ -1329,7 +1330,7 @@ __global__ __launch_bounds__(256) void k_sw_pivot(double *A, int ld, int k, doub
         }
     }
     __syncthreads();
-    if (!reg_sweep<SW_T * SW_T / 256, 4>(sT, SW_T, ck)) {  // 4 row groups of 64 columns, 16 rows each
+    if (!reg_sweep<SW_T * SW_T / 1024, 16>(sT, SW_T, ck)) {  // 16 row groups of 64 columns, 4 rows each
         if (threadIdx.x == 0) atomicOr(fail, 1);
         return;
     }
@@ -1350,12 +1351,25 @@ __device__ inline void tile_mm(const double *X, const double *Y, double acc[4][4
     }
 }
 
+// The sweep keeps the lower triangle of tiles only (the matrix stays symmetric through every step): an upper tile
+// (i, j), i < j, is read as the transpose of the lower tile (j, i).
+__device__ inline void tile_load_sym(double *dst, const double *A, int ld, int ti, int tj) {
+    if (ti >= tj) {
+        tile_load(dst, A, ld, ti, tj);
+        return;
+    }
+    for (int e = threadIdx.x; e < SW_T * SW_T; e += blockDim.x) {
+        const int i = e / SW_T, j = e - SW_T * (e / SW_T);
+        dst[e] = A[(size_t)(SW_T * tj + j) * ld + SW_T * ti + i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sw_strip(const double *A, int ld, int k, const double *P, double *S,
                                                   const int32_t *fail) {
     __shared__ double sX[SW_T * SW_T], sY[SW_T * SW_T];
     const int i = blockIdx.x;
     if (*fail || i == k) return;
-    tile_load(sX, A, ld, i, k);
+    tile_load_sym(sX, A, ld, i, k);
     for (int e = threadIdx.x; e < SW_T * SW_T; e += blockDim.x) sY[e] = P[e];
     __syncthreads();
     double acc[4][4];
@@ -1365,13 +1379,17 @@ __global__ __launch_bounds__(256) void k_sw_strip(const double *A, int ld, int k
         for (int b = 0; b < 4; b++) S[(size_t)(SW_T * i + 4 * ti + a) * SW_T + 4 * tj + b] = acc[a][b];
 }
 
+// the lower tiles (i >= j, neither k): A_ij -= S_i A_kj; one workgroup per tile, x = i (i + 1) / 2 + j
 __global__ __launch_bounds__(256) void k_sw_update(double *A, int ld, int k, const double *S, const int32_t *fail) {
     __shared__ double sX[SW_T * SW_T], sY[SW_T * SW_T];
-    const int nt = ld / SW_T;
-    const int i = blockIdx.x / nt, j = blockIdx.x - nt * (blockIdx.x / nt);
+    const int x = blockIdx.x;
+    int i = (int)((sqrtf(8.0f * (float)x + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= x) i++;
+    while (i * (i + 1) / 2 > x) i--;
+    const int j = x - i * (i + 1) / 2;
     if (*fail || i == k || j == k) return;
     for (int e = threadIdx.x; e < SW_T * SW_T; e += blockDim.x) sX[e] = S[(size_t)SW_T * SW_T * i + e];
-    tile_load(sY, A, ld, k, j);
+    tile_load_sym(sY, A, ld, k, j);
     __syncthreads();
     double acc[4][4];
     tile_mm(sX, sY, acc);
@@ -1380,6 +1398,7 @@ __global__ __launch_bounds__(256) void k_sw_update(double *A, int ld, int k, con
         for (int b = 0; b < 4; b++) A[(size_t)(SW_T * i + 4 * ti + a) * ld + SW_T * j + 4 * tj + b] -= acc[a][b];
 }
 
+// column k of the lower triangle: A_ik = S_i below the diagonal tile, A_ki = S_i^T left of it; A_kk = -P
 __global__ void k_sw_finish(double *A, int ld, int k, const double *P, const double *S, const int32_t *fail) {
     const int i = blockIdx.x;
     if (*fail) return;
@@ -1389,19 +1408,28 @@ __global__ void k_sw_finish(double *A, int ld, int k, const double *P, const dou
             A[(size_t)(SW_T * k + a) * ld + SW_T * k + b] = -P[e];
         } else {
             const double v = S[(size_t)(SW_T * i + a) * SW_T + b];
-            A[(size_t)(SW_T * i + a) * ld + SW_T * k + b] = v;
-            A[(size_t)(SW_T * k + b) * ld + SW_T * i + a] = v;
+            if (i > k) A[(size_t)(SW_T * i + a) * ld + SW_T * k + b] = v;
+            else A[(size_t)(SW_T * k + b) * ld + SW_T * i + a] = v;
         }
     }
 }
 
-// Einv = -A / (d d^T) on [0, nc)^2 (A the swept scaled matrix)
+// Einv = -A / (d d^T) on [0, nc)^2 (A the swept scaled matrix): the lower tiles in place, then (k_coarse_mirror)
+// the upper tiles from them
 __global__ void k_coarse_unscale(int nc, int ld, double *A, const double *dsc) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)nc * ld) return;
     const int i = (int)(e / ld), j = (int)(e - (int64_t)i * ld);
-    if (j >= nc) return;
+    if (j >= nc || i / SW_T < j / SW_T) return;
     A[e] = -A[e] / (dsc[i] * dsc[j]);
+}
+
+__global__ void k_coarse_mirror(int nc, int ld, double *A) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nc * ld) return;
+    const int i = (int)(e / ld), j = (int)(e - (int64_t)i * ld);
+    if (j >= nc || i / SW_T >= j / SW_T) return;
+    A[e] = A[(size_t)j * ld + i];
 }
 
 __global__ void k_gba_brow(int nk, const int32_t *rowptr, int32_t *brow) {
@@ -1469,6 +1497,14 @@ __global__ void k_to_keys(int n, const double *v, uint64_t *key) {
     if (i >= n) return;
     const uint64_t b = (uint64_t)__double_as_longlong(v[i]);
     key[i] = (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+// cnt += the entries of v above thr
+__global__ void k_count_above(int n, const double *v, double thr, int32_t *cnt) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += v[i] > thr;
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, XBA_WAVE);
+    if ((threadIdx.x & (XBA_WAVE - 1)) == 0 && c) atomicAdd(cnt, c);
 }
 
 __global__ void k_scale(int n, const double *a, double f, double *out) {
@@ -1841,12 +1877,13 @@ class Solver {
         k_coarse_scale<<<nblk((int64_t)ldE * ldE, 256), 256, 0, s>>>(nc, ldE, E, dsc);
         const int ntl = ldE / SW_T;
         for (int k = 0; k < ntl; k++) {
-            k_sw_pivot<<<1, 256, 0, s>>>(E, ldE, k, Pt, flag);
+            k_sw_pivot<<<1, 1024, 0, s>>>(E, ldE, k, Pt, flag);
             k_sw_strip<<<ntl, 256, 0, s>>>(E, ldE, k, Pt, St, flag);
-            k_sw_update<<<ntl * ntl, 256, 0, s>>>(E, ldE, k, St, flag);
+            k_sw_update<<<ntl * (ntl + 1) / 2, 256, 0, s>>>(E, ldE, k, St, flag);
             k_sw_finish<<<ntl, 256, 0, s>>>(E, ldE, k, Pt, St, flag);
         }
         k_coarse_unscale<<<nblk((int64_t)nc * ldE, 256), 256, 0, s>>>(nc, ldE, E, dsc);
+        k_coarse_mirror<<<nblk((int64_t)nc * ldE, 256), 256, 0, s>>>(nc, ldE, E);
         GCK(hipGetLastError());
         const int es1 = mark();
         if (es0 >= 0) setup_ev.push_back({es0, es1});
@@ -2163,16 +2200,11 @@ extern "C" int xba_gba_solve(const xba_gba_problem *q, const xba_gba_options *o,
         stats[XBA_GBA_ST_NSEG] = S.nseg;
         stats[XBA_GBA_ST_SEGLEN] = S.m;
         stats[XBA_GBA_ST_NNZB] = (double)P1.nnzb;
-        {
-            int ns = 0;
-            if (P1.rd) {
-                // disparity rows: d > 1e-6
-                std::vector<double> d(N1);
-                GCK(hipMemcpyAsync(d.data(), P1.rd, sizeof(double) * N1, hipMemcpyDeviceToHost, s));
-                GCK(hipStreamSynchronize(s));
-                for (double v : d) ns += v > 1e-6;
-            }
-            stats[XBA_GBA_ST_STEREO] = ns;
+        stats[XBA_GBA_ST_STEREO] = 0;
+        if (P1.rd && N1) {  // disparity rows: d > 1e-6, counted on the device
+            GCK(hipMemsetAsync(S.flag + 6, 0, sizeof(int32_t), s));
+            k_count_above<<<std::min(nblk(N1, 256), 1024), 256, 0, s>>>(N1, P1.rd, 1e-6, S.flag + 6);
+            stats[XBA_GBA_ST_STEREO] = S.read1(S.flag + 6);
         }
         int nh = 0;
         PassStats ps1;
