@@ -868,10 +868,11 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
                 while (l + nl < ch.l1 && nl < FB_LM) {
                     const int n = B.ob_start[l + nl + 1] - B.ob_start[l + nl];
                     const int k = std::min(n + 1, ch.R);
-                    if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP || PP + k * (k + 1) / 2 > FB_PAIRS)) break;
+                    const int pp = 1 + (k - 1) * (k - 2) / 2;  // the pair loop's blocks: (h,h) and target pairs
+                    if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP || PP + pp > FB_PAIRS)) break;
                     S += n;
                     K += k;
-                    PP += k * (k + 1) / 2;
+                    PP += pp;
                     nl++;
                     cost[ci] += (double)(n + 1) * (n + 1) + 4.0 * n + 8.0;
                 }

@@ -19,7 +19,7 @@ namespace xba {
 #define FB_HLP 64   // sum over a pass's landmarks of their touched slots
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
 #define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, masked-in flag
-#define FB_PAIRS 512  // sum over a pass's landmarks of k (k + 1) / 2 (their touched slot pairs)
+#define FB_PAIRS 512  // sum over a pass's landmarks of 1 + (k - 1)(k - 2) / 2 (their pair-loop blocks)
 enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_N = 10 };  // per-landmark accumulator: Hll (6), bl (3), active rows
 enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
 
@@ -37,7 +37,7 @@ struct FastLayout {
     int sc, sc64, wc, wd, tile, btile, ctl, wave, wave_stride, total;
     // offsets inside one wavefront's region
     int o_oblm, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
-        o_lrow, o_lh, o_kown, o_ptab;
+        o_lrow, o_lh, o_ptab;
 };
 
 // rts: bytes per record entry (the observation record's storage type); everything else is FP64.
@@ -76,7 +76,6 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
         L.o_lrow = L.o_hlp + FB_OBS * 8;
     }
     L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
-    L.o_kown = w;  w = align16(w + (trial ? 0 : FB_HLP));                  // per touched-slot entry: its landmark
     L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: first entry, si, sj
     L.wave_stride = w;
     L.total = o + nw * w;
@@ -357,7 +356,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
     double *lacc = (double *)(wb + FL.o_lrow);
     int *lh = (int *)(wb + FL.o_lh);
-    uint8_t *kown = (uint8_t *)(wb + FL.o_kown);
     uint16_t *ptab = (uint16_t *)(wb + FL.o_ptab);
     int *ctl = (int *)(smem + FL.ctl);
 
@@ -401,10 +399,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             WAVE_SYNC();
             // ---- observations (one lane each, kept in registers): whitened residual, Huber weight and cost,
-            //      whitened Jacobians; the row's own target blocks (t,t), (t,h) and target gradient straight into
-            //      the tile.  The per-landmark sums (Hll, bl, active rows, touched slots) and the host blocks (h,h)
-            //      and host gradient (one host slot per landmark) are summed over the landmark's lanes first and
-            //      written once per landmark by its last lane.
+            //      whitened Jacobians.  The per-landmark sums (Hll, bl, active rows, touched slots) and the host
+            //      blocks (h,h) and host gradient (one host slot per landmark) are summed over the landmark's lanes
+            //      and written once per landmark by its last lane.  The row's own target blocks (t,t), (t,h) and
+            //      target gradient are added later, together with their Schur parts.
             ObsRow<RT> ob;
             ob.zero();
             ob.t = 0;
@@ -421,46 +419,24 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     cq = obs_row<RT>(A, k, ob_h, A.lmp[3 * (size_t)l], A.lmp[3 * (size_t)l + 1],
                                      A.lmp[3 * (size_t)l + 2], sc64, sc, wc, wd, ob);
                 rcq[lane] = cq;
-                const double w = (double)ob.w;
-                const int t = ob.t, h = ob_h;
-                if (w != 0.0 && t != h) {
-                    double xt[2][6], xh[2][6];
-                    for (int r = 0; r < 2; r++)
-                        for (int j = 0; j < 6; j++) {
-                            xt[r][j] = (double)ob.jt[6 * r + j];
-                            xh[r][j] = ob.jh(r, j);
-                        }
-                    const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
-                    for (int j = 0; j < 6; j++) lds_add<double>(btile + 6 * t + j, xt[0][j] * e0 + xt[1][j] * e1);
-                    double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK;
-                    for (int i = 0; i < 6; i++)
-                        for (int j = 0; j <= i; j++)
-                            lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]));
-                    const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
-                    double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
-                    for (int i = 0; i < 6; i++)
-                        for (int j = 0; j < 6; j++) {
-                            const double v = tlo ? xh[0][i] * xt[0][j] + xh[1][i] * xt[1][j]
-                                                 : xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j];
-                            lds_add<double>(dth + 6 * i + j, w * v);
-                        }
-                }
             }
+            // the first active cross row of each (landmark, target slot) adds that slot's Schur parts
+            bool ob_first = false;
             {
                 // The lane sums below need finite values in every lane (lanes with nothing to add hold zeros).  A
                 // row with non-finite entries (from non-finite inputs) is left out of them and marks its landmark's
                 // Hll non-finite instead, so that the landmark fails the determinant guard as it would with the
-                // row summed in; its target terms above already carry the non-finite values into the tile.
+                // row summed in; its own target terms still carry the non-finite values into the tile.
                 const bool bad = ob.w != (RT)0 && !ob.finite();
-                if (bad) ob.zero();
-                const double w = (double)ob.w;
+                const double w = bad ? 0.0 : (double)ob.w;
                 const bool cross = w != 0.0 && ob.t != ob_h;
                 const bool head = seg_last;
                 double s[LA_N + 1];
                 {
-                    const double l00 = ob.jl[0], l01 = ob.jl[1], l02 = ob.jl[2];
-                    const double l10 = ob.jl[3], l11 = ob.jl[4], l12 = ob.jl[5];
-                    const double e0 = ob.e[0], e1 = ob.e[1];
+                    const double l00 = bad ? 0.0 : (double)ob.jl[0], l01 = bad ? 0.0 : (double)ob.jl[1];
+                    const double l02 = bad ? 0.0 : (double)ob.jl[2], l10 = bad ? 0.0 : (double)ob.jl[3];
+                    const double l11 = bad ? 0.0 : (double)ob.jl[4], l12 = bad ? 0.0 : (double)ob.jl[5];
+                    const double e0 = bad ? 0.0 : (double)ob.e[0], e1 = bad ? 0.0 : (double)ob.e[1];
                     s[LA_H + 0] = w * (l00 * l00 + l10 * l10);
                     s[LA_H + 1] = w * (l00 * l01 + l10 * l11);
                     s[LA_H + 2] = w * (l00 * l02 + l10 * l12);
@@ -475,7 +451,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 }
                 seg_sum(s, lane, seg_ss);
                 if (s[LA_N] != 0.0) s[LA_H] = (double)NAN;
+                // an inclusive scan over the landmark's lanes: its last lane holds the touched-slot mask, and a
+                // cross row is the first of its target slot when the scan up to the lane before it lacks that slot
                 const unsigned m = seg_or(cross ? (1u << ob.t) | (1u << ob_h) : 0u, lane, seg_ss);
+                const unsigned mprev = (unsigned)__builtin_amdgcn_ds_bpermute(((lane + XBA_WAVE - 1) % XBA_WAVE) << 2,
+                                                                              (int)m);
+                ob_first = cross && !(lane > seg_ss && ((mprev >> ob.t) & 1u));
                 if (head) {
                     double *la = lacc + LA_N * ob_bb;
                     for (int i = 0; i < LA_N; i++) la[i] = s[i];
@@ -485,7 +466,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 for (int r = 0; r < 2; r++)
                     for (int j = 0; j < 6; j++) xh[r][j] = cross ? ob.jh(r, j) : 0.0;
                 double g[6];
-                const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
+                const double e0 = cross ? w * (double)ob.e[0] : 0.0, e1 = cross ? w * (double)ob.e[1] : 0.0;
                 for (int j = 0; j < 6; j++) g[j] = xh[0][j] * e0 + xh[1][j] * e1;
                 seg_sum(g, lane, seg_ss);
                 if (head && m)
@@ -585,7 +566,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             const int koff = wave_incl_scan(kb, lane) - kb;
             const int ktot = __builtin_amdgcn_readlane(koff + kb, B - 1);
-            const int npb = lok ? kb * (kb + 1) / 2 : 0;  // a landmark that failed the det guard has no Schur part
+            // the pair loop's share of the landmark's Schur blocks: the host block (h,h) and the blocks of two
+            // different target slots; a landmark that failed the det guard has no Schur part, one without cross
+            // rows touches no slot
+            const int npb = (lok && kb > 0) ? 1 + (kb - 1) * (kb - 2) / 2 : 0;
             const int pof = wave_incl_scan(npb, lane) - npb;
             const int ptot = __builtin_amdgcn_readlane(pof + npb, B - 1);
             if (lane < B) {
@@ -595,12 +579,17 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 int i = 0;
                 while (m) {
                     slots[koff + i] = __ffs(m) - 1;
-                    kown[koff + i] = (uint8_t)lane;
                     m &= m - 1u;
                     i++;
                 }
-                for (int si = 0, o = pof; si < kb && lok; si++)
-                    for (int sj = 0; sj <= si; sj++, o++) ptab[o] = (uint16_t)((koff << 10) | (si << 5) | sj);
+                if (npb > 0) {
+                    const int he = __popc(msk & ((1u << lm_tile(pl_li)) - 1u));  // the host's touched-slot entry
+                    int o = pof;
+                    ptab[o++] = (uint16_t)((koff << 10) | (he << 5) | he);
+                    for (int si = 0; si < kb; si++)
+                        for (int sj = 0; sj < si; sj++)
+                            if (si != he && sj != he) ptab[o++] = (uint16_t)((koff << 10) | (si << 5) | sj);
+                }
             }
             const double pcost = wave_sum(lcost, lane);
             if (lane == 0) A.pass_cost[p] = pcost;
@@ -642,13 +631,65 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 if (seg_last && lr[LR_OK] != 0.0 && ((m >> h) & 1u)) {
                     T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
                     for (int i = 0; i < 18; i++) Uh[i] = uh[i];
+                    // the host gradient's Schur part -U_h^T g
+                    for (int j = 0; j < 6; j++)
+                        lds_add<double>(btile + 6 * h + j,
+                                        -(uh[j] * lr[LR_G] + uh[6 + j] * lr[LR_G + 1] + uh[12 + j] * lr[LR_G + 2]));
                 }
             }
             WAVE_SYNC();
-            // ---- reduced system: one lane per (landmark, touched block pair) of the landmarks that passed the det
-            //      guard (the table entry holds the landmark's first touched-slot entry and the pair), -U2^T U1 in
-            //      two halves of three rows held in registers, then the half's atomics (lower triangle only on the
-            //      diagonal blocks)
+            // ---- the rows' own blocks: every active cross row adds w Jt^T Jt into (t,t), its (t,h) block and
+            //      w Jt^T e into the gradient of t; the first row of each (landmark, target slot) of a landmark that
+            //      passed the det guard adds the slot's Schur parts with them: -U_t^T U_t, -U_t^T U_h (stored under
+            //      the larger slot, so -U_h^T U_t when t < h) and -U_t^T g
+            if (lane < nob && ob.w != (RT)0 && ob.t != ob_h) {
+                const int bb = ob_bb, t = ob.t, h = ob_h;
+                const T *lr = lrec + bb * FB_LREC;
+                double ut[18], uu[18], gl[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < 18; i++) ut[i] = uu[i] = 0.0;
+                if (ob_first && lr[LR_OK] != 0.0) {
+                    const unsigned m = lmask[bb];
+                    const T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
+                    const T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
+                    for (int i = 0; i < 18; i++) {
+                        ut[i] = Ut[i];
+                        uu[i] = Uh[i];
+                    }
+                    for (int c = 0; c < 3; c++) gl[c] = lr[LR_G + c];
+                }
+                const double w = (double)ob.w;
+                double xt[2][6], xh[2][6];
+                for (int r = 0; r < 2; r++)
+                    for (int j = 0; j < 6; j++) {
+                        xt[r][j] = (double)ob.jt[6 * r + j];
+                        xh[r][j] = ob.jh(r, j);
+                    }
+                const double e0 = w * (double)ob.e[0], e1 = w * (double)ob.e[1];
+                for (int j = 0; j < 6; j++)
+                    lds_add<double>(btile + 6 * t + j, (xt[0][j] * e0 + xt[1][j] * e1) -
+                                                           (ut[j] * gl[0] + ut[6 + j] * gl[1] + ut[12 + j] * gl[2]));
+                double *dtt = tile + (t * (t + 1) / 2 + t) * TBLK;
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j <= i; j++)
+                        lds_add<double>(dtt + 6 * i + j, w * (xt[0][i] * xt[0][j] + xt[1][i] * xt[1][j]) -
+                                                             (ut[i] * ut[j] + ut[6 + i] * ut[6 + j] +
+                                                              ut[12 + i] * ut[12 + j]));
+                const bool tlo = t < h;  // the (t,h) block is stored under the larger slot
+                double *dth = tile + (tlo ? h * (h + 1) / 2 + t : t * (t + 1) / 2 + h) * TBLK;
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) {
+                        const double v = tlo ? w * (xh[0][i] * xt[0][j] + xh[1][i] * xt[1][j]) -
+                                                   (uu[i] * ut[j] + uu[6 + i] * ut[6 + j] + uu[12 + i] * ut[12 + j])
+                                             : w * (xt[0][i] * xh[0][j] + xt[1][i] * xh[1][j]) -
+                                                   (ut[i] * uu[j] + ut[6 + i] * uu[6 + j] + ut[12 + i] * uu[12 + j]);
+                        lds_add<double>(dth + 6 * i + j, v);
+                    }
+            }
+            // ---- reduced system, the remaining blocks: one lane per (landmark, block pair) of the landmarks that
+            //      passed the det guard, the host block (h,h) and the blocks of two different target slots (the
+            //      table entry holds the landmark's first touched-slot entry and the pair), -U2^T U1 in two halves
+            //      of three rows held in registers, then the half's atomics (lower triangle only on the diagonal
+            //      blocks)
             for (int o = lane; o < ptot; o += XBA_WAVE) {
                 const int pk = ptab[o];
                 const int base = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
@@ -671,18 +712,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                             if (si != sj || jj <= i) lds_add<double>(dst + 6 * i + jj, acc[6 * ii + jj]);
                     }
                 }
-            }
-            // ---- gradient: one lane per (landmark, touched slot), its Schur part -U^T g (6 entries)
-            for (int o = lane; o < ktot; o += XBA_WAVE) {
-                const int bb = kown[o];
-                const T *lr = lrec + bb * FB_LREC;
-                if (lr[LR_OK] == 0.0) continue;
-                const int s = slots[o];
-                const T *U = hlp + o * 18;
-                T acc[6];
-                for (int i = 0; i < 6; i++)
-                    acc[i] = -(U[i] * lr[LR_G] + U[6 + i] * lr[LR_G + 1] + U[12 + i] * lr[LR_G + 2]);
-                for (int i = 0; i < 6; i++) lds_add<double>(btile + 6 * s + i, (double)acc[i]);
             }
             WAVE_SYNC();
         }
