@@ -251,6 +251,20 @@ DArgs dargs(xba_ctx *c) {
 bool mixed(const xba_ctx *c) { return c->opt.precision == XBA_MIXED; }
 
 // The distinct window slots landmark l touches (its host and the slots of its observations), ascending.
+// Number of distinct slots among landmark l's host and observation rows (an upper bound of the slots its rows touch
+// in the reduced system).
+int distinct_slots(const xba_batch &B, int l) {
+    int v[XBA_MAX_OBS_PER_LM + 1], n = 0;
+    v[n++] = B.lm_host[l];
+    for (int k = B.ob_start[l]; k < B.ob_start[l + 1] && n <= XBA_MAX_OBS_PER_LM; k++) {
+        const int s = B.ob_pose[k];
+        int i = 0;
+        while (i < n && v[i] != s) i++;
+        if (i == n) v[n++] = s;
+    }
+    return n;
+}
+
 std::vector<int> touched_slots(const xba_batch &B, int l) {
     std::vector<int> v(1, B.lm_host[l]);
     for (int k = B.ob_start[l]; k < B.ob_start[l + 1]; k++) v.push_back(B.ob_pose[k]);
@@ -867,7 +881,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
                 pass_l0.push_back(l);
                 while (l + nl < ch.l1 && nl < FB_LM) {
                     const int n = B.ob_start[l + nl + 1] - B.ob_start[l + nl];
-                    const int k = std::min(n + 1, ch.R);
+                    const int k = std::min(distinct_slots(B, l + nl), ch.R);
                     const int pp = 1 + (k - 1) * (k - 2) / 2;  // the pair loop's blocks: (h,h) and target pairs
                     if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP || PP + pp > FB_PAIRS)) break;
                     S += n;

@@ -18,10 +18,13 @@ namespace xba {
 #define FB_LM 32    // landmarks per pass
 #define FB_HLP 64   // sum over a pass's landmarks of their touched slots
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
-#define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, masked-in flag
+#define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, unique-target flag
 #define FB_PAIRS 512  // sum over a pass's landmarks of 1 + (k - 1)(k - 2) / 2 (their pair-loop blocks)
-enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_N = 10 };  // per-landmark accumulator: Hll (6), bl (3), active rows
-enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_MASKED = 10 };
+// per-landmark accumulator: Hll (6), bl (3), active rows, active rows off the host slot
+enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_NX = 10, LA_N = 11 };
+// LR_UNIQ: 1 when each of the landmark's active cross rows has a target slot of its own (then each row alone forms
+// its slot's U entry)
+enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_UNIQ = 10 };
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
 // plus a wait on the LDS counter makes every lane's earlier LDS writes visible to every lane's later reads.
@@ -447,6 +450,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     s[LA_B + 1] = w * (l01 * e0 + l11 * e1);
                     s[LA_B + 2] = w * (l02 * e0 + l12 * e1);
                     s[LA_NACT] = w != 0.0 ? 1.0 : 0.0;
+                    s[LA_NX] = cross ? 1.0 : 0.0;
                     s[LA_N] = bad ? 1.0 : 0.0;
                 }
                 seg_sum(s, lane, seg_ss);
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 const int l = lb + lane, li = pl_li;
                 T *lr = lrec + lane * FB_LREC;
                 lr[LR_OK] = 0.0;
-                lr[LR_MASKED] = 0.0;
+                lr[LR_UNIQ] = 0.0;
                 lh[lane] = lm_tile(li);
                 if ((li >> 12) & 1) {
                     const int q0 = lobs0[lane], n = lnobs[lane];
@@ -550,7 +554,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     }
                     lok = ok;
                     lr[LR_OK] = ok ? 1.0 : 0.0;
-                    lr[LR_MASKED] = 1.0;  // masked in: its pose-pose blocks are added even when the det guard fails
+                    lr[LR_UNIQ] = (int)la[LA_NX] == kb - 1 ? 1.0 : 0.0;
                     A.lmok[l] = ok ? 1 : 0;
                     if (A.ok_out) A.ok_out[l] = ok ? 1 : 0;
                     T *ho = A.lmHi + 6 * (size_t)l;
@@ -598,8 +602,13 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             //      target term is added into U of its target slot, the host terms are summed over the landmark's
             //      lanes and stored once into U of the host slot by the landmark's last lane (the slot's entry is
             //      its rank in the landmark's touched-slot mask)
-            for (int e = lane; e < ktot * 18; e += XBA_WAVE) hlp[e] = 0.0;
+            // the entries are summed with atomics from cleared storage only for landmarks whose rows share a target
+            // slot; otherwise every target entry is stored once by its row and the host entry by the last lane
+            if (__ballot(lane < B && lok && kb > 0 && lrec[lane * FB_LREC + LR_UNIQ] == 0.0) != 0)
+                for (int e = lane; e < ktot * 18; e += XBA_WAVE) hlp[e] = 0.0;
             WAVE_SYNC();
+            double ut[18];  // the row's term G Jt of its target slot's U entry (the whole entry when LR_UNIQ)
+            for (int i = 0; i < 18; i++) ut[i] = 0.0;
             {
                 double uh[18];
                 for (int i = 0; i < 18; i++) uh[i] = 0.0;
@@ -621,10 +630,14 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                             const T t0 = ob.jt[j], t1 = ob.jt[6 + j];
                             const T h0 = ob.jh(0, j), h1 = ob.jh(1, j);
                             for (int c = 0; c < 3; c++) {
-                                lds_add<double>(Ut + 6 * c + j, G[c][0] * t0 + G[c][1] * t1);
+                                ut[6 * c + j] = G[c][0] * t0 + G[c][1] * t1;
                                 uh[6 * c + j] = G[c][0] * h0 + G[c][1] * h1;
                             }
                         }
+                        if (lr[LR_UNIQ] != 0.0)
+                            for (int i = 0; i < 18; i++) Ut[i] = ut[i];
+                        else
+                            for (int i = 0; i < 18; i++) lds_add<double>(Ut + i, ut[i]);
                     }
                 }
                 seg_sum(uh, lane, seg_ss);
@@ -645,17 +658,19 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             if (lane < nob && ob.w != (RT)0 && ob.t != ob_h) {
                 const int bb = ob_bb, t = ob.t, h = ob_h;
                 const T *lr = lrec + bb * FB_LREC;
-                double ut[18], uu[18], gl[3] = {0.0, 0.0, 0.0};
-                for (int i = 0; i < 18; i++) ut[i] = uu[i] = 0.0;
+                double uu[18], gl[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < 18; i++) uu[i] = 0.0;
                 if (ob_first && lr[LR_OK] != 0.0) {
                     const unsigned m = lmask[bb];
-                    const T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
                     const T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
-                    for (int i = 0; i < 18; i++) {
-                        ut[i] = Ut[i];
-                        uu[i] = Uh[i];
+                    for (int i = 0; i < 18; i++) uu[i] = Uh[i];
+                    if (lr[LR_UNIQ] == 0.0) {  // the entry summed over the rows sharing the slot
+                        const T *Ut = hlp + (loff[bb] + __popc(m & ((1u << t) - 1u))) * 18;
+                        for (int i = 0; i < 18; i++) ut[i] = Ut[i];
                     }
                     for (int c = 0; c < 3; c++) gl[c] = lr[LR_G + c];
+                } else {
+                    for (int i = 0; i < 18; i++) ut[i] = 0.0;
                 }
                 const double w = (double)ob.w;
                 double xt[2][6], xh[2][6];
