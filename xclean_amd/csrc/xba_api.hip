@@ -882,7 +882,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
                 while (l + nl < ch.l1 && nl < FB_LM) {
                     const int n = B.ob_start[l + nl + 1] - B.ob_start[l + nl];
                     const int k = std::min(distinct_slots(B, l + nl), ch.R);
-                    const int pp = 1 + (k - 1) * (k - 2) / 2;  // the pair loop's blocks: (h,h) and target pairs
+                    const int pp = (k - 1) * (k - 2) / 2;  // the pair loop's pairs of target slots
                     if (nl > 0 && (S + n > FB_OBS || K + k > FB_HLP || PP + pp > FB_PAIRS)) break;
                     S += n;
                     K += k;

@@ -19,7 +19,7 @@ namespace xba {
 #define FB_HLP 64   // sum over a pass's landmarks of their touched slots
 #define TBLK 37     // tile block stride (doubles): 36 entries + 1 pad, spreads a wave's atomics over the banks
 #define FB_LREC 12  // per landmark (FP64): ok, L^-1 (00 10 11 20 21 22), g = L^-1 bl, unique-target flag
-#define FB_PAIRS 512  // sum over a pass's landmarks of 1 + (k - 1)(k - 2) / 2 (their pair-loop blocks)
+#define FB_PAIRS 512  // sum over a pass's landmarks of (k - 1)(k - 2) / 2 (their pairs of target slots)
 // per-landmark accumulator: Hll (6), bl (3), active rows, active rows off the host slot
 enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_NX = 10, LA_N = 11 };
 // LR_UNIQ: 1 when each of the landmark's active cross rows has a target slot of its own (then each row alone forms
@@ -40,7 +40,7 @@ struct FastLayout {
     int sc, sc64, wc, wd, tile, btile, ctl, wave, wave_stride, total;
     // offsets inside one wavefront's region
     int o_oblm, o_rcq, o_u, o_lmask, o_lobs0, o_lnobs, o_loff, o_poff, o_slots, o_hlp, o_lrec, o_ldbl,
-        o_lrow, o_lh, o_ptab;
+        o_lrow, o_ptab;
 };
 
 // rts: bytes per record entry (the observation record's storage type); everything else is FP64.
@@ -78,7 +78,6 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
         L.o_rcq = L.o_hlp;
         L.o_lrow = L.o_hlp + FB_OBS * 8;
     }
-    L.o_lh = w;    w = align16(w + FB_LM * 4);                             // per landmark: host slot in the tile
     L.o_ptab = w;  w = align16(w + (trial ? 0 : FB_PAIRS * 2));            // per slot pair: first entry, si, sj
     L.wave_stride = w;
     L.total = o + nw * w;
@@ -358,7 +357,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
     int *slots = (int *)(wb + FL.o_slots);
     T *hlp = (T *)(wb + FL.o_hlp), *lrec = (T *)(wb + FL.o_lrec);
     double *lacc = (double *)(wb + FL.o_lrow);
-    int *lh = (int *)(wb + FL.o_lh);
     uint16_t *ptab = (uint16_t *)(wb + FL.o_ptab);
     int *ctl = (int *)(smem + FL.ctl);
 
@@ -388,15 +386,16 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
             int obase;
             const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm, &obase);
+            const int pl_l = lb + lane;  // this lane's landmark (lane < B)
             // the landmark lanes' own loads, issued now so that they arrive during the observation phase, and their
             // accumulators cleared
             int pl_li = 0;
             double pl_rho = 0.0, pl_rm = 0.0, pl_ri = 0.0;
             if (lane < B) {
-                pl_li = A.lm_info[lb + lane];
-                pl_rho = A.lmp[3 * (size_t)(lb + lane) + 2];
-                pl_rm = A.rho_meas[lb + lane];
-                pl_ri = (double)A.rho_isig[lb + lane];
+                pl_li = A.lm_info[pl_l];
+                pl_rho = A.lmp[3 * (size_t)pl_l + 2];
+                pl_rm = A.rho_meas[pl_l];
+                pl_ri = (double)A.rho_isig[pl_l];
                 for (int i = 0; i < LA_N; i++) lacc[LA_N * lane + i] = 0.0;
                 lmask[lane] = 0u;
             }
@@ -425,6 +424,9 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             // the first active cross row of each (landmark, target slot) adds that slot's Schur parts
             bool ob_first = false;
+            // the host block (h,h) and host gradient of the landmark's rows, summed over its lanes (complete in its
+            // last lane, which adds them to the tile together with their Schur parts)
+            double hg[6], hh[21];
             {
                 // The lane sums below need finite values in every lane (lanes with nothing to add hold zeros).  A
                 // row with non-finite entries (from non-finite inputs) is left out of them and marks its landmark's
@@ -469,21 +471,12 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 double xh[2][6];
                 for (int r = 0; r < 2; r++)
                     for (int j = 0; j < 6; j++) xh[r][j] = cross ? ob.jh(r, j) : 0.0;
-                double g[6];
                 const double e0 = cross ? w * (double)ob.e[0] : 0.0, e1 = cross ? w * (double)ob.e[1] : 0.0;
-                for (int j = 0; j < 6; j++) g[j] = xh[0][j] * e0 + xh[1][j] * e1;
-                seg_sum(g, lane, seg_ss);
-                if (head && m)
-                    for (int j = 0; j < 6; j++) lds_add<double>(btile + 6 * ob_h + j, g[j]);
-                double hh[21];
+                for (int j = 0; j < 6; j++) hg[j] = xh[0][j] * e0 + xh[1][j] * e1;
+                seg_sum(hg, lane, seg_ss);
                 for (int i = 0, o = 0; i < 6; i++)
                     for (int j = 0; j <= i; j++, o++) hh[o] = w * (xh[0][i] * xh[0][j] + xh[1][i] * xh[1][j]);
                 seg_sum(hh, lane, seg_ss);
-                if (head && m) {
-                    double *dhh = tile + (ob_h * (ob_h + 1) / 2 + ob_h) * TBLK;
-                    for (int i = 0, o = 0; i < 6; i++)
-                        for (int j = 0; j <= i; j++, o++) lds_add<double>(dhh + 6 * i + j, hh[o]);
-                }
             }
             WAVE_SYNC();
             // ---- landmarks (one lane each): touched slots, Hll, bl, prior, damping, det guard, FP64 inverse,
@@ -493,11 +486,10 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             bool lok = false;
             unsigned msk = 0u;
             if (lane < B) {
-                const int l = lb + lane, li = pl_li;
+                const int l = pl_l, li = pl_li;
                 T *lr = lrec + lane * FB_LREC;
                 lr[LR_OK] = 0.0;
                 lr[LR_UNIQ] = 0.0;
-                lh[lane] = lm_tile(li);
                 if ((li >> 12) & 1) {
                     const int q0 = lobs0[lane], n = lnobs[lane];
                     for (int q = q0; q < q0 + n; q++) lcost += rcq[q];
@@ -570,10 +562,9 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
             }
             const int koff = wave_incl_scan(kb, lane) - kb;
             const int ktot = __builtin_amdgcn_readlane(koff + kb, B - 1);
-            // the pair loop's share of the landmark's Schur blocks: the host block (h,h) and the blocks of two
-            // different target slots; a landmark that failed the det guard has no Schur part, one without cross
-            // rows touches no slot
-            const int npb = (lok && kb > 0) ? 1 + (kb - 1) * (kb - 2) / 2 : 0;
+            // the pair loop's share of the landmark's Schur blocks: the blocks (s2, s1) of two different target slots
+            // s2 > s1; a landmark that failed the det guard has no Schur part, one without cross rows touches no slot
+            const int npb = (lok && kb > 0) ? (kb - 1) * (kb - 2) / 2 : 0;
             const int pof = wave_incl_scan(npb, lane) - npb;
             const int ptot = __builtin_amdgcn_readlane(pof + npb, B - 1);
             if (lane < B) {
@@ -589,7 +580,6 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 if (npb > 0) {
                     const int he = __popc(msk & ((1u << lm_tile(pl_li)) - 1u));  // the host's touched-slot entry
                     int o = pof;
-                    ptab[o++] = (uint16_t)((koff << 10) | (he << 5) | he);
                     for (int si = 0; si < kb; si++)
                         for (int sj = 0; sj < si; sj++)
                             if (si != he && sj != he) ptab[o++] = (uint16_t)((koff << 10) | (si << 5) | sj);
@@ -641,13 +631,22 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                     }
                 }
                 seg_sum(uh, lane, seg_ss);
-                if (seg_last && lr[LR_OK] != 0.0 && ((m >> h) & 1u)) {
-                    T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
-                    for (int i = 0; i < 18; i++) Uh[i] = uh[i];
-                    // the host gradient's Schur part -U_h^T g
+                if (seg_last && m != 0u) {
+                    // the landmark's host block and host gradient: the rows' sums, and for a landmark that passed the
+                    // det guard their Schur parts -U_h^T U_h and -U_h^T g (U_h is zero otherwise)
+                    const bool ok = lr[LR_OK] != 0.0;
+                    if (ok) {
+                        T *Uh = hlp + (loff[bb] + __popc(m & ((1u << h) - 1u))) * 18;
+                        for (int i = 0; i < 18; i++) Uh[i] = uh[i];
+                    }
+                    const double g0 = ok ? lr[LR_G] : 0.0, g1 = ok ? lr[LR_G + 1] : 0.0, g2 = ok ? lr[LR_G + 2] : 0.0;
                     for (int j = 0; j < 6; j++)
-                        lds_add<double>(btile + 6 * h + j,
-                                        -(uh[j] * lr[LR_G] + uh[6 + j] * lr[LR_G + 1] + uh[12 + j] * lr[LR_G + 2]));
+                        lds_add<double>(btile + 6 * h + j, hg[j] - (uh[j] * g0 + uh[6 + j] * g1 + uh[12 + j] * g2));
+                    double *dhh = tile + (h * (h + 1) / 2 + h) * TBLK;
+                    for (int i = 0, o = 0; i < 6; i++)
+                        for (int j = 0; j <= i; j++, o++)
+                            lds_add<double>(dhh + 6 * i + j,
+                                            hh[o] - (uh[i] * uh[j] + uh[6 + i] * uh[6 + j] + uh[12 + i] * uh[12 + j]));
                 }
             }
             WAVE_SYNC();
@@ -700,32 +699,25 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                         lds_add<double>(dth + 6 * i + j, v);
                     }
             }
-            // ---- reduced system, the remaining blocks: one lane per (landmark, block pair) of the landmarks that
-            //      passed the det guard, the host block (h,h) and the blocks of two different target slots (the
-            //      table entry holds the landmark's first touched-slot entry and the pair), -U2^T U1 in two halves
-            //      of three rows held in registers, then the half's atomics (lower triangle only on the diagonal
-            //      blocks)
+            // ---- reduced system, the remaining blocks: one lane per (landmark, pair of target slots s2 > s1) of the
+            //      landmarks that passed the det guard (the table entry holds the landmark's first touched-slot entry
+            //      and the pair); both U entries held in registers, -U2^T U1 formed one row at a time, each row's
+            //      atomics issued after its products
             for (int o = lane; o < ptot; o += XBA_WAVE) {
                 const int pk = ptab[o];
                 const int base = pk >> 10, si = (pk >> 5) & 31, sj = pk & 31;
                 const int s2 = slots[base + si], s1 = slots[base + sj];
                 const T *U2 = hlp + (base + si) * 18, *U1 = hlp + (base + sj) * 18;
-                T a1[18];
-                for (int e = 0; e < 18; e++) a1[e] = U1[e];
+                T u2[18], a1[18];
+                for (int e = 0; e < 18; e++) {
+                    u2[e] = U2[e];
+                    a1[e] = U1[e];
+                }
                 double *dst = tile + (s2 * (s2 + 1) / 2 + s1) * TBLK;
-                for (int hh = 0; hh < 2; hh++) {
-                    T acc[18];
-                    for (int ii = 0; ii < 3; ii++) {
-                        const int i = 3 * hh + ii;
-                        const T c0 = U2[i], c1 = U2[6 + i], c2 = U2[12 + i];
-                        for (int jj = 0; jj < 6; jj++)
-                            acc[6 * ii + jj] = -(c0 * a1[jj] + c1 * a1[6 + jj] + c2 * a1[12 + jj]);
-                    }
-                    for (int ii = 0; ii < 3; ii++) {
-                        const int i = 3 * hh + ii;
-                        for (int jj = 0; jj < 6; jj++)
-                            if (si != sj || jj <= i) lds_add<double>(dst + 6 * i + jj, acc[6 * ii + jj]);
-                    }
+                for (int i = 0; i < 6; i++) {
+                    const T c0 = u2[i], c1 = u2[6 + i], c2 = u2[12 + i];
+                    for (int jj = 0; jj < 6; jj++)
+                        lds_add<double>(dst + 6 * i + jj, -(c0 * a1[jj] + c1 * a1[6 + jj] + c2 * a1[12 + jj]));
                 }
             }
             WAVE_SYNC();
