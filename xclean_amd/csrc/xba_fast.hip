@@ -128,15 +128,41 @@ __device__ inline void lds_add(T *p, T v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// One pass (the B landmarks from lb, planned on the host so that their observations fit one lane each and their
-// touched slots fit the pass's U storage): per landmark its first record row and observation count, per record row
-// its landmark.  Returns the pass's observation count.
-__device__ inline int load_pass(const int32_t *ob_start, int lb, int B, int lane, int *lobs0, int *lnobs, int *ob_lm,
-                                int *base) {
-    const int l = lb + lane;
+// A pass's plan (the B landmarks from lb, planned on the host so that their observations fit one lane each and
+// their touched slots fit the pass's U storage) and its landmarks' row offsets (lane < B: o0, o1 = ob_start of
+// landmark lb + lane and the next), loaded one pass ahead: each wavefront takes its next pass and issues these loads
+// while it works on the current one.
+struct PassAhead {
+    int p, lb, B, o0, o1;
+};
+
+__device__ inline int next_pass(int *ctl, int lane);
+
+// Takes the wavefront's next pass of the work item and issues the loads of its plan entries (the scalar part; the
+// row offsets follow in pass_rows once the plan has arrived).
+__device__ inline void pass_take(const KArgs<double> &A, const Chunk &ch, int *ctl, int lane, PassAhead &n) {
+    n.p = next_pass(ctl, lane);
+    n.lb = 0;
+    n.B = 0;
+    if (n.p < ch.p1) {
+        n.lb = A.pass_l0[n.p];
+        n.B = (n.p + 1 < ch.p1 ? A.pass_l0[n.p + 1] : ch.l1) - n.lb;
+    }
+}
+
+__device__ inline void pass_rows(const int32_t *ob_start, int lane, PassAhead &n) {
+    const bool in = lane < n.B;
+    n.o0 = in ? ob_start[n.lb + lane] : 0;
+    n.o1 = in ? ob_start[n.lb + lane + 1] : 0;
+}
+
+// One pass from its plan: per landmark its first record row and observation count, per record row its landmark.
+// Returns the pass's observation count.
+__device__ inline int load_pass(const PassAhead &pa, int lane, int *lobs0, int *lnobs, int *ob_lm, int *base) {
+    const int B = pa.B;
     const bool in = lane < B;
-    const int o0 = in ? ob_start[l] : 0;
-    const int n = in ? (ob_start[l + 1] - o0) : 0;
+    const int o0 = pa.o0;
+    const int n = in ? (pa.o1 - o0) : 0;
     const int S = wave_incl_scan(n, lane);
     if (in) {
         lobs0[lane] = S - n;
@@ -379,13 +405,17 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
         for (int i = tid; i < nblk * TBLK; i += blockDim.x) tile[i] = 0.0;
         for (int i = tid; i < 6 * R; i += blockDim.x) btile[i] = 0.0;
         __syncthreads();
-        // the chunk's passes, taken by the wavefronts in turn
+        // the chunk's passes, taken by the wavefronts in turn (each one pass ahead)
+        PassAhead pa;
+        pass_take(A, ch, ctl, lane, pa);
+        pass_rows(A.ob_start, lane, pa);
         for (;;) {
-            const int p = next_pass(ctl, lane);
+            const int p = pa.p;
             if (p >= ch.p1) break;
-            const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
+            const int lb = pa.lb, B = pa.B;
             int obase;
-            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm, &obase);
+            const int nob = load_pass(pa, lane, lobs0, lnobs, ob_lm, &obase);
+            pass_take(A, ch, ctl, lane, pa);  // the next pass; its row offsets are loaded after the next wave sync
             const int pl_l = lb + lane;  // this lane's landmark (lane < B)
             // the landmark lanes' own loads, issued now so that they arrive during the observation phase, and their
             // accumulators cleared
@@ -400,6 +430,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
                 lmask[lane] = 0u;
             }
             WAVE_SYNC();
+            pass_rows(A.ob_start, lane, pa);
             // ---- observations (one lane each, kept in registers): whitened residual, Huber weight and cost,
             //      whitened Jacobians.  The per-landmark sums (Hll, bl, active rows, touched slots) and the host
             //      blocks (h,h) and host gradient (one host slot per landmark) are summed over the landmark's lanes
@@ -785,11 +816,14 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
         for (int s = tid; s < R; s += blockDim.x) dir_slot<T>(dps + 6 * s, sc + s * SC_STRIDE, wc);
         __syncthreads();
         for (;;) {
-            const int p = next_pass(ctl, lane);
+            PassAhead pa;
+            pass_take(A, ch, ctl, lane, pa);
+            const int p = pa.p;
             if (p >= ch.p1) break;
-            const int lb = A.pass_l0[p], B = (p + 1 < ch.p1 ? A.pass_l0[p + 1] : ch.l1) - lb;
+            pass_rows(A.ob_start, lane, pa);
+            const int lb = pa.lb, B = pa.B;
             int obase;
-            const int nob = load_pass(A.ob_start, lb, B, lane, lobs0, lnobs, ob_lm, &obase);
+            const int nob = load_pass(pa, lane, lobs0, lnobs, ob_lm, &obase);
             WAVE_SYNC();
             // ---- observations at the current state: u_q = w Jl^T (Jt dp_t + Jh dp_h)
             int q_l = 0, q_k = 0, q_t = 0, q_h = 0, q_cam = 0, q_valid = 0;
