@@ -115,6 +115,8 @@ typedef struct xba_options {
     int32_t big_n;              /* windows whose reduced system is larger than big_n are factored by the tiled
                                    multi-workgroup Cholesky (64 x 64 tiles, trailing updates on MFMA); smaller ones
                                    by one workgroup each.  0 = automatic (192); -1 = never tiled */
+    int32_t big_fp64;           /* mixed precision: 1 factors the tiled windows in FP64 (one solve, no refinement)
+                                   instead of FP32 with FP64 refinement; ignored in FP64 */
 } xba_options;
 
 typedef struct xba_ctx xba_ctx;

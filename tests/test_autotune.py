@@ -8,8 +8,9 @@ from xclean_amd import autotune as AT
 
 
 def test_coordinate_descent_finds_the_separable_optimum_and_skips_rejected_plans():
-    best = dict(slot_tile=12, lm_per_chunk=64, big_n=128, grid=0)
-    space = dict(slot_tile=[0, 8, 12, 20], lm_per_chunk=[0, 64, 256], big_n=[0, 128, -1], grid=[0, 512])
+    best = dict(slot_tile=12, lm_per_chunk=64, big_n=128, big_fp64=1, grid=0)
+    space = dict(slot_tile=[0, 8, 12, 20], lm_per_chunk=[0, 64, 256], big_n=[0, 128, -1], big_fp64=[0, 1],
+                 grid=[0, 512])
     seen = []
 
     def measure(o):
@@ -35,6 +36,9 @@ def test_candidate_space_holds_the_defaults_and_only_moving_thresholds():
     for k in AT.ORDER:
         assert sp[k][0] == 0
     assert sp["big_n"] == [0, 64, 96, 112, 128, 144, 160]    # 256 / 384 / 512 / never change nothing here
+    assert sp["big_fp64"] == [0, 1]
     sp2 = AT.candidate_space(nlm=200_000, ncu=256, ns=np.array([900]))
     assert sp2["big_n"] == [0, -1]                           # the tiled solve or one workgroup
+    assert AT.candidate_space(nlm=200_000, ncu=256, ns=np.array([900]), mixed=False)["big_fp64"] == [0]
+    assert AT.candidate_space(nlm=2_000, ncu=256, ns=np.array([30, 60]))["big_fp64"] == [0]  # nothing tiled
     assert all(8 <= v <= 4096 for v in sp2["lm_per_chunk"][1:])

@@ -36,7 +36,7 @@ class XbaBatch(C.Structure):
 class XbaOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("lm_per_chunk", C.c_int32), ("slot_tile", C.c_int32),
                 ("grid", C.c_int32), ("use_graph", C.c_int32), ("want_chi2", C.c_int32),
-                ("deterministic", C.c_int32), ("big_n", C.c_int32)]
+                ("deterministic", C.c_int32), ("big_n", C.c_int32), ("big_fp64", C.c_int32)]
 
 
 class XbaNpProblem(C.Structure):

@@ -7,6 +7,8 @@ Options tuned (``xba_options`` in include/xba.h):
   lm_per_chunk  landmarks per work item (0: the solver's rule, landmarks / (4 x compute units))
   big_n         reduced-system size above which a window's Cholesky is tiled over many workgroups (64 x 64 tiles,
                 trailing updates on the matrix cores) instead of one workgroup per window
+  big_fp64      mixed precision: the tiled windows factored in FP64 (one solve) instead of FP32 with FP64
+                refinement (several solves)
   grid          persistent grid of the assembly kernels (0: one workgroup per compute unit)
 
 Search: coordinate descent over the options in that order, starting from the solver's defaults.  A candidate is
@@ -25,13 +27,13 @@ import time
 import numpy as np
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), ".autotune_cache.json")
-ORDER = ("slot_tile", "lm_per_chunk", "big_n", "grid")
-DEFAULTS = dict(slot_tile=0, lm_per_chunk=0, big_n=0, grid=0)
+ORDER = ("slot_tile", "lm_per_chunk", "big_n", "big_fp64", "grid")
+DEFAULTS = dict(slot_tile=0, lm_per_chunk=0, big_n=0, big_fp64=0, grid=0)
 
 
-def candidate_space(nlm, ncu, ns):
+def candidate_space(nlm, ncu, ns, mixed=True):
     """Candidate values per option for a batch of ``nlm`` landmarks whose reduced systems have sizes ``ns``, on a
-    GPU with ``ncu`` compute units.  0 is the solver's own default in every list."""
+    GPU with ``ncu`` compute units, in mixed precision or FP64.  0 is the solver's own default in every list."""
     auto_lpc = max(8, min(4096, int(round(nlm / (4.0 * ncu)))))
     ns = np.asarray(ns)
     big = [0]
@@ -44,6 +46,8 @@ def candidate_space(nlm, ncu, ns):
         lm_per_chunk=[0] + sorted({max(8, auto_lpc // 2), min(4096, auto_lpc * 2), min(4096, auto_lpc * 4)}
                                   - {auto_lpc}),
         big_n=big,
+        # the tiled windows' factor precision matters only in mixed precision and when some window can be tiled
+        big_fp64=[0, 1] if (mixed and len(big) > 1 or (mixed and (ns > 192).any())) else [0],
         grid=[0, 2 * ncu],
     )
 
@@ -107,7 +111,7 @@ def tune(solver, repeats=3, cache_path=DEFAULT_CACHE, use_cache=True, space=None
         return dict(db[key], cached=True, seconds=0.0)
     if space is None:
         ns = np.diff(solver.win_n0)
-        space = candidate_space(solver.nlm, props.multi_processor_count, ns)
+        space = candidate_space(solver.nlm, props.multi_processor_count, ns, mixed=solver.prec != 0)
     t0 = time.time()
 
     def measure(opts):

@@ -100,8 +100,8 @@ struct xba_ctx {
     int nchunks = 0, Rmax = 1, maxn = 0, grid = 1;
     int64_t nstate = 0, nh = 0, npartH = 0, npartb = 0;
     int lin_lds = 0, solve_lds_cap = 0, fast_nw = 8, fast_nw_trial = 8, fast_lds = 0, fast_lds_trial = 0;
-    int nnarrow = 0, Rnarrow = 1, Rwide = 1, wide_nw = 8, wide_nw_trial = 8, wide_lds = 0, wide_lds_trial = 0, grid_wide = 1;
-    int grid_trial = 1, grid_wide_trial = 1;  // the trial kernel runs up to XBA_TRIAL_PER_CU workgroups per CU
+    int nnarrow = 0, Rnarrow = 1, Rwide = 1, wide_nw = 8, wide_lds = 0, grid_wide = 1;
+    int grid_trial = 1, Rtrial = 1;  // the trial kernel runs up to XBA_TRIAL_PER_CU workgroups per CU
     int npass = 0;
     int32_t *pass_l0 = nullptr, *sched = nullptr, *sched_ctr = nullptr;
     double *pass_cost = nullptr;
@@ -132,6 +132,7 @@ struct xba_ctx {
     float *fact32 = nullptr;
     // windows factored by the tiled Cholesky
     int nbig = 0, big_npmax = 0, big_nbmax = 0, big_nmax = 0;
+    bool big64 = false;  // mixed precision with the tiled windows factored in FP64
     BigWin *bigs = nullptr;
     float *bigF32 = nullptr;
     double *bigF64 = nullptr, *bigS = nullptr, *bigV = nullptr;
@@ -235,6 +236,7 @@ DArgs dargs(xba_ctx *c) {
     D.partH64 = c->partH64;
     D.partb64 = c->partb64;
     D.nbig = c->nbig;
+    D.big64 = c->big64 ? 1 : 0;
     D.bigs = c->bigs;
     D.bigF32 = c->bigF32;
     D.bigF64 = c->bigF64;
@@ -336,18 +338,15 @@ void launch_lin_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = 
     }
 }
 
+// The trial: one launch over every work item (narrow and wide), handed out largest first.  `part` 1 (the profile's
+// wide share) launches nothing.
 template <typename T>
 void launch_trial_kernel(xba_ctx *c, const KArgs<T> &A, hipStream_t s, int part = -1) {
     if (c->fast) {
         const KArgs<double> F = fast_args<T>(c, A);
-        const bool fork = fork_wide(c, s, part);
-        if (c->nnarrow > 0 && part != 1)
+        if (c->nchunks > 0 && part != 1)
             hipLaunchKernelGGL((k_trial_fast<0>), dim3(c->grid_trial), dim3(XBA_WAVE * c->fast_nw_trial),
-                               c->fast_lds_trial, s, F, c->Rnarrow, 0, c->nnarrow, 2);
-        if (c->nchunks > c->nnarrow && part != 0)
-            hipLaunchKernelGGL((k_trial_fast<1>), dim3(c->grid_wide_trial), dim3(XBA_WAVE * c->wide_nw_trial),
-                               c->wide_lds_trial, fork ? c->side : s, F, c->Rwide, c->nnarrow, c->nchunks, 3);
-        if (fork) join_wide(c, s);
+                               c->fast_lds_trial, s, F, c->Rtrial, c->nchunks, 2 * c->nchunks, 2);
     } else if (part != 1) {
         hipLaunchKernelGGL(k_trial<T>, dim3(c->grid), dim3(XBA_WAVE), c->lin_lds, s, A, c->Rmax);
     }
@@ -375,6 +374,26 @@ void launch_lin(xba_ctx *c, const KArgs<T> &A, hipStream_t s, double *imu_chi2 =
     launch_terms(c, D, s, 0);
 }
 
+// The tiled Cholesky sequence of the large windows in precision T (FP64: one solve; FP32: refined in FP64).
+template <typename T>
+void launch_big(xba_ctx *c, const DArgs &D, hipStream_t s) {
+    hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
+    for (int k = 0; k < c->big_nbmax; k++) {
+        const int below = c->big_nbmax - k - 1;
+        hipLaunchKernelGGL(k_big_panel<T>, dim3(1 + (below + 3) / 4, c->nbig), dim3(256), 0, s, D, k);
+        if (below > 0)
+            hipLaunchKernelGGL(k_big_update<T>, dim3(below * (below + 1) / 2, c->nbig), dim3(256), 0, s, D, k);
+    }
+    const int maxit = sizeof(T) == 8 ? 1 : D.nref_max;
+    for (int it = 0; it < maxit; it++) {
+        hipLaunchKernelGGL(k_big_trsv<T>, dim3(c->nbig), dim3(1024), 0, s, D, it);
+        if (it + 1 < maxit)
+            hipLaunchKernelGGL(k_big_resid, dim3((c->big_nmax + XBA_RES_ROWS - 1) / XBA_RES_ROWS, c->nbig),
+                               dim3(256), 0, s, D);
+    }
+    hipLaunchKernelGGL(k_big_finish, dim3(c->nbig), dim3(256), 0, s, D);
+}
+
 // The damped solve and trial state of every window: the tiled Cholesky sequence for the large windows, one
 // workgroup per window for the others.
 template <typename T>
@@ -387,21 +406,10 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
         (void)hipEventRecord(c->ev_join, c->side);
     }
     if (c->nbig > 0) {
-        hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
-        for (int k = 0; k < c->big_nbmax; k++) {
-            const int below = c->big_nbmax - k - 1;
-            hipLaunchKernelGGL(k_big_panel<T>, dim3(1 + (below + 3) / 4, c->nbig), dim3(256), 0, s, D, k);
-            if (below > 0)
-                hipLaunchKernelGGL(k_big_update<T>, dim3(below * (below + 1) / 2, c->nbig), dim3(256), 0, s, D, k);
-        }
-        const int maxit = sizeof(T) == 8 ? 1 : D.nref_max;
-        for (int it = 0; it < maxit; it++) {
-            hipLaunchKernelGGL(k_big_trsv<T>, dim3(c->nbig), dim3(1024), 0, s, D, it);
-            if (it + 1 < maxit)
-                hipLaunchKernelGGL(k_big_resid, dim3((c->big_nmax + XBA_RES_ROWS - 1) / XBA_RES_ROWS, c->nbig),
-                                   dim3(256), 0, s, D);
-        }
-        hipLaunchKernelGGL(k_big_finish, dim3(c->nbig), dim3(256), 0, s, D);
+        if (sizeof(T) == 8 || c->big64)
+            launch_big<double>(c, D, s);
+        else
+            launch_big<float>(c, D, s);
     }
     if (fork)
         (void)hipStreamWaitEvent(s, c->ev_join, 0);
@@ -894,11 +902,15 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             }
             ch.p1 = (int)pass_l0.size();
         }
-        sched.resize(chunks.size());
-        for (size_t ci = 0; ci < chunks.size(); ci++) sched[ci] = (int32_t)ci;
+        // the assembly launches hand out each class (narrow, wide) largest first; the trial launch, which takes
+        // every work item, hands them all out largest first from the second half of the list
+        const size_t nc = chunks.size();
+        sched.resize(2 * nc);
+        for (size_t ci = 0; ci < nc; ci++) sched[ci] = sched[nc + ci] = (int32_t)ci;
         auto by_cost = [&](int32_t a, int32_t b) { return cost[a] > cost[b] || (cost[a] == cost[b] && a < b); };
         std::sort(sched.begin(), sched.begin() + nn, by_cost);
-        std::sort(sched.begin() + nn, sched.end(), by_cost);
+        std::sort(sched.begin() + nn, sched.begin() + nc, by_cost);
+        std::sort(sched.begin() + nc, sched.end(), by_cost);
     }
     c->npass = (int)pass_l0.size();
     // tile slot of every observation and landmark host inside its work item (the multi-wave kernels read them from
@@ -935,8 +947,9 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
             ldst = fast_layout(R, tsz, nwt, true).total;
             return true;
         };
-        if (!pick(c->Rnarrow, c->fast_nw, c->fast_lds, c->fast_nw_trial, c->fast_lds_trial)) return fail(XBA_E_RANGE);
-        if (!pick(c->Rwide, c->wide_nw, c->wide_lds, c->wide_nw_trial, c->wide_lds_trial)) return fail(XBA_E_RANGE);
+        int nwt_, ldst_;  // (the trial's shape is picked below, for every work item at once)
+        if (!pick(c->Rnarrow, c->fast_nw, c->fast_lds, nwt_, ldst_)) return fail(XBA_E_RANGE);
+        if (!pick(c->Rwide, c->wide_nw, c->wide_lds, nwt_, ldst_)) return fail(XBA_E_RANGE);
         {  // the mixed fallback's re-assembly: one launch over every work item, tile of the widest
             int nwt, ldst;
             c->fb_R = std::max(c->Rnarrow, c->Rwide);
@@ -945,10 +958,16 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         }
         per_cu = 1;
         c->grid_wide = std::max(1, std::min(c->nchunks - c->nnarrow, ncu));
-        // k_trial_fast is compiled for 4 waves per SIMD: two 8-wave workgroups per CU where the LDS holds both
+        // the trial: one launch over every work item with the widest tile (its LDS is small: the slot poses, steps
+        // and per-wavefront row arrays); k_trial_fast is compiled for 4 waves per SIMD, two 8-wave workgroups per CU
+        // where the LDS holds both
+        {
+            int nw, lds;
+            c->Rtrial = std::max(c->Rnarrow, c->Rwide);
+            if (!pick(c->Rtrial, nw, lds, c->fast_nw_trial, c->fast_lds_trial)) return fail(XBA_E_RANGE);
+        }
         auto tpc = [&](int lds) { return std::max(1, std::min(XBA_TRIAL_PER_CU, lds_max / std::max(lds, 1))); };
-        c->grid_wide_trial = std::max(1, std::min(c->nchunks - c->nnarrow, ncu * tpc(c->wide_lds_trial)));
-        c->grid_trial = std::max(1, std::min(c->nnarrow, ncu * tpc(c->fast_lds_trial)));
+        c->grid_trial = std::max(1, std::min(c->nchunks, ncu * tpc(c->fast_lds_trial)));
     } else {
         per_cu = std::max(1, std::min(8, lds_max / std::max(c->lin_lds, 1)));
     }
@@ -1023,8 +1042,9 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->imu_color = dalloc<int32_t>(std::max(c->nimu, 1), O);
     c->wstate = dalloc<double>((size_t)NWS * nwin, O);
     c->bigs = dalloc<BigWin>(bigs.size(), O);
-    c->bigF32 = (mx && c->nbig) ? dalloc<float>(bigF, O) : dalloc<float>(1, O);
-    c->bigF64 = (!mx && c->nbig) ? dalloc<double>(bigF, O) : dalloc<double>(1, O);
+    c->big64 = mx && opt->big_fp64 != 0;
+    c->bigF32 = (mx && !c->big64 && c->nbig) ? dalloc<float>(bigF, O) : dalloc<float>(1, O);
+    c->bigF64 = ((!mx || c->big64) && c->nbig) ? dalloc<double>(bigF, O) : dalloc<double>(1, O);
     c->bigS = dalloc<double>(bigSv, O);
     c->bigV = dalloc<double>(3 * bigSv, O);
     for (void *p : O)
@@ -1062,7 +1082,6 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
         HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         HIPCK(hipFuncSetAttribute((const void *)k_lin_fast<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        HIPCK(hipFuncSetAttribute((const void *)k_trial_fast<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
     if (!c->fast && c->lin_lds > 64 * 1024) {  // the single-wavefront kernels' tile (their launch only)
         if (c->lin_lds > 160 * 1024) return fail(XBA_E_RANGE);

@@ -458,8 +458,8 @@ __global__ __launch_bounds__(256) void k_big_finish(DArgs D) {
     const int n = B.n, tid = threadIdx.x, nt = blockDim.x;
     double *ws = D.wstate + (size_t)NWS * B.win;
     const bool ok = ws[WS_BIGOK] != 0.0;
-    // mixed: converged by the residual or the correction test; FP64 (one solve, no refinement): always
-    const bool conv = !D.fallback_on || ws[WS_BIGCONV] != 0.0;
+    // mixed: converged by the residual or the correction test; an FP64 factor (one solve, no refinement): always
+    const bool conv = !D.fallback_on || D.big64 || ws[WS_BIGCONV] != 0.0;
     const bool fb = D.fallback_on && !(ok && conv);  // left to the FP64 fallback (k_solve<double>, fallback mode)
     const double *x = D.bigV + B.v0;
     if (!fb) {

@@ -265,6 +265,7 @@ struct DArgs {
     double *bigF64;                       // tiled factors (sum np^2), FP64
     double *bigS;                         // (sum np) Jacobi scaling of the tiled factors
     double *bigV;                         // (sum 3 np) refinement vectors of the tiled solves
+    int32_t big64;                        // mixed precision: the tiled windows are factored in FP64 (one solve)
     const int32_t *chunk_slots;           // tile slot lists of the work items with smap >= 0
     // reduction lists: for every slot pair (s2 >= s1) of a window, the partial blocks (index into the partial
     // buffer in units of 36 entries) that hold it, in work-item order; for every slot, its partial gradient

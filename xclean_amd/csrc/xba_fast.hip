@@ -958,6 +958,5 @@ template __global__ void k_lin_fast<float, 1>(KArgs<double>, int, int, int, int)
 template __global__ void k_lin_fast<double, 0>(KArgs<double>, int, int, int, int);
 template __global__ void k_lin_fast<double, 1>(KArgs<double>, int, int, int, int);
 template __global__ void k_trial_fast<0>(KArgs<double>, int, int, int, int);
-template __global__ void k_trial_fast<1>(KArgs<double>, int, int, int, int);
 
 }  // namespace xba

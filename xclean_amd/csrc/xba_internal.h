@@ -78,8 +78,10 @@ enum {
 // right-hand-side maxima) at v0 of the refinement buffer (3 np entries).
 #define XBA_TILE 64
 // Mixed-precision refinement stops once the last FP64-accumulated correction is at most this fraction of the
-// solution (max norms), or once the FP64 residual is at most 1e-15 of the right-hand side.
-#define XBA_REFINE_TOL 1e-10
+// solution (max norms), or once the FP64 residual is at most 1e-15 of the right-hand side.  The error left after
+// that solve is the correction times the refinement's contraction factor (below 1 whenever it converges), so at
+// most 1e-8 of the solution, two orders below the mixed path's 1e-6 error target.
+#define XBA_REFINE_TOL 1e-8
 #define XBA_RES_ROWS 4  // rows per workgroup of the refinement residual
 struct BigWin {
     int32_t win, n, np, nb;  // window, state dimension, padded dimension, tiles per side (np / XBA_TILE)

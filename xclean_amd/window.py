@@ -145,9 +145,10 @@ class BatchSolver:
     """Solve many windows at once on the GPU.  ``problems`` is a list of ``Problem``s (mutated by ``download``)."""
 
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
-                 device="cuda", deterministic=False, with_weights=False, big_n=0, tuned=None):
-        """``tuned``: a dict of plan options (lm_per_chunk, slot_tile, grid, big_n) chosen by
-        ``xclean_amd.autotune``, overriding the individual arguments."""
+                 device="cuda", deterministic=False, with_weights=False, big_n=0, big_fp64=0, tuned=None):
+        """``tuned``: a dict of plan options (lm_per_chunk, slot_tile, grid, big_n, big_fp64) chosen by
+        ``xclean_amd.autotune``, overriding the individual arguments.  ``big_fp64`` (mixed precision): the windows
+        above ``big_n`` are factored in FP64 instead of FP32 with FP64 refinement."""
         import torch
         _lib.require_gpu()
         self.torch = torch
@@ -236,7 +237,7 @@ class BatchSolver:
         self.use_graph = bool(use_graph)
         self.deterministic = bool(deterministic)
         self.ctx = None
-        plan = dict(lm_per_chunk=lm_per_chunk, slot_tile=slot_tile, grid=grid, big_n=big_n)
+        plan = dict(lm_per_chunk=lm_per_chunk, slot_tile=slot_tile, grid=grid, big_n=big_n, big_fp64=big_fp64)
         plan.update(tuned or {})
         self.replan(**plan)
         ns, nh = C.c_int64(), C.c_int64()
@@ -245,16 +246,17 @@ class BatchSolver:
         self.nobs = int(self.win_obs0[-1])
         self.nlm = int(self.win_lm0[-1])
 
-    def replan(self, lm_per_chunk=0, slot_tile=0, grid=0, big_n=0):
+    def replan(self, lm_per_chunk=0, slot_tile=0, grid=0, big_n=0, big_fp64=0):
         """(Re)create the solver context with these plan options over the same device data (the state is
         reloaded by the next ``upload``)."""
         self._destroy()
         o = _lib.XbaOptions()
         o.precision, o.use_graph, o.deterministic = self.prec, int(self.use_graph), int(self.deterministic)
         o.lm_per_chunk, o.slot_tile, o.grid, o.big_n = int(lm_per_chunk), int(slot_tile), int(grid), int(big_n)
+        o.big_fp64 = int(big_fp64)
         self.opts = o
         self.plan_options = dict(lm_per_chunk=int(lm_per_chunk), slot_tile=int(slot_tile), grid=int(grid),
-                                 big_n=int(big_n))
+                                 big_n=int(big_n), big_fp64=int(big_fp64))
         ctx = C.c_void_p()
         _lib.check(self.L.xba_create(C.byref(ctx), C.byref(self.batch), C.byref(o)), "xba_create")
         self.ctx = ctx
