@@ -588,17 +588,29 @@ def main():
     torch.cuda.synchronize(dev)
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / max(args.steps, 1)
+    # the dominant kernel's launch duration: the same steps replayed at once after the timed region, with HIP events
+    # on the solver's stream around the assembly launch of every LM iteration (the iteration then runs as two
+    # graphs, the assembly and the rest, which costs ~1 % of the step: not in the timed region above)
+    bs.set_assembly_timing(True)
+    bs.assembly_times()
+    for _ in range(args.steps):
+        step()
+    asm_ms, asm_n = bs.assembly_times()
+    bs.set_assembly_timing(False)
     value = M * LM_ITERS / (ms_step * 1e-3)
     log("bench: %.3f ms per step (%d LM iterations), %.3e obs/s per LM iteration" % (ms_step, LM_ITERS, value))
     st = bs.stats()
 
-    # per-kernel durations of one iteration (events on the solver's stream), from the initial state
+    # per-kernel durations of one iteration (events on the solver's stream around each launch, eager), each sample
+    # queued right behind three replayed iterations so that the GPU's clocks are those of the steady state
     prof = []
-    for _ in range(3):
+    for _ in range(5):
         bs.upload(1e-4)
+        bs.iterate(3)
         prof.append(bs.profile_iteration())
     kern = {k: float(np.mean([p[k] for p in prof])) for k in prof[0]}
     log("bench: kernel ms", kern)
+    log("bench: assembly launch over the replayed steps: %.4f ms (mean of %d)" % (asm_ms, asm_n))
 
     # the same batch on the FP64 path (its own autotuned plan, same step protocol): batched FP64 throughput
     bs64 = BatchSolver([pb.copy() for pb in pbs], precision="fp64", use_graph=True)
@@ -664,6 +676,9 @@ def main():
         L_n += int(narrow.sum())
         ks_n.append(roofline.touched_slots(w.lm_host, w.ob_start, w.ob_pose)[narrow])
     ks_n = np.concatenate(ks_n)
+    # the narrow launch's duration: HIP events on the solver's stream right around the launch (the eager profile,
+    # steady clocks, mean of 5); cross-checked by the assembly graph's span over the replayed steps, which also holds
+    # the graph boundary
     t_lin = kern["lin_schur"] * 1e-3
     # the launch reads the measurements and whitening in FP64 and computes in FP64: the FP64 figures apply
     lin_bytes = roofline.lin_kernel_bytes(M_n, L_n, ns, fp32=False)
@@ -737,7 +752,12 @@ def main():
         "roofline": {"bound": "valu", "achieved": tfl, "peak": roofline.FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tfl / roofline.FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_lin_fast<float, 0> (linearise + Schur, narrow work items)",
-                     "launch_ms": kern["lin_schur"], "launch_obs_rows": M_n, "launch_landmarks": L_n,
+                     "launch_ms": t_lin * 1e3, "launch_timing": "HIP events on the solver's stream right around the "
+                     "narrow launch, mean of 5 eager iterations each queued behind 3 replayed ones (steady clocks)",
+                     "span_ms": asm_ms, "span_timing": "mean of %d LM iterations: the timed steps replayed once more "
+                     "with HIP events around the assembly graph (the narrow launch, the wide one beside it on the side "
+                     "stream ending first, and the graph boundary)" % asm_n,
+                     "launch_obs_rows": M_n, "launch_landmarks": L_n,
                      "algorithmic_flops": lin_flops, "algorithmic_bytes": lin_bytes,
                      "hbm_achieved_gbs": gbs, "hbm_frac": gbs / roofline.HBM_PEAK_GBS,
                      "note": "compute-bound on the FP64 VALU (no MFMA in this kernel): intensity %.0f FLOP/B is "

@@ -164,6 +164,18 @@ int xba_sizes(const xba_ctx *ctx, int64_t *n_state, int64_t *n_hentries);
 #define XBA_NPROF 10
 int xba_profile_iteration(xba_ctx *ctx, double *ms_out, void *stream);
 
+/* Assembly timing over the replays of xba_iterate (no reference counterpart; a measurement aid).  on = 1: every LM
+ * iteration xba_iterate runs from now on records two HIP events on the caller's stream around its assembly launch
+ * (the narrow linearise+Schur kernel, with the wide one beside it on the side stream ending first); with graphs the
+ * iteration is captured as two graphs, the assembly and the rest, and the events are recorded between their
+ * launches.  Between two reads the first XBA_TIMING_SLOTS iterations are recorded.  on = 0: one graph, no events.
+ * Synchronous. */
+#define XBA_TIMING_SLOTS 4096
+int xba_set_assembly_timing(xba_ctx *ctx, int32_t on);
+/* The recorded assembly durations since the last call: their sum in ms and their number (host out); waits for the
+ * last recorded event and starts a new record. */
+int xba_assembly_times(xba_ctx *ctx, double *ms_sum, int32_t *count);
+
 /* Plan summary (host out, 8): work items, persistent grid, max slot tile, LDS bytes of the landmark kernels, LDS
  * bytes of the dense solve, partial-tile entries, partial-b entries, max state dimension. */
 int xba_plan_info(const xba_ctx *ctx, int64_t *out);

@@ -87,6 +87,7 @@ class XbaError(RuntimeError):
 # every symbol include/xba.h declares
 EXPORTS = ("xba_create", "xba_destroy", "xba_upload", "xba_iterate", "xba_download", "xba_stats", "xba_last_dx",
            "xba_linearize", "xba_sizes", "xba_profile_iteration", "xba_plan_info", "xba_gn_step", "xba_backward",
+           "xba_set_assembly_timing", "xba_assembly_times",
            "xba_np_solve", "xba_np_solve_noalloc", "xba_np_id_solve", "xba_preintegrate", "xba_sqrt_information",
            "xba_marg_schur", "xba_prior_from_info", "xba_build_pairs", "xba_gba_solve",
            "xba_copy_kernel", "xba_strerror", "xba_version")
@@ -111,6 +112,8 @@ def load():
     L.xba_linearize.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
     L.xba_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xba_profile_iteration.argtypes = [vp, vp, vp]
+    L.xba_set_assembly_timing.argtypes = [vp, C.c_int32]
+    L.xba_assembly_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.xba_plan_info.argtypes = [vp, vp]
     L.xba_gn_step.argtypes = [vp, vp, vp]
     L.xba_backward.argtypes = [vp, vp, vp, vp, vp]

@@ -139,6 +139,14 @@ struct xba_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t gstream = nullptr;
+    // assembly timing (xba_set_assembly_timing): the iteration as two graphs, the assembly (gexec_a) and the rest
+    // (gexec_b), with the event pair tev[2k], tev[2k + 1] recorded around the assembly of the k-th iteration since the
+    // last read (tcount iterations so far)
+    bool timing = false;
+    hipGraph_t graph_a = nullptr, graph_b = nullptr;
+    hipGraphExec_t gexec_a = nullptr, gexec_b = nullptr;
+    std::vector<hipEvent_t> tev;
+    int tcount = 0;
     // a second stream for independent launches of one iteration: the one-workgroup solves of the small windows
     // beside the tiled sequence of the large ones, the wide work items' assembly and trial beside the narrow ones
     hipStream_t side = nullptr;
@@ -432,11 +440,15 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     }
 }
 
+// One LM iteration (part -1), or its assembly launch (part 0: narrow and wide linearise+Schur) or the rest (part 1).
 template <typename T>
-void launch_iter(xba_ctx *c, hipStream_t s) {
+void launch_iter(xba_ctx *c, hipStream_t s, int part = -1) {
     KArgs<T> A = kargs<T>(c);
-    launch_lin<T>(c, A, s);
+    if (part != 1) launch_lin_kernel<T>(c, A, s);
+    if (part == 0) return;
     DArgs D = dargs(c);
+    launch_reduce<T>(c, D, s);
+    launch_terms(c, D, s, 0);
     launch_solve<T>(c, D, s);
     launch_trial_kernel<T>(c, A, s);
     launch_terms(c, D, s, 1);
@@ -555,6 +567,11 @@ int xba_destroy(xba_ctx *c) {
     if (!c) return XBA_OK;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gexec_a) (void)hipGraphExecDestroy(c->gexec_a);
+    if (c->graph_a) (void)hipGraphDestroy(c->graph_a);
+    if (c->gexec_b) (void)hipGraphExecDestroy(c->gexec_b);
+    if (c->graph_b) (void)hipGraphDestroy(c->graph_b);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -1157,6 +1174,23 @@ int xba_iterate(xba_ctx *c, int32_t iters, void *stream) {
     if (!c->uploaded) return XBA_E_STATE;
     c->gn_ready = false;
     hipStream_t s = (hipStream_t)stream;
+    // with assembly timing, the event pair of the next iteration (none once the record is full)
+    auto ev_next = [&]() -> hipEvent_t * {
+        return (c->timing && c->tcount < XBA_TIMING_SLOTS) ? &c->tev[2 * c->tcount] : nullptr;
+    };
+    if (c->opt.use_graph && c->timing) {
+        for (int it = 0; it < iters; it++) {
+            hipEvent_t *ev = ev_next();
+            if (ev) HIPCK(hipEventRecord(ev[0], s));
+            HIPCK(hipGraphLaunch(c->gexec_a, s));
+            if (ev) {
+                HIPCK(hipEventRecord(ev[1], s));
+                c->tcount++;
+            }
+            HIPCK(hipGraphLaunch(c->gexec_b, s));
+        }
+        return XBA_OK;
+    }
     if (c->opt.use_graph) {
         if (!c->gexec || c->gstream != s) {
             if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1183,12 +1217,70 @@ int xba_iterate(xba_ctx *c, int32_t iters, void *stream) {
         return XBA_OK;
     }
     for (int it = 0; it < iters; it++) {
+        hipEvent_t *ev = ev_next();
+        if (ev) HIPCK(hipEventRecord(ev[0], s));
         if (mixed(c))
-            launch_iter<float>(c, s);
+            launch_iter<float>(c, s, 0);
         else
-            launch_iter<double>(c, s);
+            launch_iter<double>(c, s, 0);
+        if (ev) {
+            HIPCK(hipEventRecord(ev[1], s));
+            c->tcount++;
+        }
+        if (mixed(c))
+            launch_iter<float>(c, s, 1);
+        else
+            launch_iter<double>(c, s, 1);
         HIPCK(hipGetLastError());
     }
+    return XBA_OK;
+}
+
+int xba_set_assembly_timing(xba_ctx *c, int32_t on) {
+    if (!c) return XBA_E_ARG;
+    const bool t = on != 0;
+    if (t && c->tev.empty()) {
+        c->tev.resize(2 * XBA_TIMING_SLOTS, nullptr);
+        for (auto &e : c->tev) HIPCK(hipEventCreate(&e));
+    }
+    if (t && c->opt.use_graph && !c->gexec_a) {
+        // the assembly and the rest captured as two graphs (on a private stream: a captured graph can be launched
+        // on any stream), uploaded so that their first launches cost what the later ones do
+        hipStream_t cs;
+        HIPCK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        for (int part = 0; part < 2; part++) {
+            hipGraph_t *g = part == 0 ? &c->graph_a : &c->graph_b;
+            hipGraphExec_t *e = part == 0 ? &c->gexec_a : &c->gexec_b;
+            HIPCK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            if (mixed(c))
+                launch_iter<float>(c, cs, part);
+            else
+                launch_iter<double>(c, cs, part);
+            HIPCK(hipStreamEndCapture(cs, g));
+            HIPCK(hipGraphInstantiate(e, *g, nullptr, nullptr, 0));
+            HIPCK(hipGraphUpload(*e, cs));
+        }
+        HIPCK(hipStreamSynchronize(cs));
+        HIPCK(hipStreamDestroy(cs));
+    }
+    c->timing = t;
+    c->tcount = 0;
+    return XBA_OK;
+}
+
+int xba_assembly_times(xba_ctx *c, double *ms_sum, int32_t *count) {
+    if (!c || !ms_sum || !count) return XBA_E_ARG;
+    double sum = 0.0;
+    const int n = c->tcount;
+    if (n > 0) HIPCK(hipEventSynchronize(c->tev[2 * n - 1]));
+    for (int k = 0; k < n; k++) {
+        float ms = 0.f;
+        HIPCK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
+        sum += ms;
+    }
+    *ms_sum = sum;
+    *count = n;
+    c->tcount = 0;
     return XBA_OK;
 }
 

@@ -365,6 +365,17 @@ class BatchSolver:
         _lib.check(self.L.xba_profile_iteration(self.ctx, out.ctypes.data, self.stream), "xba_profile_iteration")
         return dict(zip(self.KERNELS, out.tolist()))
 
+    def set_assembly_timing(self, on=True):
+        """Record HIP events around the assembly launch of every LM iteration ``iterate`` runs from now on (the
+        iteration's graph is split into the assembly and the rest); ``assembly_times`` reads them."""
+        _lib.check(self.L.xba_set_assembly_timing(self.ctx, int(bool(on))), "xba_set_assembly_timing")
+
+    def assembly_times(self):
+        """(mean ms, count) of the assembly launches recorded since the last call (waits for them)."""
+        ms, n = C.c_double(), C.c_int32()
+        _lib.check(self.L.xba_assembly_times(self.ctx, C.byref(ms), C.byref(n)), "xba_assembly_times")
+        return (ms.value / n.value if n.value else float("nan")), n.value
+
     def plan_info(self):
         out = np.zeros(8, np.int64)
         _lib.check(self.L.xba_plan_info(self.ctx, out.ctypes.data), "xba_plan_info")
