@@ -105,6 +105,33 @@ __device__ inline double big_bcast(double v, int jj) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// A tile row (XBA_TILE consecutive entries, 16-byte aligned: rows start at multiples of XBA_TILE entries of the
+// padded matrix) moved in 16-byte pieces, so that a wavefront whose lanes each move their own row touches each
+// cache line of it once per piece rather than once per entry.
+template <typename T>
+__device__ inline void big_row_load(const T *p, T (&v)[XBA_TILE]) {
+    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int W = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < XBA_TILE / W; q++) {
+        const V4 x = *(const V4 *)(p + W * q);
+#pragma unroll
+        for (int e = 0; e < W; e++) v[W * q + e] = x[e];
+    }
+}
+template <typename T>
+__device__ inline void big_row_store(T *p, const T (&v)[XBA_TILE]) {
+    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int W = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < XBA_TILE / W; q++) {
+        V4 x;
+#pragma unroll
+        for (int e = 0; e < W; e++) x[e] = v[W * q + e];
+        *(V4 *)(p + W * q) = x;
+    }
+}
+
 #define BIG_WAVE_SYNC()                                        \
     do {                                                       \
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
@@ -114,14 +141,82 @@ __device__ inline double big_bcast(double v, int jj) {
     } while (0)
 
 // --------------------------------------------------------------------------------------------- k_big_panel --
+// Row j of the transposed factor sLt (entries m > j) times x, subtracted from v[m]: the row read in 16-entry vector
+// chunks from LDS ahead of the chunk's multiply-adds (every lane reads the same addresses: broadcasts).
+template <typename T, int J>
+__device__ inline void big_row_axpy(const T (*sLt)[XBA_TILE], T x, T (&v)[XBA_TILE]) {
+    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int W = 16 / sizeof(T), C = 16;
+#pragma unroll
+    for (int c0 = ((J + 1) / C) * C; c0 < XBA_TILE; c0 += C) {
+        T l[C];
+#pragma unroll
+        for (int q = 0; q < C / W; q++) {
+            const V4 r = *(const V4 *)(&sLt[J][c0 + W * q]);
+#pragma unroll
+            for (int e = 0; e < W; e++) l[W * q + e] = r[e];
+        }
+#pragma unroll
+        for (int e = 0; e < C; e++)
+            if (c0 + e > J) v[c0 + e] -= x * l[e];
+    }
+}
+
+// sum over m > J of v[m] sLt[J][m], the row read likewise.
+template <typename T, int J>
+__device__ inline T big_row_dot(const T (*sLt)[XBA_TILE], const T (&v)[XBA_TILE]) {
+    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int W = 16 / sizeof(T), C = 16;
+    T acc = (T)0;
+#pragma unroll
+    for (int c0 = ((J + 1) / C) * C; c0 < XBA_TILE; c0 += C) {
+        T l[C];
+#pragma unroll
+        for (int q = 0; q < C / W; q++) {
+            const V4 r = *(const V4 *)(&sLt[J][c0 + W * q]);
+#pragma unroll
+            for (int e = 0; e < W; e++) l[W * q + e] = r[e];
+        }
+#pragma unroll
+        for (int e = 0; e < C; e++)
+            if (c0 + e > J) acc += v[c0 + e] * l[e];
+    }
+    return acc;
+}
+
+template <typename T, int J>
+struct BigSteps {
+    // forward substitution x L^T = v (row J of the lane's tile row), J = 0 .. XBA_TILE - 1
+    __device__ static inline void trsm(const T (*sLt)[XBA_TILE], const T *sinv, T (&v)[XBA_TILE]) {
+        const T xj = v[J] * sinv[J];
+        v[J] = xj;
+        big_row_axpy<T, J>(sLt, xj, v);
+        BigSteps<T, J + 1>::trsm(sLt, sinv, v);
+    }
+    // back substitution x L = e_lane (row `lane` of L^-1), columns J = XBA_TILE - 1 .. 0: call with J = XBA_TILE - 1
+    __device__ static inline void inv(const T (*sLt)[XBA_TILE], const T *sinv, int lane, T (&x)[XBA_TILE]) {
+        x[J] = (((lane == J) ? (T)1 : (T)0) - big_row_dot<T, J>(sLt, x)) * sinv[J];
+        BigSteps<T, J - 1>::inv(sLt, sinv, lane, x);
+    }
+};
+template <typename T>
+struct BigSteps<T, XBA_TILE> {
+    __device__ static inline void trsm(const T (*)[XBA_TILE], const T *, T (&)[XBA_TILE]) {}
+};
+template <typename T>
+struct BigSteps<T, -1> {
+    __device__ static inline void inv(const T (*)[XBA_TILE], const T *, int, T (&)[XBA_TILE]) {}
+};
+
 // grid (1 + ceil((max nb - k - 1) / 4), nbig), 256 threads.  Every workgroup factors the diagonal tile (k,k) in
-// registers (wavefront 0, lane r holding row r); workgroup 0 then inverts it into the side buffer, and wavefront w
-// of workgroup x >= 1 solves tile row i = k + 1 + 4 (x - 1) + w.
+// registers (wavefront 0, lane r holding row r, each pivot's column broadcast from the lanes' registers); workgroup
+// 0 then inverts it into the side buffer, and wavefront w of workgroup x >= 1 solves tile row i = k + 1 + 4 (x - 1)
+// + w.  The FP32 factor (the mixed solve's, refined in FP64 against the FP64 matrix) takes its pivots' inverse
+// square roots from the hardware approximation (within an ulp); the FP64 factor's are correctly rounded.
 template <typename T>
 __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
     __shared__ __align__(16) T sLt[XBA_TILE][XBA_TILE];  // the factor transposed: sLt[j][m] = L[m][j] (0, m < j)
-    __shared__ __align__(16) T scol[XBA_TILE];
-    __shared__ T sinv[XBA_TILE];
+    __shared__ __align__(16) T sinv[XBA_TILE];
     __shared__ int sok;
     const BigWin B = D.bigs[blockIdx.y];
     if (k >= B.nb) return;
@@ -139,8 +234,9 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
         // lane r holds row r of the diagonal tile (lower part; 0 above the diagonal)
         const T *Ar = F + (size_t)(k * XBA_TILE + lane) * np + k * XBA_TILE;
         T a[XBA_TILE];
+        big_row_load(Ar, a);
 #pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? Ar[m] : (T)0;
+        for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? a[m] : (T)0;
         int ok = 1;
         T dinv = (T)0;
 #pragma unroll
@@ -150,15 +246,20 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
                 ok = 0;
                 break;
             }
-            const T sd = sqrt(d), inv = (T)1 / sd;
+            T sd, inv;
+            if constexpr (sizeof(T) == 4) {
+                inv = __builtin_amdgcn_rsqf(d);
+                sd = d * inv;
+            } else {
+                sd = sqrt(d);
+                inv = (T)1 / sd;
+            }
             if (lane == j) dinv = inv;
             a[j] = (lane == j) ? sd : a[j] * inv;  // L[lane][j] below the diagonal (entries above it are not used)
-            scol[lane] = a[j];
-            BIG_WAVE_SYNC();
+            // a[m] -= L[lane][j] L[m][j]: L[m][j] is lane m's a[j] (lanes up to j change only unused entries)
             const T lij = (lane > j) ? a[j] : (T)0;
 #pragma unroll
-            for (int m = j + 1; m < XBA_TILE; m++) a[m] -= lij * scol[m];
-            BIG_WAVE_SYNC();
+            for (int m = j + 1; m < XBA_TILE; m++) a[m] -= lij * big_bcast(a[j], m);
         }
         if (lane == 0) sok = ok;
         if (ok) {
@@ -171,15 +272,10 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
                 BIG_WAVE_SYNC();
                 T xr[XBA_TILE];
 #pragma unroll
-                for (int j = XBA_TILE - 1; j >= 0; j--) {
-                    T v = (lane == j) ? (T)1 : (T)0;
-#pragma unroll
-                    for (int m = j + 1; m < XBA_TILE; m++) v -= xr[m] * sLt[j][m];
-                    xr[j] = v * sinv[j];
-                }
+                for (int m = 0; m < XBA_TILE; m++) xr[m] = (T)0;
+                BigSteps<T, XBA_TILE - 1>::inv(sLt, sinv, lane, xr);
                 T *Dr = F + (size_t)np * np + (size_t)(k * XBA_TILE + lane) * XBA_TILE;
-#pragma unroll
-                for (int m = 0; m < XBA_TILE; m++) Dr[m] = xr[m];
+                big_row_store(Dr, xr);
             }
         }
     }
@@ -192,17 +288,9 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
     if (x > 0 && i < B.nb) {
         T *Xr = F + (size_t)(i * XBA_TILE + lane) * np + k * XBA_TILE;
         T xv[XBA_TILE];
-#pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) xv[m] = Xr[m];
-#pragma unroll
-        for (int j = 0; j < XBA_TILE; j++) {
-            const T xj = xv[j] * sinv[j];
-            xv[j] = xj;
-#pragma unroll
-            for (int m = j + 1; m < XBA_TILE; m++) xv[m] -= xj * sLt[j][m];
-        }
-#pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) Xr[m] = xv[m];
+        big_row_load(Xr, xv);
+        BigSteps<T, 0>::trsm(sLt, sinv, xv);
+        big_row_store(Xr, xv);
     }
 }
 
