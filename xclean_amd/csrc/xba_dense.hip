@@ -602,8 +602,8 @@ __device__ inline double lane_bcast(double v, int jj) {
 // In-place lower Cholesky of the n x n row-major A (leading dimension ld) by one workgroup (a multiple of 64
 // threads), right-looking in panels of CH_BS columns:
 //   1. the panel's diagonal block in the first wavefront's registers (lane i holds row i of the block): per
-//      column the pivot is broadcast through the scalar unit and the scaled column through 32 LDS entries
-//      (sinv[CH_BS..2 CH_BS)), which every lane reads back at uniform addresses; reciprocal pivots into sinv;
+//      column the pivot and then each entry of the scaled column are broadcast from the lanes' registers through
+//      the scalar unit (no LDS round trip in the column loop); reciprocal pivots into sinv;
 //   2. the rows below solved against it, one row per thread in registers (the block read at wave-uniform
 //      addresses, so every read is a broadcast), written to the lower triangle and, transposed, to the unused
 //      upper triangle (row j0 + m holds column m of the panel, contiguous in the row index);
@@ -612,14 +612,13 @@ __device__ inline double lane_bcast(double v, int jj) {
 //      (consecutive threads then read consecutive 16-byte groups), and each loaded entry feeds 4 products.
 // The lower triangle holds the factor on return; the upper triangle is scratch.  Every thread reads the pivot
 // status, so a failure exits uniformly.  Returns false on a non-positive or non-finite pivot.  Inlined, so an
-// LDS-resident A gets LDS instructions.  sinv: CH_SCRATCH entries of LDS (the pivots' inverse square roots and
-// the current pivot column).
+// LDS-resident A gets LDS instructions.  sinv: CH_SCRATCH entries of LDS (the pivots' inverse square roots in the
+// first CH_BS).
 #define CH_BS 32
 #define CH_SCRATCH (2 * CH_BS)
 template <typename T>
 __device__ __attribute__((always_inline)) inline bool chol_wg(T *A, int n, int *sflag, T *sinv, int ld) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    T *scr = sinv + CH_BS;  // the current pivot column of the block, for the first wavefront
     for (int j0 = 0; j0 < n; j0 += CH_BS) {
         const int nb = min(CH_BS, n - j0), j1 = j0 + nb;
         if (wv == 0) {
@@ -638,21 +637,26 @@ __device__ __attribute__((always_inline)) inline bool chol_wg(T *A, int n, int *
                     ok = 0;
                     d = (T)1;
                 }
-                // l = L[lane][jj] for lane > jj; lane jj gets d / sqrt(d), the pivot (within an ulp of sqrt(d))
-                const T s = (T)1 / sqrt(d), l = row[jj] * s;
+                // l = L[lane][jj] for lane > jj; lane jj gets d / sqrt(d), the pivot (within an ulp of sqrt(d)).  The
+                // FP32 factor (refined in FP64 against the FP64 matrix) takes 1/sqrt(d) from the hardware
+                // approximation (within an ulp), the FP64 factor's is correctly rounded.
+                T s;
+                if constexpr (sizeof(T) == 4)
+                    s = __builtin_amdgcn_rsqf(d);
+                else
+                    s = (T)1 / sqrt(d);
+                const T l = row[jj] * s;
                 row[jj] = l;
-                if (lane < CH_BS) scr[lane] = l;
                 if (lane == 0) sinv[jj] = s;
-                CH_WAVE_SYNC();
-                // row[k] -= L[lane][jj] L[k][jj]; entries right of the diagonal are scratch, never stored
+                // row[k] -= L[lane][jj] L[k][jj], L[k][jj] broadcast from lane k's register; entries right of the
+                // diagonal are scratch, never stored
 #pragma unroll
                 for (int k = jj + 1; k < CH_BS; k++) {
-                    row[k] -= l * scr[k];
+                    row[k] -= l * lane_bcast(l, k);
                     // the update happens here: otherwise the compiler defers each row[k] to its use at step k and
-                    // keeps every earlier column's scratch values live until then
+                    // keeps every earlier column's values live until then
                     asm volatile("" : "+v"(row[k]));
                 }
-                CH_WAVE_SYNC();
             }
             // whole rows back: the part right of the diagonal is never read as factor
 #pragma unroll
