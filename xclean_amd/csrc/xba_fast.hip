@@ -25,6 +25,10 @@ enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_NX = 10, LA_N = 11 };
 // LR_UNIQ: 1 when each of the landmark's active cross rows has a target slot of its own (then each row alone forms
 // its slot's U entry)
 enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_UNIQ = 10 };
+// trial kernel, per landmark of a pass (FP64): state at the current point, trial state, symmetric damped inverse
+// (00 01 02 11 12 22), bl
+enum { TL_CUR = 0, TL_NEW = 3, TL_HI = 6, TL_BL = 12, TL_N = 16 };
+#define TL_LMOK (1 << 20)  // in the trial's per-landmark word (packed landmark word | this): the landmark is solvable
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
 // plus a wait on the LDS counter makes every lane's earlier LDS writes visible to every lane's later reads.
@@ -59,7 +63,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
     L.o_rcq = w;   w = align16(w + (trial ? FB_OBS * 8 : 0));
-    L.o_u = w;     w = align16(w + (trial ? FB_OBS * 3 * ts : 0));
+    L.o_u = w;     // (unused)
     L.o_lmask = w; w = align16(w + FB_LM * 4);
     L.o_lobs0 = w; w = align16(w + FB_LM * 4);
     L.o_lnobs = w; w = align16(w + FB_LM * 4);
@@ -68,7 +72,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.o_slots = w; w = align16(w + (trial ? 0 : FB_HLP * 4));
     L.o_hlp = w;   w = align16(w + (trial ? 0 : FB_HLP * 18 * ts));
     L.o_lrec = w;  w = align16(w + (trial ? 0 : FB_LM * FB_LREC * ts));
-    L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * 4 * 8 : 0));
+    L.o_ldbl = w;  w = align16(w + (trial ? FB_LM * TL_N * 8 : 0));
     L.o_lrow = w;  // (trial: unused)
     if (!trial) {
         // the observation costs and the per-landmark sums (Hll, bl, active count) live from the observation phase
@@ -784,8 +788,7 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
     int *ob_lm = (int *)(wb + FL.o_oblm);
     double *rcq = (double *)(wb + FL.o_rcq), *ldbl = (double *)(wb + FL.o_ldbl);
-    T *u = (T *)(wb + FL.o_u);
-    int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs);
+    int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs), *lmask = (int *)(wb + FL.o_lmask);
     int *ctl = (int *)(smem + FL.ctl);
 
     for (;;) {
@@ -820,30 +823,47 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
             pass_take(A, ch, ctl, lane, pa);
             const int p = pa.p;
             if (p >= ch.p1) break;
-            pass_rows(A.ob_start, lane, pa);
             const int lb = pa.lb, B = pa.B;
+            // The landmark lanes' loads depend only on the plan: issued first, together with the row offsets, and
+            // parked in LDS (landmark record, word) or registers (prior) for the two phases that read them.
+            const bool lm_in = lane < B;
+            int lw = 0;
+            double lm_rm = 0.0, lm_ri = 0.0;
+            pass_rows(A.ob_start, lane, pa);
+            if (lm_in) {
+                const int l = lb + lane;
+                double *d = ldbl + TL_N * lane;
+                for (int i = 0; i < 3; i++) d[TL_CUR + i] = A.lmp[3 * (size_t)l + i];
+                for (int i = 0; i < 6; i++) d[TL_HI + i] = (double)A.lmHi[6 * (size_t)l + i];
+                for (int i = 0; i < 3; i++) d[TL_BL + i] = (double)A.lmbl[3 * (size_t)l + i];
+                lw = A.lm_info[l] | (A.lmok[l] ? TL_LMOK : 0);
+                lmask[lane] = lw;
+                lm_rm = A.rho_meas[l];
+                lm_ri = (double)A.rho_isig[l];
+            }
             int obase;
             const int nob = load_pass(pa, lane, lobs0, lnobs, ob_lm, &obase);
-            WAVE_SYNC();
-            // ---- observations at the current state: u_q = w Jl^T (Jt dp_t + Jh dp_h)
-            int q_l = 0, q_k = 0, q_t = 0, q_h = 0, q_cam = 0, q_valid = 0;
-            bool q_mask = false;
             const bool q_in = lane < nob;
+            const int q_k = obase + lane;
+            const int info = q_in ? A.ob_info[q_k] : 0;
+            WAVE_SYNC();
+            // ---- observations at the current state: u_q = w Jl^T (Jt dp_t + Jh dp_h), summed per landmark over its
+            //      lanes with DPP moves (complete in the landmark's last lane)
+            int q_t = 0, q_h = 0, q_cam = 0, q_valid = 0, qb = 0, seg_ss = XBA_WAVE;
+            bool q_mask = false;
+            double uv[3] = {0.0, 0.0, 0.0};
             if (q_in) {
-                const int qb = ob_lm[lane];
-                q_l = lb + qb;
-                q_k = obase + lane;
-                const int li = A.lm_info[q_l];
+                qb = ob_lm[lane];
+                seg_ss = lobs0[qb];
+                const int li = lmask[qb];
                 q_mask = (li >> 12) & 1;
                 q_h = lm_tile(li);
-                const int info = A.ob_info[q_k];
                 q_t = obs_tile(info);
                 q_cam = (info >> 12) & 1;
                 q_valid = (info >> 13) & 1;
-                T u0 = 0, u1 = 0, u2 = 0;
-                if (q_mask && q_valid && A.lmok[q_l] && q_t != q_h) {
-                    const double a = A.lmp[3 * (size_t)q_l], b = A.lmp[3 * (size_t)q_l + 1],
-                                 rho = A.lmp[3 * (size_t)q_l + 2];
+                if (q_mask && q_valid && (li & TL_LMOK) && q_t != q_h) {
+                    const double *d = ldbl + TL_N * qb;
+                    const double a = d[TL_CUR], b = d[TL_CUR + 1], rho = d[TL_CUR + 2];
                     double e[2];
                     T Jl[2][3], jdv[2];
                     if (obs_dir(sc + q_h * SC_STRIDE, sc + q_t * SC_STRIDE, wdc, a, b, rho, q_cam, A.z[2 * (size_t)q_k],
@@ -861,34 +881,36 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
                             const T jd = r < nd ? jdv[r] * (isgT * isgT) : (T)0;
                             for (int i = 0; i < 3; i++) s[i] += Jl[r][i] * jd;
                         }
-                        u0 = (T)w * s[0];
-                        u1 = (T)w * s[1];
-                        u2 = (T)w * s[2];
+                        for (int i = 0; i < 3; i++) uv[i] = (T)w * s[i];
                     }
                 }
-                u[3 * lane] = u0;
-                u[3 * lane + 1] = u1;
-                u[3 * lane + 2] = u2;
             }
-            WAVE_SYNC();
+            // The lane sums need finite values in every lane: a non-finite row is left out and fails its landmark's
+            // update instead (as it would summed in: the step's rho would not be finite).
+            const bool ubad = !(isfinite(uv[0]) && isfinite(uv[1]) && isfinite(uv[2]));
+            if (ubad) uv[0] = uv[1] = uv[2] = 0.0;
+            const uint64_t badm = __ballot(ubad);
+            seg_sum(uv, lane, seg_ss);
             // ---- landmarks: back-substitution and the update rule; prior cost at the trial state
+            const int l_n = lm_in ? lnobs[lane] : 0, l_s0 = lm_in ? lobs0[lane] : 0;
+            const int src = (l_s0 + (l_n > 0 ? l_n - 1 : 0)) << 2;
+            double v[3];
+            for (int i = 0; i < 3; i++) {
+                const long long bits = __double_as_longlong(uv[i]);
+                const int lo = __builtin_amdgcn_ds_bpermute(src, (int)bits);
+                const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(bits >> 32));
+                v[i] = l_n > 0 ? __longlong_as_double(((long long)hi << 32) | (unsigned)lo) : 0.0;
+            }
             double lcost = 0.0;
-            if (lane < B) {
+            if (lm_in) {
                 const int l = lb + lane;
-                const int li = A.lm_info[l];
-                const double a = A.lmp[3 * (size_t)l], b = A.lmp[3 * (size_t)l + 1], rho = A.lmp[3 * (size_t)l + 2];
+                const double *d = ldbl + TL_N * lane;
+                const double a = d[TL_CUR], b = d[TL_CUR + 1], rho = d[TL_CUR + 2];
                 double na = a, nb = b, nr = rho;
-                if (((li >> 12) & 1) && A.lmok[l]) {
-                    T v0 = 0, v1 = 0, v2 = 0;
-                    for (int q = lobs0[lane]; q < lobs0[lane] + lnobs[lane]; q++) {
-                        v0 += u[3 * q];
-                        v1 += u[3 * q + 1];
-                        v2 += u[3 * q + 2];
-                    }
-                    const T *Hi = A.lmHi + 6 * (size_t)l, *bl = A.lmbl + 3 * (size_t)l;
-                    v0 += bl[0];
-                    v1 += bl[1];
-                    v2 += bl[2];
+                const uint64_t segm = l_n >= 64 ? ~0ull : ((1ull << l_n) - 1ull);
+                if (((lw >> 12) & 1) && (lw & TL_LMOK) && ((badm >> l_s0) & segm) == 0ull) {
+                    const double *Hi = d + TL_HI, *bl = d + TL_BL;
+                    const double v0 = v[0] + bl[0], v1 = v[1] + bl[1], v2 = v[2] + bl[2];
                     const T d0 = -(Hi[0] * v0 + Hi[1] * v1 + Hi[2] * v2);
                     const T d1 = -(Hi[1] * v0 + Hi[3] * v1 + Hi[4] * v2);
                     const T d2 = -(Hi[2] * v0 + Hi[4] * v1 + Hi[5] * v2);
@@ -902,16 +924,14 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
                 A.lmp_trial[3 * (size_t)l] = na;
                 A.lmp_trial[3 * (size_t)l + 1] = nb;
                 A.lmp_trial[3 * (size_t)l + 2] = nr;
-                ldbl[4 * lane] = na;
-                ldbl[4 * lane + 1] = nb;
-                ldbl[4 * lane + 2] = nr;
-                if ((li >> 12) & 1) {
-                    const T ris = A.rho_isig[l];
-                    if (ris > (T)0) {
-                        double r0 = (nr - A.rho_meas[l]) * (double)ris, w, cc;
-                        huber(fabs(r0), wdc[5], &w, &cc);
-                        lcost += cc;
-                    }
+                double *dn = ldbl + TL_N * lane + TL_NEW;
+                dn[0] = na;
+                dn[1] = nb;
+                dn[2] = nr;
+                if (((lw >> 12) & 1) && lm_ri > 0.0) {
+                    double r0 = (nr - lm_rm) * lm_ri, w, cc;
+                    huber(fabs(r0), wdc[5], &w, &cc);
+                    lcost += cc;
                 }
             }
             WAVE_SYNC();
@@ -919,13 +939,13 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
             if (q_in) {
                 double cq = 0.0;
                 if (q_mask && q_valid) {
-                    const int qb = ob_lm[lane];
                     double e[2];
                     T Jl[2][3], Jt[2][6], Jh[2][6];
                     bool act = obs_eval<T>(sc2 + q_h * SC_STRIDE, sc2 + q_t * SC_STRIDE, sc2 + q_h * SC_STRIDE,
-                                           sc2 + q_t * SC_STRIDE, wc, wdc, ldbl[4 * qb], ldbl[4 * qb + 1],
-                                           ldbl[4 * qb + 2], q_cam, A.z[2 * (size_t)q_k], A.z[2 * (size_t)q_k + 1],
-                                           q_t == q_h, false, e, Jl, Jt, Jh);
+                                           sc2 + q_t * SC_STRIDE, wc, wdc, ldbl[TL_N * qb + TL_NEW],
+                                           ldbl[TL_N * qb + TL_NEW + 1], ldbl[TL_N * qb + TL_NEW + 2], q_cam,
+                                           A.z[2 * (size_t)q_k], A.z[2 * (size_t)q_k + 1], q_t == q_h, false, e, Jl,
+                                           Jt, Jh);
                     if (!act) {
                         cq = 1e6;
                     } else {
