@@ -26,8 +26,8 @@ enum { LA_H = 0, LA_B = 6, LA_NACT = 9, LA_NX = 10, LA_N = 11 };
 // its slot's U entry)
 enum { LR_OK = 0, LR_L = 1, LR_G = 7, LR_UNIQ = 10 };
 // trial kernel, per landmark of a pass (FP64): state at the current point, trial state, symmetric damped inverse
-// (00 01 02 11 12 22), bl
-enum { TL_CUR = 0, TL_NEW = 3, TL_HI = 6, TL_BL = 12, TL_N = 16 };
+// (00 01 02 11 12 22), bl, inverse-depth prior (measurement, inverse sigma)
+enum { TL_CUR = 0, TL_NEW = 3, TL_HI = 6, TL_BL = 12, TL_RM = 15, TL_RI = 16, TL_N = 18 };
 #define TL_LMOK (1 << 20)  // in the trial's per-landmark word (packed landmark word | this): the landmark is solvable
 
 // Orders the wavefront's LDS traffic: LDS requests of one wavefront complete in issue order, so a compiler fence
@@ -62,7 +62,7 @@ __host__ __device__ inline FastLayout fast_layout(int R, int rts, int nw, bool t
     L.wave = o;
     int w = 0;
     L.o_oblm = w;  w = align16(w + FB_OBS * 4);
-    L.o_rcq = w;   w = align16(w + (trial ? FB_OBS * 8 : 0));
+    L.o_rcq = w;   // (the assembly kernels place it in U's storage below; the trial keeps its costs in registers)
     L.o_u = w;     // (unused)
     L.o_lmask = w; w = align16(w + FB_LM * 4);
     L.o_lobs0 = w; w = align16(w + FB_LM * 4);
@@ -224,6 +224,16 @@ __device__ inline unsigned seg_or(unsigned v, int lane, int ss) {
     v = seg_or_step<0x142, 0xa>(v, (row & 1) && lane - r - 1 >= ss);
     v = seg_or_step<0x143, 0xc>(v, row >= 2 && 31 >= ss);
     return v;
+}
+
+// The work item's robust cost from its passes' costs (written by the workgroup's wavefronts before the last
+// barrier), summed by one wavefront in a fixed order: lane i takes the passes p0 + i, p0 + i + 64, ... (its loads in
+// flight together across the lanes), then the lanes are summed.
+__device__ inline void chunk_cost(const double *pass_cost, const Chunk &ch, int lane, double *out) {
+    double cs = 0.0;
+    for (int q = ch.p0 + lane; q < ch.p1; q += XBA_WAVE) cs += pass_cost[q];
+    cs = wave_sum(cs, lane);
+    if (lane == 0) *out = cs;
 }
 
 // Work items are handed out dynamically, largest first (the host orders `sched`): thread 0 takes the next index of
@@ -761,11 +771,7 @@ __global__ __launch_bounds__(512) void k_lin_fast(KArgs<double> A, int Rmax, int
         double *pH = A.partH64 + ch.partH, *pb = A.partb64 + ch.partb;
         for (int i = tid; i < nblk * 36; i += blockDim.x) pH[i] = tile[(i / 36) * TBLK + i % 36];
         for (int i = tid; i < 6 * R; i += blockDim.x) pb[i] = btile[i];
-        if (tid == 0) {
-            double cs = 0.0;
-            for (int q = ch.p0; q < ch.p1; q++) cs += A.pass_cost[q];
-            A.partc[c] = cs;
-        }
+        if (wv == 0) chunk_cost(A.pass_cost, ch, lane, A.partc + c);
     }
 }
 
@@ -787,7 +793,7 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
     const int tid = threadIdx.x, lane = tid % XBA_WAVE, wv = tid / XBA_WAVE;
     unsigned char *wb = smem + FL.wave + wv * FL.wave_stride;
     int *ob_lm = (int *)(wb + FL.o_oblm);
-    double *rcq = (double *)(wb + FL.o_rcq), *ldbl = (double *)(wb + FL.o_ldbl);
+    double *ldbl = (double *)(wb + FL.o_ldbl);
     int *lobs0 = (int *)(wb + FL.o_lobs0), *lnobs = (int *)(wb + FL.o_lnobs), *lmask = (int *)(wb + FL.o_lmask);
     int *ctl = (int *)(smem + FL.ctl);
 
@@ -828,7 +834,6 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
             // parked in LDS (landmark record, word) or registers (prior) for the two phases that read them.
             const bool lm_in = lane < B;
             int lw = 0;
-            double lm_rm = 0.0, lm_ri = 0.0;
             pass_rows(A.ob_start, lane, pa);
             if (lm_in) {
                 const int l = lb + lane;
@@ -838,8 +843,8 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
                 for (int i = 0; i < 3; i++) d[TL_BL + i] = (double)A.lmbl[3 * (size_t)l + i];
                 lw = A.lm_info[l] | (A.lmok[l] ? TL_LMOK : 0);
                 lmask[lane] = lw;
-                lm_rm = A.rho_meas[l];
-                lm_ri = (double)A.rho_isig[l];
+                d[TL_RM] = A.rho_meas[l];
+                d[TL_RI] = (double)A.rho_isig[l];
             }
             int obase;
             const int nob = load_pass(pa, lane, lobs0, lnobs, ob_lm, &obase);
@@ -928,8 +933,8 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
                 dn[0] = na;
                 dn[1] = nb;
                 dn[2] = nr;
-                if (((lw >> 12) & 1) && lm_ri > 0.0) {
-                    double r0 = (nr - lm_rm) * lm_ri, w, cc;
+                if (((lw >> 12) & 1) && d[TL_RI] > 0.0) {
+                    double r0 = (nr - d[TL_RM]) * d[TL_RI], w, cc;
                     huber(fabs(r0), wdc[5], &w, &cc);
                     lcost += cc;
                 }
@@ -955,11 +960,8 @@ __global__ __launch_bounds__(512, XBA_TRIAL_WAVES_PER_SIMD) void k_trial_fast(KA
                         if (A.ob_w) cq *= A.ob_w[q_k];
                     }
                 }
-                rcq[lane] = cq;
+                lcost += cq;
             }
-            WAVE_SYNC();
-            if (lane < B)
-                for (int q = lobs0[lane]; q < lobs0[lane] + lnobs[lane]; q++) lcost += rcq[q];
             const double pcost = wave_sum(lcost, lane);
             if (lane == 0) A.pass_cost[p] = pcost;
             WAVE_SYNC();
