@@ -208,6 +208,50 @@ struct BigSteps<T, -1> {
     __device__ static inline void inv(const T (*)[XBA_TILE], const T *, int, T (&)[XBA_TILE]) {}
 };
 
+// a[m] -= lij L[m][J] for m = M0 .. XBA_TILE - 1, L[m][J] broadcast from lane m's a[J]: the broadcasts issued in
+// groups ahead of their multiply-adds (a broadcast's scalar result read by the next instruction stalls the
+// wavefront), each group at most BIG_BC_GROUP values (scalar registers: one per FP32, two per FP64 value).
+#define BIG_BC_GROUP 8
+template <typename T, int J, int M0>
+__device__ inline void big_rank1(T (&a)[XBA_TILE], T lij) {
+    constexpr int N = (XBA_TILE - M0 < BIG_BC_GROUP) ? XBA_TILE - M0 : BIG_BC_GROUP;
+    T bc[N];
+#pragma unroll
+    for (int e = 0; e < N; e++) bc[e] = big_bcast(a[J], M0 + e);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int e = 0; e < N; e++) a[M0 + e] -= lij * bc[e];
+    if constexpr (M0 + N < XBA_TILE) big_rank1<T, J, M0 + N>(a, lij);
+}
+
+// Column J of the diagonal tile's factor, lane r holding row r (entries above the diagonal unused), then the
+// columns after it; ok = 0 and the remaining columns skipped at a pivot that is not positive and finite (uniform).
+template <typename T, int J>
+struct BigChol {
+    __device__ static inline void step(T (&a)[XBA_TILE], int lane, T &dinv, int &ok) {
+        const T d = big_bcast(a[J], J);
+        if (!(d > (T)0) || !isfinite(d)) {
+            ok = 0;
+            return;
+        }
+        T sd, inv;
+        if constexpr (sizeof(T) == 4) {
+            inv = __builtin_amdgcn_rsqf(d);
+            sd = d * inv;
+        } else {
+            sd = sqrt(d);
+            inv = (T)1 / sd;
+        }
+        if (lane == J) dinv = inv;
+        a[J] = (lane == J) ? sd : a[J] * inv;  // L[lane][J] below the diagonal
+        // a[m] -= L[lane][J] L[m][J] (lanes up to J change only unused entries)
+        if constexpr (J + 1 < XBA_TILE) {
+            big_rank1<T, J, J + 1>(a, (lane > J) ? a[J] : (T)0);
+            BigChol<T, J + 1>::step(a, lane, dinv, ok);
+        }
+    }
+};
+
 // grid (1 + ceil((max nb - k - 1) / 4), nbig), 256 threads.  Every workgroup factors the diagonal tile (k,k) in
 // registers (wavefront 0, lane r holding row r, each pivot's column broadcast from the lanes' registers); workgroup
 // 0 then inverts it into the side buffer, and wavefront w of workgroup x >= 1 solves tile row i = k + 1 + 4 (x - 1)
@@ -239,28 +283,7 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
         for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? a[m] : (T)0;
         int ok = 1;
         T dinv = (T)0;
-#pragma unroll
-        for (int j = 0; j < XBA_TILE; j++) {
-            const T d = big_bcast(a[j], j);
-            if (!(d > (T)0) || !isfinite(d)) {  // uniform: every lane reads the same pivot
-                ok = 0;
-                break;
-            }
-            T sd, inv;
-            if constexpr (sizeof(T) == 4) {
-                inv = __builtin_amdgcn_rsqf(d);
-                sd = d * inv;
-            } else {
-                sd = sqrt(d);
-                inv = (T)1 / sd;
-            }
-            if (lane == j) dinv = inv;
-            a[j] = (lane == j) ? sd : a[j] * inv;  // L[lane][j] below the diagonal (entries above it are not used)
-            // a[m] -= L[lane][j] L[m][j]: L[m][j] is lane m's a[j] (lanes up to j change only unused entries)
-            const T lij = (lane > j) ? a[j] : (T)0;
-#pragma unroll
-            for (int m = j + 1; m < XBA_TILE; m++) a[m] -= lij * big_bcast(a[j], m);
-        }
+        BigChol<T, 0>::step(a, lane, dinv, ok);
         if (lane == 0) sok = ok;
         if (ok) {
 #pragma unroll

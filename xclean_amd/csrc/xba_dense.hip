@@ -649,13 +649,23 @@ __device__ __attribute__((always_inline)) inline bool chol_wg(T *A, int n, int *
                 row[jj] = l;
                 if (lane == 0) sinv[jj] = s;
                 // row[k] -= L[lane][jj] L[k][jj], L[k][jj] broadcast from lane k's register; entries right of the
-                // diagonal are scratch, never stored
+                // diagonal are scratch, never stored.  The broadcasts are issued in groups of 8 ahead of their
+                // multiply-adds (a broadcast's scalar result read by the next instruction stalls the wavefront).
 #pragma unroll
-                for (int k = jj + 1; k < CH_BS; k++) {
-                    row[k] -= l * lane_bcast(l, k);
-                    // the update happens here: otherwise the compiler defers each row[k] to its use at step k and
-                    // keeps every earlier column's values live until then
-                    asm volatile("" : "+v"(row[k]));
+                for (int k0 = jj + 1; k0 < CH_BS; k0 += 8) {
+                    T bc[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        if (k0 + e < CH_BS) bc[e] = lane_bcast(l, k0 + e);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        if (k0 + e < CH_BS) {
+                            row[k0 + e] -= l * bc[e];
+                            // the update happens here: otherwise the compiler defers each row[k] to its use at
+                            // step k and keeps every earlier column's values live until then
+                            asm volatile("" : "+v"(row[k0 + e]));
+                        }
                 }
             }
             // whole rows back: the part right of the diagonal is never read as factor
