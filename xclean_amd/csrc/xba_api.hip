@@ -126,7 +126,7 @@ struct xba_ctx {
     uint8_t *lmok = nullptr;
     double *Rwb = nullptr, *pwb = nullptr, *vel = nullptr, *bgs = nullptr, *bas = nullptr;
     double *R2 = nullptr, *p2 = nullptr, *v2 = nullptr, *g2 = nullptr, *a2 = nullptr;
-    double *H = nullptr, *b = nullptr, *dx = nullptr, *fact64 = nullptr, *imu_scr = nullptr, *wstate = nullptr;
+    double *H = nullptr, *b = nullptr, *dx = nullptr, *fact64 = nullptr, *wstate = nullptr;
     double *imu_out = nullptr;
     int32_t *imu_win = nullptr, *imu_color = nullptr;
     float *fact32 = nullptr;
@@ -231,7 +231,6 @@ DArgs dargs(xba_ctx *c) {
     D.dx = c->dx;
     D.fact64 = c->fact64;
     D.fact32 = c->fact32;
-    D.imu_scr = c->imu_scr;
     D.imu_out = c->imu_out;
     D.imu_win = c->imu_win;
     D.imu_color = c->imu_color;
@@ -1053,7 +1052,6 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->fact64 = dalloc<double>(h0, O);  // FP64 factor: the FP64 solve and the Gauss-Newton step of the adjoint
     c->adj_y = dalloc<double>(c->nstate, O);
     c->fact32 = mx ? dalloc<float>(h0, O) : dalloc<float>(1, O);
-    c->imu_scr = dalloc<double>(465 * (size_t)std::max(c->nimu, 1), O);
     c->imu_out = dalloc<double>(IMU_OUT * (size_t)std::max(c->nimu, 1), O);
     c->imu_win = dalloc<int32_t>(std::max(c->nimu, 1), O);
     c->imu_color = dalloc<int32_t>(std::max(c->nimu, 1), O);

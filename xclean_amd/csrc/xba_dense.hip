@@ -249,7 +249,6 @@ struct DArgs {
     double *R2, *p2, *v2, *g2, *a2;       // trial state
     double *H, *b, *dx, *fact64;          // (sum n^2), (sum n), (sum n), (sum n^2)
     float *fact32;                        // (sum n^2)
-    double *imu_scr;                      // (nimu, 465)
     double *imu_out;                      // (nimu, IMU_OUT) per-factor weighted blocks, gradients and cost
     const int32_t *imu_win;               // (nimu) window of every IMU factor
     const int32_t *imu_color;             // (nimu) colour of every IMU factor (no shared slot within a colour)
@@ -362,7 +361,23 @@ __device__ inline void reduce_window(const DArgs &D, int w, int64_t idx, const T
         Acc<T> v;
         if (ca >= 0) {
             const int s = ca >> 3, i = ca & 7, gs = W.slot0 + s;
-            for (int q = D.bslot_start[gs]; q < D.bslot_start[gs + 1]; q++) {
+            const int q1 = D.bslot_start[gs + 1];
+            int q = D.bslot_start[gs];
+            for (; q + 8 <= q1; q += 8) {  // as above: eight entries' loads issued together, added in list order
+                T h[8], hl[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int64_t e = (int64_t)D.bslot_ent[q + u] * 6 + i;
+                    h[u] = partb[e];
+                    hl[u] = kah ? partb_lo[e] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    v.add(h[u]);
+                    if (kah) v.add(hl[u]);
+                }
+            }
+            for (; q < q1; q++) {
                 const int64_t e = (int64_t)D.bslot_ent[q] * 6 + i;
                 v.add(partb[e]);
                 if (kah) v.add(partb_lo[e]);
@@ -405,6 +420,7 @@ __global__ void k_reduce(DArgs D, const T *partH, const T *partb, const T *partH
 // cost only, at the trial state).  k_terms adds them to the window's system in factor order.
 __global__ __launch_bounds__(256) void k_imu(DArgs D, int trial) {
     __shared__ double sr[15], sJi[225], sJj[225];
+    __shared__ double scr[465];  // unwhitened residual and Jacobians from the first thread
     __shared__ double se;
     const int fi = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int w = D.imu_win[fi];
@@ -413,7 +429,7 @@ __global__ __launch_bounds__(256) void k_imu(DArgs D, int trial) {
     const double *v = trial ? D.v2 : D.vel, *bg = trial ? D.g2 : D.bgs, *ba = trial ? D.a2 : D.bas;
     const double kd = D.win_par[(size_t)XBA_WPAR * w + 6];
     const double *g = D.win_par + (size_t)XBA_WPAR * w + 20;
-    double *scr = D.imu_scr + 465 * (size_t)fi, *o = D.imu_out + (size_t)IMU_OUT * fi;
+    double *o = D.imu_out + (size_t)IMU_OUT * fi;
     const double *Wm = D.imu_rec + (size_t)XBA_IMU_REC * fi + 76;
     if (tid == 0) {
         const int i = W.slot0 + D.imu_ij[2 * fi], j = W.slot0 + D.imu_ij[2 * fi + 1];
@@ -463,6 +479,18 @@ __global__ __launch_bounds__(256) void k_imu(DArgs D, int trial) {
         o[IMU_O_B + q] = wgt * acc;
     }
 }
+
+// Sum and max over a wavefront (butterfly), the result in every lane.
+__device__ inline double dwave_sum(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ inline double dwave_max(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+#define IMU_BATCH 8  // IMU block entries a thread scatters at a time
 
 // ------------------------------------------------------------------------------------------- k_terms --
 // The prior and IMU terms of one window (one workgroup per window).  trial == 0: add H, b and the cost at the
@@ -528,35 +556,81 @@ __device__ inline void terms_window(const DArgs &D, int w, int trial, double *sd
         __syncthreads();
     }
     // ---- IMU factors (vio/window.py:831-862): their weighted whitened blocks come from k_imu.  The cost is summed
-    // in factor order; the blocks are added one colour at a time (factors of one colour share no slot, colours
+    // in a fixed order; the blocks are added one colour at a time (factors of one colour share no slot, colours
     // assigned in factor order at planning), so every H entry receives its terms in a fixed order.
     const int nf = W.imu1 - W.imu0;
-    if (tid == 0)
-        for (int f = 0; f < nf; f++) cost += D.imu_out[(size_t)IMU_OUT * (W.imu0 + f) + IMU_O_COST];
-    if (!trial)
+    if (tid < 64) {  // the first wavefront sums the factors' costs in a fixed order: lane-strided, then the lanes
+        double ic = 0.0;
+        for (int f = tid; f < nf; f += 64) ic += D.imu_out[(size_t)IMU_OUT * (W.imu0 + f) + IMU_O_COST];
+        ic = dwave_sum(ic);
+        if (tid == 0) cost += ic;
+    }
+    if (!trial && nf > 0) {
+        // per factor its colour and its two slots' columns, staged in LDS (scol is free after the prior) when they
+        // fit, so that the scatter below needs one global round trip per batch
+        const bool staged = 3 * nf <= XBA_MAX_STATE;
+        int *fcol = scol, *fci = scol + nf, *fcj = scol + 2 * nf;
+        if (staged) {
+            for (int f = tid; f < nf; f += nt) {
+                const int fi = W.imu0 + f;
+                fcol[f] = D.imu_color[fi];
+                fci[f] = D.slot_off[W.slot0 + D.imu_ij[2 * fi]];
+                fcj[f] = D.slot_off[W.slot0 + D.imu_ij[2 * fi + 1]];
+            }
+            __syncthreads();
+        }
         for (int col = 0; col < W.imu_ncol; col++) {
-            for (int q = tid; q < nf * 705; q += nt) {
-                const int f = q / 705, e = q % 705, fi = W.imu0 + f;
-                if (D.imu_color[fi] != col) continue;
-                const double v = D.imu_out[(size_t)IMU_OUT * fi + e];
-                const int i = D.imu_ij[2 * fi], j = D.imu_ij[2 * fi + 1];
-                const int ci = D.slot_off[W.slot0 + i], cj = D.slot_off[W.slot0 + j];
-                if (e >= IMU_O_B) {
-                    b[(e - IMU_O_B < 15 ? ci : cj) + (e - IMU_O_B) % 15] += v;
-                    continue;
+            // IMU_BATCH entries per thread at a time: their targets and values loaded first, then the sums stored
+            // (each target entry receives one term per colour, so the order within it is free)
+            for (int q0 = tid; q0 < nf * 705; q0 += IMU_BATCH * nt) {
+                double val[IMU_BATCH], t1[IMU_BATCH], t2[IMU_BATCH];
+                double *d1[IMU_BATCH], *d2[IMU_BATCH];
+#pragma unroll
+                for (int u = 0; u < IMU_BATCH; u++) {
+                    const int q = q0 + u * nt;
+                    d1[u] = d2[u] = nullptr;
+                    val[u] = 0.0;
+                    if (q >= nf * 705) continue;
+                    const int f = q / 705, e = q % 705, fi = W.imu0 + f;
+                    int ci, cj;
+                    if (staged) {
+                        if (fcol[f] != col) continue;
+                        ci = fci[f];
+                        cj = fcj[f];
+                    } else {
+                        if (D.imu_color[fi] != col) continue;
+                        ci = D.slot_off[W.slot0 + D.imu_ij[2 * fi]];
+                        cj = D.slot_off[W.slot0 + D.imu_ij[2 * fi + 1]];
+                    }
+                    val[u] = D.imu_out[(size_t)IMU_OUT * fi + e];
+                    if (e >= IMU_O_B) {
+                        d1[u] = b + (e - IMU_O_B < 15 ? ci : cj) + (e - IMU_O_B) % 15;
+                        continue;
+                    }
+                    const int which = e / 225, ac = e % 225, a = ac / 15, c = ac % 15;
+                    if (which == 0) {
+                        d1[u] = H + (size_t)(ci + a) * n + ci + c;
+                    } else if (which == 1) {
+                        d1[u] = H + (size_t)(cj + a) * n + cj + c;
+                    } else {
+                        d1[u] = H + (size_t)(ci + a) * n + cj + c;
+                        d2[u] = H + (size_t)(cj + c) * n + ci + a;
+                    }
                 }
-                const int which = e / 225, ac = e % 225, a = ac / 15, c = ac % 15;
-                if (which == 0) {
-                    H[(size_t)(ci + a) * n + ci + c] += v;
-                } else if (which == 1) {
-                    H[(size_t)(cj + a) * n + cj + c] += v;
-                } else {
-                    H[(size_t)(ci + a) * n + cj + c] += v;
-                    H[(size_t)(cj + c) * n + ci + a] += v;
+#pragma unroll
+                for (int u = 0; u < IMU_BATCH; u++) {
+                    if (d1[u]) t1[u] = *d1[u];
+                    if (d2[u]) t2[u] = *d2[u];
+                }
+#pragma unroll
+                for (int u = 0; u < IMU_BATCH; u++) {
+                    if (d1[u]) *d1[u] = t1[u] + val[u];
+                    if (d2[u]) *d2[u] = t2[u] + val[u];
                 }
             }
             __syncthreads();
         }
+    }
     if (tid == 0) {
         double *ws = D.wstate + (size_t)NWS * w;
         if (trial) {
@@ -815,16 +889,6 @@ __device__ __attribute__((always_inline)) inline void chol_solve_wg(const T *Lm,
         }
         __syncthreads();
     }
-}
-
-// Sum and max over a wavefront (butterfly), the result in every lane.
-__device__ inline double dwave_sum(double v) {
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ inline double dwave_max(double v) {
-    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
-    return v;
 }
 
 // The trial state of a window from its update dx (vio/window.py:522-532, 945-955): R <- R Exp(dphi), p += dp, and
