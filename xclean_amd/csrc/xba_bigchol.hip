@@ -162,12 +162,13 @@ __device__ inline void big_row_axpy(const T (*sLt)[XBA_TILE], T x, T (&v)[XBA_TI
     }
 }
 
-// sum over m > J of v[m] sLt[J][m], the row read likewise.
+// sum over m > J of v[m] sLt[J][m], the row read likewise, in four interleaved partial sums (m mod 4) added at the
+// end: four independent multiply-add chains instead of one.
 template <typename T, int J>
 __device__ inline T big_row_dot(const T (*sLt)[XBA_TILE], const T (&v)[XBA_TILE]) {
     typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
     constexpr int W = 16 / sizeof(T), C = 16;
-    T acc = (T)0;
+    T acc[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
     for (int c0 = ((J + 1) / C) * C; c0 < XBA_TILE; c0 += C) {
         T l[C];
@@ -179,9 +180,9 @@ __device__ inline T big_row_dot(const T (*sLt)[XBA_TILE], const T (&v)[XBA_TILE]
         }
 #pragma unroll
         for (int e = 0; e < C; e++)
-            if (c0 + e > J) acc += v[c0 + e] * l[e];
+            if (c0 + e > J) acc[e & 3] += v[c0 + e] * l[e];
     }
-    return acc;
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 template <typename T, int J>
@@ -398,9 +399,10 @@ __device__ inline double big_red16(double v) {
     return v;
 }
 
+#define TRSV_BATCH 8  // loads a thread of the triangular solve issues together
 template <typename T>
 __global__ __launch_bounds__(1024) void k_big_trsv(DArgs D, int it) {
-    __shared__ T sv[XBA_MAX_STATE];
+    __shared__ __align__(16) T sv[XBA_MAX_STATE];
     __shared__ T st[XBA_TILE];
     __shared__ T spart[16][XBA_TILE];
     __shared__ double sred[32];
@@ -446,30 +448,98 @@ __global__ __launch_bounds__(1024) void k_big_trsv(DArgs D, int it) {
     }
     __syncthreads();
     const T *F = big_base<T>(D) + B.f0, *Dinv = F + (size_t)np * np;
-    // forward: y_I = L(I,I)^-1 (s_I - sum_{J<I} L(I,J) y_J), 16 threads per row
+    // forward: y_I = L(I,I)^-1 (s_I - sum_{J<I} L(I,J) y_J), 16 threads per row, each reading 16 bytes of every
+    // 16-vector-wide column chunk of the row, TRSV_BATCH chunks' loads issued together.  Each stage's loads of its
+    // diagonal-tile inverse row are issued at its start, and the next stage's first batch of row chunks before its
+    // barriers, so that they arrive while the stage waits.
     {
+        constexpr int V = 16 / sizeof(T), CW = 16 * V;
+        typedef T VT __attribute__((ext_vector_type(V)));
         const int i = tid >> 4, q = tid & 15;
+        VT rv[TRSV_BATCH];  // the current stage's first batch
         for (int I = 0; I < nb; I++) {
+            const int nch = I * XBA_TILE / CW;
+            const T *Dr = Dinv + (size_t)(I * XBA_TILE + i) * XBA_TILE;
+            T dv[XBA_TILE / 16];
+#pragma unroll
+            for (int u = 0; u < XBA_TILE / 16; u++) dv[u] = Dr[q + 16 * u];
             const T *row = F + (size_t)(I * XBA_TILE + i) * np;
             T acc = (T)0;
-            for (int c = q; c < I * XBA_TILE; c += 16) acc += row[c] * sv[c];
+            for (int k0 = 0; k0 < nch; k0 += TRSV_BATCH) {
+                if (k0 > 0) {
+#pragma unroll
+                    for (int u = 0; u < TRSV_BATCH; u++)
+                        if (k0 + u < nch) rv[u] = *(const VT *)(row + (k0 + u) * CW + V * q);
+                }
+#pragma unroll
+                for (int u = 0; u < TRSV_BATCH; u++)
+                    if (k0 + u < nch) {
+                        const VT sx = *(const VT *)(sv + (k0 + u) * CW + V * q);
+#pragma unroll
+                        for (int e = 0; e < V; e++) acc += rv[u][e] * sx[e];
+                    }
+            }
+            if (I + 1 < nb) {
+                const T *nrow = F + (size_t)((I + 1) * XBA_TILE + i) * np;
+                const int nn = (I + 1) * XBA_TILE / CW;
+#pragma unroll
+                for (int u = 0; u < TRSV_BATCH; u++)
+                    if (u < nn) rv[u] = *(const VT *)(nrow + u * CW + V * q);
+            }
             acc = big_red16(acc);
             if (q == 0) st[i] = sv[I * XBA_TILE + i] - acc;
             __syncthreads();
-            const T *Dr = Dinv + (size_t)(I * XBA_TILE + i) * XBA_TILE;
             T y = (T)0;
-            for (int c = q; c <= i; c += 16) y += Dr[c] * st[c];
+#pragma unroll
+            for (int u = 0; u < XBA_TILE / 16; u++) {
+                const int c = q + 16 * u;
+                if (c <= i) y += dv[u] * st[c];
+            }
             y = big_red16(y);
             if (q == 0) sv[I * XBA_TILE + i] = y;
             __syncthreads();
         }
     }
-    // backward: x_I = L(I,I)^-T (y_I - sum_{J>I} L(J,I)^T x_J), a column per lane, 16 row groups
+    // backward: x_I = L(I,I)^-T (y_I - sum_{J>I} L(J,I)^T x_J), a column per lane, 16 row groups, TRSV_BATCH rows'
+    // loads issued together; the diagonal-tile inverse entries at the stage's start and the next stage's first
+    // batch before its barriers, as above
     {
         const int c = tid & 63, g = tid >> 6;
+        T fv[TRSV_BATCH];  // the current stage's first batch
+        {
+            const int I = nb - 1;
+#pragma unroll
+            for (int u = 0; u < TRSV_BATCH; u++) {
+                const int rr = (I + 1) * XBA_TILE + g + 16 * u;
+                fv[u] = rr < np ? F[(size_t)rr * np + I * XBA_TILE + c] : (T)0;
+            }
+        }
         for (int I = nb - 1; I >= 0; I--) {
+            T dv[XBA_TILE / 16];
+#pragma unroll
+            for (int u = 0; u < XBA_TILE / 16; u++) dv[u] = Dinv[(size_t)(I * XBA_TILE + g + 16 * u) * XBA_TILE + c];
             T acc = (T)0;
-            for (int rr = (I + 1) * XBA_TILE + g; rr < np; rr += 16) acc += F[(size_t)rr * np + I * XBA_TILE + c] * sv[rr];
+            for (int r0 = (I + 1) * XBA_TILE + g; r0 < np; r0 += 16 * TRSV_BATCH) {
+                if (r0 > (I + 1) * XBA_TILE + g) {
+#pragma unroll
+                    for (int u = 0; u < TRSV_BATCH; u++) {
+                        const int rr = r0 + 16 * u;
+                        fv[u] = rr < np ? F[(size_t)rr * np + I * XBA_TILE + c] : (T)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < TRSV_BATCH; u++) {
+                    const int rr = r0 + 16 * u;
+                    if (rr < np) acc += fv[u] * sv[rr];
+                }
+            }
+            if (I > 0) {
+#pragma unroll
+                for (int u = 0; u < TRSV_BATCH; u++) {
+                    const int rr = I * XBA_TILE + g + 16 * u;
+                    fv[u] = rr < np ? F[(size_t)rr * np + (I - 1) * XBA_TILE + c] : (T)0;
+                }
+            }
             spart[g][c] = acc;
             __syncthreads();
             if (g == 0) {
@@ -479,8 +549,11 @@ __global__ __launch_bounds__(1024) void k_big_trsv(DArgs D, int it) {
             }
             __syncthreads();
             T xv = (T)0;
-            for (int i = g; i < XBA_TILE; i += 16)
-                if (i >= c) xv += Dinv[(size_t)(I * XBA_TILE + i) * XBA_TILE + c] * st[i];
+#pragma unroll
+            for (int u = 0; u < XBA_TILE / 16; u++) {
+                const int i = g + 16 * u;
+                if (i >= c) xv += dv[u] * st[i];
+            }
             spart[g][c] = xv;
             __syncthreads();
             if (g == 0) {
