@@ -387,7 +387,7 @@ void launch_big(xba_ctx *c, const DArgs &D, hipStream_t s) {
     hipLaunchKernelGGL(k_big_prep<T>, dim3(c->big_npmax, c->nbig), dim3(256), 0, s, D, 0);
     for (int k = 0; k < c->big_nbmax; k++) {
         const int below = c->big_nbmax - k - 1;
-        hipLaunchKernelGGL(k_big_panel<T>, dim3(1 + (below + 3) / 4, c->nbig), dim3(256), 0, s, D, k);
+        hipLaunchKernelGGL(k_big_panel<T>, dim3((below + 4) / 4, c->nbig), dim3(256), 0, s, D, k);
         if (below > 0)
             hipLaunchKernelGGL(k_big_update<T>, dim3(below * (below + 1) / 2, c->nbig), dim3(256), 0, s, D, k);
     }

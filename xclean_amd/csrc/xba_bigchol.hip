@@ -3,9 +3,9 @@
 //
 //   k_big_prep    Hd = H + diag(lam diag(H) + 1e-9) (vio/window.py:933-936), Jacobi-scaled in mixed precision,
 //                 copied into the padded layout (identity on the padding), lower tiles only
-//   k_big_panel   step k: the diagonal tile factored in registers by one wavefront (every workgroup of the launch
-//                 factors it; the first one also inverts it into a side buffer), then L(i,k) = A(i,k) L(k,k)^-T for
-//                 the tiles below it, one tile per wavefront with a row per lane
+//   k_big_panel   step k: L(i,k) = A(i,k) L(k,k)^-T for the tiles below the diagonal tile and L(k,k)^-1 into a
+//                 side buffer, one per wavefront with a row per lane, each wavefront factoring the diagonal tile
+//                 in its registers fused with its triangular solve
 //   k_big_update  step k: A(i,j) -= L(i,k) L(j,k)^T over the trailing lower tiles on the matrix cores (FP32 or
 //                 FP64 16x16x4 MFMA), one tile per workgroup, 32 x 32 per wavefront
 //   k_big_trsv    one solve L L^T y = s of the scaled residual with the tiled factor (one workgroup per window:
@@ -132,104 +132,39 @@ __device__ inline void big_row_store(T *p, const T (&v)[XBA_TILE]) {
     }
 }
 
-#define BIG_WAVE_SYNC()                                        \
-    do {                                                       \
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
-        __builtin_amdgcn_s_waitcnt(0);                         \
-        __builtin_amdgcn_wave_barrier();                       \
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); \
-    } while (0)
-
 // --------------------------------------------------------------------------------------------- k_big_panel --
-// Row j of the transposed factor sLt (entries m > j) times x, subtracted from v[m]: the row read in 16-entry vector
-// chunks from LDS ahead of the chunk's multiply-adds (every lane reads the same addresses: broadcasts).
-template <typename T, int J>
-__device__ inline void big_row_axpy(const T (*sLt)[XBA_TILE], T x, T (&v)[XBA_TILE]) {
-    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
-    constexpr int W = 16 / sizeof(T), C = 16;
-#pragma unroll
-    for (int c0 = ((J + 1) / C) * C; c0 < XBA_TILE; c0 += C) {
-        T l[C];
-#pragma unroll
-        for (int q = 0; q < C / W; q++) {
-            const V4 r = *(const V4 *)(&sLt[J][c0 + W * q]);
-#pragma unroll
-            for (int e = 0; e < W; e++) l[W * q + e] = r[e];
-        }
-#pragma unroll
-        for (int e = 0; e < C; e++)
-            if (c0 + e > J) v[c0 + e] -= x * l[e];
-    }
-}
-
-// sum over m > J of v[m] sLt[J][m], the row read likewise, in four interleaved partial sums (m mod 4) added at the
-// end: four independent multiply-add chains instead of one.
-template <typename T, int J>
-__device__ inline T big_row_dot(const T (*sLt)[XBA_TILE], const T (&v)[XBA_TILE]) {
-    typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
-    constexpr int W = 16 / sizeof(T), C = 16;
-    T acc[4] = {(T)0, (T)0, (T)0, (T)0};
-#pragma unroll
-    for (int c0 = ((J + 1) / C) * C; c0 < XBA_TILE; c0 += C) {
-        T l[C];
-#pragma unroll
-        for (int q = 0; q < C / W; q++) {
-            const V4 r = *(const V4 *)(&sLt[J][c0 + W * q]);
-#pragma unroll
-            for (int e = 0; e < W; e++) l[W * q + e] = r[e];
-        }
-#pragma unroll
-        for (int e = 0; e < C; e++)
-            if (c0 + e > J) acc[e & 3] += v[c0 + e] * l[e];
-    }
-    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
-}
-
-template <typename T, int J>
-struct BigSteps {
-    // forward substitution x L^T = v (row J of the lane's tile row), J = 0 .. XBA_TILE - 1
-    __device__ static inline void trsm(const T (*sLt)[XBA_TILE], const T *sinv, T (&v)[XBA_TILE]) {
-        const T xj = v[J] * sinv[J];
-        v[J] = xj;
-        big_row_axpy<T, J>(sLt, xj, v);
-        BigSteps<T, J + 1>::trsm(sLt, sinv, v);
-    }
-    // back substitution x L = e_lane (row `lane` of L^-1), columns J = XBA_TILE - 1 .. 0: call with J = XBA_TILE - 1
-    __device__ static inline void inv(const T (*sLt)[XBA_TILE], const T *sinv, int lane, T (&x)[XBA_TILE]) {
-        x[J] = (((lane == J) ? (T)1 : (T)0) - big_row_dot<T, J>(sLt, x)) * sinv[J];
-        BigSteps<T, J - 1>::inv(sLt, sinv, lane, x);
-    }
-};
-template <typename T>
-struct BigSteps<T, XBA_TILE> {
-    __device__ static inline void trsm(const T (*)[XBA_TILE], const T *, T (&)[XBA_TILE]) {}
-};
-template <typename T>
-struct BigSteps<T, -1> {
-    __device__ static inline void inv(const T (*)[XBA_TILE], const T *, int, T (&)[XBA_TILE]) {}
-};
-
-// a[m] -= lij L[m][J] for m = M0 .. XBA_TILE - 1, L[m][J] broadcast from lane m's a[J]: the broadcasts issued in
-// groups ahead of their multiply-adds (a broadcast's scalar result read by the next instruction stalls the
-// wavefront), each group at most BIG_BC_GROUP values (scalar registers: one per FP32, two per FP64 value).
+// One column step J of the diagonal tile's factor fused with the same step of a triangular solve x L^T = v, on one
+// wavefront: lane r holds row r of the tile in a (entries above the diagonal zero) and row r of the right-hand side
+// in v.  The pivot's inverse square root scales column J of L (lane m's a[J], m > J), x_J = v_J / L(J,J), and every
+// later entry takes the rank-one updates a[m] -= L(r,J) L(m,J), v[m] -= x_J L(m,J), L(m,J) broadcast from lane m's
+// register.  The broadcasts are issued in groups ahead of their multiply-adds (a broadcast's scalar result read by
+// the next instruction stalls the wavefront), each group at most BIG_BC_GROUP values (scalar registers: one per
+// FP32, two per FP64 value).  The FP32 factor (the mixed solve's, refined in FP64 against the FP64 matrix) takes its
+// pivots' inverse square roots from the hardware approximation (within an ulp); the FP64 factor's are correctly
+// rounded.  ok = 0 and the remaining steps skipped at a pivot that is not positive and finite (uniform: every lane
+// reads the same pivot).
 #define BIG_BC_GROUP 8
 template <typename T, int J, int M0>
-__device__ inline void big_rank1(T (&a)[XBA_TILE], T lij) {
+__device__ inline void big_rank1x2(T (&a)[XBA_TILE], T (&v)[XBA_TILE], T lij, T xj) {
     constexpr int N = (XBA_TILE - M0 < BIG_BC_GROUP) ? XBA_TILE - M0 : BIG_BC_GROUP;
     T bc[N];
 #pragma unroll
     for (int e = 0; e < N; e++) bc[e] = big_bcast(a[J], M0 + e);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int e = 0; e < N; e++) a[M0 + e] -= lij * bc[e];
-    if constexpr (M0 + N < XBA_TILE) big_rank1<T, J, M0 + N>(a, lij);
+    for (int e = 0; e < N; e++) {
+        a[M0 + e] -= lij * bc[e];
+        v[M0 + e] -= xj * bc[e];
+        // the updates happen here: otherwise the compiler defers each entry's update to its use at step M0 + e and
+        // keeps the broadcasts live in scalar registers until then
+        asm volatile("" : "+v"(a[M0 + e]), "+v"(v[M0 + e]));
+    }
+    if constexpr (M0 + N < XBA_TILE) big_rank1x2<T, J, M0 + N>(a, v, lij, xj);
 }
 
-// Column J of the diagonal tile's factor, lane r holding row r (entries above the diagonal unused), then the
-// columns after it; ok = 0 and the remaining columns skipped at a pivot that is not positive and finite (uniform).
 template <typename T, int J>
-struct BigChol {
-    __device__ static inline void step(T (&a)[XBA_TILE], int lane, T &dinv, int &ok) {
+struct BigPanelStep {
+    __device__ static inline void step(T (&a)[XBA_TILE], T (&v)[XBA_TILE], int lane, int &ok) {
         const T d = big_bcast(a[J], J);
         if (!(d > (T)0) || !isfinite(d)) {
             ok = 0;
@@ -243,78 +178,59 @@ struct BigChol {
             sd = sqrt(d);
             inv = (T)1 / sd;
         }
-        if (lane == J) dinv = inv;
         a[J] = (lane == J) ? sd : a[J] * inv;  // L[lane][J] below the diagonal
-        // a[m] -= L[lane][J] L[m][J] (lanes up to J change only unused entries)
+        const T xj = v[J] * inv;
+        v[J] = xj;
         if constexpr (J + 1 < XBA_TILE) {
-            big_rank1<T, J, J + 1>(a, (lane > J) ? a[J] : (T)0);
-            BigChol<T, J + 1>::step(a, lane, dinv, ok);
+            // lanes up to J change only entries of a that are not used
+            big_rank1x2<T, J, J + 1>(a, v, (lane > J) ? a[J] : (T)0, xj);
+            BigPanelStep<T, J + 1>::step(a, v, lane, ok);
         }
     }
 };
 
-// grid (1 + ceil((max nb - k - 1) / 4), nbig), 256 threads.  Every workgroup factors the diagonal tile (k,k) in
-// registers (wavefront 0, lane r holding row r, each pivot's column broadcast from the lanes' registers); workgroup
-// 0 then inverts it into the side buffer, and wavefront w of workgroup x >= 1 solves tile row i = k + 1 + 4 (x - 1)
-// + w.  The FP32 factor (the mixed solve's, refined in FP64 against the FP64 matrix) takes its pivots' inverse
-// square roots from the hardware approximation (within an ulp); the FP64 factor's are correctly rounded.
+// grid (ceil((max nb - k) / 4), nbig), 256 threads.  Wavefront w of workgroup x takes task t = 4 x + w of the
+// window's nb - k tasks: t = 0 the inverse of the diagonal tile's factor L(k,k) (right-hand side: the identity rows,
+// so lane j solves column j of L(k,k)^-1) into the side buffer, t >= 1 the tile row i = k + t: L(i,k) = A(i,k)
+// L(k,k)^-T.  Every wavefront factors the diagonal tile itself, in registers, fused with its solve: no LDS and no
+// barriers.  The wavefronts' factors are identical (the same instructions on the same values).
 template <typename T>
 __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
-    __shared__ __align__(16) T sLt[XBA_TILE][XBA_TILE];  // the factor transposed: sLt[j][m] = L[m][j] (0, m < j)
-    __shared__ __align__(16) T sinv[XBA_TILE];
     __shared__ int sok;
     const BigWin B = D.bigs[blockIdx.y];
-    if (k >= B.nb) return;
-    const int x = blockIdx.x, i = k + 1 + 4 * (x - 1) + (int)(threadIdx.x >> 6);
-    if (x > 0 && k + 1 + 4 * (x - 1) >= B.nb) return;  // no tile row of this window for this workgroup
+    const int ntask = B.nb - k;
+    if ((int)blockIdx.x * 4 >= ntask) return;  // (uniform per workgroup)
     double *ws = D.wstate + (size_t)NWS * B.win;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, np = B.np;
+    const int tid = threadIdx.x, lane = tid & 63, t = 4 * (int)blockIdx.x + (tid >> 6), np = B.np;
     // the flag is read once per workgroup: another workgroup of this launch may clear it meanwhile
     if (tid == 0) sok = ws[WS_BIGOK] != 0.0;
     __syncthreads();
-    if (!sok) return;
-    __syncthreads();  // every thread has read sok before wavefront 0 rewrites it
+    if (!sok || t >= ntask) return;
     T *F = big_base<T>(D) + B.f0;
-    if (wv == 0) {
-        // lane r holds row r of the diagonal tile (lower part; 0 above the diagonal)
-        const T *Ar = F + (size_t)(k * XBA_TILE + lane) * np + k * XBA_TILE;
-        T a[XBA_TILE];
-        big_row_load(Ar, a);
+    T a[XBA_TILE], v[XBA_TILE];
+    big_row_load(F + (size_t)(k * XBA_TILE + lane) * np + k * XBA_TILE, a);
 #pragma unroll
-        for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? a[m] : (T)0;
-        int ok = 1;
-        T dinv = (T)0;
-        BigChol<T, 0>::step(a, lane, dinv, ok);
-        if (lane == 0) sok = ok;
-        if (ok) {
+    for (int m = 0; m < XBA_TILE; m++) a[m] = (m <= lane) ? a[m] : (T)0;
+    T *Xr = F + (size_t)((k + t) * XBA_TILE + lane) * np + k * XBA_TILE;  // (t >= 1)
+    if (t == 0) {
 #pragma unroll
-            for (int m = 0; m < XBA_TILE; m++) sLt[m][lane] = (m <= lane) ? a[m] : (T)0;
-            sinv[lane] = dinv;
-            if (x == 0) {
-                // row `lane` of L(k,k)^-1 (X L = I, X lower triangular, by back substitution over the columns) into
-                // the side buffer (row-major, 64 wide)
-                BIG_WAVE_SYNC();
-                T xr[XBA_TILE];
-#pragma unroll
-                for (int m = 0; m < XBA_TILE; m++) xr[m] = (T)0;
-                BigSteps<T, XBA_TILE - 1>::inv(sLt, sinv, lane, xr);
-                T *Dr = F + (size_t)np * np + (size_t)(k * XBA_TILE + lane) * XBA_TILE;
-                big_row_store(Dr, xr);
-            }
-        }
+        for (int m = 0; m < XBA_TILE; m++) v[m] = (m == lane) ? (T)1 : (T)0;
+    } else {
+        big_row_load(Xr, v);
     }
-    __syncthreads();
-    if (!sok) {
-        if (tid == 0) ws[WS_BIGOK] = 0.0;
+    int ok = 1;
+    BigPanelStep<T, 0>::step(a, v, lane, ok);
+    if (!ok) {
+        if (lane == 0) ws[WS_BIGOK] = 0.0;
         return;
     }
-    // L(i,k) = A(i,k) L(k,k)^-T: lane r solves x L^T = a for row r of the tile by forward substitution
-    if (x > 0 && i < B.nb) {
-        T *Xr = F + (size_t)(i * XBA_TILE + lane) * np + k * XBA_TILE;
-        T xv[XBA_TILE];
-        big_row_load(Xr, xv);
-        BigSteps<T, 0>::trsm(sLt, sinv, xv);
-        big_row_store(Xr, xv);
+    if (t == 0) {
+        // lane j holds column j of L(k,k)^-1: row m of the side buffer (row-major, 64 wide) gets entry j from lane j
+        T *Dc = F + (size_t)np * np + (size_t)(k * XBA_TILE) * XBA_TILE + lane;
+#pragma unroll
+        for (int m = 0; m < XBA_TILE; m++) Dc[(size_t)m * XBA_TILE] = v[m];
+    } else {
+        big_row_store(Xr, v);
     }
 }
 
