@@ -306,12 +306,24 @@ __global__ __launch_bounds__(256) void k_big_update(DArgs D, int k) {
 // grid (nbig), 1024 threads.  Refinement step `it`: it = 0 solves for the right-hand side b from x = 0; a later step
 // first tests the residual of the previous one (|r| <= 1e-15 |b|: converged, nothing more to do), then solves for r.
 // The solve runs in the factor's precision on s = scale * rhs and adds scale * y to x (FP64).
-__device__ inline float big_red16(float v) {
-    for (int d = 8; d >= 1; d >>= 1) v += __shfl_xor(v, d, 16);
-    return v;
+// (sum over each aligned group of 16 lanes, by DPP row rotations: every lane of the group ends with its total)
+template <int CTRL>
+__device__ inline float big_ror(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
 }
-__device__ inline double big_red16(double v) {
-    for (int d = 8; d >= 1; d >>= 1) v += __shfl_xor(v, d, 16);
+template <int CTRL>
+__device__ inline double big_ror(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <typename T>
+__device__ inline T big_red16(T v) {
+    v += big_ror<0x128>(v);  // row_ror:8
+    v += big_ror<0x124>(v);  // row_ror:4
+    v += big_ror<0x122>(v);  // row_ror:2
+    v += big_ror<0x121>(v);  // row_ror:1
     return v;
 }
 
