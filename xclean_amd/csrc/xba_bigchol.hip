@@ -139,9 +139,8 @@ __device__ inline void big_row_store(T *p, const T (&v)[XBA_TILE]) {
 // later entry takes the rank-one updates a[m] -= L(r,J) L(m,J), v[m] -= x_J L(m,J), L(m,J) broadcast from lane m's
 // register.  The broadcasts are issued in groups ahead of their multiply-adds (a broadcast's scalar result read by
 // the next instruction stalls the wavefront), each group at most BIG_BC_GROUP values (scalar registers: one per
-// FP32, two per FP64 value).  The FP32 factor (the mixed solve's, refined in FP64 against the FP64 matrix) takes its
-// pivots' inverse square roots from the hardware approximation (within an ulp); the FP64 factor's are correctly
-// rounded.  ok = 0 and the remaining steps skipped at a pivot that is not positive and finite (uniform: every lane
+// FP32, two per FP64 value).  The pivots' inverse square roots come from the hardware approximation (FP32: within an
+// ulp; FP64: refined by two Newton steps to within an ulp); the pivot is d times it.  ok = 0 and the remaining steps skipped at a pivot that is not positive and finite (uniform: every lane
 // reads the same pivot).
 #define BIG_BC_GROUP 8
 template <typename T, int J, int M0>
@@ -173,11 +172,14 @@ struct BigPanelStep {
         T sd, inv;
         if constexpr (sizeof(T) == 4) {
             inv = __builtin_amdgcn_rsqf(d);
-            sd = d * inv;
         } else {
-            sd = sqrt(d);
-            inv = (T)1 / sd;
+            // the hardware approximation refined by two Newton steps y <- y + y (1/2 - (d/2) y^2): within an ulp
+            const double h = 0.5 * d;
+            inv = __builtin_amdgcn_rsq(d);
+            inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
+            inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
         }
+        sd = d * inv;
         a[J] = (lane == J) ? sd : a[J] * inv;  // L[lane][J] below the diagonal
         const T xj = v[J] * inv;
         v[J] = xj;
