@@ -239,12 +239,14 @@ __global__ __launch_bounds__(256) void k_big_panel(DArgs D, int k) {
 // -------------------------------------------------------------------------------------------- k_big_update --
 // grid (m (m + 1) / 2 for m = max nb - k - 1, nbig): workgroup x updates the trailing lower tile (i, j),
 // i = k + 1 + ti, j = k + 1 + tj, x = ti (ti + 1) / 2 + tj.  Wavefront w computes the 32 x 32 quadrant
-// (rows 32 (w / 2), cols 32 (w % 2)) as 2 x 2 MFMA blocks of 16 x 16, over the tile's k-range in two halves staged
-// in LDS.
+// (rows 32 (w / 2), cols 32 (w % 2)) as 2 x 2 MFMA blocks of 16 x 16, over the tile's k-range staged in LDS whole
+// (both tiles read 16 bytes at a time, every load of the workgroup in flight together).
 template <typename T>
 __global__ __launch_bounds__(256) void k_big_update(DArgs D, int k) {
-    constexpr int KH = XBA_TILE / 2, LD = KH + (sizeof(T) == 4 ? 4 : 2);
-    __shared__ T sI[XBA_TILE][LD], sJ[XBA_TILE][LD];
+    constexpr int LD = XBA_TILE + (sizeof(T) == 4 ? 4 : 2), W = 16 / sizeof(T);  // (rows stay 16-byte aligned)
+    typedef T VT __attribute__((ext_vector_type(16 / sizeof(T))));
+    __shared__ __align__(16) T sI[XBA_TILE][LD];
+    __shared__ __align__(16) T sJ[XBA_TILE][LD];
     typedef typename Mfma<T>::V V;
     const BigWin B = D.bigs[blockIdx.y];
     const int m = B.nb - k - 1, x = blockIdx.x;
@@ -272,26 +274,34 @@ __global__ __launch_bounds__(256) void k_big_update(DArgs D, int k) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 acc[a][b][q] = C[(size_t)(r0 + 16 * a + Mfma<T>::row(lk, q)) * np + c0 + 16 * b + lr];
-    for (int h = 0; h < 2; h++) {
-        if (h) __syncthreads();  // every wavefront has read the first half
-        for (int e = tid; e < XBA_TILE * KH; e += blockDim.x) {
-            const int r = e / KH, c = e % KH;
-            sI[r][c] = Li[(size_t)r * np + h * KH + c];
-            sJ[r][c] = Lj[(size_t)r * np + h * KH + c];
+    {
+        constexpr int RW = XBA_TILE / W, NV = XBA_TILE * RW / 256;  // vectors per row; per thread and tile
+        VT va[NV], vb[NV];
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int e = tid + 256 * u, r = e / RW, c = (e % RW) * W;
+            va[u] = *(const VT *)(Li + (size_t)r * np + c);
+            vb[u] = *(const VT *)(Lj + (size_t)r * np + c);
         }
-        __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < KH / 4; kk++) {
-            T av[2], bv[2];
-#pragma unroll
-            for (int a = 0; a < 2; a++) av[a] = -sI[r0 + 16 * a + lr][4 * kk + lk];
-#pragma unroll
-            for (int b = 0; b < 2; b++) bv[b] = sJ[c0 + 16 * b + lr][4 * kk + lk];
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::fma(av[a], bv[b], acc[a][b]);
+        for (int u = 0; u < NV; u++) {
+            const int e = tid + 256 * u, r = e / RW, c = (e % RW) * W;
+            *(VT *)&sI[r][c] = va[u];
+            *(VT *)&sJ[r][c] = vb[u];
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < XBA_TILE / 4; kk++) {
+        T av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) av[a] = -sI[r0 + 16 * a + lr][4 * kk + lk];
+#pragma unroll
+        for (int b = 0; b < 2; b++) bv[b] = sJ[c0 + 16 * b + lr][4 * kk + lk];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::fma(av[a], bv[b], acc[a][b]);
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
