@@ -116,7 +116,8 @@ typedef struct xba_options {
                                    multi-workgroup Cholesky (64 x 64 tiles, trailing updates on MFMA); smaller ones
                                    by one workgroup each.  0 = automatic (192); -1 = never tiled */
     int32_t big_fp64;           /* mixed precision: 1 factors the tiled windows in FP64 (one solve, no refinement)
-                                   instead of FP32 with FP64 refinement; ignored in FP64 */
+                                   instead of FP32 with FP64 refinement; 2 every window (the one-workgroup solves
+                                   too); ignored in FP64 */
 } xba_options;
 
 typedef struct xba_ctx xba_ctx;

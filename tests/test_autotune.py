@@ -36,9 +36,9 @@ def test_candidate_space_holds_the_defaults_and_only_moving_thresholds():
     for k in AT.ORDER:
         assert sp[k][0] == 0
     assert sp["big_n"] == [0, 64, 96, 112, 128, 144, 160]    # 256 / 384 / 512 / never change nothing here
-    assert sp["big_fp64"] == [0, 1]
+    assert sp["big_fp64"] == [0, 1, 2]
     sp2 = AT.candidate_space(nlm=200_000, ncu=256, ns=np.array([900]))
     assert sp2["big_n"] == [0, -1]                           # the tiled solve or one workgroup
     assert AT.candidate_space(nlm=200_000, ncu=256, ns=np.array([900]), mixed=False)["big_fp64"] == [0]
-    assert AT.candidate_space(nlm=2_000, ncu=256, ns=np.array([30, 60]))["big_fp64"] == [0]  # nothing tiled
+    assert AT.candidate_space(nlm=2_000, ncu=256, ns=np.array([30, 60]))["big_fp64"] == [0, 2]  # nothing tiled
     assert all(8 <= v <= 4096 for v in sp2["lm_per_chunk"][1:])

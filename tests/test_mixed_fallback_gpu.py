@@ -4,7 +4,8 @@
   system are held only by the damping: Jacobi-scaled condition ~1e9, beyond FP32 refinement, the regime of
   backend/kernels.py:12-16): the FP32 refinement cannot converge, the window is flagged and solved in FP64, and its
   update and accept decision match the FP64 oracle.  The same for a window forced onto the tiled factorisation.
-- A well-conditioned batch takes no fallback and its refinement converges.
+- A well-conditioned batch takes no fallback and its refinement converges; with the plan options that factor the
+  tiled windows or every window in FP64, those windows take one solve and the update matches the FP64 path.
 - A landmark behind one of its cameras at the initial state (vio/window.py:483-485): build_system skips the row but
   the trial cost adds 1e6, so every step is rejected and lambda grows x10 per iteration, as in the oracle.
 - The noise-free truth: every whitened chi2 of the GPU linearisation is zero to rounding (measurements made by the
@@ -99,6 +100,31 @@ def test_well_conditioned_batch_takes_no_fallback():
         for s in bs.stats():
             assert s["fallback"] == 0 and s["refine_converged"] and s["chol_ok"]
             assert 1 <= s["refine_steps"] <= 6
+
+
+@pytest.mark.parametrize("big_fp64", [1, 2])
+def test_fp64_factor_plan_options_match_the_fp64_path(big_fp64):
+    """Mixed precision with the tiled windows (big_fp64 = 1) or every window (2) factored in FP64: one solve per
+    such window, no fallback, and the first update within 1e-6 of the FP64 path (whose only difference is the FP64
+    observation record)."""
+    from xclean_amd.window import BatchSolver
+    pbs = [synth.make_window(P=P, L=L, views=(2, min(10, P)), seed=[42, i])[0]
+           for i, (P, L) in enumerate([(5, 200), (12, 700), (30, 900), (20, 600)])]
+    ref = BatchSolver([pb.copy() for pb in pbs], precision="fp64", use_graph=False, big_n=64)
+    ref.upload(1e-4)
+    ref.iterate(1)
+    dx_ref = ref.last_dx()
+    bs = BatchSolver([pb.copy() for pb in pbs], precision="mixed", use_graph=False, big_n=64, big_fp64=big_fp64)
+    bs.upload(1e-4)
+    bs.iterate(1)
+    dx = bs.last_dx()
+    for pb, a, b, st in zip(pbs, dx, dx_ref, bs.stats()):
+        assert _rel(a, b) < 1e-6
+        assert st["fallback"] == 0 and st["chol_ok"] and st["refine_converged"]
+        if pb.n_state() > 64 or big_fp64 == 2:
+            assert st["refine_steps"] == 1  # factored in FP64: one solve
+        else:
+            assert 1 <= st["refine_steps"] <= 6
 
 
 def _behind_window():

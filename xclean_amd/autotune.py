@@ -7,8 +7,8 @@ Options tuned (``xba_options`` in include/xba.h):
   lm_per_chunk  landmarks per work item (0: the solver's rule, landmarks / (4 x compute units))
   big_n         reduced-system size above which a window's Cholesky is tiled over many workgroups (64 x 64 tiles,
                 trailing updates on the matrix cores) instead of one workgroup per window
-  big_fp64      mixed precision: the tiled windows factored in FP64 (one solve) instead of FP32 with FP64
-                refinement (several solves)
+  big_fp64      mixed precision: 1, the tiled windows factored in FP64 (one solve) instead of FP32 with FP64
+                refinement (several solves); 2, every window
   grid          persistent grid of the assembly kernels (0: one workgroup per compute unit)
 
 Search: coordinate descent over the options in that order, starting from the solver's defaults.  A candidate is
@@ -46,8 +46,9 @@ def candidate_space(nlm, ncu, ns, mixed=True):
         lm_per_chunk=[0] + sorted({max(8, auto_lpc // 2), min(4096, auto_lpc * 2), min(4096, auto_lpc * 4)}
                                   - {auto_lpc}),
         big_n=big,
-        # the tiled windows' factor precision matters only in mixed precision and when some window can be tiled
-        big_fp64=[0, 1] if (mixed and len(big) > 1 or (mixed and (ns > 192).any())) else [0],
+        # the factor precision matters only in mixed precision; 1 (the tiled windows in FP64) only when some window
+        # can be tiled, 2 (every window in FP64) always
+        big_fp64=([0, 1, 2] if (len(big) > 1 or (ns > 192).any()) else [0, 2]) if mixed else [0],
         grid=[0, 2 * ncu],
     )
 

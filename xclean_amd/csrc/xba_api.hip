@@ -132,7 +132,9 @@ struct xba_ctx {
     float *fact32 = nullptr;
     // windows factored by the tiled Cholesky
     int nbig = 0, big_npmax = 0, big_nbmax = 0, big_nmax = 0;
-    bool big64 = false;  // mixed precision with the tiled windows factored in FP64
+    bool big64 = false;    // mixed precision with the tiled windows factored in FP64
+    bool small64 = false;  // mixed precision with the one-workgroup windows factored in FP64 too
+    int small64_lds = 0;   // LDS of k_solve<double> for those windows
     BigWin *bigs = nullptr;
     float *bigF32 = nullptr;
     double *bigF64 = nullptr, *bigS = nullptr, *bigV = nullptr;
@@ -406,10 +408,16 @@ void launch_big(xba_ctx *c, const DArgs &D, hipStream_t s) {
 template <typename T>
 void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     const bool fork = c->solve_fork;  // both kinds present: k_solve beside the tiled sequence
+    auto small = [&](hipStream_t st) {  // the one-workgroup windows (FP64 for every precision with small64)
+        if (sizeof(T) == 4 && c->small64)
+            hipLaunchKernelGGL(k_solve<double>, dim3(c->nwin), dim3(512), c->small64_lds, st, D, c->small64_lds, 0);
+        else
+            hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, st, D, c->solve_lds_cap, 0);
+    };
     if (fork) {
         (void)hipEventRecord(c->ev_fork, s);
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, c->side, D, c->solve_lds_cap, 0);
+        small(c->side);
         (void)hipEventRecord(c->ev_join, c->side);
     }
     if (c->nbig > 0) {
@@ -421,7 +429,7 @@ void launch_solve(xba_ctx *c, const DArgs &D, hipStream_t s) {
     if (fork)
         (void)hipStreamWaitEvent(s, c->ev_join, 0);
     else if (c->nbig < c->nwin)
-        hipLaunchKernelGGL(k_solve<T>, dim3(c->nwin), dim3(512), c->solve_lds_cap, s, D, c->solve_lds_cap, 0);
+        small(s);
     // mixed precision: the windows flagged above are linearised again with FP64 observation records (the
     // multi-wavefront kernels; the FP32 records perturb H by ~1e-7 relative, which an ill-conditioned window
     // amplifies) and solved in FP64.  Workgroups of the other windows exit at once.
@@ -1058,6 +1066,7 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     c->wstate = dalloc<double>((size_t)NWS * nwin, O);
     c->bigs = dalloc<BigWin>(bigs.size(), O);
     c->big64 = mx && opt->big_fp64 != 0;
+    c->small64 = mx && opt->big_fp64 >= 2;
     c->bigF32 = (mx && !c->big64 && c->nbig) ? dalloc<float>(bigF, O) : dalloc<float>(1, O);
     c->bigF64 = ((!mx || c->big64) && c->nbig) ? dalloc<double>(bigF, O) : dalloc<double>(1, O);
     c->bigS = dalloc<double>(bigSv, O);
@@ -1120,11 +1129,13 @@ int xba_create(xba_ctx **out, const xba_batch *batch, const xba_options *opt) {
     if (mx) {  // the FP64 fallback of the mixed solve: the matrix in LDS when it fits, else in global memory
         HIPCK(hipFuncGetAttributes(&fa, (const void *)k_solve<double>));
         const int cap64 = 160 * 1024 - (int)fa.sharedSizeBytes - 256;
-        const int64_t n64 = std::max(c->maxn, 1);
+        const int64_t n64 = std::max(c->maxn, 1), ns64 = std::max(c->maxn_small, 1);
         c->fb_lds_cap = (int)std::max<int64_t>(0, std::min<int64_t>(n64 * ((n64 + 3) & ~3) * 8, cap64));
-        if (c->fb_lds_cap > 64 * 1024)
+        // (the FP64 solve of the one-workgroup windows with small64: sized to those windows only)
+        c->small64_lds = (int)std::max<int64_t>(0, std::min<int64_t>(ns64 * ((ns64 + 3) & ~3) * 8, cap64));
+        if (c->fb_lds_cap > 64 * 1024 || c->small64_lds > 64 * 1024)
             HIPCK(hipFuncSetAttribute((const void *)k_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      c->fb_lds_cap));
+                                      std::max(c->fb_lds_cap, c->small64_lds)));
     }
     c->solve_fork = c->nbig > 0 && c->nbig < c->nwin;
     c->wide_fork = c->fast && c->nnarrow > 0 && c->nchunks > c->nnarrow;

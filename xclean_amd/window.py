@@ -147,8 +147,8 @@ class BatchSolver:
     def __init__(self, problems, precision="fp64", lm_per_chunk=0, slot_tile=0, grid=0, use_graph=True,
                  device="cuda", deterministic=False, with_weights=False, big_n=0, big_fp64=0, tuned=None):
         """``tuned``: a dict of plan options (lm_per_chunk, slot_tile, grid, big_n, big_fp64) chosen by
-        ``xclean_amd.autotune``, overriding the individual arguments.  ``big_fp64`` (mixed precision): the windows
-        above ``big_n`` are factored in FP64 instead of FP32 with FP64 refinement."""
+        ``xclean_amd.autotune``, overriding the individual arguments.  ``big_fp64`` (mixed precision): 1, the windows
+        above ``big_n`` are factored in FP64 instead of FP32 with FP64 refinement; 2, every window."""
         import torch
         _lib.require_gpu()
         self.torch = torch
