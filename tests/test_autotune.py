@@ -1,6 +1,6 @@
 """The autotuner's search and candidate space (host logic, no GPU): coordinate descent keeps the fastest value of
-each option in turn, skips plans the solver rejects, never re-times a plan, and the candidate space always holds the
-solver's defaults."""
+each option in turn (two rounds), skips plans the solver rejects, never re-times a plan, and the candidate space
+always holds the solver's defaults."""
 import numpy as np
 import pytest
 
@@ -24,6 +24,21 @@ def test_coordinate_descent_finds_the_separable_optimum_and_skips_rejected_plans
     assert t == pytest.approx(1.0)
     assert len(seen) == len(set(seen))                       # no plan timed twice
     assert any(tt is None for _o, tt in trials)              # the rejected plan is recorded as such
+
+
+def test_second_round_revisits_an_option_after_a_later_one_moved():
+    # every window in FP64 pays only with the lower tiling threshold, which alone is slower: one round stops at
+    # (0, 2), the second finds (64, 2)
+    table = {(0, 0): 1.0, (64, 0): 1.05, (0, 2): 0.99, (64, 2): 0.9}
+    space = dict(big_n=[0, 64], big_fp64=[0, 2])
+
+    def measure(o):
+        return table[(o["big_n"], o["big_fp64"])]
+
+    one, _, _ = AT.search(measure, space, start=dict(big_n=0, big_fp64=0), order=("big_n", "big_fp64"), rounds=1)
+    assert one == dict(big_n=0, big_fp64=2)
+    two, t, _ = AT.search(measure, space, start=dict(big_n=0, big_fp64=0), order=("big_n", "big_fp64"))
+    assert two == dict(big_n=64, big_fp64=2) and t == pytest.approx(0.9)
 
 
 def test_search_raises_when_the_default_plan_is_rejected():

@@ -11,7 +11,8 @@ Options tuned (``xba_options`` in include/xba.h):
                 refinement (several solves); 2, every window
   grid          persistent grid of the assembly kernels (0: one workgroup per compute unit)
 
-Search: coordinate descent over the options in that order, starting from the solver's defaults.  A candidate is
+Search: coordinate descent over the options in that order, starting from the solver's defaults, two rounds (the
+second revisits each option with the others at their first-round choices).  A candidate is
 timed as the median of ``repeats`` first LM iterations replayed from its captured graph (the initial state is
 reloaded before each, so every candidate times the same work); a candidate the solver rejects (a tile that does
 not fit the LDS) is skipped.  Results are cached in a JSON file keyed by the batch's shape and the device name, so a
@@ -53,10 +54,11 @@ def candidate_space(nlm, ncu, ns, mixed=True):
     )
 
 
-def search(measure, space, start=None, order=ORDER):
+def search(measure, space, start=None, order=ORDER, rounds=2):
     """Coordinate descent: for each option in ``order``, try every value of ``space[option]`` with the others at
-    their current best and keep the fastest.  ``measure(opts) -> seconds`` (None or an exception: rejected).
-    Returns (best options, best seconds, trials [(opts, seconds or None)])."""
+    their current best and keep the fastest; ``rounds`` passes over the options (a plan is timed once).
+    ``measure(opts) -> seconds`` (None or an exception: rejected).  Returns (best options, best seconds, trials
+    [(opts, seconds or None)])."""
     cur = dict(DEFAULTS if start is None else start)
     trials = []
     cache = {}
@@ -75,12 +77,13 @@ def search(measure, space, start=None, order=ORDER):
     best_t = timed(cur)
     if best_t is None:
         raise RuntimeError("the solver's default plan is rejected for this batch")
-    for name in order:
-        for v in space.get(name, []):
-            o = dict(cur, **{name: v})
-            t = timed(o)
-            if t is not None and t < best_t:
-                cur, best_t = o, t
+    for _ in range(rounds):
+        for name in order:
+            for v in space.get(name, []):
+                o = dict(cur, **{name: v})
+                t = timed(o)
+                if t is not None and t < best_t:
+                    cur, best_t = o, t
     return cur, best_t, trials
 
 
