@@ -570,7 +570,7 @@ def main():
     tune = None
     if not args.no_tune:  # plan options picked on this GPU before the timed region (not timed)
         from xclean_amd import autotune
-        tune = autotune.tune(bs, repeats=3, log=log)
+        tune = autotune.tune(bs, repeats=3, log=log, iters=LM_ITERS)  # the timed step's iterations
         log("bench: autotune -> %s (%.3f ms per LM iteration; %s)" % (
             tune["options"], tune["ms"], "cached" if tune["cached"] else "%.1f s" % tune["seconds"]))
     log("bench: plan", bs.plan_info())

@@ -12,10 +12,10 @@ Options tuned (``xba_options`` in include/xba.h):
   grid          persistent grid of the assembly kernels (0: one workgroup per compute unit)
 
 Search: coordinate descent over the options in that order, starting from the solver's defaults, two rounds (the
-second revisits each option with the others at their first-round choices).  A candidate is
-timed as the median of ``repeats`` first LM iterations replayed from its captured graph (the initial state is
-reloaded before each, so every candidate times the same work); a candidate the solver rejects (a tile that does
-not fit the LDS) is skipped.  Results are cached in a JSON file keyed by the batch's shape and the device name, so a
+second revisits each option with the others at their first-round choices).  A candidate is timed as the median of
+``repeats`` runs of the first ``iters`` LM iterations replayed from its captured graph (the initial state is
+reloaded before each, so every candidate times the same work); a candidate the solver rejects (a tile that does not
+fit the LDS) is skipped.  Results are cached in a JSON file keyed by the batch's shape and the device name, so a
 second call with the same workload reuses them without timing.
 """
 from __future__ import annotations
@@ -96,9 +96,11 @@ def workload_key(solver, device_name):
     return "%s|prec%d|%s" % (device_name, solver.prec, h.hexdigest()[:16])
 
 
-def tune(solver, repeats=3, cache_path=DEFAULT_CACHE, use_cache=True, space=None, log=None):
+def tune(solver, repeats=3, cache_path=DEFAULT_CACHE, use_cache=True, space=None, log=None, iters=1):
     """Pick the plan options for ``solver`` (a ``window.BatchSolver``); leaves the solver re-planned with them and
-    returns dict(options, ms, trials, cached, seconds)."""
+    returns dict(options, ms, trials, cached, seconds).  A candidate is timed over ``iters`` LM iterations from the
+    initial state (the workload a caller replays: 1 for a single LM step, the step's iteration count otherwise);
+    ``ms`` is per LM iteration."""
     torch = solver.torch
     dev = solver.dev
     props = torch.cuda.get_device_properties(dev)
@@ -127,9 +129,9 @@ def tune(solver, repeats=3, cache_path=DEFAULT_CACHE, use_cache=True, space=None
             solver.upload(1e-4)
             torch.cuda.synchronize(dev)
             a = time.perf_counter()
-            solver.iterate(1)
+            solver.iterate(iters)
             torch.cuda.synchronize(dev)
-            ts.append(time.perf_counter() - a)
+            ts.append((time.perf_counter() - a) / iters)
         t = float(np.median(ts))
         if log:
             log("autotune %s: %.3f ms" % (opts, t * 1e3))
