@@ -529,8 +529,8 @@ def main():
                          "print their JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (windows would shard across GPUs with "
-              "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (independent windows would shard across GPUs "
+              "with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
