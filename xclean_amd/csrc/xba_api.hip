@@ -228,7 +228,6 @@ DArgs dargs(xba_ctx *c) {
     D.win_par = c->B.win_par;
     D.Rwb = c->Rwb; D.pwb = c->pwb; D.vel = c->vel; D.bgs = c->bgs; D.bas = c->bas;
     D.R2 = c->R2; D.p2 = c->p2; D.v2 = c->v2; D.g2 = c->g2; D.a2 = c->a2;
-    D.lmp = (double *)c->lmpT; D.lmp_trial = (double *)c->lmp_trial;
     D.H = c->H;
     D.b = c->b;
     D.dx = c->dx;
@@ -460,7 +459,9 @@ void launch_iter(xba_ctx *c, hipStream_t s, int part = -1) {
     launch_solve<T>(c, D, s);
     launch_trial_kernel<T>(c, A, s);
     launch_terms(c, D, s, 1);
-    hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);  // (commits the accepted windows' state)
+    hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
+    int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
 }
 
 }  // namespace
@@ -1359,6 +1360,7 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
     hipEvent_t ev[XBA_NPROF + 1];
     for (int i = 0; i <= XBA_NPROF; i++) HIPCK(hipEventCreate(&ev[i]));
     DArgs D = dargs(c);
+    int gl = std::max(1, std::min((c->nlm + 255) / 256, 4096));
     auto run = [&](auto tag) {
         using T = decltype(tag);
         KArgs<T> A = kargs<T>(c);
@@ -1379,9 +1381,10 @@ int xba_profile_iteration(xba_ctx *c, double *ms_out, void *stream) {
         (void)hipEventRecord(ev[7], s);
         launch_terms(c, D, s, 1);
         (void)hipEventRecord(ev[8], s);
-        hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);  // (commits the accepted windows' state)
+        hipLaunchKernelGGL(k_accept, dim3(c->nwin), dim3(256), 0, s, D);
         (void)hipEventRecord(ev[9], s);
-        (void)hipEventRecord(ev[10], s);  // (the commit slot: part of k_accept)
+        hipLaunchKernelGGL(k_commit_lm<T>, dim3(gl), dim3(256), 0, s, A, (const int32_t *)c->lm_win);
+        (void)hipEventRecord(ev[10], s);
     };
     if (mixed(c))
         run(0.0f);

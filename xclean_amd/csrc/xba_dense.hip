@@ -247,7 +247,6 @@ struct DArgs {
     const double *imu_rec, *prior_x0, *prior_S, *prior_r, *win_par;
     double *Rwb, *pwb, *vel, *bgs, *bas;  // state (nslot,...)
     double *R2, *p2, *v2, *g2, *a2;       // trial state
-    double *lmp, *lmp_trial;              // (nlm,3) landmark state and its trial (FP64)
     double *H, *b, *dx, *fact64;          // (sum n^2), (sum n), (sum n), (sum n^2)
     float *fact32;                        // (sum n^2)
     double *imu_out;                      // (nimu, IMU_OUT) per-factor weighted blocks, gradients and cost
@@ -1109,8 +1108,6 @@ __global__ __launch_bounds__(256) void k_accept(DArgs D) {
     }
     __syncthreads();
     if (!sacc) return;
-    // the window's landmarks (vio/window.py:967-970): one contiguous range of 3 L doubles
-    for (int64_t i = tid; i < 3 * (int64_t)W.L; i += nt) D.lmp[3 * (int64_t)W.lm0 + i] = D.lmp_trial[3 * (int64_t)W.lm0 + i];
     for (int s = tid; s < W.P; s += nt) {
         size_t gs = W.slot0 + s;
         for (int i = 0; i < 9; i++) D.Rwb[9 * gs + i] = D.R2[9 * gs + i];

@@ -727,9 +727,24 @@ __global__ __launch_bounds__(XBA_WAVE) void k_trial(KArgs<T> A, int Rmax) {
     }
 }
 
+// Landmark commit of accepted windows (vio/window.py:967-970).
+template <typename T>
+__global__ void k_commit_lm(KArgs<T> A, const int32_t *lm_win) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < A.nlm; l += gridDim.x * blockDim.x) {
+        int w = lm_win[l];
+        if (A.wstate[(size_t)NWS * w + WS_ACCEPT] != 0.0) {
+            A.lmp[3 * (size_t)l] = A.lmp_trial[3 * (size_t)l];
+            A.lmp[3 * (size_t)l + 1] = A.lmp_trial[3 * (size_t)l + 1];
+            A.lmp[3 * (size_t)l + 2] = A.lmp_trial[3 * (size_t)l + 2];
+        }
+    }
+}
+
 template __global__ void k_lin<float>(KArgs<float>, int);
 template __global__ void k_lin<double>(KArgs<double>, int);
 template __global__ void k_trial<float>(KArgs<float>, int);
 template __global__ void k_trial<double>(KArgs<double>, int);
+template __global__ void k_commit_lm<float>(KArgs<float>, const int32_t *);
+template __global__ void k_commit_lm<double>(KArgs<double>, const int32_t *);
 
 }  // namespace xba
