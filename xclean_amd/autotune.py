@@ -104,7 +104,7 @@ def tune(solver, repeats=3, cache_path=DEFAULT_CACHE, use_cache=True, space=None
     torch = solver.torch
     dev = solver.dev
     props = torch.cuda.get_device_properties(dev)
-    key = workload_key(solver, props.name)
+    key = workload_key(solver, props.name) + ("" if iters == 1 else "|it%d" % iters)  # (one entry per objective)
     db = {}
     if use_cache and cache_path and os.path.exists(cache_path):
         try:
